@@ -1,0 +1,213 @@
+/*
+ * libqmctoolscl_amd.so -- C ABI of the MI355X (gfx950) QMC point-generation
+ * and transform library.
+ *
+ * Two families of entry points:
+ *
+ *  1. Drop-ins.  The 23 kernels of QMCToolsCL's C backend, with the exact
+ *     prototypes of qmctoolscl/c_funcs/<name>.c (the same symbol names the
+ *     reference's generated ctypes layer binds, pdm_build.py:79-103):
+ *     host pointers, three sizes, three batch sizes (accepted and ignored:
+ *     the GPU picks its own tiling), then the remaining arguments.  Arrays
+ *     are staged to the current HIP device and back; the call returns when
+ *     the outputs are in the caller's arrays.  They return void like the
+ *     reference; a failure is reported through qmc_last_error_code() and
+ *     qmc_last_error() (thread-local; every drop-in call clears it first).
+ *
+ *  2. Device entry points (*_dev and qmc_*_dev).  Same arguments without the
+ *     batch sizes, device pointers, and a HIP stream; they enqueue and return
+ *     a status code (QMC_OK on success) without synchronizing.
+ *
+ * Numerics: integer outputs (lattice index arithmetic, base-2 and base-b
+ * digits) and the double outputs of the lattice, integer_to_float and FWHT
+ * kernels are bit-identical to the reference C backend; FFT/IFFT outputs agree
+ * to ~1e-15 relative (the twiddle table comes from device cos/sin).
+ *
+ * Inputs the reference leaves undefined (reads past an input array, bases
+ * below 2, natural-order indices needing more than mmax columns, lattice
+ * indices at or above 2^31) are rejected with QMC_ERR_RANGE instead.
+ */
+#ifndef QMCTOOLSCL_AMD_H
+#define QMCTOOLSCL_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QMC_EXPORT __attribute__((visibility("default")))
+
+typedef struct ihipStream_t *qmc_stream_t; /* == hipStream_t */
+typedef uint64_t qmc_u64;
+
+enum {
+    QMC_OK = 0,
+    QMC_ERR_ARG = 1,
+    QMC_ERR_RANGE = 2,
+    QMC_ERR_HIP = 3,
+    QMC_ERR_NOMEM = 4
+};
+
+/* ---------------------------------------------------------------- runtime */
+QMC_EXPORT int qmc_init(void);                 /* allocate per-device state on the current device */
+QMC_EXPORT const char *qmc_version(void);
+QMC_EXPORT const char *qmc_last_error(void);
+QMC_EXPORT int qmc_last_error_code(void);
+QMC_EXPORT void qmc_clear_error(void);
+/* read (and optionally clear) the current device's status bits that kernels
+ * set on inputs the reference leaves undefined; synchronizes the device */
+QMC_EXPORT int qmc_device_status(unsigned *bits, int clear);
+
+/* ------------------------------------------------- drop-ins: lattices
+ * lat_gen_linear.c:3-13, lat_gen_natural.c:3-14, lat_gen_gray.c:3-14,
+ * lat_shift_mod_1.c:3-15 */
+QMC_EXPORT void lat_gen_linear(qmc_u64 r, qmc_u64 n, qmc_u64 d, qmc_u64 bs_r, qmc_u64 bs_n, qmc_u64 bs_d,
+                               const qmc_u64 *g, double *x);
+QMC_EXPORT void lat_gen_natural(qmc_u64 r, qmc_u64 n, qmc_u64 d, qmc_u64 bs_r, qmc_u64 bs_n, qmc_u64 bs_d,
+                                qmc_u64 n_start, const qmc_u64 *g, double *x);
+QMC_EXPORT void lat_gen_gray(qmc_u64 r, qmc_u64 n, qmc_u64 d, qmc_u64 bs_r, qmc_u64 bs_n, qmc_u64 bs_d,
+                             qmc_u64 n_start, const qmc_u64 *g, double *x);
+QMC_EXPORT void lat_shift_mod_1(qmc_u64 r, qmc_u64 n, qmc_u64 d, qmc_u64 bs_r, qmc_u64 bs_n, qmc_u64 bs_d,
+                                qmc_u64 r_x, const double *x, const double *shifts, double *xr);
+
+/* ------------------------------------------- drop-ins: base-2 digital nets
+ * dnb2_gmat_lsb_to_msb.c:3-14, dnb2_linear_matrix_scramble.c:3-16,
+ * dnb2_gen_natural.c:3-15, dnb2_gen_gray.c:3-15, dnb2_digital_shift.c:3-16,
+ * dnb2_integer_to_float.c:3-14, dnb2_interlace.c:3-17, dnb2_undo_interlace.c:3-17 */
+QMC_EXPORT void dnb2_gmat_lsb_to_msb(qmc_u64 r, qmc_u64 d, qmc_u64 mmax, qmc_u64 bs_r, qmc_u64 bs_d,
+                                     qmc_u64 bs_mmax, const qmc_u64 *tmaxes, const qmc_u64 *C_lsb, qmc_u64 *C_msb);
+QMC_EXPORT void dnb2_linear_matrix_scramble(qmc_u64 r, qmc_u64 d, qmc_u64 mmax, qmc_u64 bs_r, qmc_u64 bs_d,
+                                            qmc_u64 bs_mmax, qmc_u64 r_C, qmc_u64 tmax_new, const qmc_u64 *S,
+                                            const qmc_u64 *C, qmc_u64 *C_lms);
+QMC_EXPORT void dnb2_gen_natural(qmc_u64 r, qmc_u64 n, qmc_u64 d, qmc_u64 bs_r, qmc_u64 bs_n, qmc_u64 bs_d,
+                                 qmc_u64 n_start, qmc_u64 mmax, const qmc_u64 *C, qmc_u64 *xb);
+QMC_EXPORT void dnb2_gen_gray(qmc_u64 r, qmc_u64 n, qmc_u64 d, qmc_u64 bs_r, qmc_u64 bs_n, qmc_u64 bs_d,
+                              qmc_u64 n_start, qmc_u64 mmax, const qmc_u64 *C, qmc_u64 *xb);
+QMC_EXPORT void dnb2_digital_shift(qmc_u64 r, qmc_u64 n, qmc_u64 d, qmc_u64 bs_r, qmc_u64 bs_n, qmc_u64 bs_d,
+                                   qmc_u64 r_x, const qmc_u64 *lshifts, const qmc_u64 *xb, const qmc_u64 *shiftsb,
+                                   qmc_u64 *xrb);
+QMC_EXPORT void dnb2_integer_to_float(qmc_u64 r, qmc_u64 n, qmc_u64 d, qmc_u64 bs_r, qmc_u64 bs_n, qmc_u64 bs_d,
+                                      const qmc_u64 *tmaxes, const qmc_u64 *xb, double *x);
+QMC_EXPORT void dnb2_interlace(qmc_u64 r, qmc_u64 d_alpha, qmc_u64 mmax, qmc_u64 bs_r, qmc_u64 bs_d_alpha,
+                               qmc_u64 bs_mmax, qmc_u64 d, qmc_u64 tmax, qmc_u64 tmax_alpha, qmc_u64 alpha,
+                               const qmc_u64 *C, qmc_u64 *C_alpha);
+QMC_EXPORT void dnb2_undo_interlace(qmc_u64 r, qmc_u64 d, qmc_u64 mmax, qmc_u64 bs_r, qmc_u64 bs_d, qmc_u64 bs_mmax,
+                                    qmc_u64 d_alpha, qmc_u64 tmax, qmc_u64 tmax_alpha, qmc_u64 alpha,
+                                    const qmc_u64 *C_alpha, qmc_u64 *C);
+
+/* ----------------------------------- drop-ins: generalized digital nets
+ * gdn_gen_natural.c:3-18, gdn_gen_natural_same_base.c:3-17,
+ * gdn_linear_matrix_scramble.c:3-19, gdn_digital_permutation.c:3-19,
+ * gdn_digital_shift.c:3-19, gdn_integer_to_float.c:3-16, gdn_interlace.c:3-17,
+ * gdn_undo_interlace.c:3-17 */
+QMC_EXPORT void gdn_gen_natural(qmc_u64 r, qmc_u64 n, qmc_u64 d, qmc_u64 bs_r, qmc_u64 bs_n, qmc_u64 bs_d,
+                                qmc_u64 r_b, qmc_u64 mmax, qmc_u64 tmax, qmc_u64 n_start, const qmc_u64 *bases,
+                                const qmc_u64 *C, qmc_u64 *xdig);
+QMC_EXPORT void gdn_gen_natural_same_base(qmc_u64 r, qmc_u64 n, qmc_u64 d, qmc_u64 bs_r, qmc_u64 bs_n,
+                                          qmc_u64 bs_d, qmc_u64 mmax, qmc_u64 tmax, qmc_u64 n_start, qmc_u64 b,
+                                          const qmc_u64 *C, qmc_u64 *xdig);
+QMC_EXPORT void gdn_linear_matrix_scramble(qmc_u64 r, qmc_u64 d, qmc_u64 mmax, qmc_u64 bs_r, qmc_u64 bs_d,
+                                           qmc_u64 bs_mmax, qmc_u64 r_C, qmc_u64 r_b, qmc_u64 tmax,
+                                           qmc_u64 tmax_new, const qmc_u64 *bases, const qmc_u64 *S,
+                                           const qmc_u64 *C, qmc_u64 *C_lms);
+QMC_EXPORT void gdn_digital_permutation(qmc_u64 r, qmc_u64 n, qmc_u64 d, qmc_u64 bs_r, qmc_u64 bs_n, qmc_u64 bs_d,
+                                        qmc_u64 r_x, qmc_u64 r_b, qmc_u64 tmax, qmc_u64 tmax_new, qmc_u64 bmax,
+                                        const qmc_u64 *perms, const qmc_u64 *xdig, qmc_u64 *xdig_new);
+QMC_EXPORT void gdn_digital_shift(qmc_u64 r, qmc_u64 n, qmc_u64 d, qmc_u64 bs_r, qmc_u64 bs_n, qmc_u64 bs_d,
+                                  qmc_u64 r_x, qmc_u64 r_b, qmc_u64 tmax, qmc_u64 tmax_new, const qmc_u64 *bases,
+                                  const qmc_u64 *shifts, const qmc_u64 *xdig, qmc_u64 *xdig_new);
+QMC_EXPORT void gdn_integer_to_float(qmc_u64 r, qmc_u64 n, qmc_u64 d, qmc_u64 bs_r, qmc_u64 bs_n, qmc_u64 bs_d,
+                                     qmc_u64 r_b, qmc_u64 tmax, const qmc_u64 *bases, const qmc_u64 *xdig, double *x);
+QMC_EXPORT void gdn_interlace(qmc_u64 r, qmc_u64 d_alpha, qmc_u64 mmax, qmc_u64 bs_r, qmc_u64 bs_d_alpha,
+                              qmc_u64 bs_mmax, qmc_u64 d, qmc_u64 tmax, qmc_u64 tmax_alpha, qmc_u64 alpha,
+                              const qmc_u64 *C, qmc_u64 *C_alpha);
+QMC_EXPORT void gdn_undo_interlace(qmc_u64 r, qmc_u64 d, qmc_u64 mmax, qmc_u64 bs_r, qmc_u64 bs_d, qmc_u64 bs_mmax,
+                                   qmc_u64 d_alpha, qmc_u64 tmax, qmc_u64 tmax_alpha, qmc_u64 alpha,
+                                   const qmc_u64 *C_alpha, qmc_u64 *C);
+
+/* --------------------------------------------- drop-ins: fast transforms
+ * fwht_1d_radix2.c:3-14, fft_bro_1d_radix2.c:3-17, ifft_bro_1d_radix2.c:3-17 */
+QMC_EXPORT void fwht_1d_radix2(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, qmc_u64 bs_d1, qmc_u64 bs_d2,
+                               qmc_u64 bs_n_half, double *x);
+QMC_EXPORT void fft_bro_1d_radix2(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, qmc_u64 bs_d1, qmc_u64 bs_d2,
+                                  qmc_u64 bs_n_half, double *twiddler, double *twiddlei, double *xr, double *xi);
+QMC_EXPORT void ifft_bro_1d_radix2(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, qmc_u64 bs_d1, qmc_u64 bs_d2,
+                                   qmc_u64 bs_n_half, double *twiddler, double *twiddlei, double *xr, double *xi);
+
+/* ----------------------------------------------- device entry points
+ * Each replaces the drop-in of the same name (same citation) for callers
+ * whose buffers already live on the GPU. */
+QMC_EXPORT int lat_gen_linear_dev(qmc_u64 r, qmc_u64 n, qmc_u64 d, const qmc_u64 *g, double *x, qmc_stream_t s);
+QMC_EXPORT int lat_gen_natural_dev(qmc_u64 r, qmc_u64 n, qmc_u64 d, qmc_u64 n_start, const qmc_u64 *g, double *x,
+                                   qmc_stream_t s);
+QMC_EXPORT int lat_gen_gray_dev(qmc_u64 r, qmc_u64 n, qmc_u64 d, qmc_u64 n_start, const qmc_u64 *g, double *x,
+                                qmc_stream_t s);
+QMC_EXPORT int lat_shift_mod_1_dev(qmc_u64 r, qmc_u64 n, qmc_u64 d, qmc_u64 r_x, const double *x,
+                                   const double *shifts, double *xr, qmc_stream_t s);
+QMC_EXPORT int dnb2_gmat_lsb_to_msb_dev(qmc_u64 r, qmc_u64 d, qmc_u64 mmax, const qmc_u64 *tmaxes,
+                                        const qmc_u64 *C_lsb, qmc_u64 *C_msb, qmc_stream_t s);
+QMC_EXPORT int dnb2_linear_matrix_scramble_dev(qmc_u64 r, qmc_u64 d, qmc_u64 mmax, qmc_u64 r_C, qmc_u64 tmax_new,
+                                               const qmc_u64 *S, const qmc_u64 *C, qmc_u64 *C_lms, qmc_stream_t s);
+QMC_EXPORT int dnb2_gen_natural_dev(qmc_u64 r, qmc_u64 n, qmc_u64 d, qmc_u64 n_start, qmc_u64 mmax,
+                                    const qmc_u64 *C, qmc_u64 *xb, qmc_stream_t s);
+QMC_EXPORT int dnb2_gen_gray_dev(qmc_u64 r, qmc_u64 n, qmc_u64 d, qmc_u64 n_start, qmc_u64 mmax, const qmc_u64 *C,
+                                 qmc_u64 *xb, qmc_stream_t s);
+QMC_EXPORT int dnb2_digital_shift_dev(qmc_u64 r, qmc_u64 n, qmc_u64 d, qmc_u64 r_x, const qmc_u64 *lshifts,
+                                      const qmc_u64 *xb, const qmc_u64 *shiftsb, qmc_u64 *xrb, qmc_stream_t s);
+QMC_EXPORT int dnb2_integer_to_float_dev(qmc_u64 r, qmc_u64 n, qmc_u64 d, const qmc_u64 *tmaxes,
+                                         const qmc_u64 *xb, double *x, qmc_stream_t s);
+QMC_EXPORT int dnb2_interlace_dev(qmc_u64 r, qmc_u64 d_alpha, qmc_u64 mmax, qmc_u64 d, qmc_u64 tmax,
+                                  qmc_u64 tmax_alpha, qmc_u64 alpha, const qmc_u64 *C, qmc_u64 *C_alpha,
+                                  qmc_stream_t s);
+QMC_EXPORT int dnb2_undo_interlace_dev(qmc_u64 r, qmc_u64 d, qmc_u64 mmax, qmc_u64 d_alpha, qmc_u64 tmax,
+                                       qmc_u64 tmax_alpha, qmc_u64 alpha, const qmc_u64 *C_alpha, qmc_u64 *C,
+                                       qmc_stream_t s);
+QMC_EXPORT int gdn_gen_natural_dev(qmc_u64 r, qmc_u64 n, qmc_u64 d, qmc_u64 r_b, qmc_u64 mmax, qmc_u64 tmax,
+                                   qmc_u64 n_start, const qmc_u64 *bases, const qmc_u64 *C, qmc_u64 *xdig,
+                                   qmc_stream_t s);
+QMC_EXPORT int gdn_gen_natural_same_base_dev(qmc_u64 r, qmc_u64 n, qmc_u64 d, qmc_u64 mmax, qmc_u64 tmax,
+                                             qmc_u64 n_start, qmc_u64 b, const qmc_u64 *C, qmc_u64 *xdig,
+                                             qmc_stream_t s);
+QMC_EXPORT int gdn_linear_matrix_scramble_dev(qmc_u64 r, qmc_u64 d, qmc_u64 mmax, qmc_u64 r_C, qmc_u64 r_b,
+                                              qmc_u64 tmax, qmc_u64 tmax_new, const qmc_u64 *bases,
+                                              const qmc_u64 *S, const qmc_u64 *C, qmc_u64 *C_lms, qmc_stream_t s);
+QMC_EXPORT int gdn_digital_permutation_dev(qmc_u64 r, qmc_u64 n, qmc_u64 d, qmc_u64 r_x, qmc_u64 r_b, qmc_u64 tmax,
+                                           qmc_u64 tmax_new, qmc_u64 bmax, const qmc_u64 *perms,
+                                           const qmc_u64 *xdig, qmc_u64 *xdig_new, qmc_stream_t s);
+QMC_EXPORT int gdn_digital_shift_dev(qmc_u64 r, qmc_u64 n, qmc_u64 d, qmc_u64 r_x, qmc_u64 r_b, qmc_u64 tmax,
+                                     qmc_u64 tmax_new, const qmc_u64 *bases, const qmc_u64 *shifts,
+                                     const qmc_u64 *xdig, qmc_u64 *xdig_new, qmc_stream_t s);
+QMC_EXPORT int gdn_integer_to_float_dev(qmc_u64 r, qmc_u64 n, qmc_u64 d, qmc_u64 r_b, qmc_u64 tmax,
+                                        const qmc_u64 *bases, const qmc_u64 *xdig, double *x, qmc_stream_t s);
+QMC_EXPORT int gdn_interlace_dev(qmc_u64 r, qmc_u64 d_alpha, qmc_u64 mmax, qmc_u64 d, qmc_u64 tmax,
+                                 qmc_u64 tmax_alpha, qmc_u64 alpha, const qmc_u64 *C, qmc_u64 *C_alpha,
+                                 qmc_stream_t s);
+QMC_EXPORT int gdn_undo_interlace_dev(qmc_u64 r, qmc_u64 d, qmc_u64 mmax, qmc_u64 d_alpha, qmc_u64 tmax,
+                                      qmc_u64 tmax_alpha, qmc_u64 alpha, const qmc_u64 *C_alpha, qmc_u64 *C,
+                                      qmc_stream_t s);
+QMC_EXPORT int fwht_1d_radix2_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, double *x, qmc_stream_t s);
+QMC_EXPORT int fft_bro_1d_radix2_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, double *twiddler, double *twiddlei,
+                                     double *xr, double *xi, qmc_stream_t s);
+QMC_EXPORT int ifft_bro_1d_radix2_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, double *twiddler, double *twiddlei,
+                                      double *xr, double *xi, qmc_stream_t s);
+
+/* ------------------------------------------------------- fused pipelines
+ * No single reference symbol: each equals a chain of reference calls as the
+ * doctests chain them (docs/index.md:64-208, 419-677), in one HBM pass. */
+
+/* lat_gen_<order>(r_g, ...) then lat_shift_mod_1(r, ..., r_x = r_g, ...);
+ * order 0 linear, 1 natural, 2 gray */
+QMC_EXPORT int qmc_lat_gen_shift_dev(int order, qmc_u64 r, qmc_u64 n, qmc_u64 d, qmc_u64 r_g, qmc_u64 n_start,
+                                     const qmc_u64 *g, const double *shifts, double *xr, qmc_stream_t s);
+/* dnb2_gen_<natural|gray>(r_x, ...) -> dnb2_digital_shift(r, ..., r_x, ...) ->
+ * dnb2_integer_to_float(r, ...); xrb and/or x may be NULL */
+QMC_EXPORT int qmc_dnb2_gen_shift_float_dev(int natural, qmc_u64 r, qmc_u64 n, qmc_u64 d, qmc_u64 r_x,
+                                            qmc_u64 n_start, qmc_u64 mmax, const qmc_u64 *C,
+                                            const qmc_u64 *lshifts, const qmc_u64 *shiftsb,
+                                            const qmc_u64 *tmaxes, qmc_u64 *xrb, double *x, qmc_stream_t s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QMCTOOLSCL_AMD_H */

@@ -1,0 +1,331 @@
+// Base-2 digital nets on gfx950: generating-matrix conversion, linear matrix
+// scrambling (LMS) over GF(2), point generation in natural and Gray order,
+// digital shift, conversion to double, interlacing, and the fused
+// generate -> shift -> float pipeline.
+//
+// A point's value is the XOR of the generating-matrix columns selected by the
+// bits of its index (natural order: q = row + n_start; Gray order: Gray(k),
+// k = row + n_start).  The reference walks all rows serially
+// (dnb2_gen_natural.c:53-70); here each thread walks its own run of rows for
+// one (replication, dimension) column:
+//   Gray order   : next value = value ^ C[ctz(k+1)]            (one XOR per row)
+//   natural order: next value = value ^ C[0] ^ ... ^ C[ctz(q+1)]  (the bits that
+//                  flip when q is incremented; two columns per row on average)
+// so all runs proceed in parallel and lanes write consecutive dimensions.
+#include "qmc_common.hpp"
+
+namespace {
+
+// XOR of the columns selected by the bits of q
+__device__ __forceinline__ u64 xor_cols(const u64 *__restrict__ col, u64 q) {
+    u64 v = 0;
+    while (q) {
+        v ^= col[__ffsll((long long)q) - 1];
+        q &= q - 1;
+    }
+    return v;
+}
+
+struct RunMap {
+    u64 r, n, d, nd, n_start, runs;
+    u32 K;      // rows per run
+    int check;  // natural order over a range not known to be closed under Gray
+};
+
+// thread -> (l, run, j) with j fastest so lanes write consecutive dimensions
+__device__ __forceinline__ bool run_of(const RunMap &m, u64 t, u64 &l, u64 &run, u64 &j) {
+    if (t >= m.r * m.runs * m.d) return false;
+    const u64 rest = t / m.d;
+    j = t - rest * m.d;
+    l = rest / m.runs;
+    run = rest - l * m.runs;
+    return true;
+}
+
+// Emit xb (and the fused shift/float) for one run.  OP = 0: plain generation.
+// OP = 1: fused with digital shift and conversion to double.
+struct FuseArgs {
+    u64 r_out;              // replications after the shift (>= r of the net)
+    const u64 *lshifts;     // per net replication
+    const u64 *shiftsb;     // r_out * d
+    const u64 *tmaxes;      // per output replication
+    u64 *xrb;               // may be null
+    double *x;              // may be null
+};
+
+template <bool NATURAL, int OP>
+__global__ __launch_bounds__(kBlock) void dnb2_gen_kernel(RunMap m, u64 mmax,
+                                                          const u64 *__restrict__ C,
+                                                          u64 *__restrict__ xb, FuseArgs f) {
+    u64 l, run, j;
+    if (!run_of(m, (u64)blockIdx.x * blockDim.x + threadIdx.x, l, run, j)) return;
+    const u64 *col = C + (l * m.d + j) * mmax;
+    const u64 i0 = run * m.K;
+    const u64 i1 = (i0 + m.K < m.n) ? i0 + m.K : m.n;
+    u64 lsh = 0;
+    if (OP == 1) lsh = f.lshifts[l] & 63;  // x86 masks variable shift counts to 6 bits
+    u64 k = i0 + m.n_start;
+    u64 v = xor_cols(col, NATURAL ? k : gray_of(k));
+    for (u64 i = i0;;) {
+        bool write = true;
+        if (NATURAL && m.check) {
+            const u64 kk = gray_inverse(k);
+            write = kk >= m.n_start && kk - m.n_start < m.n;
+        }
+        if (write) {
+            const u64 e2 = i * m.d + j;
+            if (OP == 0) {
+                xb[l * m.nd + e2] = v;
+            } else {
+                const u64 sv = v << lsh;
+                for (u64 lo = l; lo < f.r_out; lo += m.r) {
+                    const u64 out = sv ^ f.shiftsb[lo * m.d + j];
+                    if (f.xrb) store_stream(f.xrb + lo * m.nd + e2, out);
+                    if (f.x) store_stream(f.x + lo * m.nd + e2, ldexp((double)out, -(int)f.tmaxes[lo]));
+                }
+            }
+        }
+        if (++i >= i1) break;
+        k++;
+        const int c = __ffsll((long long)k) - 1;  // ctz(k)
+        if (NATURAL) {
+            for (int b = 0; b <= c; b++) v ^= col[b];
+        } else {
+            v ^= col[c];
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void lsb_to_msb_kernel(u64 r, u64 dm, const u64 *__restrict__ tmaxes,
+                                                            const u64 *__restrict__ src,
+                                                            u64 *__restrict__ dst) {
+    const u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= r * dm) return;
+    const u64 tmax = tmaxes[e / dm];
+    u64 v = src[e], out = 0;
+    // dnb2_gmat_lsb_to_msb.c:34-40; the shift count is taken mod 64 as x86 does
+    for (u64 t = 0; v != 0; t++, v >>= 1)
+        if (v & 1) out += 1ull << ((tmax - t - 1) & 63);
+    dst[e] = out;
+}
+
+// C_lms[l,j,k] = sum_t parity(S[l,j,t] & C[(l,j,k) mod r_C]) << (tmax_new-1-t);
+// one block per (l, j): the rows of S are staged in LDS once and every lane
+// takes columns k.
+__global__ __launch_bounds__(64) void lms2_kernel(u64 mmax, u64 nC, u64 tmax_new,
+                                                  const u64 *__restrict__ S,
+                                                  const u64 *__restrict__ C,
+                                                  u64 *__restrict__ C_lms) {
+    __shared__ u64 rows[64];
+    const u64 lj = blockIdx.x;  // l * d + j
+    for (u64 t = threadIdx.x; t < tmax_new && t < 64; t += blockDim.x) rows[t] = S[lj * tmax_new + t];
+    __syncthreads();
+    for (u64 k = threadIdx.x; k < mmax; k += blockDim.x) {
+        const u64 e = lj * mmax + k;
+        const u64 col = C[e % nC];
+        u64 out = 0;
+        for (u64 t = 0; t < tmax_new; t++) {
+            const u64 row = t < 64 ? rows[t] : S[lj * tmax_new + t];
+            if (__popcll(row & col) & 1) out += 1ull << ((tmax_new - t - 1) & 63);
+        }
+        C_lms[e] = out;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void shift2_kernel(u64 r, u64 r_x, u64 nd, u64 d,
+                                                        const u64 *__restrict__ lshifts,
+                                                        const u64 *__restrict__ xb,
+                                                        const u64 *__restrict__ shiftsb,
+                                                        u64 *__restrict__ xrb) {
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 e2 = (u64)blockIdx.x * blockDim.x + threadIdx.x; e2 < nd; e2 += stride) {
+        const u64 j = e2 % d;
+        for (u64 lx = 0; lx < r_x && lx < r; lx++) {
+            const u64 v = xb[lx * nd + e2] << (lshifts[lx] & 63);
+            for (u64 l = lx; l < r; l += r_x) store_stream(xrb + l * nd + e2, v ^ shiftsb[l * d + j]);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void to_float2_kernel(u64 r, u64 nd, const u64 *__restrict__ tmaxes,
+                                                           const u64 *__restrict__ xb,
+                                                           double *__restrict__ x) {
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 e2 = (u64)blockIdx.x * blockDim.x + threadIdx.x; e2 < nd; e2 += stride)
+        for (u64 l = 0; l < r; l++)
+            store_stream(x + l * nd + e2, ldexp((double)xb[l * nd + e2], -(int)tmaxes[l]));
+}
+
+// dnb2_interlace.c:26-45 (undo == 0) and dnb2_undo_interlace.c:26-45 (undo == 1)
+__global__ __launch_bounds__(kBlock) void interlace2_kernel(int undo, u64 r, u64 d_out, u64 mmax, u64 d_in,
+                                                            u64 tmax, u64 tmax_alpha, u64 alpha,
+                                                            const u64 *__restrict__ src,
+                                                            u64 *__restrict__ dst) {
+    const u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= r * d_out * mmax) return;
+    const u64 k = e % mmax;
+    const u64 lj = e / mmax;
+    const u64 jo = lj % d_out;
+    const u64 l = lj / d_out;
+    u64 out = 0;
+    if (!undo) {
+        for (u64 ta = 0; ta < tmax_alpha; ta++) {
+            const u64 t = ta / alpha;
+            const u64 j = jo * alpha + ta % alpha;
+            if ((src[(l * d_in + j) * mmax + k] >> ((tmax - t - 1) & 63)) & 1)
+                out += 1ull << ((tmax_alpha - ta - 1) & 63);
+        }
+    } else {
+        const u64 ja = jo / alpha;
+        const u64 tt = jo % alpha;
+        for (u64 t = 0; t < tmax; t++) {
+            const u64 ta = t * alpha + tt;
+            if ((src[(l * d_in + ja) * mmax + k] >> ((tmax_alpha - ta - 1) & 63)) & 1)
+                out += 1ull << ((tmax - t - 1) & 63);
+        }
+    }
+    dst[e] = out;
+}
+
+bool pow2_or_zero(u64 v) { return (v & (v - 1)) == 0; }
+
+RunMap make_runs(u64 r, u64 n, u64 d, u64 n_start, int check) {
+    RunMap m;
+    m.r = r;
+    m.n = n;
+    m.d = d;
+    m.nd = n * d;
+    m.n_start = n_start;
+    m.check = check;
+    // aim for >= 64K threads so every CU has several waves, with runs of 16..256 rows
+    u64 K = (r * d * n) / (1u << 16);
+    if (K < 16) K = 16;
+    if (K > 256) K = 256;
+    m.K = (u32)K;
+    m.runs = (n + K - 1) / K;
+    return m;
+}
+
+int check_bits(const char *who, u64 n_start, u64 n, u64 mmax) {
+    if (mmax < 64 && (n_start > (1ull << mmax) || n > (1ull << mmax) - n_start))
+        return qmc_fail(QMC_ERR_RANGE, "%s: indices up to n_start+n-1 = %llu need more than mmax = %llu "
+                        "columns; the reference would read outside the generating matrix", who,
+                        (unsigned long long)(n_start + n - 1), (unsigned long long)mmax);
+    return QMC_OK;
+}
+
+template <int OP>
+int launch_gen(bool natural, u64 r, u64 n, u64 d, u64 n_start, u64 mmax, const u64 *C, u64 *xb,
+               FuseArgs f, hipStream_t s) {
+    const int check = natural && !(pow2_or_zero(n_start) && pow2_or_zero(n_start + n));
+    RunMap m = make_runs(r, n, d, n_start, check);
+    const u64 threads = r * m.runs * d;
+    dim3 grid(grid_for(threads, kBlock, 0x7fffffffu));
+    if (natural)
+        hipLaunchKernelGGL((dnb2_gen_kernel<true, OP>), grid, dim3(kBlock), 0, s, m, mmax, C, xb, f);
+    else
+        hipLaunchKernelGGL((dnb2_gen_kernel<false, OP>), grid, dim3(kBlock), 0, s, m, mmax, C, xb, f);
+    QMC_HIP_CHECK(hipGetLastError());
+    return QMC_OK;
+}
+
+unsigned ew_grid(u64 work) { return grid_for(work, kBlock, 2048 * 8); }
+
+}  // namespace
+
+extern "C" {
+
+int dnb2_gmat_lsb_to_msb_dev(u64 r, u64 d, u64 mmax, const u64 *tmaxes, const u64 *C_lsb, u64 *C_msb,
+                             hipStream_t s) {
+    const u64 total = r * d * mmax;
+    if (!total) return QMC_OK;
+    hipLaunchKernelGGL(lsb_to_msb_kernel, dim3(grid_for(total, kBlock, 0x7fffffffu)), dim3(kBlock), 0, s,
+                       r, d * mmax, tmaxes, C_lsb, C_msb);
+    QMC_HIP_CHECK(hipGetLastError());
+    return QMC_OK;
+}
+
+int dnb2_linear_matrix_scramble_dev(u64 r, u64 d, u64 mmax, u64 r_C, u64 tmax_new, const u64 *S,
+                                    const u64 *C, u64 *C_lms, hipStream_t s) {
+    if (!(r && d && mmax)) return QMC_OK;
+    if (r_C == 0) return qmc_fail(QMC_ERR_ARG, "dnb2_linear_matrix_scramble: r_C must be positive");
+    if (r * d > 0x7fffffffull) return qmc_fail(QMC_ERR_ARG, "dnb2_linear_matrix_scramble: r*d too large");
+    hipLaunchKernelGGL(lms2_kernel, dim3((unsigned)(r * d)), dim3(64), 0, s, mmax, r_C * d * mmax, tmax_new, S, C, C_lms);
+    QMC_HIP_CHECK(hipGetLastError());
+    return QMC_OK;
+}
+
+int dnb2_gen_natural_dev(u64 r, u64 n, u64 d, u64 n_start, u64 mmax, const u64 *C, u64 *xb, hipStream_t s) {
+    if (!(r && n && d)) return QMC_OK;
+    int rc = check_bits("dnb2_gen_natural", n_start, n, mmax);
+    if (rc) return rc;
+    return launch_gen<0>(true, r, n, d, n_start, mmax, C, xb, FuseArgs{}, s);
+}
+
+int dnb2_gen_gray_dev(u64 r, u64 n, u64 d, u64 n_start, u64 mmax, const u64 *C, u64 *xb, hipStream_t s) {
+    if (!(r && n && d)) return QMC_OK;
+    int rc = check_bits("dnb2_gen_gray", n_start, n, mmax);
+    if (rc) return rc;
+    return launch_gen<0>(false, r, n, d, n_start, mmax, C, xb, FuseArgs{}, s);
+}
+
+int dnb2_digital_shift_dev(u64 r, u64 n, u64 d, u64 r_x, const u64 *lshifts, const u64 *xb,
+                           const u64 *shiftsb, u64 *xrb, hipStream_t s) {
+    if (!(r && n && d)) return QMC_OK;
+    if (r_x == 0) return qmc_fail(QMC_ERR_ARG, "dnb2_digital_shift: r_x must be positive");
+    hipLaunchKernelGGL(shift2_kernel, dim3(ew_grid(n * d)), dim3(kBlock), 0, s, r, r_x, n * d, d, lshifts, xb,
+                       shiftsb, xrb);
+    QMC_HIP_CHECK(hipGetLastError());
+    return QMC_OK;
+}
+
+int dnb2_integer_to_float_dev(u64 r, u64 n, u64 d, const u64 *tmaxes, const u64 *xb, double *x,
+                              hipStream_t s) {
+    if (!(r && n && d)) return QMC_OK;
+    hipLaunchKernelGGL(to_float2_kernel, dim3(ew_grid(n * d)), dim3(kBlock), 0, s, r, n * d, tmaxes, xb, x);
+    QMC_HIP_CHECK(hipGetLastError());
+    return QMC_OK;
+}
+
+int dnb2_interlace_dev(u64 r, u64 d_alpha, u64 mmax, u64 d, u64 tmax, u64 tmax_alpha, u64 alpha,
+                       const u64 *C, u64 *C_alpha, hipStream_t s) {
+    const u64 total = r * d_alpha * mmax;
+    if (!total) return QMC_OK;
+    if (alpha == 0) return qmc_fail(QMC_ERR_ARG, "dnb2_interlace: alpha must be positive");
+    if (d_alpha * alpha > d)
+        return qmc_fail(QMC_ERR_RANGE, "dnb2_interlace: d_alpha*alpha <= d is required");
+    hipLaunchKernelGGL(interlace2_kernel, dim3(grid_for(total, kBlock, 0x7fffffffu)), dim3(kBlock), 0, s, 0, r,
+                       d_alpha, mmax, d, tmax, tmax_alpha, alpha, C, C_alpha);
+    QMC_HIP_CHECK(hipGetLastError());
+    return QMC_OK;
+}
+
+int dnb2_undo_interlace_dev(u64 r, u64 d, u64 mmax, u64 d_alpha, u64 tmax, u64 tmax_alpha, u64 alpha,
+                            const u64 *C_alpha, u64 *C, hipStream_t s) {
+    const u64 total = r * d * mmax;
+    if (!total) return QMC_OK;
+    if (alpha == 0) return qmc_fail(QMC_ERR_ARG, "dnb2_undo_interlace: alpha must be positive");
+    if ((d + alpha - 1) / alpha > d_alpha)
+        return qmc_fail(QMC_ERR_RANGE, "dnb2_undo_interlace: ceil(d/alpha) <= d_alpha is required");
+    hipLaunchKernelGGL(interlace2_kernel, dim3(grid_for(total, kBlock, 0x7fffffffu)), dim3(kBlock), 0, s, 1, r,
+                       d, mmax, d_alpha, tmax, tmax_alpha, alpha, C_alpha, C);
+    QMC_HIP_CHECK(hipGetLastError());
+    return QMC_OK;
+}
+
+// Fused dnb2_gen_<order>(r_x nets) -> dnb2_digital_shift(r, r_x) ->
+// dnb2_integer_to_float(r): one pass that writes the shifted integers (xrb,
+// optional) and/or their doubles (x, optional).  natural: 1 natural, 0 Gray.
+int qmc_dnb2_gen_shift_float_dev(int natural, u64 r, u64 n, u64 d, u64 r_x, u64 n_start, u64 mmax,
+                                 const u64 *C, const u64 *lshifts, const u64 *shiftsb,
+                                 const u64 *tmaxes, u64 *xrb, double *x, hipStream_t s) {
+    if (!(r && n && d)) return QMC_OK;
+    if (r_x == 0) return qmc_fail(QMC_ERR_ARG, "dnb2_gen_shift_float: r_x must be positive");
+    int rc = check_bits("dnb2_gen_shift_float", n_start, n, mmax);
+    if (rc) return rc;
+    const u64 rx = r_x < r ? r_x : r;
+    FuseArgs f{r, lshifts, shiftsb, tmaxes, xrb, x};
+    return launch_gen<1>(natural != 0, rx, n, d, n_start, mmax, C, nullptr, f, s);
+}
+
+}  // extern "C"

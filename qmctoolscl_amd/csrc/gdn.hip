@@ -1,0 +1,397 @@
+// Generalized (base-b) digital nets and Halton points on gfx950: generation,
+// linear matrix scrambling, digital shift, digit permutation, conversion to
+// double, and interlacing.
+//
+// Digit outputs are laid out [l][i][j][t] with t fastest, so for one
+// replication l the rows rho = i*d + j and their digits form one contiguous
+// array.  The row kernels give each block R consecutive rows: lane f of the
+// block produces digit (rho0 + f / T, f % T), so stores are fully coalesced,
+// and anything shared by a row (its base, the base-b digits of its index) is
+// computed once into LDS.
+//
+// Arithmetic follows the reference's unsigned 64-bit semantics exactly,
+// including wrap-around: gdn_gen_natural.c:40-50 reduces after every digit,
+// x = (x + dig*C) % b.  When b < 2^16 and every C entry used is < 2^32, no
+// step can wrap, so the sum is accumulated exactly and reduced once; any
+// element that does not meet that bound takes the step-by-step path.
+#include "qmc_common.hpp"
+
+namespace {
+
+constexpr int kRows = 64;        // rows per block in the row kernels
+constexpr int kMaxDigits = 64;   // base-b digits of a 64-bit index, b >= 2
+
+enum { STATUS_BAD_BASE = 1u, STATUS_OOB = 2u };
+
+__device__ __forceinline__ void flag(u32 *status, u32 bit) { atomicOr(status, bit); }
+
+// Row geometry shared by the kernels below: rho -> j via a 32-bit magic
+// division when n*d < 2^32.
+struct Rows {
+    u64 nd, d;
+    DivU32 ddiv, tdiv;
+    int small;
+};
+
+__device__ __forceinline__ u64 j_of(const Rows &g, u64 rho) {
+    if (g.small) {
+        const u32 q = div_q((u32)rho, g.ddiv);
+        return (u32)rho - q * g.ddiv.d;
+    }
+    return rho % g.d;
+}
+
+// ---------------------------------------------------------------- generation
+
+__global__ __launch_bounds__(kBlock) void gdn_gen_kernel(Rows g, u64 n_start, u64 r_b, u64 mmax, u64 tmax,
+                                                         const u64 *__restrict__ bases, u64 b_common,
+                                                         const u64 *__restrict__ C, u64 Ctotal,
+                                                         u64 *__restrict__ xdig, u32 *status) {
+    __shared__ u32 sdig[kRows][kMaxDigits];
+    __shared__ u64 sbase[kRows];
+    __shared__ u32 scount[kRows];
+    __shared__ u64 sj[kRows];
+    const u64 l = blockIdx.y;
+    const u64 rho0 = (u64)blockIdx.x * kRows;
+    const u64 nrows = (g.nd - rho0) < (u64)kRows ? (g.nd - rho0) : (u64)kRows;
+
+    // 1. base-b digits of k = i + n_start for each row of the block
+    if (threadIdx.x < nrows) {
+        const u64 rho = rho0 + threadIdx.x;
+        const u64 j = j_of(g, rho);
+        const u64 b = bases ? bases[(l % r_b) * g.d + j] : b_common;
+        u64 k = rho / g.d + n_start;
+        u32 cnt = 0;
+        if (b < 2) {
+            flag(status, STATUS_BAD_BASE);  // the reference loops forever (b=1) or divides by 0
+            k = 0;
+        }
+        if ((k >> 32) == 0 && (b >> 32) == 0) {
+            u32 kk = (u32)k, bb = (u32)b;
+            while (kk) {
+                const u32 q = kk / bb;
+                sdig[threadIdx.x][cnt++] = kk - q * bb;
+                kk = q;
+            }
+        } else {
+            while (k) {
+                const u64 q = k / b;
+                sdig[threadIdx.x][cnt++] = (u32)(k - q * b);
+                k = q;
+            }
+        }
+        sbase[threadIdx.x] = b;
+        scount[threadIdx.x] = cnt;
+        sj[threadIdx.x] = j;
+    }
+    __syncthreads();
+
+    // 2. one output digit per lane
+    const u32 T = (u32)tmax;
+    const u32 total = (u32)nrows * T;
+    u64 *out = xdig + (l * g.nd + rho0) * tmax;
+    const u64 cl = l * g.d * mmax * tmax;
+    for (u32 f = threadIdx.x; f < total; f += blockDim.x) {
+        const u32 row = div_q(f, g.tdiv);
+        const u32 t = f - row * T;
+        const u64 b = sbase[row];
+        const u32 cnt = scount[row];
+        const u64 cbase = cl + sj[row] * mmax * tmax + t;
+        u64 acc = 0;
+        // fast path: b < 2^16 and every column entry < 2^32, so no step wraps
+        bool fast = b >= 2 && b < (1u << 16);
+        if (fast) {
+            for (u32 m = 0; m < cnt; m++) {
+                const u32 dg = sdig[row][m];
+                if (!dg) continue;
+                const u64 ci = cbase + (u64)m * tmax;
+                if (ci >= Ctotal) { fast = false; break; }
+                const u64 c = C[ci];
+                if (c >> 32) { fast = false; break; }
+                acc += (u64)dg * c;
+            }
+        }
+        if (fast) {
+            acc = umod(acc, b);
+        } else if (b < 2) {
+            acc = 0;
+        } else {  // step by step, as gdn_gen_natural.c:46
+            acc = 0;
+            for (u32 m = 0; m < cnt; m++) {
+                const u64 dg = sdig[row][m];
+                if (!dg) continue;
+                const u64 ci = cbase + (u64)m * tmax;
+                u64 c = 0;
+                if (ci < Ctotal) c = C[ci];
+                else flag(status, STATUS_OOB);
+                acc = (acc + dg * c) % b;
+            }
+        }
+        store_stream(out + f, acc);
+    }
+}
+
+// -------------------------------------------------- digit-wise shift / permute
+
+template <bool PERMUTE>
+__global__ __launch_bounds__(kBlock) void gdn_digitwise_kernel(Rows g, u64 r_x, u64 r_b, u64 tmax, u64 tmax_new,
+                                                               u64 bmax, const u64 *__restrict__ bases,
+                                                               const u64 *__restrict__ tab, u64 tab_total,
+                                                               const u64 *__restrict__ xdig,
+                                                               u64 *__restrict__ xnew, u32 *status) {
+    __shared__ u64 sbase[kRows];
+    __shared__ u64 sj[kRows];
+    const u64 l = blockIdx.y;
+    const u64 rho0 = (u64)blockIdx.x * kRows;
+    const u64 nrows = (g.nd - rho0) < (u64)kRows ? (g.nd - rho0) : (u64)kRows;
+    if (threadIdx.x < nrows) {
+        const u64 j = j_of(g, rho0 + threadIdx.x);
+        sj[threadIdx.x] = j;
+        if (!PERMUTE) sbase[threadIdx.x] = bases[(l % r_b) * g.d + j];
+    }
+    __syncthreads();
+    const u32 T = (u32)tmax_new;
+    const u32 total = (u32)nrows * T;
+    const u64 *src = xdig + ((l % r_x) * g.nd + rho0) * tmax;
+    u64 *out = xnew + (l * g.nd + rho0) * tmax_new;
+    for (u32 f = threadIdx.x; f < total; f += blockDim.x) {
+        const u32 row = div_q(f, g.tdiv);
+        const u32 t = f - row * T;
+        const u64 j = sj[row];
+        u64 v;
+        if (PERMUTE) {
+            // gdn_digital_permutation.c:36-42: perms[l,j,t, digit]; perms[l,j,t,0] past tmax
+            const u64 pbase = ((l * g.d + j) * tmax_new + t) * bmax;
+            const u64 p = t < tmax ? src[(u64)row * tmax + t] : 0;
+            const u64 pi = pbase + p;
+            if (pi < tab_total) {
+                v = tab[pi];
+            } else {
+                v = 0;
+                flag(status, STATUS_OOB);
+            }
+        } else {
+            // gdn_digital_shift.c:37-42: (digit + shift) % b; the raw shift past tmax
+            const u64 sh = tab[(l * g.d + j) * tmax_new + t];
+            if (t < tmax) {
+                const u64 b = sbase[row];
+                if (b == 0) {
+                    flag(status, STATUS_BAD_BASE);
+                    v = 0;
+                } else {
+                    v = umod(src[(u64)row * tmax + t] + sh, b);
+                }
+            } else {
+                v = sh;
+            }
+        }
+        store_stream(out + f, v);
+    }
+}
+
+// -------------------------------------------------------------- to double
+
+// gdn_integer_to_float.c:32-40: v += recip*digit; recip /= b, sequential per
+// element (compiled without FMA contraction, IEEE division)
+__global__ __launch_bounds__(kBlock) void gdn_to_float_kernel(u64 r, u64 nd, u64 d, u64 r_b, u64 tmax,
+                                                              const u64 *__restrict__ bases,
+                                                              const u64 *__restrict__ xdig,
+                                                              double *__restrict__ x) {
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x; e < r * nd; e += stride) {
+        const u64 l = e / nd;
+        const u64 j = (e - l * nd) % d;
+        const double b = (double)bases[(l % r_b) * d + j];
+        const u64 *src = xdig + e * tmax;
+        double recip = 1 / b, v = 0.;
+        for (u64 t = 0; t < tmax; t++) {
+            v += recip * (double)src[t];
+            recip /= b;
+        }
+        store_stream(x + e, v);
+    }
+}
+
+// ------------------------------------------------------- matrix operations
+
+// gdn_linear_matrix_scramble.c:32-44
+__global__ __launch_bounds__(kBlock) void gdn_lms_kernel(u64 r, u64 d, u64 mmax, u64 r_C, u64 r_b, u64 tmax,
+                                                         u64 tmax_new, const u64 *__restrict__ bases,
+                                                         const u64 *__restrict__ S, const u64 *__restrict__ C,
+                                                         u64 *__restrict__ C_lms, u32 *status) {
+    const u64 total = r * d * mmax * tmax_new;
+    const u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    const u64 t = e % tmax_new;
+    const u64 rest = e / tmax_new;
+    const u64 k = rest % mmax;
+    const u64 lj = rest / mmax;
+    const u64 j = lj % d;
+    const u64 l = lj / d;
+    const u64 b = bases[(l % r_b) * d + j];
+    if (b == 0) {
+        flag(status, STATUS_BAD_BASE);
+        C_lms[e] = 0;
+        return;
+    }
+    const u64 *row = S + ((l * d + j) * tmax_new + t) * tmax;
+    const u64 *col = C + (((l % r_C) * d + j) * mmax + k) * tmax;
+    u64 acc = 0;
+    for (u64 c = 0; c < tmax; c++) acc += umod(row[c] * col[c], b);
+    C_lms[e] = acc;
+}
+
+// gdn_interlace.c:31-36 (undo == 0) and gdn_undo_interlace.c:31-36 (undo == 1)
+__global__ __launch_bounds__(kBlock) void gdn_interlace_kernel(int undo, u64 r, u64 d_out, u64 mmax, u64 t_out,
+                                                               u64 d_in, u64 t_in, u64 alpha,
+                                                               const u64 *__restrict__ src,
+                                                               u64 *__restrict__ dst) {
+    const u64 total = r * d_out * mmax * t_out;
+    const u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    const u64 to = e % t_out;
+    const u64 rest = e / t_out;
+    const u64 k = rest % mmax;
+    const u64 lj = rest / mmax;
+    const u64 jo = lj % d_out;
+    const u64 l = lj / d_out;
+    u64 ji, ti;
+    if (!undo) {  // C_alpha[l, ja, k, ta] = C[l, ja*alpha + ta%alpha, k, ta/alpha]
+        ji = jo * alpha + to % alpha;
+        ti = to / alpha;
+    } else {      // C[l, j, k, t] = C_alpha[l, j/alpha, k, t*alpha + j%alpha]
+        ji = jo / alpha;
+        ti = to * alpha + jo % alpha;
+    }
+    dst[e] = src[((l * d_in + ji) * mmax + k) * t_in + ti];
+}
+
+Rows make_rows(u64 n, u64 d, u64 tdigits) {
+    Rows g;
+    g.nd = n * d;
+    g.d = d;
+    g.small = g.nd < (1ull << 32) && d < (1ull << 31);
+    g.ddiv = make_div(g.small ? (u32)d : 1u);
+    g.tdiv = make_div((u32)tdigits);
+    return g;
+}
+
+}  // namespace
+
+u32 *qmc_status_ptr();
+
+extern "C" {
+
+static int gen_common(const char *who, u64 r, u64 n, u64 d, u64 r_b, u64 mmax, u64 tmax, u64 n_start,
+                      const u64 *bases, u64 b_common, const u64 *C, u64 *xdig, hipStream_t s) {
+    if (!(r && n && d && tmax)) return QMC_OK;
+    if (bases && r_b == 0) return qmc_fail(QMC_ERR_ARG, "%s: r_b must be positive", who);
+    if (tmax > 4096) return qmc_fail(QMC_ERR_ARG, "%s: tmax = %llu is above the supported 4096", who,
+                                     (unsigned long long)tmax);
+    if (r > 65535) return qmc_fail(QMC_ERR_ARG, "%s: r above 65535 is not supported", who);
+    u32 *status = qmc_status_ptr();
+    if (!status) return QMC_ERR_HIP;
+    Rows g = make_rows(n, d, tmax);
+    dim3 grid((unsigned)((g.nd + kRows - 1) / kRows), (unsigned)r);
+    hipLaunchKernelGGL(gdn_gen_kernel, grid, dim3(kBlock), 0, s, g, n_start, r_b, mmax, tmax, bases, b_common, C,
+                       r * d * mmax * tmax, xdig, status);
+    QMC_HIP_CHECK(hipGetLastError());
+    return QMC_OK;
+}
+
+int gdn_gen_natural_dev(u64 r, u64 n, u64 d, u64 r_b, u64 mmax, u64 tmax, u64 n_start, const u64 *bases,
+                        const u64 *C, u64 *xdig, hipStream_t s) {
+    return gen_common("gdn_gen_natural", r, n, d, r_b, mmax, tmax, n_start, bases, 0, C, xdig, s);
+}
+
+int gdn_gen_natural_same_base_dev(u64 r, u64 n, u64 d, u64 mmax, u64 tmax, u64 n_start, u64 b, const u64 *C,
+                                  u64 *xdig, hipStream_t s) {
+    return gen_common("gdn_gen_natural_same_base", r, n, d, 1, mmax, tmax, n_start, nullptr, b, C, xdig, s);
+}
+
+int gdn_linear_matrix_scramble_dev(u64 r, u64 d, u64 mmax, u64 r_C, u64 r_b, u64 tmax, u64 tmax_new,
+                                   const u64 *bases, const u64 *S, const u64 *C, u64 *C_lms, hipStream_t s) {
+    const u64 total = r * d * mmax * tmax_new;
+    if (!total) return QMC_OK;
+    if (r_C == 0 || r_b == 0) return qmc_fail(QMC_ERR_ARG, "gdn_linear_matrix_scramble: r_C and r_b must be positive");
+    u32 *status = qmc_status_ptr();
+    if (!status) return QMC_ERR_HIP;
+    hipLaunchKernelGGL(gdn_lms_kernel, dim3(grid_for(total, kBlock, 0x7fffffffu)), dim3(kBlock), 0, s, r, d, mmax,
+                       r_C, r_b, tmax, tmax_new, bases, S, C, C_lms, status);
+    QMC_HIP_CHECK(hipGetLastError());
+    return QMC_OK;
+}
+
+static int digitwise(bool permute, const char *who, u64 r, u64 n, u64 d, u64 r_x, u64 r_b, u64 tmax,
+                     u64 tmax_new, u64 bmax, const u64 *bases, const u64 *tab, const u64 *xdig, u64 *xnew,
+                     hipStream_t s) {
+    if (!(r && n && d && tmax_new)) return QMC_OK;
+    if (r_x == 0 || (!permute && r_b == 0)) return qmc_fail(QMC_ERR_ARG, "%s: r_x and r_b must be positive", who);
+    if (tmax > tmax_new)
+        return qmc_fail(QMC_ERR_ARG, "%s: tmax = %llu > tmax_new = %llu overlaps output rows in the reference",
+                        who, (unsigned long long)tmax, (unsigned long long)tmax_new);
+    if (tmax_new > 4096) return qmc_fail(QMC_ERR_ARG, "%s: tmax_new above 4096 is not supported", who);
+    if (r > 65535) return qmc_fail(QMC_ERR_ARG, "%s: r above 65535 is not supported", who);
+    u32 *status = qmc_status_ptr();
+    if (!status) return QMC_ERR_HIP;
+    Rows g = make_rows(n, d, tmax_new);
+    dim3 grid((unsigned)((g.nd + kRows - 1) / kRows), (unsigned)r);
+    if (permute)
+        hipLaunchKernelGGL(gdn_digitwise_kernel<true>, grid, dim3(kBlock), 0, s, g, r_x, r_b, tmax, tmax_new, bmax,
+                           bases, tab, r * d * tmax_new * bmax, xdig, xnew, status);
+    else
+        hipLaunchKernelGGL(gdn_digitwise_kernel<false>, grid, dim3(kBlock), 0, s, g, r_x, r_b, tmax, tmax_new, bmax,
+                           bases, tab, r * d * tmax_new, xdig, xnew, status);
+    QMC_HIP_CHECK(hipGetLastError());
+    return QMC_OK;
+}
+
+int gdn_digital_permutation_dev(u64 r, u64 n, u64 d, u64 r_x, u64 r_b, u64 tmax, u64 tmax_new, u64 bmax,
+                                const u64 *perms, const u64 *xdig, u64 *xdig_new, hipStream_t s) {
+    return digitwise(true, "gdn_digital_permutation", r, n, d, r_x, r_b, tmax, tmax_new, bmax, nullptr, perms,
+                     xdig, xdig_new, s);
+}
+
+int gdn_digital_shift_dev(u64 r, u64 n, u64 d, u64 r_x, u64 r_b, u64 tmax, u64 tmax_new, const u64 *bases,
+                          const u64 *shifts, const u64 *xdig, u64 *xdig_new, hipStream_t s) {
+    return digitwise(false, "gdn_digital_shift", r, n, d, r_x, r_b, tmax, tmax_new, 0, bases, shifts, xdig,
+                     xdig_new, s);
+}
+
+int gdn_integer_to_float_dev(u64 r, u64 n, u64 d, u64 r_b, u64 tmax, const u64 *bases, const u64 *xdig,
+                             double *x, hipStream_t s) {
+    if (!(r && n && d)) return QMC_OK;
+    if (r_b == 0) return qmc_fail(QMC_ERR_ARG, "gdn_integer_to_float: r_b must be positive");
+    hipLaunchKernelGGL(gdn_to_float_kernel, dim3(grid_for(r * n * d, kBlock, 2048 * 8)), dim3(kBlock), 0, s, r,
+                       n * d, d, r_b, tmax, bases, xdig, x);
+    QMC_HIP_CHECK(hipGetLastError());
+    return QMC_OK;
+}
+
+int gdn_interlace_dev(u64 r, u64 d_alpha, u64 mmax, u64 d, u64 tmax, u64 tmax_alpha, u64 alpha, const u64 *C,
+                      u64 *C_alpha, hipStream_t s) {
+    const u64 total = r * d_alpha * mmax * tmax_alpha;
+    if (!total) return QMC_OK;
+    if (alpha == 0) return qmc_fail(QMC_ERR_ARG, "gdn_interlace: alpha must be positive");
+    if (tmax_alpha > alpha * tmax || d_alpha * alpha > d)
+        return qmc_fail(QMC_ERR_RANGE, "gdn_interlace: tmax_alpha <= alpha*tmax and d_alpha*alpha <= d are required");
+    hipLaunchKernelGGL(gdn_interlace_kernel, dim3(grid_for(total, kBlock, 0x7fffffffu)), dim3(kBlock), 0, s, 0, r,
+                       d_alpha, mmax, tmax_alpha, d, tmax, alpha, C, C_alpha);
+    QMC_HIP_CHECK(hipGetLastError());
+    return QMC_OK;
+}
+
+int gdn_undo_interlace_dev(u64 r, u64 d, u64 mmax, u64 d_alpha, u64 tmax, u64 tmax_alpha, u64 alpha,
+                           const u64 *C_alpha, u64 *C, hipStream_t s) {
+    const u64 total = r * d * mmax * tmax;
+    if (!total) return QMC_OK;
+    if (alpha == 0) return qmc_fail(QMC_ERR_ARG, "gdn_undo_interlace: alpha must be positive");
+    if (tmax * alpha > tmax_alpha || (d + alpha - 1) / alpha > d_alpha)
+        return qmc_fail(QMC_ERR_RANGE, "gdn_undo_interlace: tmax*alpha <= tmax_alpha and ceil(d/alpha) <= d_alpha are required");
+    hipLaunchKernelGGL(gdn_interlace_kernel, dim3(grid_for(total, kBlock, 0x7fffffffu)), dim3(kBlock), 0, s, 1, r, d,
+                       mmax, tmax, d_alpha, tmax_alpha, alpha, C_alpha, C);
+    QMC_HIP_CHECK(hipGetLastError());
+    return QMC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,219 @@
+// Rank-1 lattices on gfx950: generation in linear, natural and Gray order,
+// shift mod 1, and the fused generate+shift used for randomized lattices.
+//
+// Every value is produced with the reference's own floating-point steps
+// (qmctoolscl/c_funcs/lat_gen_*.c, lat_shift_mod_1.c): the generator entry is
+// converted to double, multiplied by the index fraction, and reduced with an
+// exact fmod(., 1).  The index fraction is exact in all three orders:
+//   linear : i / (double)n                      (IEEE division, lat_gen_linear.c:25)
+//   natural: phi2(q),        q = row + n_start  (lat_gen_natural.c:37-45)
+//   gray   : phi2(Gray(k)),  k = row + n_start  (lat_gen_gray.c:37-43)
+// where phi2 is the base-2 radical inverse.  The reference walks k serially
+// and carries phi2(Gray(k)) as v / 2^p; here every element computes it in
+// closed form with one bit reversal, so rows are independent and the kernels
+// are write-bound.
+#include "qmc_common.hpp"
+
+namespace {
+
+enum { ORDER_LINEAR = 0, ORDER_NATURAL = 1, ORDER_GRAY = 2 };
+
+// radical inverse of q in base 2: reverse the 64 bits and scale by 2^-64;
+// exact while q < 2^53 (the reversed pattern spans at most 53 bits)
+__device__ __forceinline__ double phi2(u64 q) { return (double)__brevll(q) * 0x1p-64; }
+
+struct RowMap {
+    u64 n, d, nd, n_start;
+    DivU32 ddiv;   // division by d when nd < 2^32
+    int small;     // nd < 2^32
+    int check;     // natural order over a range not known to be Gray-closed
+};
+
+// row index and index fraction of element e2 of one replication slab; returns
+// false when the reference never writes this row (natural order only)
+template <int ORDER>
+__device__ __forceinline__ bool row_value(const RowMap &m, u64 e2, u64 &j, double &frac) {
+    u64 i;
+    if (m.small) {
+        u32 q = div_q((u32)e2, m.ddiv);
+        i = q;
+        j = (u32)e2 - q * m.ddiv.d;
+    } else {
+        split2(e2, m.d, i, j);
+    }
+    if (ORDER == ORDER_LINEAR) {
+        frac = (double)i / (double)m.n;
+    } else if (ORDER == ORDER_GRAY) {
+        frac = phi2(gray_of(i + m.n_start));
+    } else {
+        const u64 q = i + m.n_start;
+        if (m.check) {
+            const u64 k = gray_inverse(q);
+            if (k < m.n_start || k - m.n_start >= m.n) return false;
+        }
+        frac = phi2(q);
+    }
+    return true;
+}
+
+// x[l, row, j] = fmod((double)g[l, j] * frac(row), 1) for all r generating vectors
+template <int ORDER>
+__global__ __launch_bounds__(kBlock) void lat_gen_kernel(RowMap m, u64 r,
+                                                         const u64 *__restrict__ g,
+                                                         double *__restrict__ x) {
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 e2 = (u64)blockIdx.x * blockDim.x + threadIdx.x; e2 < m.nd; e2 += stride) {
+        u64 j;
+        double frac;
+        if (!row_value<ORDER>(m, e2, j, frac)) continue;
+        for (u64 l = 0; l < r; l++)
+            store_stream(x + l * m.nd + e2, frac_mod1((double)g[l * m.d + j] * frac));
+    }
+}
+
+// xr[l, row, j] = fmod(fmod((double)g[l % r_g, j] * frac(row), 1) + shifts[l, j], 1):
+// lat_gen_* followed by lat_shift_mod_1 with r_x = r_g, one pass over HBM.
+template <int ORDER>
+__global__ __launch_bounds__(kBlock) void lat_gen_shift_kernel(RowMap m, u64 r, u64 r_g,
+                                                               const u64 *__restrict__ g,
+                                                               const double *__restrict__ shifts,
+                                                               double *__restrict__ xr) {
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 e2 = (u64)blockIdx.x * blockDim.x + threadIdx.x; e2 < m.nd; e2 += stride) {
+        u64 j;
+        double frac;
+        if (!row_value<ORDER>(m, e2, j, frac)) continue;
+        for (u64 lg = 0; lg < r_g; lg++) {
+            const double xv = frac_mod1((double)g[lg * m.d + j] * frac);
+            for (u64 l = lg; l < r; l += r_g)
+                store_stream(xr + l * m.nd + e2, frac_mod1(xv + shifts[l * m.d + j]));
+        }
+    }
+}
+
+// xr[l, i, j] = fmod(x[l % r_x, i, j] + shifts[l, j], 1): each x element is read
+// once and feeds the r / r_x replications that share it.
+__global__ __launch_bounds__(kBlock) void lat_shift_kernel(RowMap m, u64 r, u64 r_x,
+                                                           const double *__restrict__ x,
+                                                           const double *__restrict__ shifts,
+                                                           double *__restrict__ xr) {
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    const u64 total = r_x * m.nd;
+    for (u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+        const u64 lx = e / m.nd;
+        const u64 e2 = e - lx * m.nd;
+        u64 j;
+        if (m.small) {
+            u32 q = div_q((u32)e2, m.ddiv);
+            j = (u32)e2 - q * m.ddiv.d;
+        } else {
+            j = e2 % m.d;
+        }
+        const double xv = x[e];
+        for (u64 l = lx; l < r; l += r_x)
+            store_stream(xr + l * m.nd + e2, frac_mod1(xv + shifts[l * m.d + j]));
+    }
+}
+
+RowMap make_map(u64 n, u64 d, u64 n_start, int check) {
+    RowMap m;
+    m.n = n;
+    m.d = d;
+    m.nd = n * d;
+    m.n_start = n_start;
+    m.small = (m.nd < (1ull << 32)) && d < (1ull << 31);
+    m.ddiv = make_div(m.small ? (u32)d : 1u);
+    m.check = check;
+    return m;
+}
+
+bool is_pow2_or_zero(u64 v) { return (v & (v - 1)) == 0; }
+
+// The natural-order output rows are exactly [n_start, n_start+n) when both ends
+// are 0 or powers of two (the reference's own precondition, util.py:84-86);
+// otherwise each row checks whether the reference would write it.
+int natural_check(u64 n_start, u64 n) { return !(is_pow2_or_zero(n_start) && is_pow2_or_zero(n_start + n)); }
+
+// the reference builds phi2(Gray(k)) with a 32-bit int shift, valid for k < 2^31
+int check_k_range(u64 n_start, u64 n, const char *who) {
+    if (n_start > (1ull << 31) || n > (1ull << 31) - n_start)
+        return qmc_fail(QMC_ERR_RANGE, "%s: n_start+n = %llu exceeds 2^31, where the reference's "
+                        "int shift (lat_gen_natural.c:30,57) is undefined", who,
+                        (unsigned long long)(n_start + n));
+    return QMC_OK;
+}
+
+unsigned lat_grid(u64 work) { return grid_for(work, kBlock, 2048 * 8); }
+
+}  // namespace
+
+extern "C" {
+
+int lat_gen_linear_dev(u64 r, u64 n, u64 d, const u64 *g, double *x, hipStream_t s) {
+    if (r == 0 || n == 0 || d == 0) return QMC_OK;
+    RowMap m = make_map(n, d, 0, 0);
+    hipLaunchKernelGGL(lat_gen_kernel<ORDER_LINEAR>, dim3(lat_grid(m.nd)), dim3(kBlock), 0, s, m, r, g, x);
+    QMC_HIP_CHECK(hipGetLastError());
+    return QMC_OK;
+}
+
+int lat_gen_natural_dev(u64 r, u64 n, u64 d, u64 n_start, const u64 *g, double *x, hipStream_t s) {
+    if (r == 0 || n == 0 || d == 0) return QMC_OK;
+    int rc = check_k_range(n_start, n, "lat_gen_natural");
+    if (rc) return rc;
+    RowMap m = make_map(n, d, n_start, natural_check(n_start, n));
+    hipLaunchKernelGGL(lat_gen_kernel<ORDER_NATURAL>, dim3(lat_grid(m.nd)), dim3(kBlock), 0, s, m, r, g, x);
+    QMC_HIP_CHECK(hipGetLastError());
+    return QMC_OK;
+}
+
+int lat_gen_gray_dev(u64 r, u64 n, u64 d, u64 n_start, const u64 *g, double *x, hipStream_t s) {
+    if (r == 0 || n == 0 || d == 0) return QMC_OK;
+    int rc = check_k_range(n_start, n, "lat_gen_gray");
+    if (rc) return rc;
+    RowMap m = make_map(n, d, n_start, 0);
+    hipLaunchKernelGGL(lat_gen_kernel<ORDER_GRAY>, dim3(lat_grid(m.nd)), dim3(kBlock), 0, s, m, r, g, x);
+    QMC_HIP_CHECK(hipGetLastError());
+    return QMC_OK;
+}
+
+int lat_shift_mod_1_dev(u64 r, u64 n, u64 d, u64 r_x, const double *x, const double *shifts,
+                        double *xr, hipStream_t s) {
+    if (r == 0 || n == 0 || d == 0) return QMC_OK;
+    if (r_x == 0) return qmc_fail(QMC_ERR_ARG, "lat_shift_mod_1: r_x must be positive");
+    RowMap m = make_map(n, d, 0, 0);
+    const u64 rx = r_x < r ? r_x : r;  // replications of x beyond r are never read
+    hipLaunchKernelGGL(lat_shift_kernel, dim3(lat_grid(rx * m.nd)), dim3(kBlock), 0, s, m, r, rx, x, shifts, xr);
+    QMC_HIP_CHECK(hipGetLastError());
+    return QMC_OK;
+}
+
+// order: 0 linear, 1 natural, 2 gray.  Same result as lat_gen_<order>(r_g, ...)
+// into a temporary followed by lat_shift_mod_1(r, ..., r_x = r_g, ...).
+int qmc_lat_gen_shift_dev(int order, u64 r, u64 n, u64 d, u64 r_g, u64 n_start, const u64 *g,
+                          const double *shifts, double *xr, hipStream_t s) {
+    if (r == 0 || n == 0 || d == 0) return QMC_OK;
+    if (r_g == 0) return qmc_fail(QMC_ERR_ARG, "lat_gen_shift: r_g must be positive");
+    const u64 rg = r_g < r ? r_g : r;
+    const unsigned grid = lat_grid(n * d);
+    if (order == ORDER_LINEAR) {
+        RowMap m = make_map(n, d, 0, 0);
+        hipLaunchKernelGGL(lat_gen_shift_kernel<ORDER_LINEAR>, dim3(grid), dim3(kBlock), 0, s, m, r, rg, g, shifts, xr);
+    } else if (order == ORDER_NATURAL) {
+        int rc = check_k_range(n_start, n, "lat_gen_shift");
+        if (rc) return rc;
+        RowMap m = make_map(n, d, n_start, natural_check(n_start, n));
+        hipLaunchKernelGGL(lat_gen_shift_kernel<ORDER_NATURAL>, dim3(grid), dim3(kBlock), 0, s, m, r, rg, g, shifts, xr);
+    } else if (order == ORDER_GRAY) {
+        int rc = check_k_range(n_start, n, "lat_gen_shift");
+        if (rc) return rc;
+        RowMap m = make_map(n, d, n_start, 0);
+        hipLaunchKernelGGL(lat_gen_shift_kernel<ORDER_GRAY>, dim3(grid), dim3(kBlock), 0, s, m, r, rg, g, shifts, xr);
+    } else {
+        return qmc_fail(QMC_ERR_ARG, "lat_gen_shift: order must be 0 (linear), 1 (natural) or 2 (gray)");
+    }
+    QMC_HIP_CHECK(hipGetLastError());
+    return QMC_OK;
+}
+
+}  // extern "C"

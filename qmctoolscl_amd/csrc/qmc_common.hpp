@@ -1,0 +1,86 @@
+// Shared device/host helpers for the gfx950 QMC kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/qmctoolscl_amd.h"  // status codes and the exported prototypes
+
+typedef uint64_t u64;
+typedef uint32_t u32;
+
+#define QMC_HIP_CHECK(expr)                                      \
+    do {                                                         \
+        hipError_t _e = (expr);                                  \
+        if (_e != hipSuccess) return qmc_fail_hip(_e, #expr);    \
+    } while (0)
+
+int qmc_fail_hip(hipError_t e, const char *what);
+int qmc_fail(int code, const char *fmt, ...);
+
+// ---------------------------------------------------------------- launch shape
+constexpr int kBlock = 256;
+
+static inline unsigned grid_for(u64 work, u64 per_block, unsigned cap = 1u << 20) {
+    u64 g = (work + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+// --------------------------------------------------------- exact small helpers
+
+// fmod(y, 1.0) for finite y: the fractional part is exact in binary floating
+// point (y and trunc(y) share the grid of ulp(y)), and fmod keeps the sign of y.
+__device__ __forceinline__ double frac_mod1(double y) {
+    return copysign(y - trunc(y), y);
+}
+
+// Unsigned division by a run-time 32-bit divisor via a 64-bit fractional
+// reciprocal M = floor(2^64 / d) + 1 (Lemire, Kaser, Kurz 2019): exact for all
+// 32-bit numerators, d >= 1.
+struct DivU32 {
+    u64 M;
+    u32 d;
+};
+
+static inline DivU32 make_div(u32 d) {
+    DivU32 r;
+    r.d = d;
+    r.M = (d == 1) ? 0 : (~(u64)0) / d + 1;
+    return r;
+}
+
+__device__ __forceinline__ u32 div_q(u32 n, DivU32 v) {
+    return v.d == 1 ? n : (u32)__umul64hi(v.M, (u64)n);
+}
+
+// index decomposition helpers for 64-bit flat indices (used when a slab does
+// not fit 32 bits)
+__device__ __forceinline__ void split2(u64 e, u64 d, u64 &i, u64 &j) {
+    i = e / d;
+    j = e - i * d;
+}
+
+// v mod b for a 64-bit v and a run-time b: a 32-bit fast path when both fit.
+__device__ __forceinline__ u64 umod(u64 v, u64 b) {
+    if ((v >> 32) == 0 && (b >> 32) == 0) return (u64)((u32)v % (u32)b);
+    return v % b;
+}
+
+__device__ __forceinline__ u64 gray_of(u64 k) { return k ^ (k >> 1); }
+
+__device__ __forceinline__ u64 gray_inverse(u64 g) {
+    g ^= g >> 1;
+    g ^= g >> 2;
+    g ^= g >> 4;
+    g ^= g >> 8;
+    g ^= g >> 16;
+    g ^= g >> 32;
+    return g;
+}
+
+// streaming stores for large outputs: they bypass the caches on the way to HBM
+template <typename T>
+__device__ __forceinline__ void store_stream(T *p, T v) {
+    __builtin_nontemporal_store(v, p);
+}

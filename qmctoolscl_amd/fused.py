@@ -1,0 +1,44 @@
+"""Fused pipelines on GPU tensors (no single reference symbol; each equals a
+chain of reference calls, as the reference doctests chain them, in one pass
+over HBM).  Tensors must be contiguous and on a ROCm device; uint64 data may
+be held in torch.int64 tensors.  Calls are enqueued on `stream` (default:
+torch's current stream) and do not synchronize."""
+import ctypes
+
+from . import _lib
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if not (t.is_cuda and t.is_contiguous()):
+        raise ctypes.ArgumentError("expected a contiguous GPU tensor")
+    return t.data_ptr()
+
+
+def _stream(stream, like):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream(like.device)
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def lat_gen_shift(order, r, n, d, r_g, n_start, g, shifts, xr, stream=None):
+    """xr = lat_shift_mod_1(r, n, d, r_g, lat_gen_<order>(r_g, n, d, [n_start,] g), shifts).
+    order: 0 linear, 1 natural, 2 gray.  g: (r_g, d) uint64, shifts: (r, d)
+    float64, xr: (r, n, d) float64."""
+    rc = _lib.lib().qmc_lat_gen_shift_dev(int(order), int(r), int(n), int(d), int(r_g), int(n_start),
+                                          _ptr(g), _ptr(shifts), _ptr(xr), _stream(stream, xr))
+    _lib.check(rc)
+
+
+def dnb2_gen_shift_float(natural, r, n, d, r_x, n_start, mmax, C, lshifts, shiftsb, tmaxes, xrb=None, x=None,
+                         stream=None):
+    """Base-2 net generation (natural or Gray order, r_x generating matrices
+    C of shape (r_x, d, mmax)), digital shift to r replications
+    (lshifts: (r_x,), shiftsb: (r, d)) and conversion to double (tmaxes:
+    (r,)) in one pass; writes xrb (r, n, d) uint64 and/or x (r, n, d) float64."""
+    like = xrb if xrb is not None else x
+    rc = _lib.lib().qmc_dnb2_gen_shift_float_dev(int(bool(natural)), int(r), int(n), int(d), int(r_x),
+                                                 int(n_start), int(mmax), _ptr(C), _ptr(lshifts), _ptr(shiftsb),
+                                                 _ptr(tmaxes), _ptr(xrb), _ptr(x), _stream(stream, like))
+    _lib.check(rc)

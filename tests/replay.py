@@ -51,9 +51,11 @@ def _u(v):
     return np.uint64(v)
 
 
-def replay(api, to_device=None, from_device=None):
+def replay(api, to_device=None, from_device=None, nus=True):
     """Run the doctest session on `api`.  `to_device`/`from_device` move arrays
-    to and from the buffers `api` works on (identity for numpy)."""
+    to and from the buffers `api` works on (identity for numpy).  nus=False
+    skips the two nested-uniform-scrambling blocks (they draw from their own
+    generators, so the shared stream is unaffected)."""
     td = to_device or (lambda a: a)
     fd = from_device or (lambda a: a)
     chk = Checker()
@@ -132,6 +134,13 @@ def replay(api, to_device=None, from_device=None):
     chk(617, fd(x))
 
     # ---- base-2 nested uniform scrambling (docs/index.md:682-736)
+    if nus:
+        _replay_nus2(api, chk, td, fd)
+    _replay_gdn(api, chk, td, fd, rng, nus)
+    return chk
+
+
+def _replay_nus2(api, chk, td, fd):
     r, n_start, n, d = _u(1), _u(0), _u(8), _u(4)
     C = np.array([[8, 4, 2, 1], [8, 12, 10, 15], [8, 12, 6, 9], [8, 12, 2, 5]], dtype=np.uint64)
     mmax = _u(C.shape[1])
@@ -148,6 +157,8 @@ def replay(api, to_device=None, from_device=None):
     api.dnb2_nested_uniform_scramble(r, n, d, r_x, tmax, tmax_new, rngs, roots, xb, xrb)
     chk(719, fd(xrb))
 
+
+def _replay_gdn(api, chk, td, fd, rng, nus):
     # ---- generalized digital nets (docs/index.md:761-1416)
     bases = np.array([[2, 3], [5, 7]], dtype=np.uint64)
     r_C = r_b = _u(bases.shape[0])
@@ -185,6 +196,12 @@ def replay(api, to_device=None, from_device=None):
     chk(1388, fd(x))
 
     # ---- base-b nested uniform scrambling (docs/index.md:1421-1552)
+    if nus:
+        _replay_nusb(api, chk, td, fd)
+    _replay_tail(api, chk, td, fd, rng)
+
+
+def _replay_nusb(api, chk, td, fd):
     r, n_start, n = _u(1), _u(0), _u(9)
     bases = np.array([[2, 3, 5]], dtype=np.uint64)
     r_b = _u(1)
@@ -207,6 +224,8 @@ def replay(api, to_device=None, from_device=None):
     api.gdn_integer_to_float(r, n, d, r_b, tmax_new, td(bases), xrdig, xr)
     chk(1532, fd(xr))
 
+
+def _replay_tail(api, chk, td, fd, rng):
     # ---- base-b interlacing (docs/index.md:1570-1667)
     r, d, alpha = _u(2), _u(6), _u(3)
     d_alpha = _u(d // alpha)
@@ -265,4 +284,3 @@ def replay(api, to_device=None, from_device=None):
     chk.results.append((1922, "fwht vs Hadamard/sqrt(n)", bool(np.allclose(fd(x), want, atol=1e-8)), 0))
     api.fwht_1d_radix2(_u(d1), _u(d2), _u(nn // 2), x)
     chk.results.append((1925, "fwht(fwht(x))", bool(np.allclose(fd(x), x0, atol=1e-8)), 0))
-    return chk

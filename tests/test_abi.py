@@ -1,0 +1,73 @@
+"""CPU checks of the boundary: the HIP library loads, exports every symbol
+that include/qmctoolscl_amd.h declares, and the Python surface mirrors the
+reference's argument lists and error behaviour without touching a GPU."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import qmctoolscl_amd as q
+from qmctoolscl_amd import _lib
+from qmctoolscl_amd._api import SIGNATURES
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(REPO, "include", "qmctoolscl_amd.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"QMC_EXPORT\s+[\w\s\*]+?\b(\w+)\s*\(", text)))
+
+
+def test_header_declares_every_reference_kernel():
+    syms = set(declared_symbols())
+    for name in SIGNATURES:
+        assert name in syms and name + "_dev" in syms, name
+    assert len(SIGNATURES) == 23
+
+
+def test_library_loads_and_exports_all_declared_symbols():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built (python -m qmctoolscl_amd.build)")
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [s for s in declared_symbols() if not hasattr(L, s)]
+    assert missing == []
+    L = _lib.lib()  # prototypes declare cleanly
+    assert b"gfx950" in L.qmc_version()
+
+
+def test_signatures_match_reference_prototypes_order():
+    # argument names in reference order (qmctoolscl/c_funcs/<name>.c) spot checks
+    assert [s[0] for s in SIGNATURES["gdn_digital_permutation"]] == [
+        "r", "n", "d", "r_x", "r_b", "tmax", "tmax_new", "bmax", "perms", "xdig", "xdig_new"]
+    assert [s[0] for s in SIGNATURES["dnb2_interlace"]] == [
+        "r", "d_alpha", "mmax", "d", "tmax", "tmax_alpha", "alpha", "C", "C_alpha"]
+    assert [s[0] for s in SIGNATURES["fft_bro_1d_radix2"]] == [
+        "d1", "d2", "n_half", "twiddler", "twiddlei", "xr", "xi"]
+
+
+def test_python_surface_errors_without_gpu():
+    g = np.ones((1, 2), dtype=np.uint64)
+    x = np.empty((1, 3, 2))
+    # natural-order precondition of qmctoolscl/util.py:84-86
+    with pytest.raises(Exception, match="powers of 2"):
+        q.lat_gen_natural(np.uint64(1), np.uint64(3), np.uint64(2), np.uint64(0), g, x)
+    # unknown backend, as util.py:54
+    with pytest.raises(AssertionError):
+        q.lat_gen_linear(np.uint64(1), np.uint64(3), np.uint64(2), g, x, backend="cuda")
+    # wrong argument count
+    with pytest.raises(TypeError):
+        q.lat_gen_linear(np.uint64(1), np.uint64(3), g, x)
+
+
+@pytest.mark.skipif(not os.path.exists(_lib.LIB_PATH), reason="library not built")
+def test_dtype_checks_like_ndpointer():
+    g = np.ones((1, 2), dtype=np.int64)  # wrong dtype
+    x = np.empty((1, 3, 2))
+    with pytest.raises(ctypes.ArgumentError):
+        q.lat_gen_linear(np.uint64(1), np.uint64(3), np.uint64(2), g, x)
+    g = np.ones((1, 4), dtype=np.uint64)[:, ::2]  # not C-contiguous
+    with pytest.raises(ctypes.ArgumentError):
+        q.lat_gen_linear(np.uint64(1), np.uint64(3), np.uint64(2), g, x)
