@@ -21,5 +21,8 @@ from .rand_funcs import (  # noqa: F401
     gdn_get_digital_shifts, gdn_get_digital_permutations,
 )
 from .kernels import *  # noqa: F401,F403
+from .nus import (  # noqa: F401
+    NUSNode_dnb2, NUSNode_gdn, dnb2_nested_uniform_scramble, gdn_nested_uniform_scramble,
+)
 from ._api import SIGNATURES, KERNEL_NAMES  # noqa: F401
 from ._lib import QMCError  # noqa: F401

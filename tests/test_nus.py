@@ -1,0 +1,152 @@
+"""Nested uniform scrambling: the product's native tree builder must request
+random draws in exactly the reference's order (checked on the CPU against
+the oracle, which follows qmctoolscl/rand_funcs.py:155-356 call for call),
+and the GPU tree walk must reproduce the oracle's scrambled points bit for
+bit, including across repeated calls that extend the same trees."""
+import ctypes
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import _clib
+import qmctoolscl_amd as q
+from qmctoolscl_amd import _lib, nus
+
+sys.path.insert(0, os.path.join(_clib.REPO, "oracle"))
+import nus_oracle  # noqa: E402
+
+needs_lib = pytest.mark.skipif(not os.path.exists(_lib.LIB_PATH), reason="library not built")
+
+
+class Recorder:
+    """numpy Generator proxy that records the sizes of integers()/permutation() calls."""
+
+    def __init__(self, rng):
+        self.rng = rng
+        self.calls = []
+
+    def integers(self, low, high=None, size=None, dtype=np.int64):
+        if size is None:
+            self.calls.append(("int", int(high) - int(low)))
+        return self.rng.integers(low, high, size, dtype=dtype)
+
+    def permutation(self, b):
+        self.calls.append(("perm", int(b)))
+        return self.rng.permutation(b)
+
+
+def _points2(rng, n, tmax, dup=True):
+    v = rng.integers(0, 2**tmax, size=n, dtype=np.uint64)
+    if dup and n > 4:
+        v[3] = v[1]
+        v[n - 1] = 0
+    return v
+
+
+@needs_lib
+def test_dnb2_draw_schedule_matches_reference_order():
+    L = nus._lib_nus()
+    rng = np.random.default_rng(0)
+    for tmax, tmax_new in [(4, 8), (10, 10), (16, 40), (30, 64)]:
+        pts_calls = [_points2(rng, 50, tmax), _points2(rng, 30, tmax)]
+        # oracle, recording widths of the leaf draws (root draw excluded)
+        rec = Recorder(np.random.Generator(np.random.SFC64(1)))
+        roots = np.array([[nus_oracle.Node2()]], dtype=object)
+        rngs = np.array([[rec]], dtype=object)
+        for pts in pts_calls:
+            out = np.zeros((1, len(pts), 1), dtype=np.uint64)
+            nus_oracle.dnb2_nus(1, len(pts), 1, 1, tmax, tmax_new, rngs, roots, pts.reshape(1, -1, 1), out)
+        want = [w for kind, w in rec.calls]
+        # native builder
+        h = L.qmc_nus2_tree_new(tmax_new)
+        try:
+            got = []
+            widths = np.empty(64, dtype=np.uint32)
+            for pts in pts_calls:
+                k = L.qmc_nus2_insert(h, pts.ctypes.data, len(pts), 1, tmax_new - tmax, widths.ctypes.data)
+                got += [1 << int(w) for w in widths[:k]]
+        finally:
+            L.qmc_nus2_tree_free(h)
+        assert got == want[:len(got)] and len(got) == len(want), (tmax, tmax_new)
+
+
+@needs_lib
+def test_gdn_draw_schedule_matches_reference_order():
+    L = nus._lib_nus()
+    rng = np.random.default_rng(1)
+    # tmax == tmax_new is left out: there the reference raises IndexError when a
+    # repeated point reaches a leaf below the last digit (rand_funcs.py:323-325)
+    for b, tmax, tmax_new in [(2, 3, 8), (3, 4, 6), (5, 2, 5), (7, 3, 4)]:
+        calls = []
+        for n in (20, 15):
+            dig = rng.integers(0, b, size=(n, tmax), dtype=np.uint64)
+            dig[2] = dig[0]
+            calls.append(dig)
+        rec = Recorder(np.random.Generator(np.random.SFC64(2)))
+        roots = np.array([[nus_oracle.NodeB()]], dtype=object)
+        rngs = np.array([[rec]], dtype=object)
+        bases = np.array([[b]], dtype=np.uint64)
+        for dig in calls:
+            out = np.zeros((1, len(dig), 1, tmax_new), dtype=np.uint64)
+            nus_oracle.gdn_nus(1, len(dig), 1, 1, 1, tmax, tmax_new, rngs, roots, bases,
+                               dig.reshape(1, -1, 1, tmax), out)
+        want = len([c for c in rec.calls if c[0] == "perm"]) - tmax_new  # rows after the root's
+        h = L.qmc_nusb_tree_new(tmax, tmax_new, b)
+        try:
+            got = 0
+            counts = np.empty(64, dtype=np.uint32)
+            for dig in calls:
+                k = L.qmc_nusb_insert(h, dig.ctypes.data, len(dig), tmax, 1, counts.ctypes.data)
+                got += int(counts[:k].sum())
+        finally:
+            L.qmc_nusb_tree_free(h)
+        assert got == want, (b, tmax, tmax_new)
+
+
+def _rngs(r, d, seed):
+    seeds = np.random.SeedSequence(seed).spawn(r * d)
+    return np.array([np.random.Generator(np.random.SFC64(s)) for s in seeds]).reshape(r, d)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tmax,tmax_new", [(4, 8), (12, 12), (20, 52), (32, 64)])
+def test_dnb2_nus_matches_oracle_gpu(tmax, tmax_new):
+    rng = np.random.default_rng(tmax)
+    r_x, r, n, d = 2, 3, 200, 3
+    xb = rng.integers(0, 2**tmax, size=(r_x, n, d), dtype=np.uint64)
+    xb[:, 5] = xb[:, 2]  # duplicates
+    roots_o = np.array([nus_oracle.Node2() for _ in range(r * d)]).reshape(r, d)
+    roots_p = np.array([q.NUSNode_dnb2() for _ in range(r * d)]).reshape(r, d)
+    rngs_o, rngs_p = _rngs(r, d, 9), _rngs(r, d, 9)
+    for half in (xb[:, :120], xb[:, 120:]):  # two calls extend the same trees
+        half = np.ascontiguousarray(half)
+        want = np.zeros((r, half.shape[1], d), dtype=np.uint64)
+        got = np.zeros_like(want)
+        nus_oracle.dnb2_nus(r, half.shape[1], d, r_x, tmax, tmax_new, rngs_o, roots_o, half, want)
+        q.dnb2_nested_uniform_scramble(r, half.shape[1], d, r_x, tmax, tmax_new, rngs_p, roots_p, half, got)
+        assert np.array_equal(got, want)
+    # generators stay in step with the reference's consumption
+    assert all(a.integers(0, 2**40) == b.integers(0, 2**40) for a, b in zip(rngs_o.ravel(), rngs_p.ravel()))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("bases,tmax,tmax_new", [([2, 3, 5], 3, 8), ([3, 7], 4, 6), ([2, 131], 2, 5)])
+def test_gdn_nus_matches_oracle_gpu(bases, tmax, tmax_new):
+    rng = np.random.default_rng(len(bases) + tmax)
+    d = len(bases)
+    r_x, r, n = 1, 2, 60
+    b = np.array([bases], dtype=np.uint64)
+    xdig = np.stack([rng.integers(0, bb, size=(r_x, n, tmax), dtype=np.uint64) for bb in bases], axis=2)
+    xdig = np.ascontiguousarray(xdig)
+    roots_o = np.array([nus_oracle.NodeB() for _ in range(r * d)]).reshape(r, d)
+    roots_p = np.array([q.NUSNode_gdn() for _ in range(r * d)]).reshape(r, d)
+    rngs_o, rngs_p = _rngs(r, d, 5), _rngs(r, d, 5)
+    for part in (xdig[:, :40], xdig[:, 40:]):
+        part = np.ascontiguousarray(part)
+        want = np.zeros((r, part.shape[1], d, tmax_new), dtype=np.uint64)
+        got = np.zeros_like(want)
+        nus_oracle.gdn_nus(r, part.shape[1], d, r_x, 1, tmax, tmax_new, rngs_o, roots_o, b, part, want)
+        q.gdn_nested_uniform_scramble(r, part.shape[1], d, r_x, 1, tmax, tmax_new, rngs_p, roots_p, b, part, got)
+        assert np.array_equal(got, want)
