@@ -207,6 +207,44 @@ QMC_EXPORT int qmc_dnb2_gen_shift_float_dev(int natural, qmc_u64 r, qmc_u64 n, q
                                             const qmc_u64 *lshifts, const qmc_u64 *shiftsb,
                                             const qmc_u64 *tmaxes, qmc_u64 *xrb, double *x, qmc_stream_t s);
 
+/* ------------------------------------------- nested uniform scrambling
+ * Replaces the pure-Python dnb2_nested_uniform_scramble (rand_funcs.py:162-266)
+ * and gdn_nested_uniform_scramble (rand_funcs.py:274-356).  A tree per
+ * (replication, dimension) is grown on the host in point order; *_insert
+ * returns the sizes of the random draws the new leaves need, in the order the
+ * reference draws them (base 2: bit widths, for rng.integers(0, 2**w); base b:
+ * row counts, for that many rng.permutation(b) rows); the caller appends the
+ * draw values (the root's first) and *_apply_dev scrambles on the GPU.
+ * Trees persist across calls until freed. */
+QMC_EXPORT void *qmc_nus2_tree_new(int tmax_new);
+QMC_EXPORT void qmc_nus2_tree_free(void *tree);
+QMC_EXPORT qmc_u64 qmc_nus2_tree_nodes(void *tree);
+QMC_EXPORT qmc_u64 qmc_nus2_tree_draws(void *tree);
+QMC_EXPORT int64_t qmc_nus2_insert(void *tree, const qmc_u64 *v, qmc_u64 n, qmc_u64 stride, int tdelta,
+                                   uint32_t *widths);
+QMC_EXPORT int qmc_nus2_add_draws(void *tree, const qmc_u64 *vals, qmc_u64 count);
+QMC_EXPORT int qmc_nus2_apply_dev(void **trees, qmc_u64 r, qmc_u64 n, qmc_u64 d, qmc_u64 r_x, int tmax,
+                                  int tmax_new, const qmc_u64 *xb, qmc_u64 *xrb, qmc_stream_t s);
+QMC_EXPORT void *qmc_nusb_tree_new(int tmax, int tmax_new, qmc_u64 b);
+QMC_EXPORT void qmc_nusb_tree_free(void *tree);
+QMC_EXPORT qmc_u64 qmc_nusb_tree_rows(void *tree);
+QMC_EXPORT qmc_u64 qmc_nusb_tree_rows_needed(void *tree);
+QMC_EXPORT int64_t qmc_nusb_insert(void *tree, const qmc_u64 *v, qmc_u64 n, qmc_u64 stride_pt,
+                                   qmc_u64 stride_dig, uint32_t *counts);
+QMC_EXPORT int qmc_nusb_add_rows(void *tree, const qmc_u64 *rows, qmc_u64 nrows);
+QMC_EXPORT int qmc_nusb_apply_dev(void **trees, qmc_u64 r, qmc_u64 n, qmc_u64 d, qmc_u64 r_x, qmc_u64 r_b,
+                                  int tmax, int tmax_new, const qmc_u64 *bases, const qmc_u64 *xdig,
+                                  qmc_u64 *xrdig, qmc_stream_t s);
+
+/* ------------------------------------------------ matrix-core transforms
+ * Reduced-precision counterpart of fwht_1d_radix2 (fwht_1d_radix2.c:3-14):
+ * 16-point Hadamard tiles on bf16 MFMA from a 3 x bf16 split, fp32
+ * accumulation; relative L2 error ~1e-7 vs float64.  Length 2^8 .. 2^24;
+ * `work` (fp32, d1*d2*2*n_half) may be NULL. */
+QMC_EXPORT int qmc_fwht_mfma_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, double *x, float *work, qmc_stream_t s);
+/* one 16x16x32 bf16 MFMA on lane-major operands (layout check) */
+QMC_EXPORT int qmc_mfma_probe_dev(const float *a, const float *b, float *d, qmc_stream_t s);
+
 #ifdef __cplusplus
 }
 #endif

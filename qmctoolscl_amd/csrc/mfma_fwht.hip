@@ -1,0 +1,194 @@
+// Walsh-Hadamard transform on the gfx950 matrix cores (reduced precision).
+//
+// H_N = H16 (x) H16 (x) ... (x) H_{2^rem}: the transform of a length-2^m vector
+// is a 16-point Hadamard product along every 4-bit group of the index (and
+// radix-2 butterflies for the leftover bits).  A 16-point product over 16
+// columns is one 16x16 output tile of v_mfma_f32_16x16x32_bf16:
+//
+//     D = [H16 | H16] * [s0 ; s1]  +  [H16 | H16] * [s2 ; 0]
+//
+// where x = s0 + s1 + s2 splits each fp32 value into three bf16 pieces (24
+// significant bits in all) and the +-1 Hadamard entries are exact in bf16, so
+// the only rounding is the fp32 accumulation.  Values stay in LDS (fp32)
+// between axes; a vector longer than one tile (2^14 values) takes two passes
+// over HBM with an fp32 intermediate, against up to five float64 passes of the
+// radix-2 path.  The orthonormal scale 1/sqrt(n) is applied once, in float64,
+// when the result is written.
+//
+// Accuracy is not bit-exact: every result reports its relative L2 and max
+// error against the float64 transform (see tests/test_mfma.py).
+#include "qmc_common.hpp"
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kTileLog2 = 14;  // 2^14 fp32 = 64 KiB of LDS per workgroup
+constexpr int kWaves = kBlock / 64;
+
+__device__ __forceinline__ void split3(float x, __bf16 &a, __bf16 &b, __bf16 &c) {
+    a = (__bf16)x;
+    const float r1 = x - (float)a;  // exact: a is x rounded to 8 significant bits
+    b = (__bf16)r1;
+    const float r2 = r1 - (float)b;
+    c = (__bf16)r2;
+}
+
+// Operand A of the 16x16x32 MFMA: lane (row m = lane%16, k-group g = lane/16)
+// holds A[m][8g .. 8g+7]; here A = [H16 | H16], H16[m][c] = (-1)^popcount(m&c).
+__device__ __forceinline__ bf16x8 hadamard_a(int lane) {
+    const int m = lane & 15, g = lane >> 4;
+    bf16x8 a;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const int c = (g * 8 + i) & 15;
+        a[i] = (__popc(m & c) & 1) ? (__bf16)-1.0f : (__bf16)1.0f;
+    }
+    return a;
+}
+
+// 16-point Hadamard along tile index bits [a, a+4) of a 2^T-value fp32 tile
+// in LDS.  Each wave owns whole 16-column blocks, so reads and writes of a
+// column never cross waves.
+__device__ void h16_axis(float *tile, int T, int a, int wave, int lane, bf16x8 A) {
+    const int nblk = 1 << (T - 8);
+    const int n = lane & 15, g = lane >> 4;
+    const int c0 = (g & 1) * 8;
+    for (int blk = wave; blk < nblk; blk += kWaves) {
+        const int col = blk * 16 + n;
+        const int lo = col & ((1 << a) - 1);
+        const int base = ((col >> a) << (a + 4)) | lo;
+        bf16x8 b1, b2;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            __bf16 s0, s1, s2;
+            split3(tile[base + ((c0 + i) << a)], s0, s1, s2);
+            b1[i] = g < 2 ? s0 : s1;
+            b2[i] = g < 2 ? s2 : (__bf16)0.0f;
+        }
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, b1, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, b2, acc, 0, 0, 0);
+        // D layout: lane holds D[4g + i][n], i = 0..3
+#pragma unroll
+        for (int i = 0; i < 4; i++) tile[base + ((g * 4 + i) << a)] = acc[i];
+    }
+}
+
+// unnormalized radix-2 butterflies along tile bits [a, a+k), k < 4
+__device__ void h2_axes(float *tile, int T, int a, int k) {
+    for (int s = a; s < a + k; s++) {
+        const int h = 1 << s;
+        for (int b = threadIdx.x; b < (1 << (T - 1)); b += blockDim.x) {
+            const int p = ((b >> s) << (s + 1)) | (b & (h - 1));
+            const float x1 = tile[p], x2 = tile[p + h];
+            tile[p] = x1 + x2;
+            tile[p + h] = x1 - x2;
+        }
+        __syncthreads();
+    }
+}
+
+// One pass: each workgroup owns a tile of 2^lr rows x 2^lc columns of one
+// vector (global index = v*2^m + row*row_stride + cb*2^lc + col) and applies
+// the Hadamard product along tile bits [tb0, tb0 + nb).  IN/OUT: 0 fp32, 1
+// float64 (the output is then scaled by `scale`).
+template <int IN64, int OUT64>
+__global__ __launch_bounds__(kBlock) void fwht_mfma_pass(int m, int lr, int lc, u64 row_stride, int tb0, int nb,
+                                                         const void *__restrict__ src, void *__restrict__ dst,
+                                                         double scale) {
+    extern __shared__ float tile[];
+    const int T = lr + lc;
+    const u64 cblocks = (1ull << m) >> (lr + lc);  // tiles per vector
+    const u64 v = blockIdx.x / cblocks;
+    const u64 cb = blockIdx.x - v * cblocks;
+    const u64 vbase = v << m;
+    const int cmask = (1 << lc) - 1;
+    for (int t = threadIdx.x; t < (1 << T); t += blockDim.x) {
+        const u64 gidx = vbase + (u64)(t >> lc) * row_stride + (cb << lc) + (t & cmask);
+        tile[t] = IN64 ? (float)((const double *)src)[gidx] : ((const float *)src)[gidx];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const bf16x8 A = hadamard_a(lane);
+    int a = tb0;
+    for (; a + 4 <= tb0 + nb; a += 4) {
+        h16_axis(tile, T, a, wave, lane, A);
+        __syncthreads();
+    }
+    if (a < tb0 + nb) h2_axes(tile, T, a, tb0 + nb - a);
+    for (int t = threadIdx.x; t < (1 << T); t += blockDim.x) {
+        const u64 gidx = vbase + (u64)(t >> lc) * row_stride + (cb << lc) + (t & cmask);
+        if (OUT64)
+            store_stream(((double *)dst) + gidx, (double)tile[t] * scale);
+        else
+            ((float *)dst)[gidx] = tile[t];
+    }
+}
+
+__global__ void mfma_probe_kernel(const float *a, const float *b, float *d) {
+    const int l = threadIdx.x;
+    bf16x8 av, bv;
+    for (int i = 0; i < 8; i++) {
+        av[i] = (__bf16)a[l * 8 + i];
+        bv[i] = (__bf16)b[l * 8 + i];
+    }
+    f32x4 c = {0.f, 0.f, 0.f, 0.f};
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, c, 0, 0, 0);
+    for (int i = 0; i < 4; i++) d[l * 4 + i] = c[i];
+}
+
+}  // namespace
+
+extern "C" {
+
+// Orthonormal FWHT of d1*d2 vectors of length 2*n_half (in place, float64 in
+// and out) on the matrix cores.  Needs 2*n_half >= 256.  `work` is an fp32
+// scratch of d1*d2*2*n_half values, used when the length exceeds 2^14; pass
+// NULL to let the library allocate it on the stream.
+QMC_EXPORT int qmc_fwht_mfma_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, double *x, float *work, qmc_stream_t s) {
+    const u64 V = d1 * d2;
+    if (V == 0 || n_half == 0) return QMC_OK;
+    const u64 n = 2 * n_half;
+    int m = 0;
+    while ((1ull << m) < n) m++;
+    if ((1ull << m) != n) return qmc_fail(QMC_ERR_ARG, "fwht_mfma: length is not a power of two");
+    if (m < 8) return qmc_fail(QMC_ERR_ARG, "fwht_mfma: length 2^%d is below the 2^8 minimum", m);
+    if (m > 2 * kTileLog2 - 4) return qmc_fail(QMC_ERR_ARG, "fwht_mfma: length 2^%d is above 2^%d", m, 2 * kTileLog2 - 4);
+    // 1/sqrt(2^m), exact for even m
+    const double scale = (m % 2 == 0) ? ldexp(1.0, -m / 2) : ldexp(1.0, -(m - 1) / 2) / 1.4142135623730951;
+    if (m <= kTileLog2) {
+        const size_t shm = sizeof(float) << m;
+        hipLaunchKernelGGL((fwht_mfma_pass<1, 1>), dim3((unsigned)V), dim3(kBlock), shm, s, m, 0, m, (u64)0, 0, m,
+                           (const void *)x, (void *)x, scale);
+        QMC_HIP_CHECK(hipGetLastError());
+        return QMC_OK;
+    }
+    float *buf = work;
+    if (!buf) QMC_HIP_CHECK(hipMallocAsync((void **)&buf, sizeof(float) * V * n, s));
+    // pass 1: the low kTileLog2 bits over contiguous tiles
+    const int m1 = kTileLog2, m2 = m - kTileLog2;
+    const size_t shm = sizeof(float) << kTileLog2;
+    hipLaunchKernelGGL((fwht_mfma_pass<1, 0>), dim3((unsigned)(V << (m - m1))), dim3(kBlock), shm, s, m, 0, m1,
+                       (u64)0, 0, m1, (const void *)x, (void *)buf, 1.0);
+    // pass 2: the high m2 bits; a tile is 2^m2 rows (stride 2^m1) x 2^(14-m2) columns
+    const int lc = kTileLog2 - m2;
+    hipLaunchKernelGGL((fwht_mfma_pass<0, 1>), dim3((unsigned)(V << (m - kTileLog2))), dim3(kBlock), shm, s, m, m2,
+                       lc, (u64)1 << m1, lc, m2, (const void *)buf, (void *)x, scale);
+    hipError_t e = hipGetLastError();
+    if (!work) (void)hipFreeAsync(buf, s);
+    if (e != hipSuccess) return qmc_fail_hip(e, "fwht_mfma_pass");
+    return QMC_OK;
+}
+
+// D (16x16, lane-major: lane l holds 4 values) = A (16x32) * B (32x16) on
+// one wave, with a, b given lane-major (8 values per lane, converted to
+// bf16): lets a test check the operand layouts the kernels rely on.
+QMC_EXPORT int qmc_mfma_probe_dev(const float *a, const float *b, float *d, qmc_stream_t s) {
+    hipLaunchKernelGGL(mfma_probe_kernel, dim3(1), dim3(64), 0, s, a, b, d);
+    QMC_HIP_CHECK(hipGetLastError());
+    return QMC_OK;
+}
+
+}  // extern "C"
