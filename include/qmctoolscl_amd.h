@@ -245,6 +245,31 @@ QMC_EXPORT int qmc_fwht_mfma_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, double 
 /* one 16x16x32 bf16 MFMA on lane-major operands (layout check) */
 QMC_EXPORT int qmc_mfma_probe_dev(const float *a, const float *b, float *d, qmc_stream_t s);
 
+/* ------------------------------------------------------- batched requests
+ * No reference counterpart: the reference runs one call per kernel per
+ * request (util.py:99-144).  A plan holds nreq requests (method 0 lattice +
+ * shifts, 1 dnb2 LMS + digital shift, 2 dnb2 + hash-based nested scramble,
+ * 3 Halton-type base-b LMS + digit permutation; r x 2^m x d float64 points,
+ * natural order, counter-based random inputs from seeds[q]).  Outputs go to a
+ * ring of ring_elems doubles, each replication slab contiguous at
+ * qmc_batch_slab_offset(plan, q, l).  Calls on one plan must not overlap. */
+QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t *m, const int32_t *d,
+                                const int32_t *r, const qmc_u64 *seeds, qmc_u64 points_per_tile, int grid,
+                                qmc_u64 ring_elems);
+QMC_EXPORT void qmc_batch_free(void *plan);
+QMC_EXPORT qmc_u64 qmc_batch_total_tiles(void *plan);
+QMC_EXPORT qmc_u64 qmc_batch_points(void *plan);
+QMC_EXPORT qmc_u64 qmc_batch_slab_offset(void *plan, int q, int l);
+QMC_EXPORT int qmc_batch_run_dev(void *plan, double *ring, qmc_stream_t s);          /* persistent launch */
+QMC_EXPORT int qmc_batch_graph_run_dev(void *plan, double *ring, qmc_stream_t s);    /* same, hipGraph replay */
+QMC_EXPORT int qmc_batch_run_per_call_dev(void *plan, double *ring, qmc_stream_t s); /* one launch per request */
+QMC_EXPORT void qmc_batch_lattice_inputs(qmc_u64 seed, int m, int d, int r, qmc_u64 *g, double *shifts);
+QMC_EXPORT void qmc_batch_dnb2_inputs(qmc_u64 seed, int method, int d, int r, qmc_u64 *C, qmc_u64 *S,
+                                      qmc_u64 *shifts);
+QMC_EXPORT int qmc_batch_gdn_inputs(qmc_u64 seed, int m, int l, int j, uint32_t *b, int *tn, int *mm, uint32_t *S,
+                                    uint32_t *perm_a, uint32_t *perm_c);
+QMC_EXPORT qmc_u64 qmc_owen_hash64(qmc_u64 x, qmc_u64 seed);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,227 @@
+"""Ragged batches of generation requests in one persistent, graph-replayed
+kernel, with an autotuner for its tile size and grid.
+
+A request is (method, m, d, r, seed): r x 2^m x d float64 points in natural
+order, methods
+    0 lattice + r shifts mod 1          (lat_gen_natural -> lat_shift_mod_1)
+    1 dnb2, LMS + 64-bit digital shift  (dnb2_linear_matrix_scramble ->
+                                         dnb2_gen_natural -> dnb2_digital_shift
+                                         -> dnb2_integer_to_float)
+    2 dnb2 + hash-based nested uniform scramble
+    3 Halton-type base-b net + LMS + per-level digit permutation
+with all random inputs drawn counter-based from the seed on the device (see
+csrc/batch.hip).  Output goes to a caller-provided float64 ring buffer; every
+replication slab lands contiguously at offset `slab_offset(q, l)`.
+
+    plan = Batch(methods, m, d, r, seeds, points_per_tile=1 << 16)
+    plan.run(ring)                # one persistent launch, replayed from a hipGraph
+    plan.run_per_call(ring)       # the same points, one launch per request
+"""
+import ctypes
+import json
+import os
+import time
+
+import numpy as np
+
+from . import _lib
+
+_vp = ctypes.c_void_p
+
+
+def _declare(L):
+    if getattr(L, "_batch_declared", False):
+        return
+    L.qmc_batch_plan.restype = ctypes.c_void_p
+    L.qmc_batch_plan.argtypes = [ctypes.c_int] + [_vp] * 5 + [ctypes.c_uint64, ctypes.c_int, ctypes.c_uint64]
+    L.qmc_batch_free.argtypes = [_vp]
+    L.qmc_batch_total_tiles.restype = ctypes.c_uint64
+    L.qmc_batch_total_tiles.argtypes = [_vp]
+    L.qmc_batch_points.restype = ctypes.c_uint64
+    L.qmc_batch_points.argtypes = [_vp]
+    L.qmc_batch_slab_offset.restype = ctypes.c_uint64
+    L.qmc_batch_slab_offset.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
+    for f in ("qmc_batch_run_dev", "qmc_batch_graph_run_dev", "qmc_batch_run_per_call_dev"):
+        getattr(L, f).restype = ctypes.c_int
+        getattr(L, f).argtypes = [_vp, _vp, _vp]
+    L.qmc_batch_lattice_inputs.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp]
+    L.qmc_batch_dnb2_inputs.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]
+    L.qmc_batch_gdn_inputs.restype = ctypes.c_int
+    L.qmc_batch_gdn_inputs.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [_vp] * 6
+    L.qmc_owen_hash64.restype = ctypes.c_uint64
+    L.qmc_owen_hash64.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+    L._batch_declared = True
+
+
+def _L():
+    L = _lib.lib()
+    _declare(L)
+    return L
+
+
+def sample_requests(count, seed, m_range=(8, 22), d_range=(1, 256), r_range=(1, 64), methods=(0, 1, 2, 3)):
+    """Requests with method uniform over `methods`, m, d, r uniform over the
+    inclusive ranges, and per-request seeds from SeedSequence(seed).spawn."""
+    rng = np.random.Generator(np.random.SFC64(seed))
+    meth = rng.choice(np.array(methods, dtype=np.int32), size=count).astype(np.int32)
+    m = rng.integers(m_range[0], m_range[1] + 1, size=count).astype(np.int32)
+    d = rng.integers(d_range[0], d_range[1] + 1, size=count).astype(np.int32)
+    r = rng.integers(r_range[0], r_range[1] + 1, size=count).astype(np.int32)
+    seeds = np.array([s.generate_state(1, np.uint64)[0] for s in np.random.SeedSequence(seed).spawn(count)],
+                     dtype=np.uint64)
+    return meth, m, d, r, seeds
+
+
+class Batch:
+    def __init__(self, methods, m, d, r, seeds, points_per_tile=1 << 16, grid=0, ring_elems=None):
+        self.methods = np.ascontiguousarray(methods, dtype=np.int32)
+        self.m = np.ascontiguousarray(m, dtype=np.int32)
+        self.d = np.ascontiguousarray(d, dtype=np.int32)
+        self.r = np.ascontiguousarray(r, dtype=np.int32)
+        self.seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
+        slabs = (np.int64(1) << self.m.astype(np.int64)) * self.d
+        self.points = int(np.sum(slabs * self.r))
+        self.ring_elems = int(ring_elems) if ring_elems is not None else self.points
+        self.points_per_tile = int(points_per_tile)
+        self.grid = int(grid)
+        L = _L()
+        h = L.qmc_batch_plan(len(self.m), self.methods.ctypes.data, self.m.ctypes.data, self.d.ctypes.data,
+                             self.r.ctypes.data, self.seeds.ctypes.data, self.points_per_tile, self.grid,
+                             self.ring_elems)
+        if not h:
+            _lib.check()
+        self._h = h
+        self._free = L.qmc_batch_free
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._free(self._h)
+            self._h = None
+
+    @property
+    def total_tiles(self):
+        return int(_L().qmc_batch_total_tiles(self._h))
+
+    def slab_offset(self, q, l):
+        return int(_L().qmc_batch_slab_offset(self._h, int(q), int(l)))
+
+    def _check_ring(self, ring):
+        if not (ring.is_cuda and ring.is_contiguous() and str(ring.dtype) == "torch.float64"):
+            raise ctypes.ArgumentError("ring must be a contiguous float64 GPU tensor")
+        if ring.numel() < self.ring_elems:
+            raise ValueError("ring holds %d doubles, the plan needs %d" % (ring.numel(), self.ring_elems))
+
+    def _stream(self, ring, stream):
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream(ring.device)
+        return _vp(s.cuda_stream)
+
+    def run(self, ring, graph=True, stream=None):
+        self._check_ring(ring)
+        f = _L().qmc_batch_graph_run_dev if graph else _L().qmc_batch_run_dev
+        _lib.check(f(self._h, ring.data_ptr(), self._stream(ring, stream)))
+
+    def run_per_call(self, ring, stream=None):
+        self._check_ring(ring)
+        _lib.check(_L().qmc_batch_run_per_call_dev(self._h, ring.data_ptr(), self._stream(ring, stream)))
+
+    def view(self, ring, q):
+        """(r, n, d) view of request q's points (when its slabs are consecutive)."""
+        n, d, r = 1 << int(self.m[q]), int(self.d[q]), int(self.r[q])
+        o = self.slab_offset(q, 0)
+        return ring[o:o + r * n * d].view(r, n, d)
+
+
+def lattice_inputs(seed, m, d, r):
+    g = np.empty((1, d), dtype=np.uint64)
+    sh = np.empty((r, d), dtype=np.float64)
+    _L().qmc_batch_lattice_inputs(int(seed), int(m), int(d), int(r), g.ctypes.data, sh.ctypes.data)
+    return g, sh
+
+
+def dnb2_inputs(seed, method, d, r):
+    C = np.empty((1, d, 32), dtype=np.uint64)
+    S = np.empty((r, d, 32), dtype=np.uint64)
+    sh = np.empty((r, d), dtype=np.uint64)
+    _L().qmc_batch_dnb2_inputs(int(seed), int(method), int(d), int(r), C.ctypes.data, S.ctypes.data, sh.ctypes.data)
+    return C, S, sh
+
+
+def gdn_inputs(seed, m, l, j):
+    b = ctypes.c_uint32()
+    tn = ctypes.c_int()
+    mm = ctypes.c_int()
+    S = np.zeros(52 * 32, dtype=np.uint32)
+    pa = np.zeros(52, dtype=np.uint32)
+    pc = np.zeros(52, dtype=np.uint32)
+    _L().qmc_batch_gdn_inputs(int(seed), int(m), int(l), int(j), ctypes.byref(b), ctypes.byref(tn), ctypes.byref(mm),
+                              S.ctypes.data, pa.ctypes.data, pc.ctypes.data)
+    return b.value, tn.value, mm.value, S[:tn.value * mm.value].reshape(tn.value, mm.value), pa[:tn.value], pc[:tn.value]
+
+
+def owen_hash64(x, seed):
+    return int(_L().qmc_owen_hash64(int(x), int(seed)))
+
+
+# ------------------------------------------------------------------ autotuner
+
+DEFAULT_TILES = (1 << 13, 1 << 14, 1 << 15, 1 << 16, 1 << 17)
+DEFAULT_GRIDS_PER_CU = (2, 4, 8)
+
+
+def _cache_path():
+    base = os.environ.get("QMC_AMD_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "qmctoolscl_amd"))
+    return os.path.join(base, "batch_autotune.json")
+
+
+def autotune(methods, m, d, r, seeds, ring, tiles=DEFAULT_TILES, grids_per_cu=DEFAULT_GRIDS_PER_CU, reps=2,
+             cache=True):
+    """Time the persistent batch for every (points_per_tile, grid) candidate on
+    this batch and return the fastest as {"points_per_tile", "grid", "ms",
+    "table"}.  Results are cached per (device, batch signature)."""
+    import torch
+    dev = ring.device
+    props = torch.cuda.get_device_properties(dev)
+    cus = props.multi_processor_count
+    sig = "%s|%d|%d|%s" % (props.name, len(m), int(np.sum((np.int64(1) << np.asarray(m, np.int64)) *
+                                                          np.asarray(d) * np.asarray(r))),
+                           hash(np.asarray(seeds).tobytes()))
+    path = _cache_path()
+    if cache and os.path.exists(path):
+        try:
+            with open(path) as f:
+                hit = json.load(f).get(sig)
+            if hit:
+                return hit
+        except (OSError, ValueError):
+            pass
+    table = []
+    best = None
+    for p in tiles:
+        for g in grids_per_cu:
+            plan = Batch(methods, m, d, r, seeds, points_per_tile=p, grid=g * cus, ring_elems=ring.numel())
+            plan.run(ring)  # warm: captures the graph
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                plan.run(ring)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3 / reps
+            table.append({"points_per_tile": p, "grid": g * cus, "ms": round(ms, 3)})
+            if best is None or ms < best["ms"]:
+                best = {"points_per_tile": p, "grid": g * cus, "ms": round(ms, 3)}
+            del plan
+    best["table"] = table
+    if cache:
+        try:
+            os.makedirs(os.path.dirname(path), exist_ok=True)
+            data = {}
+            if os.path.exists(path):
+                with open(path) as f:
+                    data = json.load(f)
+            data[sig] = best
+            with open(path, "w") as f:
+                json.dump(data, f)
+        except (OSError, ValueError):
+            pass
+    return best
