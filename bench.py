@@ -1,21 +1,31 @@
-"""Benchmark of the QMC point-generation hot path on one MI355X.
+"""Benchmark of the QMC point-generation and transform hot path on one MI355X.
 
-Workload (one step): a randomized rank-1 lattice -- n = 2^20 points in
-natural order, d = 64, one generating vector of odd integers in [1, 2^20),
+Headline workload (one step): a randomized rank-1 lattice -- n = 2^20 points
+in natural order, d = 64, one generating vector of odd integers in [1, 2^20),
 r = 16 uniform random shifts mod 1 -- produced as the r x n x d float64 array
-(1.07e9 points, 8 GiB) by the fused generate+shift kernel; inputs resident in
-HBM before timing.  The same points through the two drop-in kernels
-(lat_gen_natural, then lat_shift_mod_1) are timed beside it ("per_call").
+(1.07e9 points, 8 GiB) by the fused generate+shift kernel, inputs resident in
+HBM before timing.  value = points/s (Gpts/s).  The same points through the
+two drop-in kernels (lat_gen_natural, then lat_shift_mod_1) are timed beside it.
 
-Printed: one JSON line (rank 0) with value = points/s in Gpts/s, the
-dominant kernel's HBM roofline (algorithmic bytes: 8 B written per point;
-duration from HIP events around the launches on the launching stream), and a
-CPU baseline: the reference C backend compiled from its own sources
-(oracle/_ref), or this project's C restatement of it where that build is
-absent, on a bounded sample of the same workload, single-threaded as the
-reference C backend is.
+Further sections (same JSON line, under "sections"):
+  dnb2     base-2 net, n=2^16, d=8, r=4: LMS (drop-in kernel) then the fused
+           natural-order generation + 64-bit digital shift + float, writing the
+           uint64 and float64 outputs; vs the four drop-in kernels chained.
+  batch    512 ragged requests (method in {lattice, dnb2 LMS+shift, dnb2 NUS,
+           Halton LMS+perm}, n=2^m with m in [8,22], d in [1,256], r in [1,64])
+           in one persistent hipGraph-replayed launch vs one launch per request,
+           streamed through a 32 GiB ring; tile size and grid from the autotuner.
+  transforms  FWHT (float64 radix-2, bit-exact; and bf16-MFMA with 3-way split)
+           and FFT (float64) at 4096 x 2^12, 64 x 2^20, 4 x 2^24, with the MFMA
+           result's relative L2 / max error against the float64 transform.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+Roofline: the headline kernel's algorithmic bytes (8 B written per point)
+over its duration from HIP events on the launching stream, against 8 TB/s.
+CPU baseline: the reference C backend built from its own sources (oracle/_ref)
+-- or this project's C restatement of it where absent -- single-threaded as
+the reference C backend is, on a bounded sample of the headline workload.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--sections main,dnb2,batch,transforms]
 """
 import argparse
 import ctypes
@@ -30,8 +40,13 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
 N_LOG2, D, R = 20, 64, 16
 PROFILE_TRAFFIC = os.path.join(HERE, "profiles", "lat_gen_shift_traffic.json")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
 
 
 def cpu_baseline(g, shifts, rows):
@@ -67,6 +82,143 @@ def cpu_baseline(g, shifts, rows):
                       "(n_start=0), d=%d: %.3g points in %.2f s" % (rows, N_LOG2, D, pts, dt)}
 
 
+class Timer:
+    def __init__(self, torch, stream):
+        self.torch = torch
+        self.stream = stream
+
+    def __call__(self, fn, steps, warmup):
+        """Average ms per call from HIP events recorded on the launching stream."""
+        torch = self.torch
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        start = torch.cuda.Event(enable_timing=True)
+        end = torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        start.record(self.stream)
+        for _ in range(steps):
+            fn()
+        end.record(self.stream)
+        torch.cuda.synchronize()
+        return start.elapsed_time(end) / steps, time.perf_counter() - t0
+
+
+def t64(torch, a, dev):
+    return torch.from_numpy(a.view(np.int64) if a.dtype == np.uint64 else a).to(dev)
+
+
+def section_dnb2(torch, q, fused, timed, dev):
+    rng = np.random.Generator(np.random.SFC64(0))
+    n, d, r, t = 1 << 16, 8, 4, 32
+    # upper-triangular unit-diagonal generating matrices, MSB-first columns
+    C = np.empty((1, d, t), dtype=np.uint64)
+    for j in range(d):
+        for k in range(t):
+            above = rng.integers(0, 2, size=k, dtype=np.uint64)
+            C[0, j, k] = (1 << (t - 1 - k)) + sum(int(b) << (t - 1 - c) for c, b in enumerate(above))
+    S = q.dnb2_get_linear_scramble_matrix(rng, r, d, t, t, False)
+    shiftsb = q.dnb2_get_digital_shifts(rng, r, d, 64)
+    Cd, Sd, shd = t64(torch, C, dev), t64(torch, S, dev), t64(torch, shiftsb, dev)
+    lsh = t64(torch, np.full(r, 32, np.uint64), dev)
+    tm = t64(torch, np.full(r, 64, np.uint64), dev)
+    C_lms = torch.empty((r, d, t), dtype=torch.int64, device=dev)
+    xb = torch.empty((r, n, d), dtype=torch.int64, device=dev)
+    xrb = torch.empty((r, n, d), dtype=torch.int64, device=dev)
+    x = torch.empty((r, n, d), dtype=torch.float64, device=dev)
+    U = np.uint64
+
+    def fused_step():
+        q.dnb2_linear_matrix_scramble(U(r), U(d), U(t), U(1), U(t), Sd, Cd, C_lms, wait=False)
+        fused.dnb2_gen_shift_float(1, r, n, d, r, 0, t, C_lms, lsh, shd, tm, xrb, x)
+
+    def chain_step():
+        q.dnb2_linear_matrix_scramble(U(r), U(d), U(t), U(1), U(t), Sd, Cd, C_lms, wait=False)
+        q.dnb2_gen_natural(U(r), U(n), U(d), U(0), U(t), C_lms, xb, wait=False)
+        q.dnb2_digital_shift(U(r), U(n), U(d), U(r), lsh, xb, shd, xrb, wait=False)
+        q.dnb2_integer_to_float(U(r), U(n), U(d), tm, xrb, x, wait=False)
+
+    ms_f, _ = timed(fused_step, 200, 10)
+    ms_c, _ = timed(chain_step, 200, 10)
+    pts = r * n * d
+    return {"workload": "dnb2 LMS + natural order + 64-bit digital shift + float, n=2^16, d=8, r=4, tmax=32",
+            "points": pts, "fused_ms": round(ms_f, 4), "fused_Gpts_s": round(pts / ms_f / 1e6, 2),
+            "fused_GBs_written": round(16 * pts / ms_f / 1e6, 1),
+            "per_call_ms": round(ms_c, 4), "per_call_Gpts_s": round(pts / ms_c / 1e6, 2)}
+
+
+def section_batch(torch, dev, timed, quick):
+    from qmctoolscl_amd import batch as B
+    meth, m, d, r, seeds = B.sample_requests(512, 2024)
+    if quick:
+        m = np.minimum(m, 16).astype(np.int32)
+    ring_elems = 4 << 30  # 32 GiB of doubles
+    ring = torch.empty(ring_elems, dtype=torch.float64, device=dev)
+    # tune on the same requests at 1/16 of the points per replication
+    m_small = np.maximum(m - 4, 4).astype(np.int32)
+    tune = B.autotune(meth, m_small, d, r, seeds, ring, tiles=(1 << 14, 1 << 15, 1 << 16, 1 << 17),
+                      grids_per_cu=(4, 8), reps=1, cache=False)
+    plan = B.Batch(meth, m, d, r, seeds, points_per_tile=tune["points_per_tile"], grid=tune["grid"],
+                   ring_elems=ring_elems)
+    pts = plan.points
+    log("batch: %d requests, %.3g points (%.2f TB written)" % (len(m), pts, 8 * pts / 1e12))
+    ms_b, _ = timed(lambda: plan.run(ring, graph=True), 2, 1)
+    ms_p, _ = timed(lambda: plan.run_per_call(ring), 2, 1)
+    per_method = {}
+    for mm in range(4):
+        sel = meth == mm
+        per_method[["lattice", "dnb2_lms_shift", "dnb2_nus_hash", "gdn_lms_perm"][mm]] = int(
+            np.sum((np.int64(1) << m[sel].astype(np.int64)) * d[sel] * r[sel]))
+    gbs = 8 * pts / ms_b / 1e6
+    return {"workload": "512 ragged requests, m in [%d,%d], d in [1,256], r in [1,64], counter-based inputs"
+                        % (int(m.min()), int(m.max())),
+            "points": pts, "points_by_method": per_method, "tiles": plan.total_tiles,
+            "autotune": {k: tune[k] for k in ("points_per_tile", "grid")}, "autotune_table": tune["table"],
+            "batched_ms": round(ms_b, 2), "batched_Gpts_s": round(pts / ms_b / 1e6, 2),
+            "batched_GBs_written": round(gbs, 1), "batched_frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
+            "per_call_ms": round(ms_p, 2), "per_call_Gpts_s": round(pts / ms_p / 1e6, 2),
+            "speedup_vs_per_call": round(ms_p / ms_b, 3)}
+
+
+def section_transforms(torch, q, dev, timed):
+    from qmctoolscl_amd.mfma import fwht_mfma, relative_errors
+    out = []
+    U = np.uint64
+    for m, V in ((12, 4096), (20, 64), (24, 4)):
+        n = 1 << m
+        rng = np.random.Generator(np.random.SFC64(m))
+        x0 = torch.from_numpy(rng.standard_normal((V, n))).to(dev)
+        x = x0.clone()
+        ms64, _ = timed(lambda: q.fwht_1d_radix2(U(1), U(V), U(n // 2), x, wait=False), 5, 1)
+        # accuracy: one transform of the same input each way
+        ref = x0.clone()
+        q.fwht_1d_radix2(U(1), U(V), U(n // 2), ref)
+        xm = x0.clone()
+        fwht_mfma(1, V, n // 2, xm)
+        torch.cuda.synchronize()
+        l2, mx = relative_errors(xm.cpu().numpy(), ref.cpu().numpy())
+        work = torch.empty(V * n, dtype=torch.float32, device=dev)
+        msm, _ = timed(lambda: fwht_mfma(1, V, n // 2, x, work), 5, 1)
+        xr, xi = x0.clone(), x0.clone()
+        twr = torch.empty(n, dtype=torch.float64, device=dev)
+        twi = torch.empty(n, dtype=torch.float64, device=dev)
+        msf, _ = timed(lambda: q.fft_bro_1d_radix2(U(1), U(V), U(n // 2), twr, twi, xr, xi, wait=False), 5, 1)
+        pts = V * n
+        # MFMA work: per 16-point axis, two 16x16x32 products per 256 outputs
+        axes = m // 4
+        mfma_flops = pts * axes * 2 * 2 * 32 * 16 / 16
+        out.append({"n": "2^%d" % m, "vectors": V,
+                    "fwht_fp64_ms": round(ms64, 4), "fwht_fp64_GBs": round(16 * pts / ms64 / 1e6, 1),
+                    "fwht_mfma_ms": round(msm, 4), "fwht_mfma_GBs": round(16 * pts / msm / 1e6, 1),
+                    "mfma_speedup": round(ms64 / msm, 3),
+                    "mfma_rel_l2": float("%.3g" % l2), "mfma_rel_max": float("%.3g" % mx),
+                    "mfma_util": float("%.3g" % (mfma_flops / (msm * 1e-3) / (BF16_PEAK_TFLOPS * 1e12))),
+                    "fft_fp64_ms": round(msf, 4), "fft_fp64_GBs": round(32 * pts / msf / 1e6, 1)})
+        del x0, x, ref, xm, work, xr, xi
+        torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,54 +226,44 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-rows", type=int, default=1 << 19)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sections", default="main,dnb2,batch,transforms")
+    ap.add_argument("--quick-batch", action="store_true", help="cap request sizes at n=2^16")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the lattice workload would "
-              "shard over replications or n_start chunks with no exchange); run with --gpus 1",
+        print("bench.py measures one GPU: this build has no multi-GPU path (generation shards over "
+              "replications or n_start chunks, transforms over the batch, with no exchange); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
+    sections = set(args.sections.split(","))
 
     import torch
     from qmctoolscl_amd import fused
     import qmctoolscl_amd as q
 
     dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    timed = Timer(torch, stream)
     n = 1 << N_LOG2
     rng = np.random.Generator(np.random.SFC64(0))
     g = (2 * rng.integers(0, 2**19, size=(1, D), dtype=np.uint64) + 1)
     shifts = q.lat_get_shifts(rng, R, D)
-    g_t = torch.from_numpy(g.view(np.int64)).to(dev)
-    s_t = torch.from_numpy(shifts).to(dev)
+    g_t = t64(torch, g, dev)
+    s_t = t64(torch, shifts, dev)
     xr = torch.empty((R, n, D), dtype=torch.float64, device=dev)
     x1 = torch.empty((1, n, D), dtype=torch.float64, device=dev)
     pts = R * n * D
-    stream = torch.cuda.current_stream(dev)
+    U = np.uint64
 
     def step_fused():
         fused.lat_gen_shift(1, R, n, D, 1, 0, g_t, s_t, xr)
 
     def step_per_call():
-        q.lat_gen_natural(np.uint64(1), np.uint64(n), np.uint64(D), np.uint64(0), g_t, x1, wait=False)
-        q.lat_shift_mod_1(np.uint64(R), np.uint64(n), np.uint64(D), np.uint64(1), x1, s_t, xr, wait=False)
+        q.lat_gen_natural(U(1), U(n), U(D), U(0), g_t, x1, wait=False)
+        q.lat_shift_mod_1(U(R), U(n), U(D), U(1), x1, s_t, xr, wait=False)
 
-    def timed(fn, steps, warmup):
-        for _ in range(warmup):
-            fn()
-        torch.cuda.synchronize()
-        start = torch.cuda.Event(enable_timing=True)
-        end = torch.cuda.Event(enable_timing=True)
-        t0 = time.perf_counter()
-        start.record(stream)
-        for _ in range(steps):
-            fn()
-        end.record(stream)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-        return start.elapsed_time(end) / steps, wall
-
+    log("main: lattice n=2^20 d=64 r=16")
     ms_fused, wall = timed(step_fused, args.steps, args.warmup)
     ms_pc, _ = timed(step_per_call, max(3, args.steps // 4), 1)
-
     achieved = 8.0 * pts / (ms_fused * 1e-3) / 1e9  # GB/s written, algorithmic
     traffic = None
     if os.path.exists(PROFILE_TRAFFIC):
@@ -148,8 +290,23 @@ def main():
         "per_call": {"ms_per_step": round(ms_pc, 4), "value": round(pts / (ms_pc * 1e-3) / 1e9, 3),
                      "unit": "Gpts/s", "kernels": "lat_gen_natural + lat_shift_mod_1"},
         "wall_s_timed": round(wall, 3),
+        "sections": {},
     }
+    del x1
+    torch.cuda.empty_cache()
+    if "dnb2" in sections:
+        log("section: dnb2")
+        out["sections"]["dnb2"] = section_dnb2(torch, q, fused, timed, dev)
+    if "transforms" in sections:
+        log("section: transforms")
+        out["sections"]["transforms"] = section_transforms(torch, q, dev, timed)
+    if "batch" in sections:
+        del xr
+        torch.cuda.empty_cache()
+        log("section: batch")
+        out["sections"]["batch"] = section_batch(torch, dev, timed, args.quick_batch)
     if not args.no_cpu:
+        log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(g, shifts, args.cpu_rows)
     print(json.dumps(out))
 
