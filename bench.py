@@ -181,7 +181,7 @@ def section_batch(torch, dev, timed, quick):
 
 
 def section_transforms(torch, q, dev, timed):
-    from qmctoolscl_amd.mfma import fwht_mfma, relative_errors
+    from qmctoolscl_amd.mfma import fft_bro_mfma, fwht_mfma, relative_errors
     out = []
     U = np.uint64
     for m, V in ((12, 4096), (20, 64), (24, 4)):
@@ -203,6 +203,18 @@ def section_transforms(torch, q, dev, timed):
         twr = torch.empty(n, dtype=torch.float64, device=dev)
         twi = torch.empty(n, dtype=torch.float64, device=dev)
         msf, _ = timed(lambda: q.fft_bro_1d_radix2(U(1), U(V), U(n // 2), twr, twi, xr, xi, wait=False), 5, 1)
+        # FFT accuracy and time on the matrix cores
+        yr, yi = x0.clone(), x0.flip(-1).contiguous()
+        q.fft_bro_1d_radix2(U(1), U(V), U(n // 2), twr, twi, yr, yi)
+        zr, zi = x0.clone(), x0.flip(-1).contiguous()
+        fft_bro_mfma(1, V, n // 2, zr, zi)
+        torch.cuda.synchronize()
+        fl2, fmx = relative_errors(np.stack([zr.cpu().numpy(), zi.cpu().numpy()]),
+                                   np.stack([yr.cpu().numpy(), yi.cpu().numpy()]))
+        del yr, yi, zr, zi
+        fwork = torch.empty(2 * V * n, dtype=torch.float32, device=dev)
+        msfm, _ = timed(lambda: fft_bro_mfma(1, V, n // 2, xr, xi, fwork), 5, 1)
+        del fwork
         pts = V * n
         # MFMA work: per 16-point axis, two 16x16x32 products per 256 outputs
         axes = m // 4
@@ -213,7 +225,10 @@ def section_transforms(torch, q, dev, timed):
                     "mfma_speedup": round(ms64 / msm, 3),
                     "mfma_rel_l2": float("%.3g" % l2), "mfma_rel_max": float("%.3g" % mx),
                     "mfma_util": float("%.3g" % (mfma_flops / (msm * 1e-3) / (BF16_PEAK_TFLOPS * 1e12))),
-                    "fft_fp64_ms": round(msf, 4), "fft_fp64_GBs": round(32 * pts / msf / 1e6, 1)})
+                    "fft_fp64_ms": round(msf, 4), "fft_fp64_GBs": round(32 * pts / msf / 1e6, 1),
+                    "fft_mfma_ms": round(msfm, 4), "fft_mfma_GBs": round(32 * pts / msfm / 1e6, 1),
+                    "fft_mfma_speedup": round(msf / msfm, 3),
+                    "fft_mfma_rel_l2": float("%.3g" % fl2), "fft_mfma_rel_max": float("%.3g" % fmx)})
         del x0, x, ref, xm, work, xr, xi
         torch.cuda.empty_cache()
     return out

@@ -242,6 +242,13 @@ QMC_EXPORT int qmc_nusb_apply_dev(void **trees, qmc_u64 r, qmc_u64 n, qmc_u64 d,
  * accumulation; relative L2 error ~1e-7 vs float64.  Length 2^8 .. 2^24;
  * `work` (fp32, d1*d2*2*n_half) may be NULL. */
 QMC_EXPORT int qmc_fwht_mfma_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, double *x, float *work, qmc_stream_t s);
+/* Reduced-precision counterpart of fft_bro_1d_radix2 (fft_bro_1d_radix2.c:3-17):
+ * bit-reversed input, natural output, orthonormal; radix-16 DFT tiles
+ * (FP64-generated, 3 x bf16 split) on MFMA, fp32 accumulation and fp32
+ * inter-level twiddles.  Length 2^8 .. 2^28; `work` (fp32, 2*d1*d2*2*n_half)
+ * may be NULL.  Writes no twiddle table. */
+QMC_EXPORT int qmc_fft_bro_mfma_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, double *xr, double *xi, float *work,
+                                    qmc_stream_t s);
 /* one 16x16x32 bf16 MFMA on lane-major operands (layout check) */
 QMC_EXPORT int qmc_mfma_probe_dev(const float *a, const float *b, float *d, qmc_stream_t s);
 

@@ -6,6 +6,9 @@ values), computed with 16-point Hadamard tiles on bf16 MFMA from a 3 x bf16
 split of each value with fp32 accumulation.  Not bit-exact: relative L2 error
 about 1e-7 against the float64 transform (see tests/test_mfma.py); use
 fwht_1d_radix2 for the bit-exact result.  Lengths 2^8 .. 2^24.
+
+fft_bro_mfma(d1, d2, n_half, xr, xi): fft_bro_1d_radix2's transform (input
+bit-reversed, output natural, orthonormal) with radix-16 DFT tiles on MFMA.
 """
 import ctypes
 
@@ -19,6 +22,8 @@ def _declare(L):
         return
     L.qmc_fwht_mfma_dev.restype = ctypes.c_int
     L.qmc_fwht_mfma_dev.argtypes = [ctypes.c_uint64] * 3 + [ctypes.c_void_p] * 3
+    L.qmc_fft_bro_mfma_dev.restype = ctypes.c_int
+    L.qmc_fft_bro_mfma_dev.argtypes = [ctypes.c_uint64] * 3 + [ctypes.c_void_p] * 4
     L.qmc_mfma_probe_dev.restype = ctypes.c_int
     L.qmc_mfma_probe_dev.argtypes = [ctypes.c_void_p] * 4
     L._mfma_declared = True
@@ -43,6 +48,22 @@ def fwht_mfma(d1, d2, n_half, x, work=None, stream=None):
         raise ctypes.ArgumentError("work must be a contiguous float32 GPU tensor")
     rc = _L().qmc_fwht_mfma_dev(int(d1), int(d2), int(n_half), x.data_ptr(),
                                 work.data_ptr() if work is not None else None, _stream(x, stream))
+    _lib.check(rc)
+
+
+def fft_bro_mfma(d1, d2, n_half, xr, xi, work=None, stream=None):
+    """In-place orthonormal FFT of bit-reversed input (fft_bro_1d_radix2's
+    transform) on the matrix cores: radix-16 DFT tiles (FP64-generated, 3 x
+    bf16 split) on MFMA with fp32 accumulation and fp32 inter-level twiddles.
+    xr, xi: contiguous float64 GPU tensors of d1*d2*2*n_half values; lengths
+    2^8 .. 2^28; relative L2 error ~1e-7 vs float64."""
+    for t in (xr, xi):
+        if not (t.is_cuda and t.is_contiguous()) or str(t.dtype) != "torch.float64":
+            raise ctypes.ArgumentError("xr, xi must be contiguous float64 GPU tensors")
+    if work is not None and not (work.is_cuda and work.is_contiguous() and str(work.dtype) == "torch.float32"):
+        raise ctypes.ArgumentError("work must be a contiguous float32 GPU tensor")
+    rc = _L().qmc_fft_bro_mfma_dev(int(d1), int(d2), int(n_half), xr.data_ptr(), xi.data_ptr(),
+                                   work.data_ptr() if work is not None else None, _stream(xr, stream))
     _lib.check(rc)
 
 

@@ -80,3 +80,19 @@ def test_fwht_mfma_smooth_integrand(torch):
     fwht_mfma(1, 1, n // 2, t)
     l2, mx = relative_errors(t.cpu().numpy(), want)
     assert l2 <= L2_TOL and mx <= MAX_TOL, (l2, mx)
+
+
+@pytest.mark.parametrize("m,batch", [(8, 3), (11, 2), (13, 2), (14, 2), (16, 2), (19, 1), (20, 1), (21, 1)])
+def test_fft_bro_mfma_accuracy(torch, m, batch):
+    from qmctoolscl_amd.mfma import fft_bro_mfma, relative_errors
+    rng = np.random.default_rng(100 + m)
+    n = 1 << m
+    xr = rng.standard_normal((batch, n))
+    xi = rng.standard_normal((batch, n))
+    ir = np.array([int(format(i, "0%db" % m)[::-1], 2) for i in range(n)])
+    want = np.fft.fft((xr + 1j * xi)[:, ir], norm="ortho")
+    tr, ti = torch.from_numpy(xr.copy()).cuda(), torch.from_numpy(xi.copy()).cuda()
+    fft_bro_mfma(1, batch, n // 2, tr, ti)
+    got = tr.cpu().numpy() + 1j * ti.cpu().numpy()
+    l2, mx = relative_errors(np.stack([got.real, got.imag]), np.stack([want.real, want.imag]))
+    assert l2 <= L2_TOL and mx <= MAX_TOL, (m, l2, mx)
