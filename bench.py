@@ -243,6 +243,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sections", default="main,dnb2,batch,transforms")
     ap.add_argument("--quick-batch", action="store_true", help="cap request sizes at n=2^16")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run one plain 2 GiB device-to-device copy (a known-size kernel that scales "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (generation shards over "
@@ -276,6 +279,13 @@ def main():
         q.lat_gen_natural(U(1), U(n), U(D), U(0), g_t, x1, wait=False)
         q.lat_shift_mod_1(U(R), U(n), U(D), U(1), x1, s_t, xr, wait=False)
 
+    if args.calibrate_copy:
+        a = torch.ones(1 << 28, dtype=torch.float64, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+        torch.cuda.empty_cache()
     log("main: lattice n=2^20 d=64 r=16")
     ms_fused, wall = timed(step_fused, args.steps, args.warmup)
     ms_pc, _ = timed(step_per_call, max(3, args.steps // 4), 1)
