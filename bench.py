@@ -147,6 +147,45 @@ def section_dnb2(torch, q, fused, timed, dev):
             "per_call_ms": round(ms_c, 4), "per_call_Gpts_s": round(pts / ms_c / 1e6, 2)}
 
 
+def section_gdn(torch, q, dev, timed, r=8):
+    """Halton-type generalized net: bases = first 32 primes, n = 3^12, d = 32,
+    Halton identity C (mmax = 20), LMS to tmax_new = 52 digits, per-level digit
+    permutations, then float.  Digits are materialized (r*n*d*52 uint64 per copy)."""
+    primes = [2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61, 67, 71,
+              73, 79, 83, 89, 97, 101, 103, 107, 109, 113, 127, 131]
+    n, d, mmax, tnew = 3 ** 12, 32, 20, 52
+    rng = np.random.Generator(np.random.SFC64(2))
+    bases = np.array([primes], dtype=np.uint64)
+    C = q.gdn_get_halton_generating_matrix(1, d, mmax)
+    S = q.gdn_get_linear_scramble_matrix(rng, r, d, mmax, tnew, 1, bases)
+    perms = q.gdn_get_digital_permutations(rng, r, d, tnew, 1, bases)
+    bmax = int(bases.max())
+    Bd, Cd, Sd, Pd = (t64(torch, a, dev) for a in (bases, C, S, perms))
+    C_lms = torch.empty((r, d, mmax, tnew), dtype=torch.int64, device=dev)
+    xdig = torch.empty((r, n, d, tnew), dtype=torch.int64, device=dev)
+    xperm = torch.empty_like(xdig)
+    x = torch.empty((r, n, d), dtype=torch.float64, device=dev)
+    U = np.uint64
+    q.gdn_linear_matrix_scramble(U(r), U(d), U(mmax), U(1), U(1), U(mmax), U(tnew), Bd, Sd, Cd, C_lms)
+    ms_gen, _ = timed(lambda: q.gdn_gen_natural(U(r), U(n), U(d), U(1), U(mmax), U(tnew), U(0), Bd, C_lms, xdig,
+                                                wait=False), 3, 1)
+    ms_perm, _ = timed(lambda: q.gdn_digital_permutation(U(r), U(n), U(d), U(r), U(1), U(tnew), U(tnew), U(bmax),
+                                                         Pd, xdig, xperm, wait=False), 3, 1)
+    ms_flt, _ = timed(lambda: q.gdn_integer_to_float(U(r), U(n), U(d), U(1), U(tnew), Bd, xperm, x, wait=False), 3, 1)
+    pts = r * n * d
+    dig_bytes = 8 * pts * tnew
+    out = {"workload": "Halton-type net, bases = first 32 primes, n=3^12, d=32, r=%d, LMS to 52 digits, "
+                       "digit permutations, float" % r,
+           "points": pts, "digit_bytes_per_copy": dig_bytes,
+           "gen_ms": round(ms_gen, 3), "gen_GBs_written": round(dig_bytes / ms_gen / 1e6, 1),
+           "perm_ms": round(ms_perm, 3), "perm_GBs": round(2 * dig_bytes / ms_perm / 1e6, 1),
+           "to_float_ms": round(ms_flt, 3), "to_float_GBs": round((dig_bytes + 8 * pts) / ms_flt / 1e6, 1),
+           "chain_Gpts_s": round(pts / (ms_gen + ms_perm + ms_flt) / 1e6, 2)}
+    del xdig, xperm, x, C_lms
+    torch.cuda.empty_cache()
+    return out
+
+
 def section_batch(torch, dev, timed, quick):
     from qmctoolscl_amd import batch as B
     meth, m, d, r, seeds = B.sample_requests(512, 2024)
@@ -241,7 +280,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-rows", type=int, default=1 << 19)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--sections", default="main,dnb2,batch,transforms")
+    ap.add_argument("--sections", default="main,dnb2,transforms,gdn,batch")
     ap.add_argument("--quick-batch", action="store_true", help="cap request sizes at n=2^16")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one plain 2 GiB device-to-device copy (a known-size kernel that scales "
@@ -325,6 +364,9 @@ def main():
     if "transforms" in sections:
         log("section: transforms")
         out["sections"]["transforms"] = section_transforms(torch, q, dev, timed)
+    if "gdn" in sections:
+        log("section: gdn")
+        out["sections"]["gdn"] = section_gdn(torch, q, dev, timed)
     if "batch" in sections:
         del xr
         torch.cuda.empty_cache()

@@ -162,23 +162,33 @@ __host__ __device__ __forceinline__ void gdn_perm(u64 seed, u64 l, u64 j, int t,
 // lattice : g[d] | shift[r*d] (double bits)
 // dnb2 LMS: pre[r*d*32] (prefix XOR of the LMS columns) | cols[r*d*32] | shift[r*d]
 // dnb2 NUS: pre[d*32] | cols[d*32] | nseed[r*d]
-// gdn     : per j: b, tn, mm, S offset (4 words) | S (u32 pairs packed in words) | perm (a,c) per (l,j,t)
+// gdn     : per j: b, tn, mm, S offset, ceil(2^64/b), b^tn (double bits) (8 words)
+//           | S (u32, two per word) | perm (a,c) per (l,j,t) | base-2 row masks per (l,t)
+
+constexpr int kJInfo = 8;
 
 struct GdnLayout {
-    u64 jinfo;   // 4 words per j
+    u64 jinfo;   // kJInfo words per j
     u64 S;       // r * SJ u32 entries, packed two per word
     u64 perm;    // r * d * kTNmax words: (a << 32) | c
+    u64 mask2;   // r * kTNmax words: row t of S for base 2 as a bit mask over digits
     u64 words;
 };
 
 __host__ __device__ __forceinline__ GdnLayout gdn_layout(int d, int r, u64 SJ) {
     GdnLayout g;
     g.jinfo = 0;
-    g.S = 4ull * d;
+    g.S = (u64)kJInfo * d;
     g.perm = g.S + (r * SJ + 1) / 2;
-    g.words = g.perm + (u64)r * d * kTNmax;
+    g.mask2 = g.perm + (u64)r * d * kTNmax;
+    g.words = g.mask2 + (u64)r * kTNmax;
     return g;
 }
+
+__host__ __device__ __forceinline__ u64 recip_magic(u32 b) { return b == 1 ? 0 : (~0ull) / b + 1; }
+
+// x mod b and x / b for x < 2^32 with M = ceil(2^64 / b) (exact, Lemire et al. 2019)
+__device__ __forceinline__ u32 fast_div(u32 x, u64 M) { return (u32)__umul64hi(M, (u64)x); }
 
 u64 table_words(const Req &q, u64 &SJ) {
     const u64 d = q.d, r = q.r;
@@ -247,10 +257,16 @@ __global__ __launch_bounds__(kBlock) void batch_setup_kernel(const Req *__restri
             u64 off = 0;
             for (u64 jj = 0; jj < j; jj++) off += (u64)gdn_tn(c_primes[jj]) * gdn_mm(c_primes[jj], q.m);
             const u32 b = c_primes[j];
-            tab[L.jinfo + 4 * j + 0] = b;
-            tab[L.jinfo + 4 * j + 1] = gdn_tn(b);
-            tab[L.jinfo + 4 * j + 2] = gdn_mm(b, q.m);
-            tab[L.jinfo + 4 * j + 3] = off;
+            const int tn = gdn_tn(b);
+            double scale = 1.0;
+            for (int t = 0; t < tn; t++) scale *= (double)b;
+            u64 *ji = tab + L.jinfo + kJInfo * j;
+            ji[0] = b;
+            ji[1] = tn;
+            ji[2] = gdn_mm(b, q.m);
+            ji[3] = off;
+            ji[4] = recip_magic(b);
+            ji[5] = (u64)__double_as_longlong(scale);
         }
         for (u64 e = tid; e < r * d; e += step) {
             const u64 l = e / d, j = e % d;
@@ -260,7 +276,13 @@ __global__ __launch_bounds__(kBlock) void batch_setup_kernel(const Req *__restri
             for (u64 jj = 0; jj < j; jj++) off += (u64)gdn_tn(c_primes[jj]) * gdn_mm(c_primes[jj], q.m);
             u32 *Slj = S + l * SJ + off;
             for (int t = 0; t < tn; t++) {
-                for (int k = 0; k < mm; k++) Slj[t * mm + k] = k <= t ? gdn_s(q.seed, l, j, t, k, b) : 0;
+                u64 mask = 0;
+                for (int k = 0; k < mm; k++) {
+                    const u32 s = k <= t ? gdn_s(q.seed, l, j, t, k, b) : 0;
+                    Slj[t * mm + k] = s;
+                    mask |= (u64)(s & 1) << k;
+                }
+                if (b == 2) tab[L.mask2 + l * kTNmax + t] = mask;
                 u32 a, c;
                 gdn_perm(q.seed, l, j, t, b, a, c);
                 tab[L.perm + (l * d + j) * kTNmax + t] = ((u64)a << 32) | c;
@@ -280,7 +302,9 @@ __device__ __forceinline__ u64 xor_cols32(const u64 *__restrict__ cols, u64 q) {
     return v;
 }
 
-__device__ void run_tile(const Req &q, u64 t, const u64 *__restrict__ ws, double *__restrict__ out) {
+constexpr int kMaxDigB = 21;  // base-b digits of a 32-bit index, b >= 3
+
+__device__ void run_tile(const Req &q, u64 t, const u64 *__restrict__ ws, double *__restrict__ out, u32 *sdig) {
     const u64 d = q.d;
     const u64 l = t / q.tiles_per_l;
     const u64 row0 = (t - l * q.tiles_per_l) * (u64)q.rows_per_tile;
@@ -304,23 +328,28 @@ __device__ void run_tile(const Req &q, u64 t, const u64 *__restrict__ ws, double
         const u64 set = lms ? l * d : 0;
         // threads -> (j, run); runs of consecutive rows walk natural order with
         // one prefix-XOR per step (value(q+1) = value(q) ^ C0 ^ ... ^ C_ctz(q+1))
-        u32 runs = 1;
-        while (runs * 2 * d <= blockDim.x && runs * 2 <= q.rows_per_tile) runs *= 2;
-        const u32 K = q.rows_per_tile / runs;
+        // as many runs as fill the block: runs * d <= blockDim (or one run per row)
+        u32 runs = blockDim.x / (u32)d;
+        if (runs < 1) runs = 1;
+        if (runs > q.rows_per_tile) runs = q.rows_per_tile;
+        const u32 K = (q.rows_per_tile + runs - 1) / runs;
         const u64 *extra = tab + 2 * ncol * kTB + l * d;  // shifts or NUS seeds
         for (u32 w = threadIdx.x; w < runs * d; w += blockDim.x) {
             const u32 j = w % (u32)d;
             const u32 run = w / (u32)d;
+            const u32 r0 = run * K;
+            if (r0 >= q.rows_per_tile) continue;
+            const u32 cnt = (q.rows_per_tile - r0) < K ? (q.rows_per_tile - r0) : K;
             const u64 *pre = tab + (set + j) * kTB;
             const u64 *cols = tab + ncol * kTB + (set + j) * kTB;
             const u64 xs = extra[j];
-            u64 row = row0 + (u64)run * K;
+            u64 row = row0 + r0;
             u64 v = xor_cols32(cols, row);
             for (u32 s = 0;; s++) {
                 const u64 xb = v << 32;  // lshift 64 - tmax (dnb2_digital_shift.c:32)
                 const u64 y = lms ? (xb ^ xs) : owen_hash64(xb, xs);
                 store_stream(slab + row * d + j, ldexp((double)y, -64));  // dnb2_integer_to_float.c:29
-                if (s + 1 == K) break;
+                if (s + 1 == cnt) break;
                 row++;
                 v ^= pre[__ffsll((long long)row) - 1];
             }
@@ -330,34 +359,52 @@ __device__ void run_tile(const Req &q, u64 t, const u64 *__restrict__ ws, double
         const GdnLayout L = gdn_layout(q.d, q.r, SJ);
         const u32 *S = (const u32 *)(tab + L.S) + l * SJ;
         const u64 *perm = tab + L.perm + l * d * kTNmax;
-        for (u32 e = threadIdx.x; e < pts; e += blockDim.x) {
-            const u32 rr = div_q(e, q.ddiv);
-            const u32 j = e - rr * q.d;
-            const u64 row = row0 + rr;
-            const u32 b = (u32)tab[L.jinfo + 4 * j];
-            const int tn = (int)tab[L.jinfo + 4 * j + 1];
-            const int mm = (int)tab[L.jinfo + 4 * j + 2];
-            const u32 *Sj = S + tab[L.jinfo + 4 * j + 3];
-            u32 dig[kTB];
-            u32 k = (u32)row;
-            for (int i = 0; i < mm; i++) {
-                dig[i] = k % b;
-                k /= b;
-            }
+        const u64 *mask2 = tab + L.mask2 + l * kTNmax;
+        u32 *dig = sdig + threadIdx.x;  // digit i at dig[i * kBlock]
+        // One dimension per wave (units of 64 rows x 1 dimension): the digit
+        // counts, LMS rows and permutations are then uniform across the wave,
+        // so no lane waits on a smaller base's longer loops.
+        const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+        const u32 chunks = (q.rows_per_tile + 63) / 64;
+        for (u32 unit = wave; unit < chunks * (u32)d; unit += nwaves) {
+            const u32 j = unit % (u32)d;
+            const u32 rr = (unit / (u32)d) * 64 + lane;
+            if (rr >= q.rows_per_tile) continue;
+            const u32 row = (u32)(row0 + rr);
+            const u64 *ji = tab + L.jinfo + kJInfo * j;
+            const u32 b = (u32)ji[0];
+            const int tn = (int)ji[1];
+            const int mm = (int)ji[2];
+            const u32 *Sj = S + ji[3];
+            const u64 M = ji[4];
             const u64 *pj = perm + (u64)j * kTNmax;
             u64 Y = 0;
-            double scale = 1.0;
-            for (int tt = 0; tt < tn; tt++) {
-                u64 acc = 0;
-                const int kmax = tt < mm ? tt + 1 : mm;
-                for (int i = 0; i < kmax; i++) acc += (u64)Sj[tt * mm + i] * dig[i];
-                const u32 xd = (u32)(acc % b);
-                const u64 pc = pj[tt];
-                const u32 y = (u32)(((pc >> 32) * xd + (pc & 0xffffffffull)) % b);
-                Y = Y * b + y;
-                scale *= (double)b;
+            if (b == 2) {
+                // digit t = parity of (row t of S) & row: one popcount per digit
+                for (int tt = 0; tt < tn; tt++) {
+                    const u32 xd = (u32)__popcll(mask2[tt] & (u64)row) & 1u;
+                    const u64 pc = pj[tt];
+                    Y = 2 * Y + (((u32)(pc >> 32) * xd + (u32)pc) & 1u);
+                }
+            } else {
+                u32 k = row;
+                for (int i = 0; i < mm; i++) {
+                    const u32 qq = fast_div(k, M);
+                    dig[i * kBlock] = k - qq * b;
+                    k = qq;
+                }
+                for (int tt = 0; tt < tn; tt++) {
+                    u32 acc = 0;  // < 32 * b^2 < 2^27
+                    const int kmax = tt < mm ? tt + 1 : mm;
+                    const u32 *Srow = Sj + tt * mm;
+                    for (int i = 0; i < kmax; i++) acc += Srow[i] * dig[i * kBlock];
+                    const u32 xd = acc - fast_div(acc, M) * b;
+                    const u64 pc = pj[tt];
+                    const u32 v = (u32)(pc >> 32) * xd + (u32)pc;
+                    Y = Y * b + (v - fast_div(v, M) * b);
+                }
             }
-            store_stream(slab + row * d + j, (double)Y / scale);
+            store_stream(slab + (u64)row * d + j, (double)Y / __longlong_as_double((long long)ji[5]));
         }
     }
 }
@@ -370,6 +417,7 @@ __global__ __launch_bounds__(kBlock) void batch_persistent_kernel(const Req *__r
                                                                   double *__restrict__ out) {
     __shared__ u64 s_tile;
     __shared__ int s_req;
+    __shared__ u32 sdig[kMaxDigB * kBlock];
     for (;;) {
         if (threadIdx.x == 0) {
             const u64 t = atomicAdd(counter, 1ull);
@@ -390,7 +438,7 @@ __global__ __launch_bounds__(kBlock) void batch_persistent_kernel(const Req *__r
         __syncthreads();
         if (t >= total_tiles) break;  // every wave of every block reaches this
         const Req q = reqs[qi];
-        run_tile(q, t - q.tile0, ws, out);
+        run_tile(q, t - q.tile0, ws, out, sdig);
     }
 }
 
@@ -399,8 +447,9 @@ __global__ void reset_counter_kernel(unsigned long long *counter) { *counter = 0
 // one request, one tile per block (the per-call form)
 __global__ __launch_bounds__(kBlock) void batch_single_kernel(const Req *__restrict__ req, const u64 *__restrict__ ws,
                                                               double *__restrict__ out) {
+    __shared__ u32 sdig[kMaxDigB * kBlock];
     const Req q = *req;
-    run_tile(q, blockIdx.x, ws, out);
+    run_tile(q, blockIdx.x, ws, out, sdig);
 }
 
 // ----------------------------------------------------------- host plan

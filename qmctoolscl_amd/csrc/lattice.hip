@@ -56,62 +56,121 @@ __device__ __forceinline__ bool row_value(const RowMap &m, u64 e2, u64 &j, doubl
     return true;
 }
 
+typedef double double2_t __attribute__((ext_vector_type(2)));
+
+// Each thread owns the element pair (2p, 2p+1) of a replication slab and
+// writes both with one 16-byte store when the slab length is even (so every
+// pair is 16-byte aligned) and both elements are written; otherwise with two
+// 8-byte stores.  Pairs halve the store instructions, which lets the kernels
+// reach the write bandwidth of a plain fill.
+struct Pair {
+    u64 j0, j1;
+    double f0, f1;
+    bool ok0, ok1;  // the reference writes this element (natural order may skip rows)
+};
+
+template <int ORDER>
+__device__ __forceinline__ Pair pair_at(const RowMap &m, u64 e2) {
+    Pair P;
+    P.ok0 = row_value<ORDER>(m, e2, P.j0, P.f0);
+    P.ok1 = false;
+    P.j1 = 0;
+    P.f1 = 0.0;
+    if (e2 + 1 < m.nd) {
+        if (P.j0 + 1 < m.d) {  // same row
+            P.j1 = P.j0 + 1;
+            P.f1 = P.f0;
+            P.ok1 = P.ok0;
+        } else {
+            P.ok1 = row_value<ORDER>(m, e2 + 1, P.j1, P.f1);
+        }
+    }
+    return P;
+}
+
+__device__ __forceinline__ void store_pair(double *p, bool vec, bool ok0, bool ok1, double v0, double v1) {
+    if (vec && ok0 && ok1) {
+        double2_t v = {v0, v1};
+        *(double2_t *)p = v;
+    } else {
+        if (ok0) p[0] = v0;
+        if (ok1) p[1] = v1;
+    }
+}
+
 // x[l, row, j] = fmod((double)g[l, j] * frac(row), 1) for all r generating vectors
 template <int ORDER>
-__global__ __launch_bounds__(kBlock) void lat_gen_kernel(RowMap m, u64 r,
+__global__ __launch_bounds__(kBlock) void lat_gen_kernel(RowMap m, u64 r, bool vec,
                                                          const u64 *__restrict__ g,
                                                          double *__restrict__ x) {
     const u64 stride = (u64)gridDim.x * blockDim.x;
-    for (u64 e2 = (u64)blockIdx.x * blockDim.x + threadIdx.x; e2 < m.nd; e2 += stride) {
-        u64 j;
-        double frac;
-        if (!row_value<ORDER>(m, e2, j, frac)) continue;
+    const u64 pairs = (m.nd + 1) / 2;
+    for (u64 p = (u64)blockIdx.x * blockDim.x + threadIdx.x; p < pairs; p += stride) {
+        const u64 e2 = 2 * p;
+        const Pair P = pair_at<ORDER>(m, e2);
         for (u64 l = 0; l < r; l++)
-            store_stream(x + l * m.nd + e2, frac_mod1((double)g[l * m.d + j] * frac));
+            store_pair(x + l * m.nd + e2, vec, P.ok0, P.ok1, frac_mod1((double)g[l * m.d + P.j0] * P.f0),
+                       P.ok1 ? frac_mod1((double)g[l * m.d + P.j1] * P.f1) : 0.0);
     }
 }
 
 // xr[l, row, j] = fmod(fmod((double)g[l % r_g, j] * frac(row), 1) + shifts[l, j], 1):
 // lat_gen_* followed by lat_shift_mod_1 with r_x = r_g, one pass over HBM.
 template <int ORDER>
-__global__ __launch_bounds__(kBlock) void lat_gen_shift_kernel(RowMap m, u64 r, u64 r_g,
+__global__ __launch_bounds__(kBlock) void lat_gen_shift_kernel(RowMap m, u64 r, u64 r_g, bool vec,
                                                                const u64 *__restrict__ g,
                                                                const double *__restrict__ shifts,
                                                                double *__restrict__ xr) {
     const u64 stride = (u64)gridDim.x * blockDim.x;
-    for (u64 e2 = (u64)blockIdx.x * blockDim.x + threadIdx.x; e2 < m.nd; e2 += stride) {
-        u64 j;
-        double frac;
-        if (!row_value<ORDER>(m, e2, j, frac)) continue;
+    const u64 pairs = (m.nd + 1) / 2;
+    for (u64 p = (u64)blockIdx.x * blockDim.x + threadIdx.x; p < pairs; p += stride) {
+        const u64 e2 = 2 * p;
+        const Pair P = pair_at<ORDER>(m, e2);
         for (u64 lg = 0; lg < r_g; lg++) {
-            const double xv = frac_mod1((double)g[lg * m.d + j] * frac);
+            const double x0 = frac_mod1((double)g[lg * m.d + P.j0] * P.f0);
+            const double x1 = P.ok1 ? frac_mod1((double)g[lg * m.d + P.j1] * P.f1) : 0.0;
             for (u64 l = lg; l < r; l += r_g)
-                store_stream(xr + l * m.nd + e2, frac_mod1(xv + shifts[l * m.d + j]));
+                store_pair(xr + l * m.nd + e2, vec, P.ok0, P.ok1, frac_mod1(x0 + shifts[l * m.d + P.j0]),
+                           frac_mod1(x1 + shifts[l * m.d + P.j1]));
         }
     }
 }
 
 // xr[l, i, j] = fmod(x[l % r_x, i, j] + shifts[l, j], 1): each x element is read
 // once and feeds the r / r_x replications that share it.
-__global__ __launch_bounds__(kBlock) void lat_shift_kernel(RowMap m, u64 r, u64 r_x,
+__global__ __launch_bounds__(kBlock) void lat_shift_kernel(RowMap m, u64 r, u64 r_x, bool vec,
                                                            const double *__restrict__ x,
                                                            const double *__restrict__ shifts,
                                                            double *__restrict__ xr) {
     const u64 stride = (u64)gridDim.x * blockDim.x;
-    const u64 total = r_x * m.nd;
-    for (u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
-        const u64 lx = e / m.nd;
-        const u64 e2 = e - lx * m.nd;
-        u64 j;
+    const u64 pairs = (m.nd + 1) / 2;
+    for (u64 t = (u64)blockIdx.x * blockDim.x + threadIdx.x; t < r_x * pairs; t += stride) {
+        const u64 lx = t / pairs;
+        const u64 e2 = 2 * (t - lx * pairs);
+        u64 j0, i0;
         if (m.small) {
-            u32 q = div_q((u32)e2, m.ddiv);
-            j = (u32)e2 - q * m.ddiv.d;
+            const u32 q = div_q((u32)e2, m.ddiv);
+            i0 = q;
+            j0 = (u32)e2 - q * m.ddiv.d;
         } else {
-            j = e2 % m.d;
+            split2(e2, m.d, i0, j0);
         }
-        const double xv = x[e];
+        (void)i0;
+        const bool two = e2 + 1 < m.nd;
+        const u64 j1 = (j0 + 1 < m.d) ? j0 + 1 : 0;
+        const double *src = x + lx * m.nd + e2;
+        double x0, x1 = 0.0;
+        if (vec && two) {
+            const double2_t v = *(const double2_t *)src;
+            x0 = v.x;
+            x1 = v.y;
+        } else {
+            x0 = src[0];
+            if (two) x1 = src[1];
+        }
         for (u64 l = lx; l < r; l += r_x)
-            store_stream(xr + l * m.nd + e2, frac_mod1(xv + shifts[l * m.d + j]));
+            store_pair(xr + l * m.nd + e2, vec, true, two, frac_mod1(x0 + shifts[l * m.d + j0]),
+                       frac_mod1(x1 + shifts[l * m.d + j1]));
     }
 }
 
@@ -143,7 +202,33 @@ int check_k_range(u64 n_start, u64 n, const char *who) {
     return QMC_OK;
 }
 
-unsigned lat_grid(u64 work) { return grid_for(work, kBlock, 2048 * 8); }
+// grid for `pairs` element pairs: one pair per thread up to 65536 workgroups
+unsigned pair_grid(u64 nd) { return grid_for((nd + 1) / 2, kBlock, 65536); }
+
+bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
+
+// 16-byte stores need even slabs (pairs never straddle replications) and aligned bases
+bool use_vec(u64 nd, const void *a, const void *b = nullptr) {
+    return (nd % 2 == 0) && aligned16(a) && (b == nullptr || aligned16(b));
+}
+
+template <int ORDER>
+int launch_gen(u64 r, u64 n, u64 d, u64 n_start, const u64 *g, double *x, hipStream_t s, int check) {
+    RowMap m = make_map(n, d, n_start, check);
+    hipLaunchKernelGGL(lat_gen_kernel<ORDER>, dim3(pair_grid(m.nd)), dim3(kBlock), 0, s, m, r, use_vec(m.nd, x), g, x);
+    QMC_HIP_CHECK(hipGetLastError());
+    return QMC_OK;
+}
+
+template <int ORDER>
+int launch_gen_shift(u64 r, u64 n, u64 d, u64 rg, u64 n_start, const u64 *g, const double *shifts, double *xr,
+                     hipStream_t s, int check) {
+    RowMap m = make_map(n, d, n_start, check);
+    hipLaunchKernelGGL(lat_gen_shift_kernel<ORDER>, dim3(pair_grid(m.nd)), dim3(kBlock), 0, s, m, r, rg,
+                       use_vec(m.nd, xr), g, shifts, xr);
+    QMC_HIP_CHECK(hipGetLastError());
+    return QMC_OK;
+}
 
 }  // namespace
 
@@ -151,30 +236,21 @@ extern "C" {
 
 int lat_gen_linear_dev(u64 r, u64 n, u64 d, const u64 *g, double *x, hipStream_t s) {
     if (r == 0 || n == 0 || d == 0) return QMC_OK;
-    RowMap m = make_map(n, d, 0, 0);
-    hipLaunchKernelGGL(lat_gen_kernel<ORDER_LINEAR>, dim3(lat_grid(m.nd)), dim3(kBlock), 0, s, m, r, g, x);
-    QMC_HIP_CHECK(hipGetLastError());
-    return QMC_OK;
+    return launch_gen<ORDER_LINEAR>(r, n, d, 0, g, x, s, 0);
 }
 
 int lat_gen_natural_dev(u64 r, u64 n, u64 d, u64 n_start, const u64 *g, double *x, hipStream_t s) {
     if (r == 0 || n == 0 || d == 0) return QMC_OK;
     int rc = check_k_range(n_start, n, "lat_gen_natural");
     if (rc) return rc;
-    RowMap m = make_map(n, d, n_start, natural_check(n_start, n));
-    hipLaunchKernelGGL(lat_gen_kernel<ORDER_NATURAL>, dim3(lat_grid(m.nd)), dim3(kBlock), 0, s, m, r, g, x);
-    QMC_HIP_CHECK(hipGetLastError());
-    return QMC_OK;
+    return launch_gen<ORDER_NATURAL>(r, n, d, n_start, g, x, s, natural_check(n_start, n));
 }
 
 int lat_gen_gray_dev(u64 r, u64 n, u64 d, u64 n_start, const u64 *g, double *x, hipStream_t s) {
     if (r == 0 || n == 0 || d == 0) return QMC_OK;
     int rc = check_k_range(n_start, n, "lat_gen_gray");
     if (rc) return rc;
-    RowMap m = make_map(n, d, n_start, 0);
-    hipLaunchKernelGGL(lat_gen_kernel<ORDER_GRAY>, dim3(lat_grid(m.nd)), dim3(kBlock), 0, s, m, r, g, x);
-    QMC_HIP_CHECK(hipGetLastError());
-    return QMC_OK;
+    return launch_gen<ORDER_GRAY>(r, n, d, n_start, g, x, s, 0);
 }
 
 int lat_shift_mod_1_dev(u64 r, u64 n, u64 d, u64 r_x, const double *x, const double *shifts,
@@ -183,7 +259,8 @@ int lat_shift_mod_1_dev(u64 r, u64 n, u64 d, u64 r_x, const double *x, const dou
     if (r_x == 0) return qmc_fail(QMC_ERR_ARG, "lat_shift_mod_1: r_x must be positive");
     RowMap m = make_map(n, d, 0, 0);
     const u64 rx = r_x < r ? r_x : r;  // replications of x beyond r are never read
-    hipLaunchKernelGGL(lat_shift_kernel, dim3(lat_grid(rx * m.nd)), dim3(kBlock), 0, s, m, r, rx, x, shifts, xr);
+    hipLaunchKernelGGL(lat_shift_kernel, dim3(grid_for(rx * ((m.nd + 1) / 2), kBlock, 65536)), dim3(kBlock), 0, s, m,
+                       r, rx, use_vec(m.nd, x, xr), x, shifts, xr);
     QMC_HIP_CHECK(hipGetLastError());
     return QMC_OK;
 }
@@ -195,25 +272,14 @@ int qmc_lat_gen_shift_dev(int order, u64 r, u64 n, u64 d, u64 r_g, u64 n_start, 
     if (r == 0 || n == 0 || d == 0) return QMC_OK;
     if (r_g == 0) return qmc_fail(QMC_ERR_ARG, "lat_gen_shift: r_g must be positive");
     const u64 rg = r_g < r ? r_g : r;
-    const unsigned grid = lat_grid(n * d);
-    if (order == ORDER_LINEAR) {
-        RowMap m = make_map(n, d, 0, 0);
-        hipLaunchKernelGGL(lat_gen_shift_kernel<ORDER_LINEAR>, dim3(grid), dim3(kBlock), 0, s, m, r, rg, g, shifts, xr);
-    } else if (order == ORDER_NATURAL) {
-        int rc = check_k_range(n_start, n, "lat_gen_shift");
-        if (rc) return rc;
-        RowMap m = make_map(n, d, n_start, natural_check(n_start, n));
-        hipLaunchKernelGGL(lat_gen_shift_kernel<ORDER_NATURAL>, dim3(grid), dim3(kBlock), 0, s, m, r, rg, g, shifts, xr);
-    } else if (order == ORDER_GRAY) {
-        int rc = check_k_range(n_start, n, "lat_gen_shift");
-        if (rc) return rc;
-        RowMap m = make_map(n, d, n_start, 0);
-        hipLaunchKernelGGL(lat_gen_shift_kernel<ORDER_GRAY>, dim3(grid), dim3(kBlock), 0, s, m, r, rg, g, shifts, xr);
-    } else {
+    if (order == ORDER_LINEAR) return launch_gen_shift<ORDER_LINEAR>(r, n, d, rg, 0, g, shifts, xr, s, 0);
+    if (order != ORDER_NATURAL && order != ORDER_GRAY)
         return qmc_fail(QMC_ERR_ARG, "lat_gen_shift: order must be 0 (linear), 1 (natural) or 2 (gray)");
-    }
-    QMC_HIP_CHECK(hipGetLastError());
-    return QMC_OK;
+    int rc = check_k_range(n_start, n, "lat_gen_shift");
+    if (rc) return rc;
+    if (order == ORDER_NATURAL)
+        return launch_gen_shift<ORDER_NATURAL>(r, n, d, rg, n_start, g, shifts, xr, s, natural_check(n_start, n));
+    return launch_gen_shift<ORDER_GRAY>(r, n, d, rg, n_start, g, shifts, xr, s, 0);
 }
 
 }  // extern "C"

@@ -27,8 +27,13 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kT = 13;  // 2^13 complex fp32 = 64 KiB of LDS
+constexpr int kT = 13;  // 2^13 complex fp32 (68 KiB of LDS with padding)
 constexpr int kWaves = kBlock / 64;
+
+// LDS word of tile index t: one pad word per 16 keeps the 16-point columns of
+// the lowest level (addresses 16 apart) on distinct banks
+__device__ __forceinline__ int pad(int t) { return t + (t >> 4); }
+static inline size_t padded_bytes(int T) { return 2 * sizeof(float) * (((size_t)1 << T) + ((size_t)1 << T) / 16); }
 
 __device__ __forceinline__ void split3f(float x, __bf16 &a, __bf16 &b, __bf16 &c) {
     a = (__bf16)x;
@@ -110,7 +115,7 @@ __device__ void dft16_level(float *re, float *im, const PassGeom &G, u64 cb, int
 #pragma unroll
         for (int i = 0; i < 8; i++) {
             __bf16 s0, s1, s2;
-            split3f(src[base + ((c0 + i) << ta)], s0, s1, s2);
+            split3f(src[pad(base + ((c0 + i) << ta))], s0, s1, s2);
             b[0][i] = s0;
             b[1][i] = s1;
             b[2][i] = s2;
@@ -145,8 +150,8 @@ __device__ void dft16_level(float *re, float *im, const PassGeom &G, u64 cb, int
                 vi = vr * s + vi * c;
                 vr = tr;
             }
-            re[t] = vr;
-            im[t] = vi;
+            re[pad(t)] = vr;
+            im[pad(t)] = vi;
         }
     }
 }
@@ -160,8 +165,8 @@ __device__ void dft_small_level(float *re, float *im, int T, int ta, int rb) {
         const int base = ((gi >> ta) << (ta + rb)) | lo;
         float xr[8], xi[8];
         for (int j = 0; j < R; j++) {
-            xr[j] = re[base + (j << ta)];
-            xi[j] = im[base + (j << ta)];
+            xr[j] = re[pad(base + (j << ta))];
+            xi[j] = im[pad(base + (j << ta))];
         }
         for (int k = 0; k < R; k++) {
             float ar = 0.f, ai = 0.f;
@@ -172,8 +177,8 @@ __device__ void dft_small_level(float *re, float *im, int T, int ta, int rb) {
                 ar += xr[j] * c - xi[j] * s;
                 ai += xr[j] * s + xi[j] * c;
             }
-            re[base + (k << ta)] = ar;
-            im[base + (k << ta)] = ai;
+            re[pad(base + (k << ta))] = ar;
+            im[pad(base + (k << ta))] = ai;
         }
     }
 }
@@ -186,7 +191,7 @@ __global__ __launch_bounds__(kBlock) void fft_mfma_pass(PassGeom G, int glo, int
     extern __shared__ float lds[];
     const int T = G.lr + G.lc;
     float *re = lds;
-    float *im = lds + (1 << T);
+    float *im = lds + (1 << T) + (1 << T) / 16;
     const u64 cblocks = (1ull << G.m) >> T;
     const u64 v = blockIdx.x / cblocks;
     const u64 cb = blockIdx.x - v * cblocks;
@@ -194,11 +199,11 @@ __global__ __launch_bounds__(kBlock) void fft_mfma_pass(PassGeom G, int glo, int
     for (int t = threadIdx.x; t < (1 << T); t += blockDim.x) {
         const u64 gi = vbase + gpos(G, cb, t);
         if (IN64) {
-            re[t] = (float)((const double *)sr)[gi];
-            im[t] = (float)((const double *)si)[gi];
+            re[pad(t)] = (float)((const double *)sr)[gi];
+            im[pad(t)] = (float)((const double *)si)[gi];
         } else {
-            re[t] = ((const float *)sr)[gi];
-            im[t] = ((const float *)si)[gi];
+            re[pad(t)] = ((const float *)sr)[gi];
+            im[pad(t)] = ((const float *)si)[gi];
         }
     }
     __syncthreads();
@@ -217,11 +222,11 @@ __global__ __launch_bounds__(kBlock) void fft_mfma_pass(PassGeom G, int glo, int
     for (int t = threadIdx.x; t < (1 << T); t += blockDim.x) {
         const u64 gi = vbase + gpos(G, cb, t);
         if (OUT64) {
-            store_stream(((double *)dr) + gi, (double)re[t] * scale);
-            store_stream(((double *)di) + gi, (double)im[t] * scale);
+            store_stream(((double *)dr) + gi, (double)re[pad(t)] * scale);
+            store_stream(((double *)di) + gi, (double)im[pad(t)] * scale);
         } else {
-            ((float *)dr)[gi] = re[t];
-            ((float *)di)[gi] = im[t];
+            ((float *)dr)[gi] = re[pad(t)];
+            ((float *)di)[gi] = im[pad(t)];
         }
     }
 }
@@ -244,10 +249,19 @@ QMC_EXPORT int qmc_fft_bro_mfma_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, doub
     if ((1ull << m) != n) return qmc_fail(QMC_ERR_ARG, "fft_mfma: length is not a power of two");
     if (m < 8 || m > 28) return qmc_fail(QMC_ERR_ARG, "fft_mfma: length 2^%d outside 2^8 .. 2^28", m);
     const double scale = (m % 2 == 0) ? ldexp(1.0, -m / 2) : ldexp(1.0, -(m - 1) / 2) / 1.4142135623730951;
-    const size_t shm = 2 * sizeof(float) << kT;
+    static bool attr_set = false;
+    if (!attr_set) {  // padded tiles need more than the default 64 KiB of LDS
+        const int bytes = (int)padded_bytes(kT);
+        QMC_HIP_CHECK(hipFuncSetAttribute((const void *)fft_mfma_pass<1, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+        QMC_HIP_CHECK(hipFuncSetAttribute((const void *)fft_mfma_pass<1, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+        QMC_HIP_CHECK(hipFuncSetAttribute((const void *)fft_mfma_pass<0, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+        QMC_HIP_CHECK(hipFuncSetAttribute((const void *)fft_mfma_pass<0, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+        attr_set = true;
+    }
+    const size_t shm = padded_bytes(kT);
     if (m <= kT) {
         PassGeom G{m, 0, m, m, m};
-        hipLaunchKernelGGL((fft_mfma_pass<1, 1>), dim3((unsigned)V), dim3(kBlock), 2 * sizeof(float) << m, s, G, 0, m,
+        hipLaunchKernelGGL((fft_mfma_pass<1, 1>), dim3((unsigned)V), dim3(kBlock), padded_bytes(m), s, G, 0, m,
                            (const void *)xr, (const void *)xi, (void *)xr, (void *)xi, scale);
         QMC_HIP_CHECK(hipGetLastError());
         return QMC_OK;

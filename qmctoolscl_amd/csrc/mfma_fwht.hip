@@ -24,8 +24,13 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kTileLog2 = 14;  // 2^14 fp32 = 64 KiB of LDS per workgroup
+constexpr int kTileLog2 = 14;  // 2^14 fp32 (68 KiB of LDS with padding) per workgroup
 constexpr int kWaves = kBlock / 64;
+
+// LDS word of tile index t: one pad word per 16 keeps the 16-point columns of
+// the lowest axis (addresses 16 apart) on distinct banks
+__device__ __forceinline__ int pad(int t) { return t + (t >> 4); }
+static inline size_t padded_bytes(int T) { return sizeof(float) * (((size_t)1 << T) + ((size_t)1 << T) / 16); }
 
 __device__ __forceinline__ void split3(float x, __bf16 &a, __bf16 &b, __bf16 &c) {
     a = (__bf16)x;
@@ -63,7 +68,7 @@ __device__ void h16_axis(float *tile, int T, int a, int wave, int lane, bf16x8 A
 #pragma unroll
         for (int i = 0; i < 8; i++) {
             __bf16 s0, s1, s2;
-            split3(tile[base + ((c0 + i) << a)], s0, s1, s2);
+            split3(tile[pad(base + ((c0 + i) << a))], s0, s1, s2);
             b1[i] = g < 2 ? s0 : s1;
             b2[i] = g < 2 ? s2 : (__bf16)0.0f;
         }
@@ -72,7 +77,7 @@ __device__ void h16_axis(float *tile, int T, int a, int wave, int lane, bf16x8 A
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, b2, acc, 0, 0, 0);
         // D layout: lane holds D[4g + i][n], i = 0..3
 #pragma unroll
-        for (int i = 0; i < 4; i++) tile[base + ((g * 4 + i) << a)] = acc[i];
+        for (int i = 0; i < 4; i++) tile[pad(base + ((g * 4 + i) << a))] = acc[i];
     }
 }
 
@@ -82,9 +87,9 @@ __device__ void h2_axes(float *tile, int T, int a, int k) {
         const int h = 1 << s;
         for (int b = threadIdx.x; b < (1 << (T - 1)); b += blockDim.x) {
             const int p = ((b >> s) << (s + 1)) | (b & (h - 1));
-            const float x1 = tile[p], x2 = tile[p + h];
-            tile[p] = x1 + x2;
-            tile[p + h] = x1 - x2;
+            const float x1 = tile[pad(p)], x2 = tile[pad(p + h)];
+            tile[pad(p)] = x1 + x2;
+            tile[pad(p + h)] = x1 - x2;
         }
         __syncthreads();
     }
@@ -107,7 +112,7 @@ __global__ __launch_bounds__(kBlock) void fwht_mfma_pass(int m, int lr, int lc, 
     const int cmask = (1 << lc) - 1;
     for (int t = threadIdx.x; t < (1 << T); t += blockDim.x) {
         const u64 gidx = vbase + (u64)(t >> lc) * row_stride + (cb << lc) + (t & cmask);
-        tile[t] = IN64 ? (float)((const double *)src)[gidx] : ((const float *)src)[gidx];
+        tile[pad(t)] = IN64 ? (float)((const double *)src)[gidx] : ((const float *)src)[gidx];
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -121,9 +126,9 @@ __global__ __launch_bounds__(kBlock) void fwht_mfma_pass(int m, int lr, int lc, 
     for (int t = threadIdx.x; t < (1 << T); t += blockDim.x) {
         const u64 gidx = vbase + (u64)(t >> lc) * row_stride + (cb << lc) + (t & cmask);
         if (OUT64)
-            store_stream(((double *)dst) + gidx, (double)tile[t] * scale);
+            store_stream(((double *)dst) + gidx, (double)tile[pad(t)] * scale);
         else
-            ((float *)dst)[gidx] = tile[t];
+            ((float *)dst)[gidx] = tile[pad(t)];
     }
 }
 
@@ -158,8 +163,16 @@ QMC_EXPORT int qmc_fwht_mfma_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, double 
     if (m > 2 * kTileLog2 - 4) return qmc_fail(QMC_ERR_ARG, "fwht_mfma: length 2^%d is above 2^%d", m, 2 * kTileLog2 - 4);
     // 1/sqrt(2^m), exact for even m
     const double scale = (m % 2 == 0) ? ldexp(1.0, -m / 2) : ldexp(1.0, -(m - 1) / 2) / 1.4142135623730951;
+    static bool attr_set = false;
+    if (!attr_set) {  // tiles of 2^14 padded values need more than the default 64 KiB of LDS
+        const int bytes = (int)padded_bytes(kTileLog2);
+        QMC_HIP_CHECK(hipFuncSetAttribute((const void *)fwht_mfma_pass<1, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+        QMC_HIP_CHECK(hipFuncSetAttribute((const void *)fwht_mfma_pass<1, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+        QMC_HIP_CHECK(hipFuncSetAttribute((const void *)fwht_mfma_pass<0, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+        attr_set = true;
+    }
     if (m <= kTileLog2) {
-        const size_t shm = sizeof(float) << m;
+        const size_t shm = padded_bytes(m);
         hipLaunchKernelGGL((fwht_mfma_pass<1, 1>), dim3((unsigned)V), dim3(kBlock), shm, s, m, 0, m, (u64)0, 0, m,
                            (const void *)x, (void *)x, scale);
         QMC_HIP_CHECK(hipGetLastError());
@@ -169,7 +182,7 @@ QMC_EXPORT int qmc_fwht_mfma_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, double 
     if (!buf) QMC_HIP_CHECK(hipMallocAsync((void **)&buf, sizeof(float) * V * n, s));
     // pass 1: the low kTileLog2 bits over contiguous tiles
     const int m1 = kTileLog2, m2 = m - kTileLog2;
-    const size_t shm = sizeof(float) << kTileLog2;
+    const size_t shm = padded_bytes(kTileLog2);
     hipLaunchKernelGGL((fwht_mfma_pass<1, 0>), dim3((unsigned)(V << (m - m1))), dim3(kBlock), shm, s, m, 0, m1,
                        (u64)0, 0, m1, (const void *)x, (void *)buf, 1.0);
     // pass 2: the high m2 bits; a tile is 2^m2 rows (stride 2^m1) x 2^(14-m2) columns

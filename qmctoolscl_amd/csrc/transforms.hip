@@ -80,35 +80,72 @@ __global__ __launch_bounds__(kBlock) void tile_kernel(u64 total, int m, int tlog
     const u64 base = (u64)blockIdx.x << tlog2;
     const u64 len = (total - base) < tile ? (total - base) : tile;
     double *sr = lds;
-    double *si = lds + tile;
+    double *si = lds + tile + tile / 16;
     const u64 nmask = (1ull << m) - 1;
+    // one pad double per 16 keeps 16-value groups of the lowest bits off a single bank
+    auto P = [](u64 t) { return t + (t >> 4); };
     for (u64 t = threadIdx.x; t < len; t += blockDim.x) {
-        sr[t] = xr[base + t];
-        if (KIND != KIND_FWHT) si[t] = xi[base + t];
+        sr[P(t)] = xr[base + t];
+        if (KIND != KIND_FWHT) si[P(t)] = xi[base + t];
     }
     __syncthreads();
-    for (int s = s_first;; s += dir) {
-        const u64 h = 1ull << s;
-        for (u64 b = threadIdx.x; b < len / 2; b += blockDim.x) {
-            const u64 p = ((b >> s) << (s + 1)) | (b & (h - 1));
-            const u64 q = p + h;
-            if (KIND == KIND_FWHT) {
-                bfly_fwht(sr[p], sr[q]);
-            } else {
-                const u64 tw = ((base + p) & nmask & (h - 1)) << (m - 1 - s);
-                const double c = twr[tw], sn = twi[tw];
-                if (KIND == KIND_FFT)
-                    bfly_dit(sr[p], si[p], sr[q], si[q], c, sn);
-                else
-                    bfly_dif(sr[p], si[p], sr[q], si[q], c, sn);
+    // Rounds of up to four consecutive stages: a lane loads the 16 values that
+    // differ in the round's index bits into registers, runs the stages there
+    // (same butterflies, same order) and stores them back: one LDS round trip
+    // and one barrier per four stages.
+    int s = s_first;
+    const int nstages = (s_last - s_first) * dir + 1;
+    for (int done = 0; done < nstages;) {
+        const int cnt = (nstages - done) < kGroupLog2 ? (nstages - done) : kGroupLog2;
+        const int lo = dir > 0 ? s : s - cnt + 1;  // lowest index bit of the round
+        const u64 groups = len >> cnt;
+        for (u64 g = threadIdx.x; g < groups; g += blockDim.x) {
+            const u64 gb = ((g >> lo) << (lo + cnt)) | (g & ((1ull << lo) - 1));
+            double re[1 << kGroupLog2], im[1 << kGroupLog2];
+#pragma unroll
+            for (int c = 0; c < (1 << kGroupLog2); c++) {
+                if (c < (1 << cnt)) {
+                    re[c] = sr[P(gb + ((u64)c << lo))];
+                    if (KIND != KIND_FWHT) im[c] = si[P(gb + ((u64)c << lo))];
+                }
+            }
+#pragma unroll
+            for (int st = 0; st < kGroupLog2; st++) {
+                if (st >= cnt) break;
+                const int bit = dir > 0 ? st : cnt - 1 - st;
+                const int ss = lo + bit;  // this stage's stride is 2^ss
+#pragma unroll
+                for (int c1 = 0; c1 < (1 << kGroupLog2); c1++) {
+                    if (c1 >= (1 << cnt) || ((c1 >> bit) & 1)) continue;
+                    const int c2 = c1 | (1 << bit);
+                    if (KIND == KIND_FWHT) {
+                        bfly_fwht(re[c1], re[c2]);
+                    } else {
+                        const u64 p = gb + ((u64)c1 << lo);
+                        const u64 tw = ((base + p) & nmask & ((1ull << ss) - 1)) << (m - 1 - ss);
+                        const double cc = twr[tw], sn = twi[tw];
+                        if (KIND == KIND_FFT)
+                            bfly_dit(re[c1], im[c1], re[c2], im[c2], cc, sn);
+                        else
+                            bfly_dif(re[c1], im[c1], re[c2], im[c2], cc, sn);
+                    }
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < (1 << kGroupLog2); c++) {
+                if (c < (1 << cnt)) {
+                    sr[P(gb + ((u64)c << lo))] = re[c];
+                    if (KIND != KIND_FWHT) si[P(gb + ((u64)c << lo))] = im[c];
+                }
             }
         }
         __syncthreads();
-        if (s == s_last) break;
+        done += cnt;
+        s += cnt * dir;
     }
     for (u64 t = threadIdx.x; t < len; t += blockDim.x) {
-        xr[base + t] = sr[t];
-        if (KIND != KIND_FWHT) xi[base + t] = si[t];
+        xr[base + t] = sr[P(t)];
+        if (KIND != KIND_FWHT) xi[base + t] = si[P(t)];
     }
 }
 
@@ -150,8 +187,10 @@ __global__ __launch_bounds__(kBlock) void group_kernel(u64 groups, int m, int a,
             if (KIND == KIND_FWHT) {
                 bfly_fwht(re[c1], re[c2]);
             } else {
+                // stage-major table: entry 2^s + t holds twiddle t << (m-1-s), so
+                // lanes with consecutive indices read consecutive entries
                 const u64 i1 = in_vec + ((u64)c1 << a);
-                const u64 tw = (i1 & ((1ull << s) - 1)) << (m - 1 - s);
+                const u64 tw = (1ull << s) + (i1 & ((1ull << s) - 1));
                 const double cc = twr[tw], sn = twi[tw];
                 if (KIND == KIND_FFT)
                     bfly_dit(re[c1], im[c1], re[c2], im[c2], cc, sn);
@@ -169,6 +208,18 @@ __global__ __launch_bounds__(kBlock) void group_kernel(u64 groups, int m, int a,
     }
 }
 
+// stage-major copy of the twiddle table: ts[2^s + t] = tw[t << (m-1-s)], t < 2^s
+__global__ __launch_bounds__(kBlock) void stage_twiddles_kernel(int m, const double *__restrict__ twr,
+                                                                const double *__restrict__ twi,
+                                                                double *__restrict__ tsr, double *__restrict__ tsi) {
+    const u64 k = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0 || k >= (1ull << m)) return;
+    const int s = 63 - __clzll((long long)k);
+    const u64 src = (k - (1ull << s)) << (m - 1 - s);
+    tsr[k] = twr[src];
+    tsi[k] = twi[src];
+}
+
 int log2_exact(u64 n) {
     int m = 0;
     while ((1ull << m) < n) m++;
@@ -179,7 +230,16 @@ template <int KIND>
 int launch_tile(u64 total, int m, int tlog2, int s_first, int s_last, int dir, double *xr, double *xi,
                 const double *twr, const double *twi, hipStream_t s) {
     const u64 tiles = (total + (1ull << tlog2) - 1) >> tlog2;
-    const size_t shm = (size_t)(KIND == KIND_FWHT ? 1 : 2) * sizeof(double) << tlog2;
+    const size_t words = ((size_t)1 << tlog2) + ((size_t)1 << tlog2) / 16;  // padded
+    const size_t shm = (size_t)(KIND == KIND_FWHT ? 1 : 2) * sizeof(double) * words;
+    static bool attr_set = false;
+    if (!attr_set) {  // padded 64 KiB tiles need a little more than the default LDS limit
+        const size_t most = (size_t)(KIND == KIND_FWHT ? 1 : 2) * sizeof(double) *
+                            (((size_t)1 << (KIND == KIND_FWHT ? kTileLog2Real : kTileLog2Cplx)) * 17 / 16);
+        QMC_HIP_CHECK(hipFuncSetAttribute((const void *)tile_kernel<KIND>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          (int)most));
+        attr_set = true;
+    }
     hipLaunchKernelGGL(tile_kernel<KIND>, dim3((unsigned)tiles), dim3(kBlock), shm, s, total, m, tlog2, s_first,
                        s_last, dir, xr, xi, twr, twi);
     QMC_HIP_CHECK(hipGetLastError());
@@ -214,29 +274,33 @@ int run_transform(const char *who, u64 d1, u64 d2, u64 n_half, double *twr, doub
     }
     const int tcap = KIND == KIND_FWHT ? kTileLog2Real : kTileLog2Cplx;
     const int tl = m < tcap ? m : tcap;  // LDS stages: strides 2^0 .. 2^(tl-1)
-    int rc;
+    double *tsr = nullptr, *tsi = nullptr;
+    if (KIND != KIND_FWHT && m > tl) {
+        QMC_HIP_CHECK(hipMallocAsync((void **)&tsr, 2 * sizeof(double) * n, s));
+        tsi = tsr + n;
+        hipLaunchKernelGGL(stage_twiddles_kernel, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, m,
+                           twr, twi, tsr, tsi);
+    }
+    int rc = QMC_OK;
     if (KIND == KIND_FFT) {
         // strides ascend: LDS stages first, then register groups
         rc = launch_tile<KIND>(total, m, tl, 0, tl - 1, +1, xr, xi, twr, twi, s);
-        if (rc) return rc;
-        for (int a = tl; a < m; a += kGroupLog2) {
+        for (int a = tl; a < m && !rc; a += kGroupLog2) {
             const int R = (m - a) < kGroupLog2 ? (m - a) : kGroupLog2;
-            rc = launch_group<KIND>(total, m, a, R, 1, xr, xi, twr, twi, s);
-            if (rc) return rc;
+            rc = launch_group<KIND>(total, m, a, R, 1, xr, xi, tsr, tsi, s);
         }
     } else {
         // strides descend: register groups from the top, then LDS stages
         int top = m;
-        while (top > tl) {
+        while (top > tl && !rc) {
             const int R = (top - tl) < kGroupLog2 ? (top - tl) : kGroupLog2;
-            rc = launch_group<KIND>(total, m, top - R, R, 0, xr, xi, twr, twi, s);
-            if (rc) return rc;
+            rc = launch_group<KIND>(total, m, top - R, R, 0, xr, xi, tsr, tsi, s);
             top -= R;
         }
-        rc = launch_tile<KIND>(total, m, tl, tl - 1, 0, -1, xr, xi, twr, twi, s);
-        if (rc) return rc;
+        if (!rc) rc = launch_tile<KIND>(total, m, tl, tl - 1, 0, -1, xr, xi, twr, twi, s);
     }
-    return QMC_OK;
+    if (tsr) (void)hipFreeAsync(tsr, s);
+    return rc;
 }
 
 }  // namespace

@@ -203,3 +203,23 @@ def test_empty_inputs_are_no_ops():
     xb = np.empty((0, 4, 2), dtype=np.uint64)
     q.dnb2_gen_gray(np.uint64(0), np.uint64(4), np.uint64(2), np.uint64(0), np.uint64(8),
                     np.ones((0, 2, 8), dtype=np.uint64), xb)
+
+
+@pytest.mark.parametrize("m,V", [(12, 3), (13, 2), (14, 2), (16, 2), (18, 1), (20, 1)])
+def test_large_transforms_match_oracle(m, V):
+    """Lengths past one LDS tile exercise the register passes (and the
+    stage-major twiddle table): FWHT still bit-exact, FFT/IFFT within 1e-12."""
+    rng = np.random.default_rng(m)
+    n = 1 << m
+    x = rng.standard_normal((1, V, n))
+    want = x.copy()
+    _clib.call_oracle("fwht_1d_radix2", dict(d1=1, d2=V, n_half=n // 2, x=want))
+    got = run_product("fwht_1d_radix2", dict(d1=1, d2=V, n_half=n // 2, x=x.copy()), "device")["x"]
+    assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), m
+    for name in ("fft_bro_1d_radix2", "ifft_bro_1d_radix2"):
+        case = dict(d1=1, d2=V, n_half=n // 2, twiddler=np.empty(n), twiddlei=np.empty(n),
+                    xr=rng.standard_normal((1, V, n)), xi=rng.standard_normal((1, V, n)))
+        want = cases.copy_case(case)
+        _clib.call_oracle(name, want)
+        got = run_product(name, cases.copy_case(case), "device")
+        compare(name, got, want)
