@@ -49,7 +49,12 @@ struct Req {
     u32 tiles_per_l;     // n / rows_per_tile
     u64 tab;             // word offset of the request's tables in the workspace
     u64 slabs;           // word offset of r slab bases (output element offsets)
-    u64 sj;              // gdn: LMS entries per replication (sum over j of tn_j * mm_j)
+    u32 clog;            // dnb2: log2 of the row stride between a chain's points
+    u32 gK;              // gdn: rows per chain
+    u32 gG, gCH;         // gdn: lanes per row (consecutive columns), chains per wave
+    u32 gng;             // gdn: column groups of gG
+    u32 gtw;             // gdn: u32 words of one replication's digit tables
+    u64 gtab;            // gdn: word offset (in the request's tables) of the replication tables
     DivU32 ddiv;         // division by d
 };
 
@@ -160,46 +165,66 @@ __host__ __device__ __forceinline__ void gdn_perm(u64 seed, u64 l, u64 j, int t,
 
 // ----------------------------------------------------------- table layouts
 // lattice : g[d] | shift[r*d] (double bits)
-// dnb2 LMS: pre[r*d*32] (prefix XOR of the LMS columns) | cols[r*d*32] | shift[r*d]
-// dnb2 NUS: pre[d*32] | cols[d*32] | nseed[r*d]
-// gdn     : per j: b, tn, mm, S offset, ceil(2^64/b), b^tn (double bits) (8 words)
-//           | S (u32, two per word) | perm (a,c) per (l,j,t) | base-2 row masks per (l,t)
+// dnb2    : per column set (l, j) -- r*d sets for LMS, d for NUS -- 64 u32:
+//           pre[32] (prefix XOR of the columns) then cols[32] | shift or NUS seed[r*d]
+// gdn     : per j kGJ words of dimension info | per replication l, gtw u32 of
+//           digit tables (layout below), one block per j at u32 offset qoff_j
+//
+// gdn digit tables.  Point k of dimension j has base-b digits dig_i(k); the
+// scrambled digit t is z_t(k) = (a_t * sum_i S[t][i] dig_i(k) + c_t) mod b.
+// Going from k to k+1 turns the c trailing digits equal to b-1 into 0 and adds
+// 1 to digit c, so every z_t moves by Q_c[t] = a_t * sum_{i<=c} S[t][i] mod b
+// (-(b-1) = 1 mod b).  A chain of consecutive points is then one table row
+// added per point, digitwise mod b.
+//   b >= 3: rows 0..mm of W u32 each; row 0 holds c_t, row 1+c holds Q_c.
+//           Digits are u16 halves, two per u32: padded digit u = t + (tn & 1)
+//           (odd tn gets a leading zero digit) sits in pair u/2, the more
+//           significant digit (even u) in the low half.  Rows are right-aligned
+//           to the width W of the widest expansion among the dimension's lane
+//           group (np = ceil(tn/2) <= W pairs, zero pairs in front), so every
+//           lane of a wave walks the same register pairs.
+//   b == 2: u64 words (a == 1): cmask, then Q_0..Q_{mm-1}, digit t at bit 51-t.
 
-constexpr int kJInfo = 8;
-
-struct GdnLayout {
-    u64 jinfo;   // kJInfo words per j
-    u64 S;       // r * SJ u32 entries, packed two per word
-    u64 perm;    // r * d * kTNmax words: (a << 32) | c
-    u64 mask2;   // r * kTNmax words: row t of S for base 2 as a bit mask over digits
-    u64 words;
-};
-
-__host__ __device__ __forceinline__ GdnLayout gdn_layout(int d, int r, u64 SJ) {
-    GdnLayout g;
-    g.jinfo = 0;
-    g.S = (u64)kJInfo * d;
-    g.perm = g.S + (r * SJ + 1) / 2;
-    g.mask2 = g.perm + (u64)r * d * kTNmax;
-    g.words = g.mask2 + (u64)r * kTNmax;
-    return g;
-}
+// words of dimension info: b | L<<16 | qoff<<32,  tn | mm<<8 | np<<16 | W<<24 | b^2<<32,  ceil(2^64/b),  b^tn
+constexpr int kGJ = 4;
+constexpr int kNPmax = 17;   // u32 digit pairs of the longest b >= 3 expansion (b = 3: 33 digits + 1 pad)
 
 __host__ __device__ __forceinline__ u64 recip_magic(u32 b) { return b == 1 ? 0 : (~0ull) / b + 1; }
 
 // x mod b and x / b for x < 2^32 with M = ceil(2^64 / b) (exact, Lemire et al. 2019)
 __device__ __forceinline__ u32 fast_div(u32 x, u64 M) { return (u32)__umul64hi(M, (u64)x); }
 
-u64 table_words(const Req &q, u64 &SJ) {
+// lane groups of the gdn tile: gG consecutive columns; the group's widest
+// expansion is that of its smallest odd base
+inline u32 gdn_group_first_odd(const Req &q, int j) {
+    const int g0 = (j / (int)q.gG) * (int)q.gG;
+    return h_primes[g0 == 0 ? (q.d > 1 ? 1 : 0) : g0];
+}
+inline u32 gdn_group_max_base(const Req &q, int j) {
+    const int g1 = (j / (int)q.gG + 1) * (int)q.gG;
+    return h_primes[(g1 < q.d ? g1 : q.d) - 1];
+}
+inline u32 gdn_np(u32 b) { return b == 2 ? 0 : ((u32)gdn_tn(b) + 1) / 2; }
+
+inline u32 gdn_words(const Req &q, int j) {
+    const u32 b = h_primes[j];
+    const u32 mm = (u32)gdn_mm(b, q.m);
+    const u32 w = b == 2 ? 2 + 2 * mm : (mm + 1) * gdn_np(gdn_group_first_odd(q, j));
+    return (w + 1) & ~1u;  // even: every block starts 8-byte aligned
+}
+
+u64 table_words(Req &q) {
     const u64 d = q.d, r = q.r;
-    SJ = 0;
     switch (q.method) {
         case M_LATTICE: return d + r * d;
-        case M_DNB2_LMS: return 2 * r * d * kTB + r * d;
-        case M_DNB2_NUS: return 2 * d * kTB + r * d;
+        case M_DNB2_LMS: return 32 * r * d + r * d;
+        case M_DNB2_NUS: return 32 * d + r * d;
         default: {
-            for (int j = 0; j < q.d; j++) SJ += (u64)gdn_tn(h_primes[j]) * gdn_mm(h_primes[j], q.m);
-            return gdn_layout(q.d, q.r, SJ).words;
+            u32 w = 0;
+            for (int j = 0; j < q.d; j++) w += gdn_words(q, j);
+            q.gtw = w;
+            q.gtab = (u64)kGJ * d;
+            return q.gtab + r * (u64)(w / 2);
         }
     }
 }
@@ -222,13 +247,14 @@ __global__ __launch_bounds__(kBlock) void batch_setup_kernel(const Req *__restri
             }
         }
     } else if (q.method == M_DNB2_LMS || q.method == M_DNB2_NUS) {
-        const u64 ncol = (q.method == M_DNB2_LMS ? r : 1) * d;  // (l, j) columns sets
-        for (u64 c = tid; c < ncol + r * d; c += step) {
-            if (c < ncol) {
+        const u64 nset = (q.method == M_DNB2_LMS ? r : 1) * d;  // (l, j) column sets
+        u32 *T32 = (u32 *)tab;
+        for (u64 c = tid; c < nset + r * d; c += step) {
+            if (c < nset) {
                 const u64 l = c / d, j = c % d;
-                u64 *pre = tab + c * kTB;
-                u64 *cols = tab + ncol * kTB + c * kTB;
-                u64 acc = 0;
+                u32 *pre = T32 + c * 64;
+                u32 *cols = pre + 32;
+                u32 acc = 0;
                 for (int k = 0; k < kTB; k++) {
                     u64 col = dnb2_col(q.seed, j, k);
                     if (q.method == M_DNB2_LMS) {  // C_lms[k] = S * C[k] over GF(2) (dnb2_linear_matrix_scramble.c:33-48)
@@ -237,55 +263,64 @@ __global__ __launch_bounds__(kBlock) void batch_setup_kernel(const Req *__restri
                             out |= (u64)(__popcll(dnb2_srow(q.seed, l, j, t) & col) & 1) << (kTB - 1 - t);
                         col = out;
                     }
-                    cols[k] = col;
-                    acc ^= col;
+                    cols[k] = (u32)col;
+                    acc ^= (u32)col;
                     pre[k] = acc;
                 }
             } else {
-                const u64 f = c - ncol;
-                tab[2 * ncol * kTB + f] = q.method == M_DNB2_LMS ? dnb2_shift(q.seed, f / d, f % d)
-                                                                 : nus_seed(q.seed, f / d, f % d);
+                const u64 f = c - nset;
+                tab[32 * nset + f] = q.method == M_DNB2_LMS ? dnb2_shift(q.seed, f / d, f % d)
+                                                           : nus_seed(q.seed, f / d, f % d);
             }
         }
     } else {
-        // gdn: j info, S rows, permutations
-        const u64 SJ = q.sj;
-        const GdnLayout L = gdn_layout(q.d, q.r, SJ);
-        u32 *S = (u32 *)(tab + L.S);
-        // j info (offsets are cumulative; computed by every thread for its own j)
-        for (u64 j = tid; j < d; j += step) {
-            u64 off = 0;
-            for (u64 jj = 0; jj < j; jj++) off += (u64)gdn_tn(c_primes[jj]) * gdn_mm(c_primes[jj], q.m);
-            const u32 b = c_primes[j];
-            const int tn = gdn_tn(b);
-            double scale = 1.0;
-            for (int t = 0; t < tn; t++) scale *= (double)b;
-            u64 *ji = tab + L.jinfo + kJInfo * j;
-            ji[0] = b;
-            ji[1] = tn;
-            ji[2] = gdn_mm(b, q.m);
-            ji[3] = off;
-            ji[4] = recip_magic(b);
-            ji[5] = (u64)__double_as_longlong(scale);
-        }
-        for (u64 e = tid; e < r * d; e += step) {
-            const u64 l = e / d, j = e % d;
-            const u32 b = c_primes[j];
-            const int tn = gdn_tn(b), mm = gdn_mm(b, q.m);
-            u64 off = 0;
-            for (u64 jj = 0; jj < j; jj++) off += (u64)gdn_tn(c_primes[jj]) * gdn_mm(c_primes[jj], q.m);
-            u32 *Slj = S + l * SJ + off;
-            for (int t = 0; t < tn; t++) {
-                u64 mask = 0;
-                for (int k = 0; k < mm; k++) {
-                    const u32 s = k <= t ? gdn_s(q.seed, l, j, t, k, b) : 0;
-                    Slj[t * mm + k] = s;
-                    mask |= (u64)(s & 1) << k;
+        // gdn digit tables, one thread per (l, j, t); the dimension info was
+        // written by the host
+        u32 *T = (u32 *)(tab + q.gtab);
+        for (u64 e = tid; e < r * d * kTNmax; e += step) {
+            const int t = (int)(e % kTNmax);
+            const u64 lj = e / kTNmax, l = lj / d, j = lj % d;
+            const u64 w0 = tab[kGJ * j], w1 = tab[kGJ * j + 1];
+            const u32 b = (u32)w0 & 0xffffu;
+            const int tn = (int)(w1 & 0xff), mm = (int)((w1 >> 8) & 0xff), np = (int)((w1 >> 16) & 0xff);
+            const int W = (int)((w1 >> 24) & 0xff);
+            const u32 qoff = (u32)(w0 >> 32);
+            const u32 qend = j + 1 < d ? (u32)(tab[kGJ * (j + 1)] >> 32) : q.gtw;
+            const u32 need = b == 2 ? 2 * (u32)(mm + 1) : (u32)((mm + 1) * W);
+            if (qoff + need > qend) continue;  // never: the host sizes every block (keeps writes in it)
+            u32 *Tj = T + l * q.gtw + qoff;
+            if (b == 2) {
+                if (t) continue;
+                u64 *Q2 = (u64 *)Tj;
+                for (int i = 0; i <= mm; i++) Q2[i] = 0;
+                u64 cm = 0;
+                for (int tt = 0; tt < tn; tt++) {
+                    u32 a, cc;
+                    gdn_perm(q.seed, l, j, tt, 2, a, cc);
+                    cm |= (u64)cc << (51 - tt);
+                    u32 run = 0;
+                    for (int i = 0; i < mm; i++) {
+                        if (i <= tt) run ^= gdn_s(q.seed, l, j, tt, i, 2) & 1;
+                        Q2[1 + i] |= (u64)run << (51 - tt);
+                    }
                 }
-                if (b == 2) tab[L.mask2 + l * kTNmax + t] = mask;
-                u32 a, c;
-                gdn_perm(q.seed, l, j, t, b, a, c);
-                tab[L.perm + (l * d + j) * kTNmax + t] = ((u64)a << 32) | c;
+                Q2[0] = cm;
+            } else {
+                if (t >= tn) continue;
+                // digit u of row w at H[w * 2W + 2(W - np) + u]; zero pairs in front
+                unsigned short *H = (unsigned short *)Tj + 2 * (W - np);
+                const int u = t + (tn & 1);
+                if (t == 0)
+                    for (int w = 0; w <= mm; w++)
+                        for (int h = -2 * (W - np); h < (tn & 1); h++) H[w * 2 * W + h] = 0;
+                u32 a, cc;
+                gdn_perm(q.seed, l, j, t, b, a, cc);
+                H[u] = (unsigned short)cc;
+                u32 run = 0;
+                for (int i = 0; i < mm; i++) {
+                    if (i <= t) run = (run + a * gdn_s(q.seed, l, j, t, i, b)) % b;  // a, S < b <= 2^11
+                    H[(1 + i) * 2 * W + u] = (unsigned short)run;
+                }
             }
         }
     }
@@ -293,8 +328,8 @@ __global__ __launch_bounds__(kBlock) void batch_setup_kernel(const Req *__restri
 
 // ------------------------------------------------------------- one tile
 
-__device__ __forceinline__ u64 xor_cols32(const u64 *__restrict__ cols, u64 q) {
-    u64 v = 0;
+__device__ __forceinline__ u32 xor_cols(const u32 *__restrict__ cols, u64 q) {
+    u32 v = 0;
     while (q) {
         v ^= cols[__ffsll((long long)q) - 1];
         q &= q - 1;
@@ -302,16 +337,208 @@ __device__ __forceinline__ u64 xor_cols32(const u64 *__restrict__ cols, u64 q) {
     return v;
 }
 
-constexpr int kMaxDigB = 21;  // base-b digits of a 32-bit index, b >= 3
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
-__device__ void run_tile(const Req &q, u64 t, const u64 *__restrict__ ws, double *__restrict__ out, u32 *sdig) {
+__device__ __forceinline__ u16x2 as_pair(u32 w) { return __builtin_bit_cast(u16x2, w); }
+__device__ __forceinline__ u32 as_word(u16x2 p) { return __builtin_bit_cast(u32, p); }
+
+// dnb2 output of column value v (the top 32 of 64 bits): LMS adds the 64-bit
+// digital shift xs, NUS hashes; then x = y * 2^-64 (dnb2_integer_to_float.c:29).
+// For LMS, y = (v ^ xs_hi) * 2^32 + xs_lo and one fma of exact terms rounds
+// y * 2^-64 once, as the u64 -> f64 conversion does.
+template <bool LMS>
+__device__ __forceinline__ double dnb2_emit(u32 v, u64 xs, double lo) {
+    if (LMS) return __fma_rn((double)(v ^ (u32)(xs >> 32)), 0x1p-32, lo);
+    return ldexp((double)owen_hash64((u64)v << 32, xs), -64);
+}
+
+// Chains of a dnb2 tile: the tile's R*d (row, column) slots of its first R rows
+// (R = 2^clog) each walk down the tile in steps of R rows, so that at every
+// step the block writes R*d consecutive doubles.  From chain step g to g+1 the
+// row index changes in bits clog..clog+c (c = ctz(g+1)), so the value changes by
+// pre[clog+c] ^ pre[clog-1] (pre = prefix XOR of the columns).  Steps come in
+// groups of 8 whose first 7 transitions use c = 0,1,0,2,0,1,0.
+template <bool LMS>
+__device__ __forceinline__ void dnb2_tile(const Req &q, u64 l, u64 row0, const u64 *__restrict__ tab,
+                                          double *__restrict__ slab) {
+    const u32 d = (u32)q.d;
+    const u32 s = q.clog;
+    const u32 R = 1u << s, H = q.rows_per_tile >> s;
+    const u64 set = LMS ? l * d : 0;
+    const u64 nset = (LMS ? (u64)q.r : 1) * d;
+    const u32 *T32 = (const u32 *)tab;
+    const u64 *extra = tab + 32 * nset + l * d;  // shifts or NUS seeds
+    const u64 stride = (u64)R * d;
+    const u64 g0 = row0 >> s;
+    for (u32 sig = threadIdx.x; sig < R * d; sig += blockDim.x) {
+        const u32 rr0 = div_q(sig, q.ddiv);
+        const u32 j = sig - rr0 * d;
+        const u32 *pre = T32 + (set + j) * 64;
+        const u32 *cols = pre + 32;
+        const u64 xs = extra[j];
+        const double lo = (double)(u32)xs * 0x1p-64;
+        const u32 base = s ? pre[s - 1] : 0u;
+        u32 v = xor_cols(cols, row0 + rr0);
+        double *dst = slab + (row0 + rr0) * d + j;
+        if (H >= 8) {
+            const u32 x0 = pre[s] ^ base, x1 = pre[s + 1] ^ base, x2 = pre[s + 2] ^ base;
+            for (u32 h = 0; h < H; h += 8) {
+                store_stream(dst, dnb2_emit<LMS>(v, xs, lo));
+                v ^= x0;
+                store_stream(dst + stride, dnb2_emit<LMS>(v, xs, lo));
+                v ^= x1;
+                store_stream(dst + 2 * stride, dnb2_emit<LMS>(v, xs, lo));
+                v ^= x0;
+                store_stream(dst + 3 * stride, dnb2_emit<LMS>(v, xs, lo));
+                v ^= x2;
+                store_stream(dst + 4 * stride, dnb2_emit<LMS>(v, xs, lo));
+                v ^= x0;
+                store_stream(dst + 5 * stride, dnb2_emit<LMS>(v, xs, lo));
+                v ^= x1;
+                store_stream(dst + 6 * stride, dnb2_emit<LMS>(v, xs, lo));
+                v ^= x0;
+                store_stream(dst + 7 * stride, dnb2_emit<LMS>(v, xs, lo));
+                dst += 8 * stride;
+                if (h + 8 < H) v ^= pre[s + 3 + __ffsll((long long)((g0 + h + 8) >> 3)) - 1] ^ base;
+            }
+        } else {
+            for (u32 h = 0;; h++) {
+                store_stream(dst, dnb2_emit<LMS>(v, xs, lo));
+                if (h + 1 == H) break;
+                v ^= pre[s + __ffsll((long long)(g0 + h + 1)) - 1] ^ base;
+                dst += stride;
+            }
+        }
+    }
+}
+
+// z + q digitwise mod b for two u16 digits per word (both < b <= 2^11):
+// min(z + q, z + q - b) with 16-bit wrap-around
+__device__ __forceinline__ u32 pk_add_mod(u32 z, u32 q, u32 bb) {
+    u32 sum, dif, out;
+    asm("v_pk_add_u16 %0, %1, %2" : "=v"(sum) : "v"(z), "v"(q));
+    asm("v_pk_sub_u16 %0, %1, %2" : "=v"(dif) : "v"(sum), "v"(bb));
+    asm("v_pk_min_u16 %0, %1, %2" : "=v"(out) : "v"(sum), "v"(dif));
+    return out;
+}
+
+// One gdn chain of a base-2 column: len consecutive points from k, row stride d.
+__device__ __forceinline__ void gdn_chain2(const u64 *__restrict__ Q, u32 k, u32 len, u32 d, double *dst) {
+    // Q = cmask, Q_0..Q_{mm-1}; point k's digits: cmask ^ XOR of columns S_i = Q_i ^ Q_{i-1}
+    u64 z = Q[0];
+    for (u32 kk = k; kk; kk &= kk - 1) {
+        const int i = __ffs(kk) - 1;
+        z ^= Q[1 + i] ^ (i ? Q[i] : 0ull);
+    }
+    for (u32 h = 0;; h++) {
+        store_stream(dst, (double)z * 0x1p-52);  // z < 2^52: exact, = z / 2^tn
+        if (h + 1 == len) break;
+        z ^= Q[__ffs(k + 1)];  // Q_c at Q[1 + c], c = ctz(k + 1)
+        k++;
+        dst += d;
+    }
+}
+
+// One gdn chain of a column with odd base b: len consecutive points from k,
+// row stride d.  T = the column's digit tables in LDS, W pairs per row (the
+// lane group's width, as a template so every pair lives in a named register);
+// L = pairs in the low part of the floating-point Horner split.
+template <int W, int L>
+__device__ __forceinline__ void gdn_chain(const u64 *__restrict__ gj, const u32 *T, u32 k, u32 len, u32 d,
+                                          double *dst) {
+    const u64 w1 = gj[1];
+    const u32 b = (u32)gj[0] & 0xffffu;
+    const u32 mm = (u32)((w1 >> 8) & 0xff);
+    const u32 b2 = (u32)(w1 >> 32);
+    const u64 M = gj[2];
+    const double scale = __longlong_as_double((long long)gj[3]);
+    // start: z = c + sum_i dig_i * S'_i  with  S'_i = Q_i - Q_{i-1}
+    //      = c + sum_i Q_i * e_i,  e_i = dig_i - dig_{i+1} mod b  (dig_mm = 0)
+    u32 acc_lo[W], acc_hi[W];
+#pragma unroll
+    for (int P = 0; P < W; P++) {
+        const u32 w = T[P];
+        acc_lo[P] = w & 0xffffu;
+        acc_hi[P] = w >> 16;
+    }
+    u32 qd = fast_div(k, M);
+    u32 dig = k - qd * b;
+    const u32 dlow0 = dig;
+    for (u32 i = 0; i < mm; i++) {
+        const u32 qn = fast_div(qd, M);
+        const u32 dn = qd - qn * b;
+        const u32 e = dig >= dn ? dig - dn : dig + b - dn;
+        const u32 *Qi = T + (1 + i) * W;
+#pragma unroll
+        for (int P = 0; P < W; P++) {
+            const u32 w = Qi[P];
+            acc_lo[P] += e * (w & 0xffffu);  // < (1 + mm (b-1)) b < 2^32
+            acc_hi[P] += e * (w >> 16);
+        }
+        qd = qn;
+        dig = dn;
+    }
+    u32 z[W];
+#pragma unroll
+    for (int P = 0; P < W; P++) {
+        const u32 lo = acc_lo[P] - fast_div(acc_lo[P], M) * b;
+        const u32 hi = acc_hi[P] - fast_div(acc_hi[P], M) * b;
+        z[P] = lo | (hi << 16);
+    }
+    const u32 bb = b | (b << 16);
+    const double bd2 = (double)b2;
+    double BL = bd2;
+#pragma unroll
+    for (int i = 1; i < L; i++) BL *= bd2;  // (b^2)^L < 2^53: exact
+    u32 dlow = dlow0;
+    for (u32 h = 0;; h++) {
+        // Y = sum of digit pairs in base b^2 as Yhi * (b^2)^L + Ylo, both parts
+        // exact doubles (< 2^53); one fma rounds Y once, as u64 -> f64 does
+        double Yhi = 0.0, Ylo = 0.0;
+#pragma unroll
+        for (int P = 0; P < W; P++) {
+            const double pv = (double)((z[P] & 0xffffu) * b + (z[P] >> 16));
+            if (P < W - L)
+                Yhi = __fma_rn(Yhi, bd2, pv);
+            else
+                Ylo = __fma_rn(Ylo, bd2, pv);
+        }
+        store_stream(dst, __fma_rn(Yhi, BL, Ylo) / scale);
+        if (h + 1 == len) break;
+        // carries of k -> k+1: c trailing digits equal to b-1
+        u32 c = 0;
+        if (dlow == b - 1) {
+            dlow = 0;
+            c = 1;
+            u32 u = fast_div(k, M);
+            for (;;) {
+                const u32 un = fast_div(u, M);
+                if (u - un * b != b - 1) break;
+                c++;
+                u = un;
+            }
+        } else {
+            dlow++;
+        }
+        k++;
+        const u32 *Qc = T + (1 + c) * W;
+        u32 inc[W];
+#pragma unroll
+        for (int P = 0; P < W; P++) inc[P] = Qc[P];
+#pragma unroll
+        for (int P = 0; P < W; P++) z[P] = pk_add_mod(z[P], inc[P], bb);
+        dst += d;
+    }
+}
+
+__device__ void run_tile(const Req &q, u64 t, const u64 *__restrict__ ws, double *__restrict__ out, u32 *lds) {
     const u64 d = q.d;
     const u64 l = t / q.tiles_per_l;
     const u64 row0 = (t - l * q.tiles_per_l) * (u64)q.rows_per_tile;
     const u64 *tab = ws + q.tab;
     double *slab = out + ws[q.slabs + l];
-    const u32 pts = q.rows_per_tile * (u32)d;
     if (q.method == M_LATTICE) {
+        const u32 pts = q.rows_per_tile * (u32)d;
         const u64 *g = tab;
         const double *sh = (const double *)(tab + d) + l * d;
         for (u32 e = threadIdx.x; e < pts; e += blockDim.x) {
@@ -322,89 +549,45 @@ __device__ void run_tile(const Req &q, u64 t, const u64 *__restrict__ ws, double
             const double x = frac_mod1((double)g[j] * phi);      // lat_gen_natural.c:45
             store_stream(slab + row * d + j, frac_mod1(x + sh[j]));  // lat_shift_mod_1.c:31
         }
-    } else if (q.method == M_DNB2_LMS || q.method == M_DNB2_NUS) {
-        const bool lms = q.method == M_DNB2_LMS;
-        const u64 ncol = (lms ? (u64)q.r : 1) * d;
-        const u64 set = lms ? l * d : 0;
-        // threads -> (j, run); runs of consecutive rows walk natural order with
-        // one prefix-XOR per step (value(q+1) = value(q) ^ C0 ^ ... ^ C_ctz(q+1))
-        // as many runs as fill the block: runs * d <= blockDim (or one run per row)
-        u32 runs = blockDim.x / (u32)d;
-        if (runs < 1) runs = 1;
-        if (runs > q.rows_per_tile) runs = q.rows_per_tile;
-        const u32 K = (q.rows_per_tile + runs - 1) / runs;
-        const u64 *extra = tab + 2 * ncol * kTB + l * d;  // shifts or NUS seeds
-        for (u32 w = threadIdx.x; w < runs * d; w += blockDim.x) {
-            const u32 j = w % (u32)d;
-            const u32 run = w / (u32)d;
-            const u32 r0 = run * K;
-            if (r0 >= q.rows_per_tile) continue;
-            const u32 cnt = (q.rows_per_tile - r0) < K ? (q.rows_per_tile - r0) : K;
-            const u64 *pre = tab + (set + j) * kTB;
-            const u64 *cols = tab + ncol * kTB + (set + j) * kTB;
-            const u64 xs = extra[j];
-            u64 row = row0 + r0;
-            u64 v = xor_cols32(cols, row);
-            for (u32 s = 0;; s++) {
-                const u64 xb = v << 32;  // lshift 64 - tmax (dnb2_digital_shift.c:32)
-                const u64 y = lms ? (xb ^ xs) : owen_hash64(xb, xs);
-                store_stream(slab + row * d + j, ldexp((double)y, -64));  // dnb2_integer_to_float.c:29
-                if (s + 1 == cnt) break;
-                row++;
-                v ^= pre[__ffsll((long long)row) - 1];
-            }
-        }
+    } else if (q.method == M_DNB2_LMS) {
+        dnb2_tile<true>(q, l, row0, tab, slab);
+    } else if (q.method == M_DNB2_NUS) {
+        dnb2_tile<false>(q, l, row0, tab, slab);
     } else {
-        const u64 SJ = q.sj;
-        const GdnLayout L = gdn_layout(q.d, q.r, SJ);
-        const u32 *S = (const u32 *)(tab + L.S) + l * SJ;
-        const u64 *perm = tab + L.perm + l * d * kTNmax;
-        const u64 *mask2 = tab + L.mask2 + l * kTNmax;
-        u32 *dig = sdig + threadIdx.x;  // digit i at dig[i * kBlock]
-        // One dimension per wave (units of 64 rows x 1 dimension): the digit
-        // counts, LMS rows and permutations are then uniform across the wave,
-        // so no lane waits on a smaller base's longer loops.
+        // this replication's digit tables into LDS
+        const u32 *src = (const u32 *)(tab + q.gtab) + l * q.gtw;
+        for (u32 e = threadIdx.x; e < q.gtw; e += blockDim.x) lds[e] = src[e];
+        __syncthreads();
+        // units = (column group, block of gCH chains of gK rows); lanes = (chain, column)
+        const u32 G = q.gG, CH = q.gCH, K = q.gK, ng = q.gng;
         const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
-        const u32 chunks = (q.rows_per_tile + 63) / 64;
-        for (u32 unit = wave; unit < chunks * (u32)d; unit += nwaves) {
-            const u32 j = unit % (u32)d;
-            const u32 rr = (unit / (u32)d) * 64 + lane;
-            if (rr >= q.rows_per_tile) continue;
-            const u32 row = (u32)(row0 + rr);
-            const u64 *ji = tab + L.jinfo + kJInfo * j;
-            const u32 b = (u32)ji[0];
-            const int tn = (int)ji[1];
-            const int mm = (int)ji[2];
-            const u32 *Sj = S + ji[3];
-            const u64 M = ji[4];
-            const u64 *pj = perm + (u64)j * kTNmax;
-            u64 Y = 0;
-            if (b == 2) {
-                // digit t = parity of (row t of S) & row: one popcount per digit
-                for (int tt = 0; tt < tn; tt++) {
-                    const u32 xd = (u32)__popcll(mask2[tt] & (u64)row) & 1u;
-                    const u64 pc = pj[tt];
-                    Y = 2 * Y + (((u32)(pc >> 32) * xd + (u32)pc) & 1u);
-                }
+        const u32 jj = lane % G, ch = lane / G;
+        const u32 cbs = (q.rows_per_tile + CH * K - 1) / (CH * K);
+        for (u32 u = wave; u < ng * cbs; u += nwaves) {
+            const u32 jg = u % ng, cb = u / ng;
+            // the group's widest expansion (smallest odd base): row width and Horner split
+            const u32 jodd = jg * G + (jg == 0 && d > 1 ? 1u : 0u);
+            const u32 Ww = __builtin_amdgcn_readfirstlane((u32)((tab[kGJ * jodd + 1] >> 24) & 0xff));
+            const u32 Lw = __builtin_amdgcn_readfirstlane((u32)(tab[kGJ * jodd] >> 16) & 0xffffu);
+            const u32 j = jg * G + jj;
+            const u32 rs = cb * CH * K + ch * K;
+            if (ch >= CH || j >= d || rs >= q.rows_per_tile) continue;
+            const u32 len = q.rows_per_tile - rs < K ? q.rows_per_tile - rs : K;
+            const u64 k = row0 + rs;
+            const u64 *gj = tab + kGJ * j;
+            const u32 *T = lds + (u32)(gj[0] >> 32);
+            double *dst = slab + k * d + j;
+            if (j == 0) {
+                gdn_chain2((const u64 *)T, (u32)k, len, (u32)d, dst);
+            } else if (Ww == 17 && Lw == 3) {
+                gdn_chain<17, 3>(gj, T, (u32)k, len, (u32)d, dst);
+            } else if (Ww == 4 && Lw == 3) {
+                gdn_chain<4, 3>(gj, T, (u32)k, len, (u32)d, dst);
+            } else if (Ww == 4 && Lw == 2) {
+                gdn_chain<4, 2>(gj, T, (u32)k, len, (u32)d, dst);
             } else {
-                u32 k = row;
-                for (int i = 0; i < mm; i++) {
-                    const u32 qq = fast_div(k, M);
-                    dig[i * kBlock] = k - qq * b;
-                    k = qq;
-                }
-                for (int tt = 0; tt < tn; tt++) {
-                    u32 acc = 0;  // < 32 * b^2 < 2^27
-                    const int kmax = tt < mm ? tt + 1 : mm;
-                    const u32 *Srow = Sj + tt * mm;
-                    for (int i = 0; i < kmax; i++) acc += Srow[i] * dig[i * kBlock];
-                    const u32 xd = acc - fast_div(acc, M) * b;
-                    const u64 pc = pj[tt];
-                    const u32 v = (u32)(pc >> 32) * xd + (u32)pc;
-                    Y = Y * b + (v - fast_div(v, M) * b);
-                }
+                gdn_chain<3, 2>(gj, T, (u32)k, len, (u32)d, dst);  // the host admits no other shape
             }
-            store_stream(slab + (u64)row * d + j, (double)Y / __longlong_as_double((long long)ji[5]));
         }
     }
 }
@@ -417,7 +600,7 @@ __global__ __launch_bounds__(kBlock) void batch_persistent_kernel(const Req *__r
                                                                   double *__restrict__ out) {
     __shared__ u64 s_tile;
     __shared__ int s_req;
-    __shared__ u32 sdig[kMaxDigB * kBlock];
+    extern __shared__ u32 lds_tables[];  // gdn digit tables of the current tile
     for (;;) {
         if (threadIdx.x == 0) {
             const u64 t = atomicAdd(counter, 1ull);
@@ -438,7 +621,7 @@ __global__ __launch_bounds__(kBlock) void batch_persistent_kernel(const Req *__r
         __syncthreads();
         if (t >= total_tiles) break;  // every wave of every block reaches this
         const Req q = reqs[qi];
-        run_tile(q, t - q.tile0, ws, out, sdig);
+        run_tile(q, t - q.tile0, ws, out, lds_tables);
     }
 }
 
@@ -447,12 +630,43 @@ __global__ void reset_counter_kernel(unsigned long long *counter) { *counter = 0
 // one request, one tile per block (the per-call form)
 __global__ __launch_bounds__(kBlock) void batch_single_kernel(const Req *__restrict__ req, const u64 *__restrict__ ws,
                                                               double *__restrict__ out) {
-    __shared__ u32 sdig[kMaxDigB * kBlock];
+    extern __shared__ u32 lds_tables[];
     const Req q = *req;
-    run_tile(q, blockIdx.x, ws, out, sdig);
+    run_tile(q, blockIdx.x, ws, out, lds_tables);
 }
 
 // ----------------------------------------------------------- host plan
+
+// dnb2 chain stride 2^s for a tile of RT rows x d columns: U = ceil(2^s d / 256)
+// chains per thread of RT / 2^s points each, plus about 3 points' worth of
+// work to start a chain
+u32 dnb2_chain_log(u32 RT, u32 d) {
+    u32 best = 0;
+    u64 best_cost = ~0ull;
+    for (u32 s = 0; (1u << s) <= RT; s++) {
+        const u64 U = (((u64)d << s) + kBlock - 1) / kBlock;
+        const u64 cost = U * ((RT >> s) + 3);
+        if (cost < best_cost) {
+            best_cost = cost;
+            best = s;
+        }
+    }
+    return best;
+}
+
+// gdn lanes: 64 consecutive columns per wave when d > 32, else gCH = 64/d
+// chains of d columns; chains of gK rows, about 8 units of work per tile
+void gdn_shape(Req &q) {
+    const u32 d = (u32)q.d;
+    q.gG = d > 32 ? 64 : d;
+    q.gCH = 64 / q.gG;
+    q.gng = (d + q.gG - 1) / q.gG;
+    u64 want = (u64)q.rows_per_tile * q.gng / ((u64)q.gCH * 8);
+    u32 K = 1;
+    while (K * 2 <= want && K < 256) K *= 2;
+    if (K > q.rows_per_tile) K = q.rows_per_tile;
+    q.gK = K;
+}
 
 struct Plan {
     std::vector<Req> reqs;
@@ -464,6 +678,7 @@ struct Plan {
     unsigned long long *d_counter = nullptr;
     int grid = 0;
     int device = 0;
+    u32 lds_bytes = 0;               // dynamic LDS of the generation kernels (gdn tables)
     hipStream_t cap = nullptr;       // private stream for graph capture
     hipGraphExec_t gexec = nullptr;  // [counter reset, persistent kernel]
     double *gring = nullptr;         // ring the graph was captured for
@@ -471,7 +686,8 @@ struct Plan {
 
 void launch_batch(Plan *P, double *ring, hipStream_t s) {
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, P->d_counter);
-    hipLaunchKernelGGL(batch_persistent_kernel, dim3(P->grid), dim3(kBlock), 0, s, P->d_reqs, (int)P->reqs.size(),
+    hipLaunchKernelGGL(batch_persistent_kernel, dim3(P->grid), dim3(kBlock), P->lds_bytes, s, P->d_reqs,
+                       (int)P->reqs.size(),
                        P->total_tiles, P->d_counter, P->d_ws, ring);
 }
 
@@ -506,6 +722,7 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
         }
         const u64 n = 1ull << q.m;
         u64 rows = points_per_tile / (u64)q.d;
+        if (q.method == M_GDN_LMS && rows < 1024) rows = 1024;  // digit tables are loaded once per tile
         u64 p2 = 1;
         while (p2 * 2 <= rows) p2 *= 2;
         if (p2 > n) p2 = n;
@@ -514,10 +731,19 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
         q.tile0 = tile;
         tile += (u64)q.r * q.tiles_per_l;
         q.ddiv = make_div((u32)q.d);
-        u64 SJ;
+        if (q.method == M_DNB2_LMS || q.method == M_DNB2_NUS) q.clog = dnb2_chain_log(q.rows_per_tile, (u32)q.d);
+        if (q.method == M_GDN_LMS) gdn_shape(q);
         q.tab = words;
-        words += table_words(q, SJ);
-        q.sj = SJ;
+        words += table_words(q);
+        if (q.method == M_GDN_LMS) {
+            if ((u64)q.gtw * 4 > 65536) {
+                qmc_fail(QMC_ERR_ARG, "batch: gdn request %d needs %u bytes of digit tables (max 65536)", i,
+                         q.gtw * 4);
+                delete P;
+                return nullptr;
+            }
+            if (q.gtw * 4 > P->lds_bytes) P->lds_bytes = q.gtw * 4;
+        }
         q.slabs = words;
         words += q.r;
         P->reqs.push_back(q);
@@ -536,6 +762,37 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, P->device);
     P->grid = grid > 0 ? grid : 4 * cus;
     std::vector<u64> ws_host(words, 0);
+    // gdn dimension info (the digit tables themselves are built on the device)
+    for (auto &q : P->reqs) {
+        if (q.method != M_GDN_LMS) continue;
+        u32 qoff = 0;
+        for (int j = 0; j < q.d; j++) {
+            const u32 b = h_primes[j];
+            const u64 tn = (u64)gdn_tn(b), mm = (u64)gdn_mm(b, q.m), np = gdn_np(b);
+            const u64 W = gdn_np(gdn_group_first_odd(q, j));
+            // Horner split: Lw low pairs, (b^2)^Lw < 2^53 for the group's largest base
+            const double bmax = (double)gdn_group_max_base(q, j);
+            const u64 L = bmax * bmax * bmax * bmax * bmax * bmax < 0x1p53 ? 3 : 2;
+            const bool shape_ok = (W == 17 && L == 3) || (W == 4 && L == 3) || (W == 4 && L == 2) ||
+                                  (W == 3 && L == 2) || b == 2;
+            if (!shape_ok || (b != 2 && !((double)b * b * b * b < 0x1p53 &&
+                                          std::pow((double)b * b, (double)(np > L ? np - L : 0)) < 0x1p53))) {
+                qmc_fail(QMC_ERR_ARG, "batch: gdn base %u has no kernel shape (W=%u, L=%u)", b, (u32)W, (u32)L);
+                delete P;
+                return nullptr;
+            }
+            double scale = 1.0;
+            for (u64 t = 0; t < tn; t++) scale *= (double)b;
+            u64 sbits;
+            std::memcpy(&sbits, &scale, sizeof sbits);
+            u64 *ji = ws_host.data() + q.tab + (u64)kGJ * j;
+            ji[0] = b | (L << 16) | ((u64)qoff << 32);
+            ji[1] = tn | (mm << 8) | (np << 16) | (W << 24) | ((u64)b * b << 32);
+            ji[2] = recip_magic(b);
+            ji[3] = sbits;
+            qoff += gdn_words(q, j);
+        }
+    }
     // slab bases in the ring: consecutive, wrapping when a slab would run past the end
     u64 pos = 0;
     for (auto &q : P->reqs) {
@@ -632,7 +889,8 @@ QMC_EXPORT int qmc_batch_run_per_call_dev(void *h, double *ring, qmc_stream_t s)
     for (size_t i = 0; i < P->reqs.size(); i++) {
         const Req &q = P->reqs[i];
         const u64 tiles = (u64)q.r * q.tiles_per_l;
-        hipLaunchKernelGGL(batch_single_kernel, dim3((unsigned)tiles), dim3(kBlock), 0, s, P->d_reqs + i, P->d_ws,
+        hipLaunchKernelGGL(batch_single_kernel, dim3((unsigned)tiles), dim3(kBlock), P->lds_bytes, s, P->d_reqs + i,
+                           P->d_ws,
                            ring);
     }
     QMC_HIP_CHECK(hipGetLastError());

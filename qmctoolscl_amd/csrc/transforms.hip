@@ -12,27 +12,50 @@
 // reference, and the FFT/IFFT differ only through the twiddle table (device
 // cos/sin of the same double angle, within an ulp of the host libm).
 //
-// Passes: the stages whose stride is below the tile (8192 reals / 4096
-// complex values, 64 KiB of LDS) run in one LDS-resident pass over contiguous
-// tiles; the stages with larger strides run in register passes of up to four
-// stages, each lane holding 16 elements spaced by the smallest stride of the
-// group, so lanes read consecutive addresses.
+// Passes: the stages whose stride is below the tile (4096 values, 32 KiB of
+// LDS per component) run in one LDS-resident pass over contiguous tiles, in
+// register rounds of four stages; the stages with larger strides run in
+// register passes of up to four stages straight from HBM, each lane holding
+// 16 elements spaced by the smallest stride of the group, so lanes read
+// consecutive addresses.  Division by sqrt(2) is a product and two fma
+// corrections that round exactly as the division does.
 #include "qmc_common.hpp"
 
 namespace {
 
-constexpr double kSqrt2 = 1.4142135623730951;  // sqrt(2.0), correctly rounded
-constexpr double kPi = 3.141592653589793;      // acos(-1.0)
-constexpr int kTileLog2Real = 13;
+constexpr double kSqrt2 = 1.4142135623730951;      // sqrt(2.0), correctly rounded
+constexpr double kRcpSqrt2 = 0x1.6a09e667f3bccp-1;  // 1.0 / kSqrt2, correctly rounded
+constexpr double kPi = 3.141592653589793;           // acos(-1.0)
+constexpr int kTileLog2Real = 12;
 constexpr int kTileLog2Cplx = 12;
 constexpr int kGroupLog2 = 4;
 
 enum { KIND_FWHT = 0, KIND_FFT = 1, KIND_IFFT = 2 };
 
+// x / sqrt(2) rounded to nearest, as IEEE division rounds it, from a product
+// and two Markstein corrections with y = RN(1/s), s = RN(sqrt 2):
+//   q0 = RN(x y) is within 1.21 ulp of x/s, so r0 = x - q0 s is exact (fma);
+//   q1 = RN(q0 + r0 y) is within 0.5 + 2^-53 ulp, r1 = x - q1 s exact;
+//   q2 = RN(q1 + r1 y) = RN(x/s + r1 (y - 1/s)) with |r1 (y - 1/s)| below
+//   2^-54.5 ulp, while x/s is never closer than 2^-53.5 ulp to a midpoint m
+//   (x - m s is a nonzero multiple of 2^(e-105): s has an odd 53-bit
+//   significand, m an odd 54-bit one), so q2 = RN(x/s).
+// Zeros, infinities, NaN and |x| outside [2^-960, 2^1000] take the division.
+__device__ __forceinline__ double div_sqrt2(double x) {
+    const double ax = fabs(x);
+    if (__builtin_expect(!(ax >= 0x1p-960 && ax <= 0x1p1000), 0)) return x / kSqrt2;
+    const double q0 = x * kRcpSqrt2;
+    const double r0 = __fma_rn(-q0, kSqrt2, x);
+    const double q1 = __fma_rn(r0, kRcpSqrt2, q0);
+    const double r1 = __fma_rn(-q1, kSqrt2, x);
+    return __fma_rn(r1, kRcpSqrt2, q1);
+}
+
+// fwht_1d_radix2.c:44-47
 __device__ __forceinline__ void bfly_fwht(double &a, double &b) {
     const double x1 = a, x2 = b;
-    a = (x1 + x2) / kSqrt2;
-    b = (x1 - x2) / kSqrt2;
+    a = div_sqrt2(x1 + x2);
+    b = div_sqrt2(x1 - x2);
 }
 
 // fft_bro_1d_radix2.c:66-75
@@ -40,19 +63,19 @@ __device__ __forceinline__ void bfly_dit(double &ar, double &ai, double &br, dou
     const double yr = br * c - bi * s;
     const double yi = br * s + bi * c;
     const double xr1 = ar, xi1 = ai;
-    ar = (xr1 + yr) / kSqrt2;
-    ai = (xi1 + yi) / kSqrt2;
-    br = (xr1 - yr) / kSqrt2;
-    bi = (xi1 - yi) / kSqrt2;
+    ar = div_sqrt2(xr1 + yr);
+    ai = div_sqrt2(xi1 + yi);
+    br = div_sqrt2(xr1 - yr);
+    bi = div_sqrt2(xi1 - yi);
 }
 
 // ifft_bro_1d_radix2.c:66-75
 __device__ __forceinline__ void bfly_dif(double &ar, double &ai, double &br, double &bi, double c, double s) {
     const double yr = ar - br, yi = ai - bi;
-    ar = (ar + br) / kSqrt2;
-    ai = (ai + bi) / kSqrt2;
-    br = (yr * c - yi * s) / kSqrt2;
-    bi = (yr * s + yi * c) / kSqrt2;
+    ar = div_sqrt2(ar + br);
+    ai = div_sqrt2(ai + bi);
+    br = div_sqrt2(yr * c - yi * s);
+    bi = div_sqrt2(yr * s + yi * c);
 }
 
 // twiddles as fft_bro_1d_radix2.c:32-42: angle = (sign*pi*i)/n, cos and sin
@@ -67,81 +90,92 @@ __global__ __launch_bounds__(kBlock) void twiddle_kernel(u64 n, double sign, dou
     twi[i] = s;
 }
 
-// Stages with strides 2^s_first .. 2^s_last (stepping by dir = +1 or -1),
-// all below the tile, over contiguous tiles of 2^tlog2 values of the flat
-// array (tiles hold whole vectors or aligned parts of one vector).
-template <int KIND>
-__global__ __launch_bounds__(kBlock) void tile_kernel(u64 total, int m, int tlog2, int s_first, int s_last,
-                                                      int dir, double *__restrict__ xr, double *__restrict__ xi,
-                                                      const double *__restrict__ twr,
-                                                      const double *__restrict__ twi) {
+// The butterflies of CNT consecutive stages on 2^CNT values held in
+// registers: value c sits at index c << lo of a group whose other index bits
+// are fixed; stage bit order ascending (FFT) or descending (FWHT, IFFT).
+// i0 = the group's index within its vector; stage-major twiddles
+// ts[2^s + t] = tw[t << (m-1-s)].
+template <int KIND, bool ASC, int CNT>
+__device__ __forceinline__ void run_stages(double *re, double *im, u64 i0, int lo, const double *__restrict__ tsr,
+                                           const double *__restrict__ tsi) {
+#pragma unroll
+    for (int st = 0; st < CNT; st++) {
+        const int bit = ASC ? st : CNT - 1 - st;
+        const int ss = lo + bit;  // stride 2^ss
+#pragma unroll
+        for (int c1 = 0; c1 < (1 << CNT); c1++) {
+            if ((c1 >> bit) & 1) continue;
+            const int c2 = c1 | (1 << bit);
+            if (KIND == KIND_FWHT) {
+                bfly_fwht(re[c1], re[c2]);
+            } else {
+                const u64 i1 = i0 + ((u64)c1 << lo);
+                const u64 tw = (1ull << ss) + (i1 & ((1ull << ss) - 1));
+                const double cc = tsr[tw], sn = tsi[tw];
+                if (KIND == KIND_FFT)
+                    bfly_dit(re[c1], im[c1], re[c2], im[c2], cc, sn);
+                else
+                    bfly_dif(re[c1], im[c1], re[c2], im[c2], cc, sn);
+            }
+        }
+    }
+}
+
+// one register round of the LDS tile: stages on index bits lo .. lo+CNT-1
+template <int KIND, bool ASC, int CNT>
+__device__ __forceinline__ void tile_round(double *sr, double *si, u64 len, int lo, u64 vbase,
+                                           const double *__restrict__ tsr, const double *__restrict__ tsi) {
+    auto P = [](u64 t) { return t + (t >> 4); };  // one pad double per 16 values
+    const u64 groups = len >> CNT;
+    for (u64 g = threadIdx.x; g < groups; g += blockDim.x) {
+        const u64 gb = ((g >> lo) << (lo + CNT)) | (g & ((1ull << lo) - 1));
+        double re[1 << CNT], im[1 << CNT];
+#pragma unroll
+        for (int c = 0; c < (1 << CNT); c++) {
+            re[c] = sr[P(gb + ((u64)c << lo))];
+            if (KIND != KIND_FWHT) im[c] = si[P(gb + ((u64)c << lo))];
+        }
+        run_stages<KIND, ASC, CNT>(re, im, vbase + gb, lo, tsr, tsi);
+#pragma unroll
+        for (int c = 0; c < (1 << CNT); c++) {
+            sr[P(gb + ((u64)c << lo))] = re[c];
+            if (KIND != KIND_FWHT) si[P(gb + ((u64)c << lo))] = im[c];
+        }
+    }
+}
+
+// The nst stages with strides 2^0 .. 2^(nst-1) (all below the tile), over
+// contiguous tiles of 2^tlog2 values of the flat array (tiles hold whole
+// vectors or aligned parts of one vector), in rounds of up to four stages:
+// one LDS round trip and one barrier per round.
+template <int KIND, bool ASC>
+__global__ __launch_bounds__(kBlock) void tile_kernel(u64 total, int m, int tlog2, int nst, double *__restrict__ xr,
+                                                      double *__restrict__ xi, const double *__restrict__ tsr,
+                                                      const double *__restrict__ tsi) {
     extern __shared__ double lds[];
     const u64 tile = 1ull << tlog2;
     const u64 base = (u64)blockIdx.x << tlog2;
     const u64 len = (total - base) < tile ? (total - base) : tile;
     double *sr = lds;
     double *si = lds + tile + tile / 16;
-    const u64 nmask = (1ull << m) - 1;
-    // one pad double per 16 keeps 16-value groups of the lowest bits off a single bank
     auto P = [](u64 t) { return t + (t >> 4); };
     for (u64 t = threadIdx.x; t < len; t += blockDim.x) {
         sr[P(t)] = xr[base + t];
         if (KIND != KIND_FWHT) si[P(t)] = xi[base + t];
     }
     __syncthreads();
-    // Rounds of up to four consecutive stages: a lane loads the 16 values that
-    // differ in the round's index bits into registers, runs the stages there
-    // (same butterflies, same order) and stores them back: one LDS round trip
-    // and one barrier per four stages.
-    int s = s_first;
-    const int nstages = (s_last - s_first) * dir + 1;
-    for (int done = 0; done < nstages;) {
-        const int cnt = (nstages - done) < kGroupLog2 ? (nstages - done) : kGroupLog2;
-        const int lo = dir > 0 ? s : s - cnt + 1;  // lowest index bit of the round
-        const u64 groups = len >> cnt;
-        for (u64 g = threadIdx.x; g < groups; g += blockDim.x) {
-            const u64 gb = ((g >> lo) << (lo + cnt)) | (g & ((1ull << lo) - 1));
-            double re[1 << kGroupLog2], im[1 << kGroupLog2];
-#pragma unroll
-            for (int c = 0; c < (1 << kGroupLog2); c++) {
-                if (c < (1 << cnt)) {
-                    re[c] = sr[P(gb + ((u64)c << lo))];
-                    if (KIND != KIND_FWHT) im[c] = si[P(gb + ((u64)c << lo))];
-                }
-            }
-#pragma unroll
-            for (int st = 0; st < kGroupLog2; st++) {
-                if (st >= cnt) break;
-                const int bit = dir > 0 ? st : cnt - 1 - st;
-                const int ss = lo + bit;  // this stage's stride is 2^ss
-#pragma unroll
-                for (int c1 = 0; c1 < (1 << kGroupLog2); c1++) {
-                    if (c1 >= (1 << cnt) || ((c1 >> bit) & 1)) continue;
-                    const int c2 = c1 | (1 << bit);
-                    if (KIND == KIND_FWHT) {
-                        bfly_fwht(re[c1], re[c2]);
-                    } else {
-                        const u64 p = gb + ((u64)c1 << lo);
-                        const u64 tw = ((base + p) & nmask & ((1ull << ss) - 1)) << (m - 1 - ss);
-                        const double cc = twr[tw], sn = twi[tw];
-                        if (KIND == KIND_FFT)
-                            bfly_dit(re[c1], im[c1], re[c2], im[c2], cc, sn);
-                        else
-                            bfly_dif(re[c1], im[c1], re[c2], im[c2], cc, sn);
-                    }
-                }
-            }
-#pragma unroll
-            for (int c = 0; c < (1 << kGroupLog2); c++) {
-                if (c < (1 << cnt)) {
-                    sr[P(gb + ((u64)c << lo))] = re[c];
-                    if (KIND != KIND_FWHT) si[P(gb + ((u64)c << lo))] = im[c];
-                }
-            }
+    const u64 vbase = base & ((1ull << m) - 1);  // the tile's offset within its vector
+    for (int done = 0; done < nst;) {
+        const int cnt = (nst - done) < kGroupLog2 ? (nst - done) : kGroupLog2;
+        const int lo = ASC ? done : nst - done - cnt;
+        switch (cnt) {
+            case 4: tile_round<KIND, ASC, 4>(sr, si, len, lo, vbase, tsr, tsi); break;
+            case 3: tile_round<KIND, ASC, 3>(sr, si, len, lo, vbase, tsr, tsi); break;
+            case 2: tile_round<KIND, ASC, 2>(sr, si, len, lo, vbase, tsr, tsi); break;
+            default: tile_round<KIND, ASC, 1>(sr, si, len, lo, vbase, tsr, tsi); break;
         }
         __syncthreads();
         done += cnt;
-        s += cnt * dir;
     }
     for (u64 t = threadIdx.x; t < len; t += blockDim.x) {
         xr[base + t] = sr[P(t)];
@@ -149,14 +183,13 @@ __global__ __launch_bounds__(kBlock) void tile_kernel(u64 total, int m, int tlog
     }
 }
 
-// Stages on index bits a .. a+R-1 (R <= 4), applied in ascending bit order
-// when ascending != 0 (FFT) and descending otherwise (FWHT, IFFT).  One lane
-// owns the 2^R values that differ only in those bits.
-template <int KIND>
-__global__ __launch_bounds__(kBlock) void group_kernel(u64 groups, int m, int a, int R, int ascending,
-                                                       double *__restrict__ xr, double *__restrict__ xi,
-                                                       const double *__restrict__ twr,
-                                                       const double *__restrict__ twi) {
+// Stages on index bits a .. a+R-1, straight from HBM: one lane owns the 2^R
+// values that differ only in those bits (lanes with consecutive indices read
+// consecutive addresses).
+template <int KIND, bool ASC, int R>
+__global__ __launch_bounds__(kBlock) void group_kernel(u64 groups, int m, int a, double *__restrict__ xr,
+                                                       double *__restrict__ xi, const double *__restrict__ tsr,
+                                                       const double *__restrict__ tsi) {
     const u64 g = (u64)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= groups) return;
     const u64 L = g & ((1ull << a) - 1);
@@ -166,45 +199,17 @@ __global__ __launch_bounds__(kBlock) void group_kernel(u64 groups, int m, int a,
     const u64 v = rest >> hib;
     const u64 in_vec = (H << (a + R)) | L;  // index of value c = 0 within its vector
     const u64 base = (v << m) + in_vec;
-    const int cnt = 1 << R;
-    double re[1 << kGroupLog2], im[1 << kGroupLog2];
+    double re[1 << R], im[1 << R];
 #pragma unroll
-    for (int c = 0; c < (1 << kGroupLog2); c++) {
-        if (c < cnt) {
-            re[c] = xr[base + ((u64)c << a)];
-            if (KIND != KIND_FWHT) im[c] = xi[base + ((u64)c << a)];
-        }
+    for (int c = 0; c < (1 << R); c++) {
+        re[c] = xr[base + ((u64)c << a)];
+        if (KIND != KIND_FWHT) im[c] = xi[base + ((u64)c << a)];
     }
+    run_stages<KIND, ASC, R>(re, im, in_vec, a, tsr, tsi);
 #pragma unroll
-    for (int st = 0; st < kGroupLog2; st++) {
-        if (st >= R) break;
-        const int bit = ascending ? st : R - 1 - st;  // bit of c handled by this stage
-        const int s = a + bit;                         // stride 2^s
-#pragma unroll
-        for (int c1 = 0; c1 < (1 << kGroupLog2); c1++) {
-            if (c1 >= cnt || (c1 >> bit) & 1) continue;
-            const int c2 = c1 | (1 << bit);
-            if (KIND == KIND_FWHT) {
-                bfly_fwht(re[c1], re[c2]);
-            } else {
-                // stage-major table: entry 2^s + t holds twiddle t << (m-1-s), so
-                // lanes with consecutive indices read consecutive entries
-                const u64 i1 = in_vec + ((u64)c1 << a);
-                const u64 tw = (1ull << s) + (i1 & ((1ull << s) - 1));
-                const double cc = twr[tw], sn = twi[tw];
-                if (KIND == KIND_FFT)
-                    bfly_dit(re[c1], im[c1], re[c2], im[c2], cc, sn);
-                else
-                    bfly_dif(re[c1], im[c1], re[c2], im[c2], cc, sn);
-            }
-        }
-    }
-#pragma unroll
-    for (int c = 0; c < (1 << kGroupLog2); c++) {
-        if (c < cnt) {
-            xr[base + ((u64)c << a)] = re[c];
-            if (KIND != KIND_FWHT) xi[base + ((u64)c << a)] = im[c];
-        }
+    for (int c = 0; c < (1 << R); c++) {
+        xr[base + ((u64)c << a)] = re[c];
+        if (KIND != KIND_FWHT) xi[base + ((u64)c << a)] = im[c];
     }
 }
 
@@ -226,9 +231,9 @@ int log2_exact(u64 n) {
     return ((1ull << m) == n) ? m : -1;
 }
 
-template <int KIND>
-int launch_tile(u64 total, int m, int tlog2, int s_first, int s_last, int dir, double *xr, double *xi,
-                const double *twr, const double *twi, hipStream_t s) {
+template <int KIND, bool ASC>
+int launch_tile(u64 total, int m, int tlog2, double *xr, double *xi, const double *tsr, const double *tsi,
+                hipStream_t s) {
     const u64 tiles = (total + (1ull << tlog2) - 1) >> tlog2;
     const size_t words = ((size_t)1 << tlog2) + ((size_t)1 << tlog2) / 16;  // padded
     const size_t shm = (size_t)(KIND == KIND_FWHT ? 1 : 2) * sizeof(double) * words;
@@ -236,22 +241,27 @@ int launch_tile(u64 total, int m, int tlog2, int s_first, int s_last, int dir, d
     if (!attr_set) {  // padded 64 KiB tiles need a little more than the default LDS limit
         const size_t most = (size_t)(KIND == KIND_FWHT ? 1 : 2) * sizeof(double) *
                             (((size_t)1 << (KIND == KIND_FWHT ? kTileLog2Real : kTileLog2Cplx)) * 17 / 16);
-        QMC_HIP_CHECK(hipFuncSetAttribute((const void *)tile_kernel<KIND>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                          (int)most));
+        QMC_HIP_CHECK(hipFuncSetAttribute((const void *)tile_kernel<KIND, ASC>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)most));
         attr_set = true;
     }
-    hipLaunchKernelGGL(tile_kernel<KIND>, dim3((unsigned)tiles), dim3(kBlock), shm, s, total, m, tlog2, s_first,
-                       s_last, dir, xr, xi, twr, twi);
+    hipLaunchKernelGGL((tile_kernel<KIND, ASC>), dim3((unsigned)tiles), dim3(kBlock), shm, s, total, m, tlog2, tlog2,
+                       xr, xi, tsr, tsi);
     QMC_HIP_CHECK(hipGetLastError());
     return QMC_OK;
 }
 
-template <int KIND>
-int launch_group(u64 total, int m, int a, int R, int ascending, double *xr, double *xi, const double *twr,
-                 const double *twi, hipStream_t s) {
+template <int KIND, bool ASC>
+int launch_group(u64 total, int m, int a, int R, double *xr, double *xi, const double *tsr, const double *tsi,
+                 hipStream_t s) {
     const u64 groups = total >> R;
-    hipLaunchKernelGGL(group_kernel<KIND>, dim3((unsigned)((groups + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
-                       groups, m, a, R, ascending, xr, xi, twr, twi);
+    const dim3 grid((unsigned)((groups + kBlock - 1) / kBlock));
+    switch (R) {
+        case 4: hipLaunchKernelGGL((group_kernel<KIND, ASC, 4>), grid, dim3(kBlock), 0, s, groups, m, a, xr, xi, tsr, tsi); break;
+        case 3: hipLaunchKernelGGL((group_kernel<KIND, ASC, 3>), grid, dim3(kBlock), 0, s, groups, m, a, xr, xi, tsr, tsi); break;
+        case 2: hipLaunchKernelGGL((group_kernel<KIND, ASC, 2>), grid, dim3(kBlock), 0, s, groups, m, a, xr, xi, tsr, tsi); break;
+        default: hipLaunchKernelGGL((group_kernel<KIND, ASC, 1>), grid, dim3(kBlock), 0, s, groups, m, a, xr, xi, tsr, tsi); break;
+    }
     QMC_HIP_CHECK(hipGetLastError());
     return QMC_OK;
 }
@@ -267,37 +277,35 @@ int run_transform(const char *who, u64 d1, u64 d2, u64 n_half, double *twr, doub
     if (m > 31) return qmc_fail(QMC_ERR_ARG, "%s: length 2^%d exceeds the reference's int stride (1<<s)", who, m);
     if (V > (1ull << 40) / n) return qmc_fail(QMC_ERR_ARG, "%s: d1*d2*n too large", who);
     const u64 total = V * n;
+    double *tsr = nullptr, *tsi = nullptr;
     if (KIND != KIND_FWHT) {
         hipLaunchKernelGGL(twiddle_kernel, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, n,
                            KIND == KIND_FFT ? -2.0 : 2.0, twr, twi);
         QMC_HIP_CHECK(hipGetLastError());
-    }
-    const int tcap = KIND == KIND_FWHT ? kTileLog2Real : kTileLog2Cplx;
-    const int tl = m < tcap ? m : tcap;  // LDS stages: strides 2^0 .. 2^(tl-1)
-    double *tsr = nullptr, *tsi = nullptr;
-    if (KIND != KIND_FWHT && m > tl) {
         QMC_HIP_CHECK(hipMallocAsync((void **)&tsr, 2 * sizeof(double) * n, s));
         tsi = tsr + n;
         hipLaunchKernelGGL(stage_twiddles_kernel, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, m,
                            twr, twi, tsr, tsi);
     }
+    const int tcap = KIND == KIND_FWHT ? kTileLog2Real : kTileLog2Cplx;
+    const int tl = m < tcap ? m : tcap;  // LDS stages: strides 2^0 .. 2^(tl-1)
     int rc = QMC_OK;
     if (KIND == KIND_FFT) {
         // strides ascend: LDS stages first, then register groups
-        rc = launch_tile<KIND>(total, m, tl, 0, tl - 1, +1, xr, xi, twr, twi, s);
+        rc = launch_tile<KIND, true>(total, m, tl, xr, xi, tsr, tsi, s);
         for (int a = tl; a < m && !rc; a += kGroupLog2) {
             const int R = (m - a) < kGroupLog2 ? (m - a) : kGroupLog2;
-            rc = launch_group<KIND>(total, m, a, R, 1, xr, xi, tsr, tsi, s);
+            rc = launch_group<KIND, true>(total, m, a, R, xr, xi, tsr, tsi, s);
         }
     } else {
         // strides descend: register groups from the top, then LDS stages
         int top = m;
         while (top > tl && !rc) {
             const int R = (top - tl) < kGroupLog2 ? (top - tl) : kGroupLog2;
-            rc = launch_group<KIND>(total, m, top - R, R, 0, xr, xi, tsr, tsi, s);
+            rc = launch_group<KIND, false>(total, m, top - R, R, xr, xi, tsr, tsi, s);
             top -= R;
         }
-        if (!rc) rc = launch_tile<KIND>(total, m, tl, tl - 1, 0, -1, xr, xi, twr, twi, s);
+        if (!rc) rc = launch_tile<KIND, false>(total, m, tl, xr, xi, tsr, tsi, s);
     }
     if (tsr) (void)hipFreeAsync(tsr, s);
     return rc;

@@ -160,3 +160,113 @@ def test_autotuner_picks_a_measured_candidate(torch, tmp_path, monkeypatch):
     assert best["ms"] == min(t["ms"] for t in best["table"])
     again = B.autotune(meth, m, d, r, seeds, ring, tiles=(1 << 10, 1 << 12), grids_per_cu=(2, 4), reps=1)
     assert again["points_per_tile"] == best["points_per_tile"]  # served from the cache
+
+
+# Requests around the kernels' shape edges: one column, columns around a wave
+# (32, 33, 64, 65), several column groups (200, 256), bases up to the 256th
+# prime, long chains with deep carries (m up to 16), ragged tiles.
+WIDE = [(3, 12, 200, 2), (3, 14, 37, 1), (3, 9, 7, 3), (3, 16, 1, 1), (3, 11, 64, 1), (3, 10, 65, 2),
+        (3, 13, 256, 1), (3, 12, 33, 1), (3, 8, 32, 2),
+        (1, 12, 200, 2), (1, 14, 37, 1), (1, 9, 7, 3), (1, 16, 1, 1), (1, 11, 64, 2), (1, 10, 256, 1),
+        (1, 13, 65, 1), (2, 13, 33, 1), (2, 10, 256, 1), (2, 15, 3, 2)]
+
+
+@pytest.fixture(scope="module", params=[1 << 11, 1 << 16])
+def wide(torch, request):
+    meth, m, d, r = (np.array(c, dtype=np.int32) for c in zip(*WIDE))
+    seeds = np.arange(101, 101 + len(WIDE), dtype=np.uint64)
+    plan = B.Batch(meth, m, d, r, seeds, points_per_tile=request.param)
+    ring = torch.full((plan.ring_elems,), float("nan"), dtype=torch.float64, device="cuda")
+    plan.run(ring)
+    torch.cuda.synchronize()
+    got = ring.cpu().numpy()
+    ring.fill_(float("nan"))
+    plan.run_per_call(ring)
+    torch.cuda.synchronize()
+    assert np.array_equal(ring.cpu().numpy().view(np.uint64), got.view(np.uint64))
+    return plan, got
+
+
+def _brev64(x):
+    x = x.astype(np.uint64)
+    for sh, mk in ((1, 0x5555555555555555), (2, 0x3333333333333333), (4, 0x0f0f0f0f0f0f0f0f),
+                   (8, 0x00ff00ff00ff00ff), (16, 0x0000ffff0000ffff)):
+        s, k = np.uint64(sh), np.uint64(mk)
+        x = ((x >> s) & k) | ((x & k) << s)
+    return (x >> np.uint64(32)) | (x << np.uint64(32))
+
+
+def owen_hash64_np(x, seed):
+    """numpy form of the library's owen_hash64 (checked against it below)"""
+    x = _brev64(x) + np.uint64(seed)
+    for c in (0x6c50b47cc8f83c2e, 0xb82f1e52a6d41b3c, 0xc7afe638e2d7f0a6, 0x8d22f6e6b1f5a3d4):
+        x ^= x * np.uint64(c)
+    return _brev64(x)
+
+
+def gdn_definition(seed, m, d, r):
+    """The gdn request on its exported inputs, every column: digits of i, LMS,
+    digit permutation, Y = sum of digits as an integer, float(Y) / b^tn."""
+    n = 1 << m
+    out = np.empty((r, n, d))
+    idx = np.arange(n, dtype=np.int64)
+    for l in range(r):
+        for j in range(d):
+            b, tn, mm, S, pa, pc = B.gdn_inputs(seed, m, l, j)
+            dig = np.empty((n, mm), dtype=np.int64)
+            k = idx.copy()
+            for i in range(mm):
+                dig[:, i] = k % b
+                k //= b
+            x = (dig @ S.astype(np.int64).T) % b
+            y = (pa.astype(np.int64) * x + pc.astype(np.int64)) % b
+            Y = np.zeros(n, dtype=np.uint64)
+            scale = 1.0
+            for t in range(tn):
+                Y = Y * np.uint64(b) + y[:, t].astype(np.uint64)
+                scale *= float(b)
+            out[l, :, j] = Y.astype(np.float64) / scale
+    return out
+
+
+def test_wide_gdn_requests_match_definition(wide):
+    plan, got = wide
+    for q in np.flatnonzero(plan.methods == 3):
+        x_gpu, n, d, r = _req(plan, got, q)
+        want = gdn_definition(int(plan.seeds[q]), int(plan.m[q]), d, r)
+        bad = np.argwhere(x_gpu.view(np.uint64) != want.view(np.uint64))
+        assert bad.size == 0, (q, int(plan.m[q]), d, r, bad[:5].tolist())
+
+
+def test_wide_dnb2_requests_match_reference_chain(wide):
+    plan, got = wide
+    for q in np.flatnonzero(plan.methods == 1):
+        x_gpu, n, d, r = _req(plan, got, q)
+        C, S, sh = B.dnb2_inputs(plan.seeds[q], 1, d, r)
+        C_lms = np.empty((r, d, 32), dtype=np.uint64)
+        _clib.call_oracle("dnb2_linear_matrix_scramble", dict(r=r, d=d, mmax=32, r_C=1, tmax_new=32, S=S, C=C,
+                                                              C_lms=C_lms))
+        xb = np.empty((r, n, d), dtype=np.uint64)
+        _clib.call_oracle("dnb2_gen_natural", dict(r=r, n=n, d=d, n_start=0, mmax=32, C=C_lms, xb=xb))
+        xrb = np.empty_like(xb)
+        _clib.call_oracle("dnb2_digital_shift", dict(r=r, n=n, d=d, r_x=r, lshifts=np.full(r, 32, np.uint64),
+                                                     xb=xb, shiftsb=sh, xrb=xrb))
+        x = np.empty((r, n, d))
+        _clib.call_oracle("dnb2_integer_to_float", dict(r=r, n=n, d=d, tmaxes=np.full(r, 64, np.uint64), xb=xrb, x=x))
+        assert np.array_equal(x_gpu.view(np.uint64), x.view(np.uint64)), q
+    for q in np.flatnonzero(plan.methods == 2):
+        x_gpu, n, d, r = _req(plan, got, q)
+        C, _, seeds = B.dnb2_inputs(plan.seeds[q], 2, d, r)
+        xb = np.empty((1, n, d), dtype=np.uint64)
+        _clib.call_oracle("dnb2_gen_natural", dict(r=1, n=n, d=d, n_start=0, mmax=32, C=C, xb=xb))
+        for l in range(r):
+            ys = np.stack([owen_hash64_np(xb[0, :, j] << np.uint64(32), seeds[l, j]) for j in range(d)], axis=1)
+            want = ys.astype(np.float64) * 2.0 ** -64
+            assert np.array_equal(x_gpu[l].view(np.uint64), want.view(np.uint64)), (q, l)
+
+
+def test_owen_hash64_np_matches_library():
+    rng = np.random.default_rng(5)
+    xs = rng.integers(0, 2**63, size=50, dtype=np.uint64) * np.uint64(2)
+    for x, s in zip(xs, rng.integers(0, 2**63, size=50, dtype=np.uint64)):
+        assert int(owen_hash64_np(np.array([x]), s)[0]) == B.owen_hash64(int(x), int(s))
