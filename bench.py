@@ -219,6 +219,96 @@ def section_batch(torch, dev, timed, quick):
             "speedup_vs_per_call": round(ms_p / ms_b, 3)}
 
 
+def section_batch_vs_chain(torch, q, dev, timed):
+    """A bounded ragged sample (lattice, dnb2 LMS+shift, gdn LMS+permutation
+    requests) three ways: the persistent batch, one fused launch per request,
+    and the reference's call sequence per request through the drop-in kernels
+    (lat_gen_natural -> lat_shift_mod_1; dnb2_linear_matrix_scramble ->
+    dnb2_gen_natural -> dnb2_digital_shift -> dnb2_integer_to_float;
+    gdn_gen_natural -> gdn_digital_permutation -> gdn_integer_to_float on
+    tmax = 52 digit arrays), all on device tensors."""
+    from qmctoolscl_amd import batch as B
+    U = np.uint64
+    meth, m, d, r, seeds = B.sample_requests(48, 7, m_range=(8, 14), d_range=(1, 64), r_range=(1, 8),
+                                             methods=(0, 1, 3))
+    plan = B.Batch(meth, m, d, r, seeds, points_per_tile=1 << 15)
+    ring = torch.empty(plan.ring_elems, dtype=torch.float64, device=dev)
+    pts = plan.points
+    ms_b, _ = timed(lambda: plan.run(ring, graph=True), 5, 2)
+    ms_f, _ = timed(lambda: plan.run_per_call(ring), 5, 2)
+    del ring
+    primes = []
+    c = 2
+    while len(primes) < 64:
+        if all(c % p for p in primes):
+            primes.append(c)
+        c += 1
+    # inputs per request, device-resident before timing; scratch shared across requests
+    n_max = int((1 << m.astype(np.int64)).max())
+    big = int(max((1 << int(mm)) * int(dd) * int(rr) for mm, dd, rr in zip(m, d, r)))
+    xdig = torch.empty(big * 52, dtype=torch.int64, device=dev)
+    xdig2 = torch.empty(big * 52, dtype=torch.int64, device=dev)
+    xb = torch.empty(big, dtype=torch.int64, device=dev)
+    xrb = torch.empty(big, dtype=torch.int64, device=dev)
+    out = torch.empty(big, dtype=torch.float64, device=dev)
+    x1 = torch.empty(n_max * 64, dtype=torch.float64, device=dev)
+    rng = np.random.default_rng(3)
+    calls = []
+    for i in range(len(m)):
+        n, dd, rr, mm = 1 << int(m[i]), int(d[i]), int(r[i]), int(m[i])
+        if meth[i] == 0:
+            g, sh = B.lattice_inputs(seeds[i], mm, dd, rr)
+            g_t, sh_t = t64(torch, g.reshape(1, dd), dev), t64(torch, sh, dev)
+            calls.append(lambda n=n, dd=dd, rr=rr, g_t=g_t, sh_t=sh_t: (
+                q.lat_gen_natural(U(1), U(n), U(dd), U(0), g_t, x1, wait=False),
+                q.lat_shift_mod_1(U(rr), U(n), U(dd), U(1), x1, sh_t, out, wait=False)))
+        elif meth[i] == 1:
+            C, S, sh = B.dnb2_inputs(seeds[i], 1, dd, rr)
+            C_t, S_t, sh_t = t64(torch, C, dev), t64(torch, S, dev), t64(torch, sh, dev)
+            Cl = torch.empty((rr, dd, 32), dtype=torch.int64, device=dev)
+            ls = t64(torch, np.full(rr, 32, np.uint64), dev)
+            tm = t64(torch, np.full(rr, 64, np.uint64), dev)
+            calls.append(lambda n=n, dd=dd, rr=rr, C_t=C_t, S_t=S_t, sh_t=sh_t, Cl=Cl, ls=ls, tm=tm: (
+                q.dnb2_linear_matrix_scramble(U(rr), U(dd), U(32), U(1), U(32), S_t, C_t, Cl, wait=False),
+                q.dnb2_gen_natural(U(rr), U(n), U(dd), U(0), U(32), Cl, xb, wait=False),
+                q.dnb2_digital_shift(U(rr), U(n), U(dd), U(rr), ls, xb, sh_t, xrb, wait=False),
+                q.dnb2_integer_to_float(U(rr), U(n), U(dd), tm, xrb, out, wait=False)))
+        else:
+            bases = np.array([primes[:dd]], dtype=np.uint64)
+            bmax = int(bases.max())
+            mmax = mm  # base 2 needs m digits of the index
+            Cg = np.zeros((rr, dd, mmax, 52), dtype=np.uint64)
+            perms = np.zeros((rr, dd, 52, bmax), dtype=np.uint64)
+            for j in range(dd):
+                b = int(bases[0, j])
+                Cg[:, j] = rng.integers(0, b, size=(rr, mmax, 52), dtype=np.uint64)
+                perms[:, j, :, :b] = rng.integers(0, b, size=(rr, 52, b), dtype=np.uint64)
+            b_t, C_t, p_t = t64(torch, bases, dev), t64(torch, Cg, dev), t64(torch, perms, dev)
+            calls.append(lambda n=n, dd=dd, rr=rr, mmax=mmax, bmax=bmax, b_t=b_t, C_t=C_t, p_t=p_t: (
+                q.gdn_gen_natural(U(rr), U(n), U(dd), U(1), U(mmax), U(52), U(0), b_t, C_t, xdig, wait=False),
+                q.gdn_digital_permutation(U(rr), U(n), U(dd), U(rr), U(1), U(52), U(52), U(bmax), p_t, xdig,
+                                          xdig2, wait=False),
+                q.gdn_integer_to_float(U(rr), U(n), U(dd), U(1), U(52), b_t, xdig2, out, wait=False)))
+
+    def chain():
+        for f in calls:
+            f()
+
+    ms_c, _ = timed(chain, 3, 1)
+    by = {}
+    for k, name in enumerate(("lattice", "dnb2_lms_shift", "gdn_lms_perm")):
+        sel = meth == (0, 1, 3)[k]
+        by[name] = int(np.sum((np.int64(1) << m[sel].astype(np.int64)) * d[sel] * r[sel]))
+    return {"workload": "48 ragged requests (lattice, dnb2 LMS+shift, gdn LMS+permutation), m in [8,14], "
+                        "d in [1,64], r in [1,8]", "points": pts, "points_by_method": by,
+            "batched_ms": round(ms_b, 4), "batched_Gpts_s": round(pts / ms_b / 1e6, 2),
+            "fused_per_request_ms": round(ms_f, 4), "fused_per_request_Gpts_s": round(pts / ms_f / 1e6, 2),
+            "dropin_chain_per_request_ms": round(ms_c, 4),
+            "dropin_chain_per_request_Gpts_s": round(pts / ms_c / 1e6, 2),
+            "speedup_vs_fused_per_request": round(ms_f / ms_b, 3),
+            "speedup_vs_dropin_chain": round(ms_c / ms_b, 3)}
+
+
 def section_transforms(torch, q, dev, timed):
     from qmctoolscl_amd.mfma import fft_bro_mfma, fwht_mfma, relative_errors
     out = []
@@ -372,6 +462,9 @@ def main():
         torch.cuda.empty_cache()
         log("section: batch")
         out["sections"]["batch"] = section_batch(torch, dev, timed, args.quick_batch)
+        torch.cuda.empty_cache()
+        log("section: batch vs per-request drop-in chains")
+        out["sections"]["batch_vs_chain"] = section_batch_vs_chain(torch, q, dev, timed)
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(g, shifts, args.cpu_rows)

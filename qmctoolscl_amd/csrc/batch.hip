@@ -39,6 +39,7 @@ namespace {
 enum { M_LATTICE = 0, M_DNB2_LMS = 1, M_DNB2_NUS = 2, M_GDN_LMS = 3 };
 
 constexpr int kTB = 32;     // base-2 digits (tmax) of the dnb2 methods
+constexpr int kMaxChainLog = 12;  // dnb2 chains start in the tile's first 2^12 rows at most
 constexpr int kTNmax = 52;  // base-b digits kept (2^-52 resolution)
 
 struct Req {
@@ -360,7 +361,7 @@ __device__ __forceinline__ double dnb2_emit(u32 v, u64 xs, double lo) {
 // groups of 8 whose first 7 transitions use c = 0,1,0,2,0,1,0.
 template <bool LMS>
 __device__ __forceinline__ void dnb2_tile(const Req &q, u64 l, u64 row0, const u64 *__restrict__ tab,
-                                          double *__restrict__ slab) {
+                                          double *__restrict__ slab, u32 *v0) {
     const u32 d = (u32)q.d;
     const u32 s = q.clog;
     const u32 R = 1u << s, H = q.rows_per_tile >> s;
@@ -370,6 +371,9 @@ __device__ __forceinline__ void dnb2_tile(const Req &q, u64 l, u64 row0, const u
     const u64 *extra = tab + 32 * nset + l * d;  // shifts or NUS seeds
     const u64 stride = (u64)R * d;
     const u64 g0 = row0 >> s;
+    // value of the tile's first row per column, shared by the column's chains
+    for (u32 j = threadIdx.x; j < d; j += blockDim.x) v0[j] = xor_cols(T32 + (set + j) * 64 + 32, row0);
+    __syncthreads();
     for (u32 sig = threadIdx.x; sig < R * d; sig += blockDim.x) {
         const u32 rr0 = div_q(sig, q.ddiv);
         const u32 j = sig - rr0 * d;
@@ -378,11 +382,17 @@ __device__ __forceinline__ void dnb2_tile(const Req &q, u64 l, u64 row0, const u
         const u64 xs = extra[j];
         const double lo = (double)(u32)xs * 0x1p-64;
         const u32 base = s ? pre[s - 1] : 0u;
-        u32 v = xor_cols(cols, row0 + rr0);
+        // row0 + rr0 with rr0 < 2^s <= the tile: the low bits add independently
+        u32 v = v0[j];
+#pragma unroll
+        for (int bit = 0; bit < kMaxChainLog; bit++)
+            if (bit < (int)s && ((rr0 >> bit) & 1)) v ^= cols[bit];
         double *dst = slab + (row0 + rr0) * d + j;
         if (H >= 8) {
             const u32 x0 = pre[s] ^ base, x1 = pre[s + 1] ^ base, x2 = pre[s + 2] ^ base;
             for (u32 h = 0; h < H; h += 8) {
+                // the transition after this group, loaded before the group's stores
+                const u32 next = h + 8 < H ? pre[s + 3 + __ffsll((long long)((g0 + h + 8) >> 3)) - 1] : base;
                 store_stream(dst, dnb2_emit<LMS>(v, xs, lo));
                 v ^= x0;
                 store_stream(dst + stride, dnb2_emit<LMS>(v, xs, lo));
@@ -399,7 +409,7 @@ __device__ __forceinline__ void dnb2_tile(const Req &q, u64 l, u64 row0, const u
                 v ^= x0;
                 store_stream(dst + 7 * stride, dnb2_emit<LMS>(v, xs, lo));
                 dst += 8 * stride;
-                if (h + 8 < H) v ^= pre[s + 3 + __ffsll((long long)((g0 + h + 8) >> 3)) - 1] ^ base;
+                v ^= next ^ base;
             }
         } else {
             for (u32 h = 0;; h++) {
@@ -550,9 +560,9 @@ __device__ void run_tile(const Req &q, u64 t, const u64 *__restrict__ ws, double
             store_stream(slab + row * d + j, frac_mod1(x + sh[j]));  // lat_shift_mod_1.c:31
         }
     } else if (q.method == M_DNB2_LMS) {
-        dnb2_tile<true>(q, l, row0, tab, slab);
+        dnb2_tile<true>(q, l, row0, tab, slab, lds);
     } else if (q.method == M_DNB2_NUS) {
-        dnb2_tile<false>(q, l, row0, tab, slab);
+        dnb2_tile<false>(q, l, row0, tab, slab, lds);
     } else {
         // this replication's digit tables into LDS
         const u32 *src = (const u32 *)(tab + q.gtab) + l * q.gtw;
@@ -643,7 +653,7 @@ __global__ __launch_bounds__(kBlock) void batch_single_kernel(const Req *__restr
 u32 dnb2_chain_log(u32 RT, u32 d) {
     u32 best = 0;
     u64 best_cost = ~0ull;
-    for (u32 s = 0; (1u << s) <= RT; s++) {
+    for (u32 s = 0; (1u << s) <= RT && s < kMaxChainLog; s++) {
         const u64 U = (((u64)d << s) + kBlock - 1) / kBlock;
         const u64 cost = U * ((RT >> s) + 3);
         if (cost < best_cost) {
@@ -757,6 +767,7 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
     }
     P->total_tiles = tile;
     P->ws_words = words;
+    if (P->lds_bytes < 256 * sizeof(u32)) P->lds_bytes = 256 * sizeof(u32);  // dnb2 first-row values
     if (hipGetDevice(&P->device) != hipSuccess) { delete P; qmc_fail(QMC_ERR_HIP, "batch: no device"); return nullptr; }
     int cus = 256;
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, P->device);

@@ -26,9 +26,11 @@ namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
 
-constexpr int kT = 13;  // 2^13 complex fp32 (68 KiB of LDS with padding)
-constexpr int kWaves = kBlock / 64;
+constexpr int kT = 13;          // 2^13 complex fp32 (68 KiB of LDS with padding)
+constexpr int kThreads = 1024;  // 16 waves: two workgroups fill a CU, each with many loads in flight
 
 // LDS word of tile index t: one pad word per 16 keeps the 16-point columns of
 // the lowest level (addresses 16 apart) on distinct banks
@@ -107,7 +109,7 @@ __device__ void dft16_level(float *re, float *im, const PassGeom &G, u64 cb, int
     const int c0 = (g & 1) * 8;
     float *src = g < 2 ? re : im;
     const int hi_bits = G.m - ga - 4;  // index bits above this level
-    for (int blk = wave; blk < nblk; blk += kWaves) {
+    for (int blk = wave; blk < nblk; blk += (int)(blockDim.x >> 6)) {
         const int col = blk * 16 + n;
         const int lo = col & ((1 << ta) - 1);
         const int base = ((col >> ta) << (ta + 4)) | lo;
@@ -185,7 +187,7 @@ __device__ void dft_small_level(float *re, float *im, int T, int ta, int rb) {
 
 // levels on global bits [glo, ghi) of tiles; first/last pass move float64
 template <int IN64, int OUT64>
-__global__ __launch_bounds__(kBlock) void fft_mfma_pass(PassGeom G, int glo, int ghi, const void *__restrict__ sr,
+__global__ __launch_bounds__(kThreads) void fft_mfma_pass(PassGeom G, int glo, int ghi, const void *__restrict__ sr,
                                                         const void *__restrict__ si, void *__restrict__ dr,
                                                         void *__restrict__ di, double scale) {
     extern __shared__ float lds[];
@@ -196,14 +198,24 @@ __global__ __launch_bounds__(kBlock) void fft_mfma_pass(PassGeom G, int glo, int
     const u64 v = blockIdx.x / cblocks;
     const u64 cb = blockIdx.x - v * cblocks;
     const u64 vbase = v << G.m;
-    for (int t = threadIdx.x; t < (1 << T); t += blockDim.x) {
+    // two consecutive values per access (G.lc >= 1: gpos(t + 1) = gpos(t) + 1 for even t)
+#pragma unroll 4
+    for (int t = 2 * threadIdx.x; t < (1 << T); t += 2 * blockDim.x) {
         const u64 gi = vbase + gpos(G, cb, t);
         if (IN64) {
-            re[pad(t)] = (float)((const double *)sr)[gi];
-            im[pad(t)] = (float)((const double *)si)[gi];
+            const f64x2 a = *(const f64x2 *)((const double *)sr + gi);
+            const f64x2 b = *(const f64x2 *)((const double *)si + gi);
+            re[pad(t)] = (float)a.x;
+            re[pad(t + 1)] = (float)a.y;
+            im[pad(t)] = (float)b.x;
+            im[pad(t + 1)] = (float)b.y;
         } else {
-            re[pad(t)] = ((const float *)sr)[gi];
-            im[pad(t)] = ((const float *)si)[gi];
+            const f32x2 a = *(const f32x2 *)((const float *)sr + gi);
+            const f32x2 b = *(const f32x2 *)((const float *)si + gi);
+            re[pad(t)] = a.x;
+            re[pad(t + 1)] = a.y;
+            im[pad(t)] = b.x;
+            im[pad(t + 1)] = b.y;
         }
     }
     __syncthreads();
@@ -219,14 +231,25 @@ __global__ __launch_bounds__(kBlock) void fft_mfma_pass(PassGeom G, int glo, int
         }
         __syncthreads();
     }
-    for (int t = threadIdx.x; t < (1 << T); t += blockDim.x) {
+#pragma unroll 4
+    for (int t = 2 * threadIdx.x; t < (1 << T); t += 2 * blockDim.x) {
         const u64 gi = vbase + gpos(G, cb, t);
         if (OUT64) {
-            store_stream(((double *)dr) + gi, (double)re[pad(t)] * scale);
-            store_stream(((double *)di) + gi, (double)im[pad(t)] * scale);
+            f64x2 a, b;
+            a.x = (double)re[pad(t)] * scale;
+            a.y = (double)re[pad(t + 1)] * scale;
+            b.x = (double)im[pad(t)] * scale;
+            b.y = (double)im[pad(t + 1)] * scale;
+            __builtin_nontemporal_store(a, (f64x2 *)((double *)dr + gi));
+            __builtin_nontemporal_store(b, (f64x2 *)((double *)di + gi));
         } else {
-            ((float *)dr)[gi] = re[pad(t)];
-            ((float *)di)[gi] = im[pad(t)];
+            f32x2 a, b;
+            a.x = re[pad(t)];
+            a.y = re[pad(t + 1)];
+            b.x = im[pad(t)];
+            b.y = im[pad(t + 1)];
+            *(f32x2 *)((float *)dr + gi) = a;
+            *(f32x2 *)((float *)di + gi) = b;
         }
     }
 }
@@ -248,6 +271,8 @@ QMC_EXPORT int qmc_fft_bro_mfma_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, doub
     while ((1ull << m) < n) m++;
     if ((1ull << m) != n) return qmc_fail(QMC_ERR_ARG, "fft_mfma: length is not a power of two");
     if (m < 8 || m > 28) return qmc_fail(QMC_ERR_ARG, "fft_mfma: length 2^%d outside 2^8 .. 2^28", m);
+    if (((uintptr_t)xr | (uintptr_t)xi | (uintptr_t)work) % 16)
+        return qmc_fail(QMC_ERR_ARG, "fft_mfma: xr, xi and work must be 16-byte aligned");
     const double scale = (m % 2 == 0) ? ldexp(1.0, -m / 2) : ldexp(1.0, -(m - 1) / 2) / 1.4142135623730951;
     static bool attr_set = false;
     if (!attr_set) {  // padded tiles need more than the default 64 KiB of LDS
@@ -261,7 +286,8 @@ QMC_EXPORT int qmc_fft_bro_mfma_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, doub
     const size_t shm = padded_bytes(kT);
     if (m <= kT) {
         PassGeom G{m, 0, m, m, m};
-        hipLaunchKernelGGL((fft_mfma_pass<1, 1>), dim3((unsigned)V), dim3(kBlock), padded_bytes(m), s, G, 0, m,
+        // one vector per workgroup: a quarter of the threads keeps four workgroups per CU
+        hipLaunchKernelGGL((fft_mfma_pass<1, 1>), dim3((unsigned)V), dim3(kThreads / 4), padded_bytes(m), s, G, 0, m,
                            (const void *)xr, (const void *)xi, (void *)xr, (void *)xi, scale);
         QMC_HIP_CHECK(hipGetLastError());
         return QMC_OK;
@@ -272,7 +298,7 @@ QMC_EXPORT int qmc_fft_bro_mfma_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, doub
     // pass 1: global bits [0, 12) over contiguous tiles of 2^13
     {
         PassGeom G{m, 0, kT, kT, kT};
-        hipLaunchKernelGGL((fft_mfma_pass<1, 0>), dim3((unsigned)(V << (m - kT))), dim3(kBlock), shm, s, G, 0, 12,
+        hipLaunchKernelGGL((fft_mfma_pass<1, 0>), dim3((unsigned)(V << (m - kT))), dim3(kThreads), shm, s, G, 0, 12,
                            (const void *)xr, (const void *)xi, (void *)br, (void *)bi, 1.0);
     }
     // then 8 bits (two levels) per pass: tiles of 2^8 rows x 2^5 columns
@@ -282,10 +308,10 @@ QMC_EXPORT int qmc_fft_bro_mfma_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, doub
         PassGeom G{m, lr, lc, lo, hi};
         const bool last = hi == m;
         if (last)
-            hipLaunchKernelGGL((fft_mfma_pass<0, 1>), dim3((unsigned)(V << (m - kT))), dim3(kBlock), shm, s, G, lo,
+            hipLaunchKernelGGL((fft_mfma_pass<0, 1>), dim3((unsigned)(V << (m - kT))), dim3(kThreads), shm, s, G, lo,
                                hi, (const void *)br, (const void *)bi, (void *)xr, (void *)xi, scale);
         else
-            hipLaunchKernelGGL((fft_mfma_pass<0, 0>), dim3((unsigned)(V << (m - kT))), dim3(kBlock), shm, s, G, lo,
+            hipLaunchKernelGGL((fft_mfma_pass<0, 0>), dim3((unsigned)(V << (m - kT))), dim3(kThreads), shm, s, G, lo,
                                hi, (const void *)br, (const void *)bi, (void *)br, (void *)bi, 1.0);
     }
     hipError_t e = hipGetLastError();

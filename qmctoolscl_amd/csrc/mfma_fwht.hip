@@ -23,9 +23,11 @@ namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
 
 constexpr int kTileLog2 = 14;  // 2^14 fp32 (68 KiB of LDS with padding) per workgroup
-constexpr int kWaves = kBlock / 64;
+constexpr int kThreads = 1024;  // 16 waves: two workgroups fill a CU, each with many loads in flight
 
 // LDS word of tile index t: one pad word per 16 keeps the 16-point columns of
 // the lowest axis (addresses 16 apart) on distinct banks
@@ -60,7 +62,7 @@ __device__ void h16_axis(float *tile, int T, int a, int wave, int lane, bf16x8 A
     const int nblk = 1 << (T - 8);
     const int n = lane & 15, g = lane >> 4;
     const int c0 = (g & 1) * 8;
-    for (int blk = wave; blk < nblk; blk += kWaves) {
+    for (int blk = wave; blk < nblk; blk += (int)(blockDim.x >> 6)) {
         const int col = blk * 16 + n;
         const int lo = col & ((1 << a) - 1);
         const int base = ((col >> a) << (a + 4)) | lo;
@@ -100,7 +102,7 @@ __device__ void h2_axes(float *tile, int T, int a, int k) {
 // the Hadamard product along tile bits [tb0, tb0 + nb).  IN/OUT: 0 fp32, 1
 // float64 (the output is then scaled by `scale`).
 template <int IN64, int OUT64>
-__global__ __launch_bounds__(kBlock) void fwht_mfma_pass(int m, int lr, int lc, u64 row_stride, int tb0, int nb,
+__global__ __launch_bounds__(kThreads) void fwht_mfma_pass(int m, int lr, int lc, u64 row_stride, int tb0, int nb,
                                                          const void *__restrict__ src, void *__restrict__ dst,
                                                          double scale) {
     extern __shared__ float tile[];
@@ -110,9 +112,22 @@ __global__ __launch_bounds__(kBlock) void fwht_mfma_pass(int m, int lr, int lc, 
     const u64 cb = blockIdx.x - v * cblocks;
     const u64 vbase = v << m;
     const int cmask = (1 << lc) - 1;
-    for (int t = threadIdx.x; t < (1 << T); t += blockDim.x) {
+    // two consecutive values per access (lc >= 1: they share a row), 8 accesses in flight per lane
+#pragma unroll 8
+    for (int t = 2 * threadIdx.x; t < (1 << T); t += 2 * blockDim.x) {
         const u64 gidx = vbase + (u64)(t >> lc) * row_stride + (cb << lc) + (t & cmask);
-        tile[pad(t)] = IN64 ? (float)((const double *)src)[gidx] : ((const float *)src)[gidx];
+        float v0, v1;
+        if (IN64) {
+            const f64x2 w = *(const f64x2 *)((const double *)src + gidx);
+            v0 = (float)w.x;
+            v1 = (float)w.y;
+        } else {
+            const f32x2 w = *(const f32x2 *)((const float *)src + gidx);
+            v0 = w.x;
+            v1 = w.y;
+        }
+        tile[pad(t)] = v0;
+        tile[pad(t + 1)] = v1;
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -123,12 +138,21 @@ __global__ __launch_bounds__(kBlock) void fwht_mfma_pass(int m, int lr, int lc, 
         __syncthreads();
     }
     if (a < tb0 + nb) h2_axes(tile, T, a, tb0 + nb - a);
-    for (int t = threadIdx.x; t < (1 << T); t += blockDim.x) {
+#pragma unroll 8
+    for (int t = 2 * threadIdx.x; t < (1 << T); t += 2 * blockDim.x) {
         const u64 gidx = vbase + (u64)(t >> lc) * row_stride + (cb << lc) + (t & cmask);
-        if (OUT64)
-            store_stream(((double *)dst) + gidx, (double)tile[pad(t)] * scale);
-        else
-            ((float *)dst)[gidx] = tile[pad(t)];
+        const float v0 = tile[pad(t)], v1 = tile[pad(t + 1)];
+        if (OUT64) {
+            f64x2 w;
+            w.x = (double)v0 * scale;
+            w.y = (double)v1 * scale;
+            __builtin_nontemporal_store(w, (f64x2 *)((double *)dst + gidx));
+        } else {
+            f32x2 w;
+            w.x = v0;
+            w.y = v1;
+            *(f32x2 *)((float *)dst + gidx) = w;
+        }
     }
 }
 
@@ -161,6 +185,8 @@ QMC_EXPORT int qmc_fwht_mfma_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, double 
     if ((1ull << m) != n) return qmc_fail(QMC_ERR_ARG, "fwht_mfma: length is not a power of two");
     if (m < 8) return qmc_fail(QMC_ERR_ARG, "fwht_mfma: length 2^%d is below the 2^8 minimum", m);
     if (m > 2 * kTileLog2 - 4) return qmc_fail(QMC_ERR_ARG, "fwht_mfma: length 2^%d is above 2^%d", m, 2 * kTileLog2 - 4);
+    if (((uintptr_t)x | (uintptr_t)work) % 16)
+        return qmc_fail(QMC_ERR_ARG, "fwht_mfma: x and work must be 16-byte aligned");
     // 1/sqrt(2^m), exact for even m
     const double scale = (m % 2 == 0) ? ldexp(1.0, -m / 2) : ldexp(1.0, -(m - 1) / 2) / 1.4142135623730951;
     static bool attr_set = false;
@@ -173,7 +199,8 @@ QMC_EXPORT int qmc_fwht_mfma_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, double 
     }
     if (m <= kTileLog2) {
         const size_t shm = padded_bytes(m);
-        hipLaunchKernelGGL((fwht_mfma_pass<1, 1>), dim3((unsigned)V), dim3(kBlock), shm, s, m, 0, m, (u64)0, 0, m,
+        // one vector per workgroup: a quarter of the threads keeps more workgroups per CU
+        hipLaunchKernelGGL((fwht_mfma_pass<1, 1>), dim3((unsigned)V), dim3(kThreads / 4), shm, s, m, 0, m, (u64)0, 0, m,
                            (const void *)x, (void *)x, scale);
         QMC_HIP_CHECK(hipGetLastError());
         return QMC_OK;
@@ -183,11 +210,11 @@ QMC_EXPORT int qmc_fwht_mfma_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, double 
     // pass 1: the low kTileLog2 bits over contiguous tiles
     const int m1 = kTileLog2, m2 = m - kTileLog2;
     const size_t shm = padded_bytes(kTileLog2);
-    hipLaunchKernelGGL((fwht_mfma_pass<1, 0>), dim3((unsigned)(V << (m - m1))), dim3(kBlock), shm, s, m, 0, m1,
+    hipLaunchKernelGGL((fwht_mfma_pass<1, 0>), dim3((unsigned)(V << (m - m1))), dim3(kThreads), shm, s, m, 0, m1,
                        (u64)0, 0, m1, (const void *)x, (void *)buf, 1.0);
     // pass 2: the high m2 bits; a tile is 2^m2 rows (stride 2^m1) x 2^(14-m2) columns
     const int lc = kTileLog2 - m2;
-    hipLaunchKernelGGL((fwht_mfma_pass<0, 1>), dim3((unsigned)(V << (m - kTileLog2))), dim3(kBlock), shm, s, m, m2,
+    hipLaunchKernelGGL((fwht_mfma_pass<0, 1>), dim3((unsigned)(V << (m - kTileLog2))), dim3(kThreads), shm, s, m, m2,
                        lc, (u64)1 << m1, lc, m2, (const void *)buf, (void *)x, scale);
     hipError_t e = hipGetLastError();
     if (!work) (void)hipFreeAsync(buf, s);
