@@ -23,8 +23,11 @@
 //
 // Scheduling: each request is cut into tiles of whole rows (about P points
 // each); a setup kernel builds every request's small tables (prefix-XOR
-// columns, shifts, LMS rows, permutation parameters) in a workspace, then a
-// persistent grid drains a global tile counter.  Outputs go to a ring buffer:
+// columns, shifts, LMS rows, permutation parameters) in a workspace, then
+// persistent grids drain tile counters: one kernel for the lattice and dnb2
+// tiles (store-bound, 28 registers, full occupancy) and one for the gdn tiles
+// (digit walks, ~110 registers), each compiled without the other's code and
+// launched back to back (one graph).  Outputs go to a ring buffer:
 // every replication slab (n*d doubles) of every request is placed contiguously
 // at an offset chosen on the host, wrapping to the start when the ring is full,
 // so batches far larger than HBM can stream through a bounded buffer.
@@ -423,13 +426,10 @@ __device__ __forceinline__ void dnb2_tile(const Req &q, u64 l, u64 row0, const u
 }
 
 // z + q digitwise mod b for two u16 digits per word (both < b <= 2^11):
-// min(z + q, z + q - b) with 16-bit wrap-around
-__device__ __forceinline__ u32 pk_add_mod(u32 z, u32 q, u32 bb) {
-    u32 sum, dif, out;
-    asm("v_pk_add_u16 %0, %1, %2" : "=v"(sum) : "v"(z), "v"(q));
-    asm("v_pk_sub_u16 %0, %1, %2" : "=v"(dif) : "v"(sum), "v"(bb));
-    asm("v_pk_min_u16 %0, %1, %2" : "=v"(out) : "v"(sum), "v"(dif));
-    return out;
+// min(z + q, z + q - b) with 16-bit wrap-around (v_pk_add/sub/min_u16)
+__device__ __forceinline__ u32 pk_add_mod(u32 z, u32 q, u16x2 bb) {
+    const u16x2 sum = as_pair(z) + as_pair(q);
+    return as_word(__builtin_elementwise_min(sum, sum - bb));
 }
 
 // One gdn chain of a base-2 column: len consecutive points from k, row stride d.
@@ -495,7 +495,8 @@ __device__ __forceinline__ void gdn_chain(const u64 *__restrict__ gj, const u32 
         const u32 hi = acc_hi[P] - fast_div(acc_hi[P], M) * b;
         z[P] = lo | (hi << 16);
     }
-    const u32 bb = b | (b << 16);
+    const u16x2 bb = as_pair(b | (b << 16));
+    const u16x2 b1 = as_pair(b | (1u << 16));  // pair value = lo * b + hi (v_dot2_u32_u16)
     const double bd2 = (double)b2;
     double BL = bd2;
 #pragma unroll
@@ -507,7 +508,7 @@ __device__ __forceinline__ void gdn_chain(const u64 *__restrict__ gj, const u32 
         double Yhi = 0.0, Ylo = 0.0;
 #pragma unroll
         for (int P = 0; P < W; P++) {
-            const double pv = (double)((z[P] & 0xffffu) * b + (z[P] >> 16));
+            const double pv = (double)__builtin_amdgcn_udot2(as_pair(z[P]), b1, 0u, false);
             if (P < W - L)
                 Yhi = __fma_rn(Yhi, bd2, pv);
             else
@@ -541,13 +542,19 @@ __device__ __forceinline__ void gdn_chain(const u64 *__restrict__ gj, const u32 
     }
 }
 
+// Tiles are drained by two persistent kernels: class 0 = lattice and dnb2
+// (HBM-write bound, few registers, full occupancy), class 1 = gdn (register
+// heavy digit walks).  Each class is compiled without the other's code.
+__host__ __device__ constexpr int method_class(int method) { return method == M_GDN_LMS ? 1 : 0; }
+
+template <int CLASS>
 __device__ void run_tile(const Req &q, u64 t, const u64 *__restrict__ ws, double *__restrict__ out, u32 *lds) {
     const u64 d = q.d;
     const u64 l = t / q.tiles_per_l;
     const u64 row0 = (t - l * q.tiles_per_l) * (u64)q.rows_per_tile;
     const u64 *tab = ws + q.tab;
     double *slab = out + ws[q.slabs + l];
-    if (q.method == M_LATTICE) {
+    if (CLASS == 0 && q.method == M_LATTICE) {
         const u32 pts = q.rows_per_tile * (u32)d;
         const u64 *g = tab;
         const double *sh = (const double *)(tab + d) + l * d;
@@ -559,9 +566,9 @@ __device__ void run_tile(const Req &q, u64 t, const u64 *__restrict__ ws, double
             const double x = frac_mod1((double)g[j] * phi);      // lat_gen_natural.c:45
             store_stream(slab + row * d + j, frac_mod1(x + sh[j]));  // lat_shift_mod_1.c:31
         }
-    } else if (q.method == M_DNB2_LMS) {
+    } else if (CLASS == 0 && q.method == M_DNB2_LMS) {
         dnb2_tile<true>(q, l, row0, tab, slab, lds);
-    } else if (q.method == M_DNB2_NUS) {
+    } else if (CLASS == 0) {
         dnb2_tile<false>(q, l, row0, tab, slab, lds);
     } else {
         // this replication's digit tables into LDS
@@ -604,13 +611,14 @@ __device__ void run_tile(const Req &q, u64 t, const u64 *__restrict__ ws, double
 
 // ------------------------------------------------------------ the kernels
 
+template <int CLASS>
 __global__ __launch_bounds__(kBlock) void batch_persistent_kernel(const Req *__restrict__ reqs, int nreq,
                                                                   u64 total_tiles, unsigned long long *counter,
                                                                   const u64 *__restrict__ ws,
                                                                   double *__restrict__ out) {
     __shared__ u64 s_tile;
     __shared__ int s_req;
-    extern __shared__ u32 lds_tables[];  // gdn digit tables of the current tile
+    extern __shared__ u32 lds_tables[];  // dnb2 first-row values or gdn digit tables of the current tile
     for (;;) {
         if (threadIdx.x == 0) {
             const u64 t = atomicAdd(counter, 1ull);
@@ -631,18 +639,22 @@ __global__ __launch_bounds__(kBlock) void batch_persistent_kernel(const Req *__r
         __syncthreads();
         if (t >= total_tiles) break;  // every wave of every block reaches this
         const Req q = reqs[qi];
-        run_tile(q, t - q.tile0, ws, out, lds_tables);
+        run_tile<CLASS>(q, t - q.tile0, ws, out, lds_tables);
     }
 }
 
-__global__ void reset_counter_kernel(unsigned long long *counter) { *counter = 0; }
+__global__ void reset_counter_kernel(unsigned long long *counter) {
+    counter[0] = 0;
+    counter[1] = 0;
+}
 
 // one request, one tile per block (the per-call form)
+template <int CLASS>
 __global__ __launch_bounds__(kBlock) void batch_single_kernel(const Req *__restrict__ req, const u64 *__restrict__ ws,
                                                               double *__restrict__ out) {
     extern __shared__ u32 lds_tables[];
     const Req q = *req;
-    run_tile(q, blockIdx.x, ws, out, lds_tables);
+    run_tile<CLASS>(q, blockIdx.x, ws, out, lds_tables);
 }
 
 // ----------------------------------------------------------- host plan
@@ -682,23 +694,32 @@ struct Plan {
     std::vector<Req> reqs;
     u64 total_tiles = 0;
     u64 ws_words = 0;
+    std::vector<Req> qreqs;          // requests regrouped by class, tile0 counted within the class
+    int qn[2] = {0, 0};              // requests per class (class 0 first in qreqs)
+    u64 qtiles[2] = {0, 0};          // tiles per class
+    Req *d_qreqs = nullptr;
     u64 ring_elems = 0;
     Req *d_reqs = nullptr;
     u64 *d_ws = nullptr;
     unsigned long long *d_counter = nullptr;
     int grid = 0;
     int device = 0;
-    u32 lds_bytes = 0;               // dynamic LDS of the generation kernels (gdn tables)
+    u32 lds_bytes = 0;               // dynamic LDS of the gdn kernel (digit tables)
     hipStream_t cap = nullptr;       // private stream for graph capture
     hipGraphExec_t gexec = nullptr;  // [counter reset, persistent kernel]
     double *gring = nullptr;         // ring the graph was captured for
 };
 
+constexpr u32 kLdsClass0 = 256 * sizeof(u32);  // dnb2 first-row values
+
 void launch_batch(Plan *P, double *ring, hipStream_t s) {
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, P->d_counter);
-    hipLaunchKernelGGL(batch_persistent_kernel, dim3(P->grid), dim3(kBlock), P->lds_bytes, s, P->d_reqs,
-                       (int)P->reqs.size(),
-                       P->total_tiles, P->d_counter, P->d_ws, ring);
+    if (P->qtiles[0])
+        hipLaunchKernelGGL(batch_persistent_kernel<0>, dim3(P->grid), dim3(kBlock), kLdsClass0, s, P->d_qreqs,
+                           P->qn[0], P->qtiles[0], P->d_counter, P->d_ws, ring);
+    if (P->qtiles[1])
+        hipLaunchKernelGGL(batch_persistent_kernel<1>, dim3(P->grid), dim3(kBlock), P->lds_bytes, s,
+                           P->d_qreqs + P->qn[0], P->qn[1], P->qtiles[1], P->d_counter + 1, P->d_ws, ring);
 }
 
 }  // namespace
@@ -767,7 +788,15 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
     }
     P->total_tiles = tile;
     P->ws_words = words;
-    if (P->lds_bytes < 256 * sizeof(u32)) P->lds_bytes = 256 * sizeof(u32);  // dnb2 first-row values
+    for (int c = 0; c < 2; c++)
+        for (const Req &q0 : P->reqs) {
+            if (method_class(q0.method) != c) continue;
+            Req q = q0;
+            q.tile0 = P->qtiles[c];
+            P->qtiles[c] += (u64)q.r * q.tiles_per_l;
+            P->qreqs.push_back(q);
+            P->qn[c]++;
+        }
     if (hipGetDevice(&P->device) != hipSuccess) { delete P; qmc_fail(QMC_ERR_HIP, "batch: no device"); return nullptr; }
     int cus = 256;
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, P->device);
@@ -817,12 +846,14 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
     if (hipMemcpyToSymbol(HIP_SYMBOL(c_primes), h_primes, sizeof(h_primes)) != hipSuccess ||
         hipMalloc((void **)&P->d_reqs, sizeof(Req) * P->reqs.size()) != hipSuccess ||
         hipMalloc((void **)&P->d_ws, sizeof(u64) * (words ? words : 1)) != hipSuccess ||
-        hipMalloc((void **)&P->d_counter, sizeof(unsigned long long)) != hipSuccess) {
+        hipMalloc((void **)&P->d_qreqs, sizeof(Req) * P->qreqs.size()) != hipSuccess ||
+        hipMalloc((void **)&P->d_counter, 2 * sizeof(unsigned long long)) != hipSuccess) {
         qmc_fail(QMC_ERR_NOMEM, "batch: device allocation failed");
         delete P;
         return nullptr;
     }
     hipMemcpy(P->d_reqs, P->reqs.data(), sizeof(Req) * P->reqs.size(), hipMemcpyHostToDevice);
+    hipMemcpy(P->d_qreqs, P->qreqs.data(), sizeof(Req) * P->qreqs.size(), hipMemcpyHostToDevice);
     hipMemcpy(P->d_ws, ws_host.data(), sizeof(u64) * words, hipMemcpyHostToDevice);
     // tables
     hipLaunchKernelGGL(batch_setup_kernel, dim3(64, (unsigned)P->reqs.size()), dim3(kBlock), 0, 0, P->d_reqs,
@@ -843,6 +874,7 @@ QMC_EXPORT void qmc_batch_free(void *h) {
     hipFree(P->d_reqs);
     hipFree(P->d_ws);
     hipFree(P->d_counter);
+    hipFree(P->d_qreqs);
     delete P;
 }
 
@@ -900,9 +932,12 @@ QMC_EXPORT int qmc_batch_run_per_call_dev(void *h, double *ring, qmc_stream_t s)
     for (size_t i = 0; i < P->reqs.size(); i++) {
         const Req &q = P->reqs[i];
         const u64 tiles = (u64)q.r * q.tiles_per_l;
-        hipLaunchKernelGGL(batch_single_kernel, dim3((unsigned)tiles), dim3(kBlock), P->lds_bytes, s, P->d_reqs + i,
-                           P->d_ws,
-                           ring);
+        if (method_class(q.method) == 0)
+            hipLaunchKernelGGL(batch_single_kernel<0>, dim3((unsigned)tiles), dim3(kBlock), kLdsClass0, s,
+                               P->d_reqs + i, P->d_ws, ring);
+        else
+            hipLaunchKernelGGL(batch_single_kernel<1>, dim3((unsigned)tiles), dim3(kBlock), P->lds_bytes, s,
+                               P->d_reqs + i, P->d_ws, ring);
     }
     QMC_HIP_CHECK(hipGetLastError());
     return QMC_OK;

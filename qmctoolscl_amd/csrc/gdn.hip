@@ -49,6 +49,8 @@ __global__ __launch_bounds__(kBlock) void gdn_gen_kernel(Rows g, u64 n_start, u6
                                                          u64 *__restrict__ xdig, u32 *status) {
     __shared__ u32 sdig[kRows][kMaxDigits];
     __shared__ u64 sbase[kRows];
+    __shared__ u64 smagic[kRows];  // ceil(2^64 / b) for b < 2^16
+    __shared__ u32 sr32[kRows];    // 2^32 mod b
     __shared__ u32 scount[kRows];
     __shared__ u64 sj[kRows];
     const u64 l = blockIdx.y;
@@ -81,6 +83,8 @@ __global__ __launch_bounds__(kBlock) void gdn_gen_kernel(Rows g, u64 n_start, u6
             }
         }
         sbase[threadIdx.x] = b;
+        smagic[threadIdx.x] = (b >= 2 && b < (1u << 16)) ? (~0ull) / b + 1 : 0;
+        sr32[threadIdx.x] = (b >= 2 && b < (1u << 16)) ? (u32)(0x100000000ull % b) : 0;
         scount[threadIdx.x] = cnt;
         sj[threadIdx.x] = j;
     }
@@ -98,21 +102,30 @@ __global__ __launch_bounds__(kBlock) void gdn_gen_kernel(Rows g, u64 n_start, u6
         const u32 cnt = scount[row];
         const u64 cbase = cl + sj[row] * mmax * tmax + t;
         u64 acc = 0;
-        // fast path: b < 2^16 and every column entry < 2^32, so no step wraps
-        bool fast = b >= 2 && b < (1u << 16);
+        // fast path: b < 2^16 and every column entry < 2^32, so no step wraps;
+        // all loads of the row are independent (checked once, after the sum)
+        bool fast = b >= 2 && b < (1u << 16) && (cnt == 0 || cbase + (u64)(cnt - 1) * tmax < Ctotal);
         if (fast) {
+            u64 orc = 0;
+            const u64 *Cp = C + cbase;
+#pragma unroll 4
             for (u32 m = 0; m < cnt; m++) {
-                const u32 dg = sdig[row][m];
-                if (!dg) continue;
-                const u64 ci = cbase + (u64)m * tmax;
-                if (ci >= Ctotal) { fast = false; break; }
-                const u64 c = C[ci];
-                if (c >> 32) { fast = false; break; }
-                acc += (u64)dg * c;
+                const u64 c = Cp[(u64)m * tmax];
+                orc |= c;
+                acc += (u64)sdig[row][m] * c;
             }
+            fast = (orc >> 32) == 0;
         }
         if (fast) {
-            acc = umod(acc, b);
+            // acc mod b with 32-bit magic divisions: acc = h 2^32 + lo, b < 2^16
+            const u64 M = smagic[row];
+            const u32 bb = (u32)b;
+            const u32 h = (u32)(acc >> 32), lo = (u32)acc;
+            const u32 hm = h - (u32)__umul64hi(M, h) * bb;
+            const u32 r32 = sr32[row];
+            const u32 lm = lo - (u32)__umul64hi(M, lo) * bb;
+            const u32 y = hm * r32 + lm;  // < b^2 <= 2^32
+            acc = y - (u32)__umul64hi(M, y) * bb;
         } else if (b < 2) {
             acc = 0;
         } else {  // step by step, as gdn_gen_natural.c:46
@@ -154,6 +167,8 @@ __global__ __launch_bounds__(kBlock) void gdn_digitwise_kernel(Rows g, u64 r_x, 
     const u32 total = (u32)nrows * T;
     const u64 *src = xdig + ((l % r_x) * g.nd + rho0) * tmax;
     u64 *out = xnew + (l * g.nd + rho0) * tmax_new;
+    u32 bad = 0;
+#pragma unroll 4
     for (u32 f = threadIdx.x; f < total; f += blockDim.x) {
         const u32 row = div_q(f, g.tdiv);
         const u32 t = f - row * T;
@@ -168,7 +183,7 @@ __global__ __launch_bounds__(kBlock) void gdn_digitwise_kernel(Rows g, u64 r_x, 
                 v = tab[pi];
             } else {
                 v = 0;
-                flag(status, STATUS_OOB);
+                bad |= STATUS_OOB;
             }
         } else {
             // gdn_digital_shift.c:37-42: (digit + shift) % b; the raw shift past tmax
@@ -176,10 +191,11 @@ __global__ __launch_bounds__(kBlock) void gdn_digitwise_kernel(Rows g, u64 r_x, 
             if (t < tmax) {
                 const u64 b = sbase[row];
                 if (b == 0) {
-                    flag(status, STATUS_BAD_BASE);
+                    bad |= STATUS_BAD_BASE;
                     v = 0;
                 } else {
-                    v = umod(src[(u64)row * tmax + t] + sh, b);
+                    const u64 x = src[(u64)row * tmax + t] + sh;  // wraps as the reference's u64 sum
+                    v = x < 2 * b && x >= sh ? (x >= b ? x - b : x) : umod(x, b);
                 }
             } else {
                 v = sh;
@@ -187,27 +203,46 @@ __global__ __launch_bounds__(kBlock) void gdn_digitwise_kernel(Rows g, u64 r_x, 
         }
         store_stream(out + f, v);
     }
+    if (bad) flag(status, bad);
 }
 
 // -------------------------------------------------------------- to double
 
 // gdn_integer_to_float.c:32-40: v += recip*digit; recip /= b, sequential per
-// element (compiled without FMA contraction, IEEE division)
+// element (compiled without FMA contraction).  The reciprocals depend only on
+// the base, so they are computed once per (l % r_b, j) by the same IEEE
+// divisions, into rt[(lb*d + j)*tmax + t]; every element then does
+// v += rt[t] * digit in the reference's order.
+__global__ __launch_bounds__(kBlock) void gdn_recip_kernel(u64 nb, u64 tmax, const u64 *__restrict__ bases,
+                                                           double *__restrict__ rt) {
+    const u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nb) return;
+    const double b = (double)bases[e];
+    double recip = 1 / b;
+    for (u64 t = 0; t < tmax; t++) {
+        rt[e * tmax + t] = recip;
+        recip /= b;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void gdn_zero_kernel(u64 total, double *__restrict__ x) {
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) x[e] = 0.;
+}
+
 __global__ __launch_bounds__(kBlock) void gdn_to_float_kernel(u64 r, u64 nd, u64 d, u64 r_b, u64 tmax,
-                                                              const u64 *__restrict__ bases,
+                                                              const double *__restrict__ rt,
                                                               const u64 *__restrict__ xdig,
                                                               double *__restrict__ x) {
     const u64 stride = (u64)gridDim.x * blockDim.x;
     for (u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x; e < r * nd; e += stride) {
         const u64 l = e / nd;
         const u64 j = (e - l * nd) % d;
-        const double b = (double)bases[(l % r_b) * d + j];
+        const double *rec = rt + ((l % r_b) * d + j) * tmax;
         const u64 *src = xdig + e * tmax;
-        double recip = 1 / b, v = 0.;
-        for (u64 t = 0; t < tmax; t++) {
-            v += recip * (double)src[t];
-            recip /= b;
-        }
+        double v = 0.;
+#pragma unroll 8
+        for (u64 t = 0; t < tmax; t++) v += rec[t] * (double)src[t];
         store_stream(x + e, v);
     }
 }
@@ -362,9 +397,22 @@ int gdn_integer_to_float_dev(u64 r, u64 n, u64 d, u64 r_b, u64 tmax, const u64 *
                              double *x, hipStream_t s) {
     if (!(r && n && d)) return QMC_OK;
     if (r_b == 0) return qmc_fail(QMC_ERR_ARG, "gdn_integer_to_float: r_b must be positive");
+    if (!tmax) {
+        hipLaunchKernelGGL(gdn_zero_kernel, dim3(grid_for(r * n * d, kBlock, 2048 * 8)), dim3(kBlock), 0, s,
+                           r * n * d, x);
+        QMC_HIP_CHECK(hipGetLastError());
+        return QMC_OK;
+    }
+    double *rt = nullptr;
+    const u64 nb = r_b * d;
+    QMC_HIP_CHECK(hipMallocAsync((void **)&rt, sizeof(double) * nb * tmax, s));
+    hipLaunchKernelGGL(gdn_recip_kernel, dim3((unsigned)((nb + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, nb, tmax,
+                       bases, rt);
     hipLaunchKernelGGL(gdn_to_float_kernel, dim3(grid_for(r * n * d, kBlock, 2048 * 8)), dim3(kBlock), 0, s, r,
-                       n * d, d, r_b, tmax, bases, xdig, x);
-    QMC_HIP_CHECK(hipGetLastError());
+                       n * d, d, r_b, tmax, rt, xdig, x);
+    const hipError_t e = hipGetLastError();
+    (void)hipFreeAsync(rt, s);
+    if (e != hipSuccess) return qmc_fail_hip(e, "gdn_integer_to_float");
     return QMC_OK;
 }
 

@@ -30,7 +30,7 @@ typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef double f64x2 __attribute__((ext_vector_type(2)));
 
 constexpr int kT = 13;          // 2^13 complex fp32 (68 KiB of LDS with padding)
-constexpr int kThreads = 1024;  // 16 waves: two workgroups fill a CU, each with many loads in flight
+constexpr int kThreads = 512;  // 8 waves; two workgroups (68 KiB of LDS, <= 128 VGPRs each) share a CU
 
 // LDS word of tile index t: one pad word per 16 keeps the 16-point columns of
 // the lowest level (addresses 16 apart) on distinct banks
@@ -101,9 +101,38 @@ __device__ __forceinline__ u64 gpos(const PassGeom &G, u64 cb, int t) {
     return (cb_hi << G.hi) | ((u64)(t >> G.lc) << G.lo) | (cb_lo << G.lc) | (u64)(t & ((1 << G.lc) - 1));
 }
 
+// Twiddles w_N^E = exp(-2 pi i E / N), N = 2^m, from two FP64 tables:
+// E = Eh 2^klo + El, w = twh[Eh] * twl[El] in FP64, rounded once to fp32.
+struct Twiddles {
+    const f64x2 *lo, *hi;  // (cos, -sin)
+    int klo;
+};
+
+__global__ __launch_bounds__(kThreads / 4) void twiddle_tables_kernel(int m, int klo, f64x2 *twl, f64x2 *twh) {
+    const u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x;
+    const u64 nlo = 1ull << klo, nhi = 1ull << (m - klo);
+    if (i >= nlo + nhi) return;
+    const u64 E = i < nlo ? i : (i - nlo) << klo;
+    double s, c;
+    sincospi(-2.0 * (double)E * ldexp(1.0, -m), &s, &c);  // exact angle: E / 2^m
+    f64x2 w;
+    w.x = c;
+    w.y = s;
+    if (i < nlo)
+        twl[i] = w;
+    else
+        twh[i - nlo] = w;
+}
+
+__device__ __forceinline__ void twiddle(const Twiddles &tw, u64 E, float &c, float &s) {
+    const f64x2 a = tw.hi[E >> tw.klo], b = tw.lo[E & ((1ull << tw.klo) - 1)];
+    c = (float)(a.x * b.x - a.y * b.y);
+    s = (float)(a.x * b.y + a.y * b.x);
+}
+
 // radix-16 level on tile bits [ta, ta+4) == global bits [ga, ga+4)
 __device__ void dft16_level(float *re, float *im, const PassGeom &G, u64 cb, int T, int ta, int ga,
-                            const Wtile &W, int wave, int lane) {
+                            const Wtile &W, int wave, int lane, const Twiddles &tw) {
     const int nblk = 1 << (T - 8);
     const int n = lane & 15, g = lane >> 4;
     const int c0 = (g & 1) * 8;
@@ -147,7 +176,7 @@ __device__ void dft16_level(float *re, float *im, const PassGeom &G, u64 cb, int
                 const u64 chi = p >> (ga + 4);
                 const u64 e = (__brevll(chi) >> (64 - hi_bits)) * (u64)k;  // < Ns = 2^(hi_bits+4)
                 float s, c;
-                sincospif(-2.0f * (float)e * __builtin_ldexpf(1.0f, -(hi_bits + 4)), &s, &c);
+                twiddle(tw, e << (G.m - hi_bits - 4), c, s);  // w_Ns^e = w_N^(e N/Ns)
                 const float tr = vr * c - vi * s;
                 vi = vr * s + vi * c;
                 vr = tr;
@@ -189,7 +218,7 @@ __device__ void dft_small_level(float *re, float *im, int T, int ta, int rb) {
 template <int IN64, int OUT64>
 __global__ __launch_bounds__(kThreads) void fft_mfma_pass(PassGeom G, int glo, int ghi, const void *__restrict__ sr,
                                                         const void *__restrict__ si, void *__restrict__ dr,
-                                                        void *__restrict__ di, double scale) {
+                                                        void *__restrict__ di, double scale, Twiddles tw) {
     extern __shared__ float lds[];
     const int T = G.lr + G.lc;
     float *re = lds;
@@ -225,7 +254,7 @@ __global__ __launch_bounds__(kThreads) void fft_mfma_pass(PassGeom G, int glo, i
         // tile bit of global bit ga
         const int ta = ga < G.lc ? ga : G.lc + (ga - G.lo);
         if (ghi - ga >= 4) {
-            dft16_level(re, im, G, cb, T, ta, ga, W, wave, lane);
+            dft16_level(re, im, G, cb, T, ta, ga, W, wave, lane, tw);
         } else {
             dft_small_level(re, im, T, ta, ghi - ga);  // only the top level can be short
         }
@@ -284,38 +313,49 @@ QMC_EXPORT int qmc_fft_bro_mfma_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, doub
         attr_set = true;
     }
     const size_t shm = padded_bytes(kT);
+    // FP64 twiddle tables (one product per twiddle), 2^klo + 2^(m-klo) entries
+    Twiddles tw;
+    tw.klo = (m + 1) / 2;
+    const u64 ntw = (1ull << tw.klo) + (1ull << (m - tw.klo));
+    f64x2 *tt = nullptr;
+    QMC_HIP_CHECK(hipMallocAsync((void **)&tt, sizeof(f64x2) * ntw, s));
+    tw.lo = tt;
+    tw.hi = tt + (1ull << tw.klo);
+    hipLaunchKernelGGL(twiddle_tables_kernel, dim3((unsigned)((ntw + kThreads / 4 - 1) / (kThreads / 4))),
+                       dim3(kThreads / 4), 0, s, m, tw.klo, tt, tt + (1ull << tw.klo));
+    float *buf = nullptr;
     if (m <= kT) {
         PassGeom G{m, 0, m, m, m};
-        // one vector per workgroup: a quarter of the threads keeps four workgroups per CU
-        hipLaunchKernelGGL((fft_mfma_pass<1, 1>), dim3((unsigned)V), dim3(kThreads / 4), padded_bytes(m), s, G, 0, m,
-                           (const void *)xr, (const void *)xi, (void *)xr, (void *)xi, scale);
-        QMC_HIP_CHECK(hipGetLastError());
-        return QMC_OK;
-    }
-    float *buf = work;
-    if (!buf) QMC_HIP_CHECK(hipMallocAsync((void **)&buf, 2 * sizeof(float) * V * n, s));
-    float *br = buf, *bi = buf + V * n;
-    // pass 1: global bits [0, 12) over contiguous tiles of 2^13
-    {
-        PassGeom G{m, 0, kT, kT, kT};
-        hipLaunchKernelGGL((fft_mfma_pass<1, 0>), dim3((unsigned)(V << (m - kT))), dim3(kThreads), shm, s, G, 0, 12,
-                           (const void *)xr, (const void *)xi, (void *)br, (void *)bi, 1.0);
-    }
-    // then 8 bits (two levels) per pass: tiles of 2^8 rows x 2^5 columns
-    for (int lo = 12; lo < m; lo += 8) {
-        const int hi = lo + 8 < m ? lo + 8 : m;
-        const int lr = hi - lo, lc = kT - lr;
-        PassGeom G{m, lr, lc, lo, hi};
-        const bool last = hi == m;
-        if (last)
-            hipLaunchKernelGGL((fft_mfma_pass<0, 1>), dim3((unsigned)(V << (m - kT))), dim3(kThreads), shm, s, G, lo,
-                               hi, (const void *)br, (const void *)bi, (void *)xr, (void *)xi, scale);
-        else
-            hipLaunchKernelGGL((fft_mfma_pass<0, 0>), dim3((unsigned)(V << (m - kT))), dim3(kThreads), shm, s, G, lo,
-                               hi, (const void *)br, (const void *)bi, (void *)br, (void *)bi, 1.0);
+        // one vector per workgroup: half the threads keeps more workgroups per CU
+        hipLaunchKernelGGL((fft_mfma_pass<1, 1>), dim3((unsigned)V), dim3(kThreads / 2), padded_bytes(m), s, G, 0, m,
+                           (const void *)xr, (const void *)xi, (void *)xr, (void *)xi, scale, tw);
+    } else {
+        buf = work;
+        if (!buf) QMC_HIP_CHECK(hipMallocAsync((void **)&buf, 2 * sizeof(float) * V * n, s));
+        float *br = buf, *bi = buf + V * n;
+        // pass 1: global bits [0, 12) over contiguous tiles of 2^13
+        {
+            PassGeom G{m, 0, kT, kT, kT};
+            hipLaunchKernelGGL((fft_mfma_pass<1, 0>), dim3((unsigned)(V << (m - kT))), dim3(kThreads), shm, s, G, 0,
+                               12, (const void *)xr, (const void *)xi, (void *)br, (void *)bi, 1.0, tw);
+        }
+        // then 8 bits (two levels) per pass: tiles of 2^8 rows x 2^5 columns
+        for (int lo = 12; lo < m; lo += 8) {
+            const int hi = lo + 8 < m ? lo + 8 : m;
+            const int lr = hi - lo, lc = kT - lr;
+            PassGeom G{m, lr, lc, lo, hi};
+            const bool last = hi == m;
+            if (last)
+                hipLaunchKernelGGL((fft_mfma_pass<0, 1>), dim3((unsigned)(V << (m - kT))), dim3(kThreads), shm, s, G,
+                                   lo, hi, (const void *)br, (const void *)bi, (void *)xr, (void *)xi, scale, tw);
+            else
+                hipLaunchKernelGGL((fft_mfma_pass<0, 0>), dim3((unsigned)(V << (m - kT))), dim3(kThreads), shm, s, G,
+                                   lo, hi, (const void *)br, (const void *)bi, (void *)br, (void *)bi, 1.0, tw);
+        }
     }
     hipError_t e = hipGetLastError();
-    if (!work) (void)hipFreeAsync(buf, s);
+    if (buf && !work) (void)hipFreeAsync(buf, s);
+    (void)hipFreeAsync(tt, s);
     if (e != hipSuccess) return qmc_fail_hip(e, "fft_mfma_pass");
     return QMC_OK;
 }

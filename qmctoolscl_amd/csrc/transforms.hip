@@ -148,8 +148,12 @@ __device__ __forceinline__ void tile_round(double *sr, double *si, u64 len, int 
 // contiguous tiles of 2^tlog2 values of the flat array (tiles hold whole
 // vectors or aligned parts of one vector), in rounds of up to four stages:
 // one LDS round trip and one barrier per round.
+// complex tiles (64 KiB of LDS, two per CU) use 512 threads: one 8-value group per lane per round
+template <int KIND>
+constexpr int tile_threads() { return KIND == KIND_FWHT ? kBlock : 2 * kBlock; }
+
 template <int KIND, bool ASC>
-__global__ __launch_bounds__(kBlock) void tile_kernel(u64 total, int m, int tlog2, int nst, double *__restrict__ xr,
+__global__ __launch_bounds__(tile_threads<KIND>()) void tile_kernel(u64 total, int m, int tlog2, int nst, double *__restrict__ xr,
                                                       double *__restrict__ xi, const double *__restrict__ tsr,
                                                       const double *__restrict__ tsi) {
     extern __shared__ double lds[];
@@ -165,8 +169,10 @@ __global__ __launch_bounds__(kBlock) void tile_kernel(u64 total, int m, int tlog
     }
     __syncthreads();
     const u64 vbase = base & ((1ull << m) - 1);  // the tile's offset within its vector
+    // complex rounds take three stages (8 values in 32 registers), real ones four
+    constexpr int kRound = KIND == KIND_FWHT ? kGroupLog2 : 3;
     for (int done = 0; done < nst;) {
-        const int cnt = (nst - done) < kGroupLog2 ? (nst - done) : kGroupLog2;
+        const int cnt = (nst - done) < kRound ? (nst - done) : kRound;
         const int lo = ASC ? done : nst - done - cnt;
         switch (cnt) {
             case 4: tile_round<KIND, ASC, 4>(sr, si, len, lo, vbase, tsr, tsi); break;
@@ -245,7 +251,8 @@ int launch_tile(u64 total, int m, int tlog2, double *xr, double *xi, const doubl
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)most));
         attr_set = true;
     }
-    hipLaunchKernelGGL((tile_kernel<KIND, ASC>), dim3((unsigned)tiles), dim3(kBlock), shm, s, total, m, tlog2, tlog2,
+    hipLaunchKernelGGL((tile_kernel<KIND, ASC>), dim3((unsigned)tiles), dim3(tile_threads<KIND>()), shm, s, total, m,
+                       tlog2, tlog2,
                        xr, xi, tsr, tsi);
     QMC_HIP_CHECK(hipGetLastError());
     return QMC_OK;
