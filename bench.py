@@ -181,8 +181,40 @@ def section_gdn(torch, q, dev, timed, r=8):
            "perm_ms": round(ms_perm, 3), "perm_GBs": round(2 * dig_bytes / ms_perm / 1e6, 1),
            "to_float_ms": round(ms_flt, 3), "to_float_GBs": round((dig_bytes + 8 * pts) / ms_flt / 1e6, 1),
            "chain_Gpts_s": round(pts / (ms_gen + ms_perm + ms_flt) / 1e6, 2)}
-    del xdig, xperm, x, C_lms
+    del xperm, x, C_lms
     torch.cuda.empty_cache()
+    # nested uniform scrambling of the same net's first 3^8 points (host trees,
+    # numpy draws in the reference's order, GPU walk), 20 -> 52 digits
+    from qmctoolscl_amd import nus
+    n8, tmax = 3 ** 8, mmax
+    xd = xdig[:1, :n8, :, :tmax].contiguous()  # the unscrambled Halton digits are not needed: regenerate
+    q.gdn_gen_natural(U(1), U(n8), U(d), U(1), U(mmax), U(tmax), U(0), Bd, Cd, xd)
+    xr_nus = torch.empty((r, n8, d, tnew), dtype=torch.int64, device=dev)
+    seqs = np.random.SeedSequence(11).spawn(r * d)
+    rngs = np.empty((r, d), dtype=object)
+    for k in range(r * d):
+        rngs[k // d, k % d] = np.random.Generator(np.random.SFC64(seqs[k]))
+    roots = np.array([[nus.NUSNode_gdn() for _ in range(d)] for _ in range(r)], dtype=object)
+    t0 = time.perf_counter()
+    nus.gdn_nested_uniform_scramble(r, n8, d, 1, 1, tmax, tnew, rngs, roots, Bd, xd, xr_nus)
+    torch.cuda.synchronize()
+    t_first = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    nus.gdn_nested_uniform_scramble(r, n8, d, 1, 1, tmax, tnew, rngs, roots, Bd, xd, xr_nus)
+    torch.cuda.synchronize()
+    t_again = time.perf_counter() - t0
+    out["nus"] = {"workload": "gdn nested uniform scramble, first 3^8 points, d=32, r=8, 20 -> 52 digits",
+                  "points": r * n8 * d, "first_call_s": round(t_first, 3), "repeat_call_s": round(t_again, 4),
+                  "first_call_Mpts_s": round(r * n8 * d / t_first / 1e6, 3)}
+    del xdig, xd, xr_nus
+    torch.cuda.empty_cache()
+    # base-2 interlacing of a d = 16 net at alpha = 2 into 8 columns of 64 digits
+    Cn = rng.integers(0, 2 ** 32, size=(r, 16, 32), dtype=np.uint64)
+    Cn_t = t64(torch, Cn, dev)
+    Ca = torch.empty((r, 8, 32), dtype=torch.int64, device=dev)
+    ms_il, _ = timed(lambda: q.dnb2_interlace(U(r), U(8), U(32), U(16), U(32), U(64), U(2), Cn_t, Ca, wait=False),
+                     20, 3)
+    out["dnb2_interlace_d16_alpha2_ms"] = round(ms_il, 4)
     return out
 
 
@@ -209,8 +241,25 @@ def section_batch(torch, dev, timed, quick):
         per_method[["lattice", "dnb2_lms_shift", "dnb2_nus_hash", "gdn_lms_perm"][mm]] = int(
             np.sum((np.int64(1) << m[sel].astype(np.int64)) * d[sel] * r[sel]))
     gbs = 8 * pts / ms_b / 1e6
+    # exactness at full size: per-slab digests of the batched and per-call runs
+    dg_b = plan.digests(ring)
+    dg_p = plan.digests(ring, per_call=True)
+    # each method's requests alone (capped at m = 16 to bound the time), same tile size
+    by_method = {}
+    mq = np.minimum(m, 16).astype(np.int32)
+    for k, name in enumerate(("lattice", "dnb2_lms_shift", "dnb2_nus_hash", "gdn_lms_perm")):
+        sel = meth == k
+        sub = B.Batch(meth[sel], mq[sel], d[sel], r[sel], seeds[sel], points_per_tile=tune["points_per_tile"],
+                      grid=tune["grid"], ring_elems=ring_elems)
+        ms_k, _ = timed(lambda: sub.run(ring, graph=True), 3, 1)
+        by_method[name] = {"points": sub.points, "ms": round(ms_k, 3), "Gpts_s": round(sub.points / ms_k / 1e6, 1),
+                           "GBs_written": round(8 * sub.points / ms_k / 1e6, 1),
+                           "frac_hbm": round(8 * sub.points / ms_k / 1e6 / HBM_PEAK_GBS, 4)}
+        del sub
     return {"workload": "512 ragged requests, m in [%d,%d], d in [1,256], r in [1,64], counter-based inputs"
                         % (int(m.min()), int(m.max())),
+            "bitexact_vs_per_call": bool(np.array_equal(dg_b, dg_p)), "slabs_checked": int(dg_b.size),
+            "by_method_m_le_16": by_method,
             "points": pts, "points_by_method": per_method, "tiles": plan.total_tiles,
             "autotune": {k: tune[k] for k in ("points_per_tile", "grid")}, "autotune_table": tune["table"],
             "batched_ms": round(ms_b, 2), "batched_Gpts_s": round(pts / ms_b / 1e6, 2),
@@ -357,7 +406,10 @@ def section_transforms(torch, q, dev, timed):
                     "fft_fp64_ms": round(msf, 4), "fft_fp64_GBs": round(32 * pts / msf / 1e6, 1),
                     "fft_mfma_ms": round(msfm, 4), "fft_mfma_GBs": round(32 * pts / msfm / 1e6, 1),
                     "fft_mfma_speedup": round(msf / msfm, 3),
-                    "fft_mfma_rel_l2": float("%.3g" % fl2), "fft_mfma_rel_max": float("%.3g" % fmx)})
+                    "fft_mfma_rel_l2": float("%.3g" % fl2), "fft_mfma_rel_max": float("%.3g" % fmx),
+                    # 12 products of 16x16x32 per 256 complex values per radix-16 level
+                    "fft_mfma_util": float("%.3g" % (pts * (m // 4) * 768 / (msfm * 1e-3) /
+                                                     (BF16_PEAK_TFLOPS * 1e12)))})
         del x0, x, ref, xm, work, xr, xi
         torch.cuda.empty_cache()
     return out

@@ -270,6 +270,13 @@ QMC_EXPORT qmc_u64 qmc_batch_slab_offset(void *plan, int q, int l);
 QMC_EXPORT int qmc_batch_run_dev(void *plan, double *ring, qmc_stream_t s);          /* persistent launch */
 QMC_EXPORT int qmc_batch_graph_run_dev(void *plan, double *ring, qmc_stream_t s);    /* same, hipGraph replay */
 QMC_EXPORT int qmc_batch_run_per_call_dev(void *plan, double *ring, qmc_stream_t s); /* one launch per request */
+/* Digests: qmc_batch_slabs() = sum of r; with qmc_batch_set_digest(plan, D) (D:
+ * that many device u64, zeroed by the caller; NULL turns it off) later runs add
+ * into D[slab] the wrap-around sum of the stored float64 bit patterns of each
+ * replication slab, so batched and per-call runs can be compared exactly at
+ * any size, whatever the ring placement. */
+QMC_EXPORT qmc_u64 qmc_batch_slabs(void *plan);
+QMC_EXPORT int qmc_batch_set_digest(void *plan, unsigned long long *digest);
 QMC_EXPORT void qmc_batch_lattice_inputs(qmc_u64 seed, int m, int d, int r, qmc_u64 *g, double *shifts);
 QMC_EXPORT void qmc_batch_dnb2_inputs(qmc_u64 seed, int method, int d, int r, qmc_u64 *C, qmc_u64 *S,
                                       qmc_u64 *shifts);

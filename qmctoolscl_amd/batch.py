@@ -39,6 +39,9 @@ def _declare(L):
     L.qmc_batch_total_tiles.argtypes = [_vp]
     L.qmc_batch_points.restype = ctypes.c_uint64
     L.qmc_batch_points.argtypes = [_vp]
+    L.qmc_batch_slabs.restype = ctypes.c_uint64
+    L.qmc_batch_slabs.argtypes = [_vp]
+    L.qmc_batch_set_digest.argtypes = [_vp, _vp]
     L.qmc_batch_slab_offset.restype = ctypes.c_uint64
     L.qmc_batch_slab_offset.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
     for f in ("qmc_batch_run_dev", "qmc_batch_graph_run_dev", "qmc_batch_run_per_call_dev"):
@@ -101,6 +104,26 @@ class Batch:
     @property
     def total_tiles(self):
         return int(_L().qmc_batch_total_tiles(self._h))
+
+    @property
+    def slabs(self):
+        return int(_L().qmc_batch_slabs(self._h))
+
+    def digests(self, ring, per_call=False, graph=True, stream=None):
+        """Run once and return the per-slab digests (wrap-around sums of the
+        stored float64 bit patterns), as a uint64 numpy array of length slabs."""
+        import torch
+        dg = torch.zeros(max(self.slabs, 1), dtype=torch.int64, device=ring.device)
+        _lib.check(_L().qmc_batch_set_digest(self._h, dg.data_ptr()))
+        try:
+            if per_call:
+                self.run_per_call(ring, stream)
+            else:
+                self.run(ring, graph=graph, stream=stream)
+            torch.cuda.synchronize(ring.device)
+        finally:
+            _lib.check(_L().qmc_batch_set_digest(self._h, None))
+        return dg[:self.slabs].cpu().numpy().view(np.uint64)
 
     def slab_offset(self, q, l):
         return int(_L().qmc_batch_slab_offset(self._h, int(q), int(l)))

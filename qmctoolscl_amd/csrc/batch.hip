@@ -31,6 +31,7 @@
 // every replication slab (n*d doubles) of every request is placed contiguously
 // at an offset chosen on the host, wrapping to the start when the ring is full,
 // so batches far larger than HBM can stream through a bounded buffer.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -43,6 +44,7 @@ enum { M_LATTICE = 0, M_DNB2_LMS = 1, M_DNB2_NUS = 2, M_GDN_LMS = 3 };
 
 constexpr int kTB = 32;     // base-2 digits (tmax) of the dnb2 methods
 constexpr int kMaxChainLog = 12;  // dnb2 chains start in the tile's first 2^12 rows at most
+constexpr int kMaxGroupLog = 8;   // dnb2 chains have at most 8 * 2^(8-1) = 1024 points
 constexpr int kTNmax = 52;  // base-b digits kept (2^-52 resolution)
 
 struct Req {
@@ -54,11 +56,12 @@ struct Req {
     u64 tab;             // word offset of the request's tables in the workspace
     u64 slabs;           // word offset of r slab bases (output element offsets)
     u32 clog;            // dnb2: log2 of the row stride between a chain's points
-    u32 gK;              // gdn: rows per chain
-    u32 gG, gCH;         // gdn: lanes per row (consecutive columns), chains per wave
-    u32 gng;             // gdn: column groups of gG
+    u32 gng;             // gdn: lane groups (ranges of consecutive columns)
+    u32 gunits;          // gdn: units of work per tile (sum over groups)
+    u64 ggrp;            // gdn: word offset (in the request's tables) of the group table
     u32 gtw;             // gdn: u32 words of one replication's digit tables
     u64 gtab;            // gdn: word offset (in the request's tables) of the replication tables
+    u32 slab0;           // index of the request's first replication among all slabs (digests)
     DivU32 ddiv;         // division by d
 };
 
@@ -198,26 +201,55 @@ __host__ __device__ __forceinline__ u64 recip_magic(u32 b) { return b == 1 ? 0 :
 // x mod b and x / b for x < 2^32 with M = ceil(2^64 / b) (exact, Lemire et al. 2019)
 __device__ __forceinline__ u32 fast_div(u32 x, u64 M) { return (u32)__umul64hi(M, (u64)x); }
 
-// lane groups of the gdn tile: gG consecutive columns; the group's widest
-// expansion is that of its smallest odd base
-inline u32 gdn_group_first_odd(const Req &q, int j) {
-    const int g0 = (j / (int)q.gG) * (int)q.gG;
-    return h_primes[g0 == 0 ? (q.d > 1 ? 1 : 0) : g0];
-}
-inline u32 gdn_group_max_base(const Req &q, int j) {
-    const int g1 = (j / (int)q.gG + 1) * (int)q.gG;
-    return h_primes[(g1 < q.d ? g1 : q.d) - 1];
-}
 inline u32 gdn_np(u32 b) { return b == 2 ? 0 : ((u32)gdn_tn(b) + 1) / 2; }
 
-inline u32 gdn_words(const Req &q, int j) {
+// Lane groups of a gdn tile: ranges [j0, j0 + gc) of consecutive columns, 64
+// per group when d > 32, else one group of d columns; a wave works on
+// ch = 64/gc chains of one group, every lane walking w digit pairs (the
+// group's widest expansion: its smallest odd base).  Chains are k rows long,
+// k a power of two giving about 8 units of work per tile.  (Finer groups by
+// cost -- narrower ranges for the small bases -- measured slower: more kernel
+// shapes and scattered row segments outweigh the saved digit work.)
+struct GdnGroup {
+    u32 j0, gc, ch, w, l, k, cb0, nu;
+};
+
+std::vector<GdnGroup> gdn_groups(int d, u32 RT) {
+    const u32 G = d > 32 ? 64 : (u32)d, ng = ((u32)d + G - 1) / G, CH = 64 / G;
+    const u64 want = (u64)RT * ng / ((u64)CH * 8);
+    u32 K = 1;
+    while (K * 2 <= want && K < 256) K *= 2;
+    if (K > RT) K = RT;
+    std::vector<GdnGroup> g(ng);
+    u32 cb0 = 0;
+    for (u32 i = 0; i < ng; i++) {
+        GdnGroup &x = g[i];
+        x.j0 = i * G;
+        x.gc = std::min<u32>(G, (u32)d - x.j0);
+        x.ch = CH;
+        u32 bmax = 0;
+        for (u32 j = x.j0; j < x.j0 + x.gc; j++) {
+            x.w = std::max(x.w, gdn_np(h_primes[j]));
+            bmax = std::max(bmax, h_primes[j]);
+        }
+        const double bm = (double)bmax;
+        x.l = bm * bm * bm * bm * bm * bm < 0x1p53 ? 3 : 2;
+        x.k = K;
+        x.nu = (RT + CH * K - 1) / (CH * K);
+        x.cb0 = cb0;
+        cb0 += x.nu;
+    }
+    return g;
+}
+
+inline u32 gdn_words(const Req &q, int j, u32 W) {
     const u32 b = h_primes[j];
     const u32 mm = (u32)gdn_mm(b, q.m);
-    const u32 w = b == 2 ? 2 + 2 * mm : (mm + 1) * gdn_np(gdn_group_first_odd(q, j));
+    const u32 w = b == 2 ? 2 + 2 * mm : (mm + 1) * W;
     return (w + 1) & ~1u;  // even: every block starts 8-byte aligned
 }
 
-u64 table_words(Req &q) {
+u64 table_words(Req &q, const std::vector<GdnGroup> &grp) {
     const u64 d = q.d, r = q.r;
     switch (q.method) {
         case M_LATTICE: return d + r * d;
@@ -225,9 +257,11 @@ u64 table_words(Req &q) {
         case M_DNB2_NUS: return 32 * d + r * d;
         default: {
             u32 w = 0;
-            for (int j = 0; j < q.d; j++) w += gdn_words(q, j);
+            for (const auto &x : grp)
+                for (u32 j = x.j0; j < x.j0 + x.gc; j++) w += gdn_words(q, (int)j, x.w);
             q.gtw = w;
-            q.gtab = (u64)kGJ * d;
+            q.ggrp = (u64)kGJ * d;
+            q.gtab = q.ggrp + 2 * grp.size();
             return q.gtab + r * (u64)(w / 2);
         }
     }
@@ -343,6 +377,25 @@ __device__ __forceinline__ u32 xor_cols(const u32 *__restrict__ cols, u64 q) {
 
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
+// Output of a tile: non-temporal stores, and with DIG a running wrap-around
+// sum of the stored bit patterns (an order-independent per-slab digest).
+template <bool DIG>
+struct Sink {
+    u64 acc = 0;
+    __device__ __forceinline__ void put(double *p, double v) {
+        store_stream(p, v);
+        if (DIG) acc += (u64)__double_as_longlong(v);
+    }
+    // one atomic per wave: every lane of the block works on the same slab
+    __device__ __forceinline__ void flush(unsigned long long *digest) {
+        if (!DIG) return;
+        u64 a = acc;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) a += (u64)__shfl_xor((long long)a, o, 64);
+        if ((threadIdx.x & 63) == 0) atomicAdd(digest, (unsigned long long)a);
+    }
+};
+
 __device__ __forceinline__ u16x2 as_pair(u32 w) { return __builtin_bit_cast(u16x2, w); }
 __device__ __forceinline__ u32 as_word(u16x2 p) { return __builtin_bit_cast(u32, p); }
 
@@ -362,9 +415,9 @@ __device__ __forceinline__ double dnb2_emit(u32 v, u64 xs, double lo) {
 // row index changes in bits clog..clog+c (c = ctz(g+1)), so the value changes by
 // pre[clog+c] ^ pre[clog-1] (pre = prefix XOR of the columns).  Steps come in
 // groups of 8 whose first 7 transitions use c = 0,1,0,2,0,1,0.
-template <bool LMS>
+template <bool LMS, bool DIG>
 __device__ __forceinline__ void dnb2_tile(const Req &q, u64 l, u64 row0, const u64 *__restrict__ tab,
-                                          double *__restrict__ slab, u32 *v0) {
+                                          double *__restrict__ slab, u32 *v0, Sink<DIG> &out) {
     const u32 d = (u32)q.d;
     const u32 s = q.clog;
     const u32 R = 1u << s, H = q.rows_per_tile >> s;
@@ -373,9 +426,20 @@ __device__ __forceinline__ void dnb2_tile(const Req &q, u64 l, u64 row0, const u
     const u32 *T32 = (const u32 *)tab;
     const u64 *extra = tab + 32 * nset + l * d;  // shifts or NUS seeds
     const u64 stride = (u64)R * d;
-    const u64 g0 = row0 >> s;
     // value of the tile's first row per column, shared by the column's chains
-    for (u32 j = threadIdx.x; j < d; j += blockDim.x) v0[j] = xor_cols(T32 + (set + j) * 64 + 32, row0);
+    // (all column loads issued at once: one memory round trip)
+    for (u32 j = threadIdx.x; j < d; j += blockDim.x) {
+        const u32 *cols = T32 + (set + j) * 64 + 32;
+        u32 v = 0;
+#pragma unroll 1
+        for (int b0 = 0; b0 < kTB; b0 += 8) {
+            if (!((row0 >> b0) & 0xff)) continue;
+#pragma unroll
+            for (int bit = b0; bit < b0 + 8; bit++)
+                if ((row0 >> bit) & 1) v ^= cols[bit];
+        }
+        v0[j] = v;
+    }
     __syncthreads();
     for (u32 sig = threadIdx.x; sig < R * d; sig += blockDim.x) {
         const u32 rr0 = div_q(sig, q.ddiv);
@@ -392,33 +456,46 @@ __device__ __forceinline__ void dnb2_tile(const Req &q, u64 l, u64 row0, const u
             if (bit < (int)s && ((rr0 >> bit) & 1)) v ^= cols[bit];
         double *dst = slab + (row0 + rr0) * d + j;
         if (H >= 8) {
-            const u32 x0 = pre[s] ^ base, x1 = pre[s + 1] ^ base, x2 = pre[s + 2] ^ base;
+            // every transition of the chain, in registers: after step h the row
+            // index flips bits s .. s+c with c = ctz(h+1) < log2(H) (the tile's
+            // first row is a multiple of H 2^s), so pre[s .. s+kMaxGroupLog+2]
+            // covers them all
+            u32 xk[3 + kMaxGroupLog];
+#pragma unroll
+            for (int k = 0; k < 3 + kMaxGroupLog; k++) xk[k] = (s + k < (u32)kTB) ? pre[s + k] ^ base : 0u;
+            const u32 x0 = xk[0], x1 = xk[1], x2 = xk[2];
             for (u32 h = 0; h < H; h += 8) {
-                // the transition after this group, loaded before the group's stores
-                const u32 next = h + 8 < H ? pre[s + 3 + __ffsll((long long)((g0 + h + 8) >> 3)) - 1] : base;
-                store_stream(dst, dnb2_emit<LMS>(v, xs, lo));
+                // the transition after this group: c = 3 + ctz((h + 8) / 8), uniform over the block
+                const int c8 = __ffs((int)((h + 8) >> 3)) - 1;
+                u32 next = xk[3];
+#pragma unroll
+                for (int k = 1; k < kMaxGroupLog; k++)
+                    if (c8 == k) next = xk[3 + k];
+                out.put(dst, dnb2_emit<LMS>(v, xs, lo));
                 v ^= x0;
-                store_stream(dst + stride, dnb2_emit<LMS>(v, xs, lo));
+                out.put(dst + stride, dnb2_emit<LMS>(v, xs, lo));
                 v ^= x1;
-                store_stream(dst + 2 * stride, dnb2_emit<LMS>(v, xs, lo));
+                out.put(dst + 2 * stride, dnb2_emit<LMS>(v, xs, lo));
                 v ^= x0;
-                store_stream(dst + 3 * stride, dnb2_emit<LMS>(v, xs, lo));
+                out.put(dst + 3 * stride, dnb2_emit<LMS>(v, xs, lo));
                 v ^= x2;
-                store_stream(dst + 4 * stride, dnb2_emit<LMS>(v, xs, lo));
+                out.put(dst + 4 * stride, dnb2_emit<LMS>(v, xs, lo));
                 v ^= x0;
-                store_stream(dst + 5 * stride, dnb2_emit<LMS>(v, xs, lo));
+                out.put(dst + 5 * stride, dnb2_emit<LMS>(v, xs, lo));
                 v ^= x1;
-                store_stream(dst + 6 * stride, dnb2_emit<LMS>(v, xs, lo));
+                out.put(dst + 6 * stride, dnb2_emit<LMS>(v, xs, lo));
                 v ^= x0;
-                store_stream(dst + 7 * stride, dnb2_emit<LMS>(v, xs, lo));
+                out.put(dst + 7 * stride, dnb2_emit<LMS>(v, xs, lo));
                 dst += 8 * stride;
-                v ^= next ^ base;
+                v ^= next;
             }
         } else {
+            // H <= 4: transitions c = ctz(h+1) <= 1
+            const u32 x0 = pre[s] ^ base, x1 = H > 1 ? pre[s + 1] ^ base : 0u;
             for (u32 h = 0;; h++) {
-                store_stream(dst, dnb2_emit<LMS>(v, xs, lo));
+                out.put(dst, dnb2_emit<LMS>(v, xs, lo));
                 if (h + 1 == H) break;
-                v ^= pre[s + __ffsll((long long)(g0 + h + 1)) - 1] ^ base;
+                v ^= (h & 1) ? x1 : x0;
                 dst += stride;
             }
         }
@@ -433,7 +510,9 @@ __device__ __forceinline__ u32 pk_add_mod(u32 z, u32 q, u16x2 bb) {
 }
 
 // One gdn chain of a base-2 column: len consecutive points from k, row stride d.
-__device__ __forceinline__ void gdn_chain2(const u64 *__restrict__ Q, u32 k, u32 len, u32 d, double *dst) {
+template <bool DIG>
+__device__ __forceinline__ void gdn_chain2(const u64 *__restrict__ Q, u32 k, u32 len, u32 d, double *dst,
+                                           Sink<DIG> &out) {
     // Q = cmask, Q_0..Q_{mm-1}; point k's digits: cmask ^ XOR of columns S_i = Q_i ^ Q_{i-1}
     u64 z = Q[0];
     for (u32 kk = k; kk; kk &= kk - 1) {
@@ -441,7 +520,7 @@ __device__ __forceinline__ void gdn_chain2(const u64 *__restrict__ Q, u32 k, u32
         z ^= Q[1 + i] ^ (i ? Q[i] : 0ull);
     }
     for (u32 h = 0;; h++) {
-        store_stream(dst, (double)z * 0x1p-52);  // z < 2^52: exact, = z / 2^tn
+        out.put(dst, (double)z * 0x1p-52);  // z < 2^52: exact, = z / 2^tn
         if (h + 1 == len) break;
         z ^= Q[__ffs(k + 1)];  // Q_c at Q[1 + c], c = ctz(k + 1)
         k++;
@@ -453,9 +532,9 @@ __device__ __forceinline__ void gdn_chain2(const u64 *__restrict__ Q, u32 k, u32
 // row stride d.  T = the column's digit tables in LDS, W pairs per row (the
 // lane group's width, as a template so every pair lives in a named register);
 // L = pairs in the low part of the floating-point Horner split.
-template <int W, int L>
+template <int W, int L, bool DIG>
 __device__ __forceinline__ void gdn_chain(const u64 *__restrict__ gj, const u32 *T, u32 k, u32 len, u32 d,
-                                          double *dst) {
+                                          double *dst, Sink<DIG> &out) {
     const u64 w1 = gj[1];
     const u32 b = (u32)gj[0] & 0xffffu;
     const u32 mm = (u32)((w1 >> 8) & 0xff);
@@ -514,7 +593,7 @@ __device__ __forceinline__ void gdn_chain(const u64 *__restrict__ gj, const u32 
             else
                 Ylo = __fma_rn(Ylo, bd2, pv);
         }
-        store_stream(dst, __fma_rn(Yhi, BL, Ylo) / scale);
+        out.put(dst, __fma_rn(Yhi, BL, Ylo) / scale);
         if (h + 1 == len) break;
         // carries of k -> k+1: c trailing digits equal to b-1
         u32 c = 0;
@@ -547,13 +626,15 @@ __device__ __forceinline__ void gdn_chain(const u64 *__restrict__ gj, const u32 
 // heavy digit walks).  Each class is compiled without the other's code.
 __host__ __device__ constexpr int method_class(int method) { return method == M_GDN_LMS ? 1 : 0; }
 
-template <int CLASS>
-__device__ void run_tile(const Req &q, u64 t, const u64 *__restrict__ ws, double *__restrict__ out, u32 *lds) {
+template <int CLASS, bool DIG>
+__device__ void run_tile(const Req &q, u64 t, const u64 *__restrict__ ws, double *__restrict__ ring, u32 *lds,
+                         unsigned long long *digest) {
     const u64 d = q.d;
     const u64 l = t / q.tiles_per_l;
     const u64 row0 = (t - l * q.tiles_per_l) * (u64)q.rows_per_tile;
     const u64 *tab = ws + q.tab;
-    double *slab = out + ws[q.slabs + l];
+    double *slab = ring + ws[q.slabs + l];
+    Sink<DIG> out;
     if (CLASS == 0 && q.method == M_LATTICE) {
         const u32 pts = q.rows_per_tile * (u32)d;
         const u64 *g = tab;
@@ -564,58 +645,64 @@ __device__ void run_tile(const Req &q, u64 t, const u64 *__restrict__ ws, double
             const u64 row = row0 + rr;
             const double phi = (double)__brevll(row) * 0x1p-64;  // phi2(row), exact
             const double x = frac_mod1((double)g[j] * phi);      // lat_gen_natural.c:45
-            store_stream(slab + row * d + j, frac_mod1(x + sh[j]));  // lat_shift_mod_1.c:31
+            out.put(slab + row * d + j, frac_mod1(x + sh[j]));  // lat_shift_mod_1.c:31
         }
     } else if (CLASS == 0 && q.method == M_DNB2_LMS) {
-        dnb2_tile<true>(q, l, row0, tab, slab, lds);
+        dnb2_tile<true>(q, l, row0, tab, slab, lds, out);
     } else if (CLASS == 0) {
-        dnb2_tile<false>(q, l, row0, tab, slab, lds);
+        dnb2_tile<false>(q, l, row0, tab, slab, lds, out);
     } else {
         // this replication's digit tables into LDS
         const u32 *src = (const u32 *)(tab + q.gtab) + l * q.gtw;
         for (u32 e = threadIdx.x; e < q.gtw; e += blockDim.x) lds[e] = src[e];
         __syncthreads();
-        // units = (column group, block of gCH chains of gK rows); lanes = (chain, column)
-        const u32 G = q.gG, CH = q.gCH, K = q.gK, ng = q.gng;
+        // units = (lane group, block of ch chains of k rows); lanes = (chain, column)
+        const u64 *grp = tab + q.ggrp;  // per group: j0 | gc<<16 | ch<<24 | w<<32 | l<<40,  k | cb0<<32 | nu<<48
         const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
-        const u32 jj = lane % G, ch = lane / G;
-        const u32 cbs = (q.rows_per_tile + CH * K - 1) / (CH * K);
-        for (u32 u = wave; u < ng * cbs; u += nwaves) {
-            const u32 jg = u % ng, cb = u / ng;
-            // the group's widest expansion (smallest odd base): row width and Horner split
-            const u32 jodd = jg * G + (jg == 0 && d > 1 ? 1u : 0u);
-            const u32 Ww = __builtin_amdgcn_readfirstlane((u32)((tab[kGJ * jodd + 1] >> 24) & 0xff));
-            const u32 Lw = __builtin_amdgcn_readfirstlane((u32)(tab[kGJ * jodd] >> 16) & 0xffffu);
-            const u32 j = jg * G + jj;
+        for (u32 u = wave; u < q.gunits; u += nwaves) {
+            u32 g = 0;
+            u64 g1 = grp[1];
+            while (u >= ((g1 >> 32) & 0xffff) + (g1 >> 48)) g1 = grp[2 * (++g) + 1];
+            const u64 g0 = grp[2 * g];
+            const u32 j0 = __builtin_amdgcn_readfirstlane((u32)(g0 & 0xffff));
+            const u32 gc = __builtin_amdgcn_readfirstlane((u32)((g0 >> 16) & 0xff));
+            const u32 CH = __builtin_amdgcn_readfirstlane((u32)((g0 >> 24) & 0xff));
+            const u32 Ww = __builtin_amdgcn_readfirstlane((u32)((g0 >> 32) & 0xff));
+            const u32 Lw = __builtin_amdgcn_readfirstlane((u32)((g0 >> 40) & 0xff));
+            const u32 K = __builtin_amdgcn_readfirstlane((u32)g1);
+            const u32 cb = u - __builtin_amdgcn_readfirstlane((u32)((g1 >> 32) & 0xffff));
+            const u32 ch = lane / gc, jj = lane - ch * gc;
+            const u32 j = j0 + jj;
             const u32 rs = cb * CH * K + ch * K;
-            if (ch >= CH || j >= d || rs >= q.rows_per_tile) continue;
+            if (ch >= CH || rs >= q.rows_per_tile) continue;
             const u32 len = q.rows_per_tile - rs < K ? q.rows_per_tile - rs : K;
             const u64 k = row0 + rs;
             const u64 *gj = tab + kGJ * j;
             const u32 *T = lds + (u32)(gj[0] >> 32);
             double *dst = slab + k * d + j;
             if (j == 0) {
-                gdn_chain2((const u64 *)T, (u32)k, len, (u32)d, dst);
-            } else if (Ww == 17 && Lw == 3) {
-                gdn_chain<17, 3>(gj, T, (u32)k, len, (u32)d, dst);
-            } else if (Ww == 4 && Lw == 3) {
-                gdn_chain<4, 3>(gj, T, (u32)k, len, (u32)d, dst);
-            } else if (Ww == 4 && Lw == 2) {
-                gdn_chain<4, 2>(gj, T, (u32)k, len, (u32)d, dst);
-            } else {
-                gdn_chain<3, 2>(gj, T, (u32)k, len, (u32)d, dst);  // the host admits no other shape
+                gdn_chain2((const u64 *)T, (u32)k, len, (u32)d, dst, out);
+                continue;
             }
+#define QMC_GDN_SHAPE(W_, L_) \
+    if (Ww == W_ && Lw == L_) { gdn_chain<W_, L_>(gj, T, (u32)k, len, (u32)d, dst, out); continue; }
+            // every (w, l) a group can have (host-checked): 17 for the group of
+            // base 3, 4 or 3 for groups of bases above 64 columns' worth of primes
+            QMC_GDN_SHAPE(17, 3) QMC_GDN_SHAPE(4, 3) QMC_GDN_SHAPE(4, 2) QMC_GDN_SHAPE(3, 2)
+#undef QMC_GDN_SHAPE
         }
     }
+    out.flush(digest + q.slab0 + l);
 }
 
 // ------------------------------------------------------------ the kernels
 
-template <int CLASS>
+template <int CLASS, bool DIG>
 __global__ __launch_bounds__(kBlock) void batch_persistent_kernel(const Req *__restrict__ reqs, int nreq,
                                                                   u64 total_tiles, unsigned long long *counter,
                                                                   const u64 *__restrict__ ws,
-                                                                  double *__restrict__ out) {
+                                                                  double *__restrict__ out,
+                                                                  unsigned long long *digest) {
     __shared__ u64 s_tile;
     __shared__ int s_req;
     extern __shared__ u32 lds_tables[];  // dnb2 first-row values or gdn digit tables of the current tile
@@ -639,7 +726,7 @@ __global__ __launch_bounds__(kBlock) void batch_persistent_kernel(const Req *__r
         __syncthreads();
         if (t >= total_tiles) break;  // every wave of every block reaches this
         const Req q = reqs[qi];
-        run_tile<CLASS>(q, t - q.tile0, ws, out, lds_tables);
+        run_tile<CLASS, DIG>(q, t - q.tile0, ws, out, lds_tables, digest);
     }
 }
 
@@ -649,12 +736,12 @@ __global__ void reset_counter_kernel(unsigned long long *counter) {
 }
 
 // one request, one tile per block (the per-call form)
-template <int CLASS>
+template <int CLASS, bool DIG>
 __global__ __launch_bounds__(kBlock) void batch_single_kernel(const Req *__restrict__ req, const u64 *__restrict__ ws,
-                                                              double *__restrict__ out) {
+                                                              double *__restrict__ out, unsigned long long *digest) {
     extern __shared__ u32 lds_tables[];
     const Req q = *req;
-    run_tile<CLASS>(q, blockIdx.x, ws, out, lds_tables);
+    run_tile<CLASS, DIG>(q, blockIdx.x, ws, out, lds_tables, digest);
 }
 
 // ----------------------------------------------------------- host plan
@@ -666,6 +753,7 @@ u32 dnb2_chain_log(u32 RT, u32 d) {
     u32 best = 0;
     u64 best_cost = ~0ull;
     for (u32 s = 0; (1u << s) <= RT && s < kMaxChainLog; s++) {
+        if ((RT >> s) > (8u << (kMaxGroupLog - 1))) continue;  // chains of at most 1024 points
         const u64 U = (((u64)d << s) + kBlock - 1) / kBlock;
         const u64 cost = U * ((RT >> s) + 3);
         if (cost < best_cost) {
@@ -674,20 +762,6 @@ u32 dnb2_chain_log(u32 RT, u32 d) {
         }
     }
     return best;
-}
-
-// gdn lanes: 64 consecutive columns per wave when d > 32, else gCH = 64/d
-// chains of d columns; chains of gK rows, about 8 units of work per tile
-void gdn_shape(Req &q) {
-    const u32 d = (u32)q.d;
-    q.gG = d > 32 ? 64 : d;
-    q.gCH = 64 / q.gG;
-    q.gng = (d + q.gG - 1) / q.gG;
-    u64 want = (u64)q.rows_per_tile * q.gng / ((u64)q.gCH * 8);
-    u32 K = 1;
-    while (K * 2 <= want && K < 256) K *= 2;
-    if (K > q.rows_per_tile) K = q.rows_per_tile;
-    q.gK = K;
 }
 
 struct Plan {
@@ -705,21 +779,52 @@ struct Plan {
     int grid = 0;
     int device = 0;
     u32 lds_bytes = 0;               // dynamic LDS of the gdn kernel (digit tables)
+    int cus = 256;
     hipStream_t cap = nullptr;       // private stream for graph capture
+    hipStream_t side = nullptr;      // the gdn kernel runs here, beside the store-bound one
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipGraphExec_t gexec = nullptr;  // [counter reset, persistent kernel]
     double *gring = nullptr;         // ring the graph was captured for
+    unsigned long long *digest = nullptr;   // per-slab digests, or null
+    unsigned long long *gdigest = nullptr;  // digest pointer the graph was captured with
+    u64 nslabs = 0;
 };
 
 constexpr u32 kLdsClass0 = 256 * sizeof(u32);  // dnb2 first-row values
 
-void launch_batch(Plan *P, double *ring, hipStream_t s) {
+// The two classes run concurrently when both have tiles: the store-bound
+// kernel on s with the full grid, the gdn kernel on the plan's side stream
+// with two workgroups per CU, so both stay resident together (28 + 2 x ~120
+// VGPRs per SIMD lane) and the digit walks fill the ALUs while the other
+// kernel waits on HBM.  Fork and join are events, so a graph capture of s
+// takes in both branches.
+template <bool DIG>
+int launch_batch_t(Plan *P, double *ring, hipStream_t s) {
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, P->d_counter);
+    const bool both = P->qtiles[0] && P->qtiles[1];
+    if (both) {
+        QMC_HIP_CHECK(hipEventRecord(P->ev_fork, s));
+        QMC_HIP_CHECK(hipStreamWaitEvent(P->side, P->ev_fork, 0));
+        const int g1 = P->grid < 2 * P->cus ? P->grid : 2 * P->cus;
+        hipLaunchKernelGGL((batch_persistent_kernel<1, DIG>), dim3(g1), dim3(kBlock), P->lds_bytes, P->side,
+                           P->d_qreqs + P->qn[0], P->qn[1], P->qtiles[1], P->d_counter + 1, P->d_ws, ring,
+                           P->digest);
+        QMC_HIP_CHECK(hipEventRecord(P->ev_join, P->side));
+    } else if (P->qtiles[1]) {
+        hipLaunchKernelGGL((batch_persistent_kernel<1, DIG>), dim3(P->grid), dim3(kBlock), P->lds_bytes, s,
+                           P->d_qreqs + P->qn[0], P->qn[1], P->qtiles[1], P->d_counter + 1, P->d_ws, ring,
+                           P->digest);
+    }
     if (P->qtiles[0])
-        hipLaunchKernelGGL(batch_persistent_kernel<0>, dim3(P->grid), dim3(kBlock), kLdsClass0, s, P->d_qreqs,
-                           P->qn[0], P->qtiles[0], P->d_counter, P->d_ws, ring);
-    if (P->qtiles[1])
-        hipLaunchKernelGGL(batch_persistent_kernel<1>, dim3(P->grid), dim3(kBlock), P->lds_bytes, s,
-                           P->d_qreqs + P->qn[0], P->qn[1], P->qtiles[1], P->d_counter + 1, P->d_ws, ring);
+        hipLaunchKernelGGL((batch_persistent_kernel<0, DIG>), dim3(P->grid), dim3(kBlock), kLdsClass0, s, P->d_qreqs,
+                           P->qn[0], P->qtiles[0], P->d_counter, P->d_ws, ring, P->digest);
+    if (both) QMC_HIP_CHECK(hipStreamWaitEvent(s, P->ev_join, 0));
+    QMC_HIP_CHECK(hipGetLastError());
+    return QMC_OK;
+}
+
+int launch_batch(Plan *P, double *ring, hipStream_t s) {
+    return P->digest ? launch_batch_t<true>(P, ring, s) : launch_batch_t<false>(P, ring, s);
 }
 
 }  // namespace
@@ -736,6 +841,7 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
     Plan *P = new Plan();
     P->ring_elems = ring_elems;
     u64 tile = 0, words = 0;
+    std::vector<std::vector<GdnGroup>> groups(nreq);
     for (int i = 0; i < nreq; i++) {
         Req q{};
         q.method = methods[i];
@@ -753,7 +859,7 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
         }
         const u64 n = 1ull << q.m;
         u64 rows = points_per_tile / (u64)q.d;
-        if (q.method == M_GDN_LMS && rows < 1024) rows = 1024;  // digit tables are loaded once per tile
+        if (q.method == M_GDN_LMS && rows < 1024) rows = 1024;  // tables loaded once per tile; long chains
         u64 p2 = 1;
         while (p2 * 2 <= rows) p2 *= 2;
         if (p2 > n) p2 = n;
@@ -763,9 +869,13 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
         tile += (u64)q.r * q.tiles_per_l;
         q.ddiv = make_div((u32)q.d);
         if (q.method == M_DNB2_LMS || q.method == M_DNB2_NUS) q.clog = dnb2_chain_log(q.rows_per_tile, (u32)q.d);
-        if (q.method == M_GDN_LMS) gdn_shape(q);
+        if (q.method == M_GDN_LMS) {
+            groups[i] = gdn_groups(q.d, q.rows_per_tile);
+            q.gng = (u32)groups[i].size();
+            q.gunits = groups[i].back().cb0 + groups[i].back().nu;
+        }
         q.tab = words;
-        words += table_words(q);
+        words += table_words(q, groups[i]);
         if (q.method == M_GDN_LMS) {
             if ((u64)q.gtw * 4 > 65536) {
                 qmc_fail(QMC_ERR_ARG, "batch: gdn request %d needs %u bytes of digit tables (max 65536)", i,
@@ -777,6 +887,8 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
         }
         q.slabs = words;
         words += q.r;
+        q.slab0 = (u32)P->nslabs;
+        P->nslabs += (u64)q.r;
         P->reqs.push_back(q);
         const u64 slab = n * (u64)q.d;
         if (slab > ring_elems) {
@@ -800,37 +912,56 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
     if (hipGetDevice(&P->device) != hipSuccess) { delete P; qmc_fail(QMC_ERR_HIP, "batch: no device"); return nullptr; }
     int cus = 256;
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, P->device);
+    P->cus = cus;
     P->grid = grid > 0 ? grid : 4 * cus;
+    if (hipStreamCreateWithFlags(&P->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&P->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&P->ev_join, hipEventDisableTiming) != hipSuccess) {
+        qmc_fail(QMC_ERR_HIP, "batch: stream or event creation failed");
+        delete P;
+        return nullptr;
+    }
     std::vector<u64> ws_host(words, 0);
-    // gdn dimension info (the digit tables themselves are built on the device)
-    for (auto &q : P->reqs) {
+    // gdn dimension info and lane groups (the digit tables are built on the device)
+    for (size_t qi = 0; qi < P->reqs.size(); qi++) {
+        const Req &q = P->reqs[qi];
         if (q.method != M_GDN_LMS) continue;
         u32 qoff = 0;
-        for (int j = 0; j < q.d; j++) {
-            const u32 b = h_primes[j];
-            const u64 tn = (u64)gdn_tn(b), mm = (u64)gdn_mm(b, q.m), np = gdn_np(b);
-            const u64 W = gdn_np(gdn_group_first_odd(q, j));
-            // Horner split: Lw low pairs, (b^2)^Lw < 2^53 for the group's largest base
-            const double bmax = (double)gdn_group_max_base(q, j);
-            const u64 L = bmax * bmax * bmax * bmax * bmax * bmax < 0x1p53 ? 3 : 2;
-            const bool shape_ok = (W == 17 && L == 3) || (W == 4 && L == 3) || (W == 4 && L == 2) ||
-                                  (W == 3 && L == 2) || b == 2;
-            if (!shape_ok || (b != 2 && !((double)b * b * b * b < 0x1p53 &&
-                                          std::pow((double)b * b, (double)(np > L ? np - L : 0)) < 0x1p53))) {
-                qmc_fail(QMC_ERR_ARG, "batch: gdn base %u has no kernel shape (W=%u, L=%u)", b, (u32)W, (u32)L);
+        for (size_t gi = 0; gi < groups[qi].size(); gi++) {
+            const GdnGroup &x = groups[qi][gi];
+            u64 *gw = ws_host.data() + q.tab + q.ggrp + 2 * gi;
+            const bool shape_ok = x.w == 0 || (x.w == 17 && x.l == 3) || (x.w == 4 && (x.l == 3 || x.l == 2)) ||
+                                  (x.w == 3 && x.l == 2);
+            if (!shape_ok) {
+                qmc_fail(QMC_ERR_ARG, "batch: gdn lane group with no kernel shape (w=%u, l=%u)", x.w, x.l);
                 delete P;
                 return nullptr;
             }
-            double scale = 1.0;
-            for (u64 t = 0; t < tn; t++) scale *= (double)b;
-            u64 sbits;
-            std::memcpy(&sbits, &scale, sizeof sbits);
-            u64 *ji = ws_host.data() + q.tab + (u64)kGJ * j;
-            ji[0] = b | (L << 16) | ((u64)qoff << 32);
-            ji[1] = tn | (mm << 8) | (np << 16) | (W << 24) | ((u64)b * b << 32);
-            ji[2] = recip_magic(b);
-            ji[3] = sbits;
-            qoff += gdn_words(q, j);
+            gw[0] = x.j0 | (x.gc << 16) | (x.ch << 24) | ((u64)x.w << 32) | ((u64)x.l << 40);
+            gw[1] = x.k | ((u64)x.cb0 << 32) | ((u64)x.nu << 48);
+            for (u32 j = x.j0; j < x.j0 + x.gc; j++) {
+                const u32 b = h_primes[j];
+                const u64 tn = (u64)gdn_tn(b), mm = (u64)gdn_mm(b, q.m), np = gdn_np(b);
+                const u64 W = x.w, L = x.l;
+                // Horner halves exact: (b^2)^L and (b^2)^(np-L) below 2^53
+                if (b != 2 && !((double)b * b * b * b < 0x1p53 && (L == 2 || std::pow((double)b, 6.0) < 0x1p53) &&
+                                std::pow((double)b * b, (double)(np > L ? np - L : 0)) < 0x1p53)) {
+                    qmc_fail(QMC_ERR_ARG, "batch: gdn base %u does not split into exact halves (W=%u, L=%u)", b,
+                             (u32)W, (u32)L);
+                    delete P;
+                    return nullptr;
+                }
+                double scale = 1.0;
+                for (u64 t = 0; t < tn; t++) scale *= (double)b;
+                u64 sbits;
+                std::memcpy(&sbits, &scale, sizeof sbits);
+                u64 *ji = ws_host.data() + q.tab + (u64)kGJ * j;
+                ji[0] = b | (L << 16) | ((u64)qoff << 32);
+                ji[1] = tn | (mm << 8) | (np << 16) | (W << 24) | ((u64)b * b << 32);
+                ji[2] = recip_magic(b);
+                ji[3] = sbits;
+                qoff += gdn_words(q, (int)j, x.w);
+            }
         }
     }
     // slab bases in the ring: consecutive, wrapping when a slab would run past the end
@@ -871,6 +1002,9 @@ QMC_EXPORT void qmc_batch_free(void *h) {
     if (!P) return;
     if (P->gexec) hipGraphExecDestroy(P->gexec);
     if (P->cap) hipStreamDestroy(P->cap);
+    if (P->side) hipStreamDestroy(P->side);
+    if (P->ev_fork) hipEventDestroy(P->ev_fork);
+    if (P->ev_join) hipEventDestroy(P->ev_join);
     hipFree(P->d_reqs);
     hipFree(P->d_ws);
     hipFree(P->d_counter);
@@ -879,6 +1013,19 @@ QMC_EXPORT void qmc_batch_free(void *h) {
 }
 
 QMC_EXPORT u64 qmc_batch_total_tiles(void *h) { return ((Plan *)h)->total_tiles; }
+
+// Number of replication slabs (sum of r over the requests): the length of a digest array.
+QMC_EXPORT u64 qmc_batch_slabs(void *h) { return ((Plan *)h)->nslabs; }
+
+// Make later runs of the plan add, for every replication slab, the wrap-around
+// sum of the stored float64 bit patterns into digest[slab] (device memory of
+// qmc_batch_slabs() u64, zeroed by the caller); NULL turns it off.  The sum is
+// order-independent, so batched and per-call runs of a plan give equal digests
+// exactly when they write equal points, whatever the ring placement.
+QMC_EXPORT int qmc_batch_set_digest(void *h, unsigned long long *digest) {
+    ((Plan *)h)->digest = digest;
+    return QMC_OK;
+}
 QMC_EXPORT u64 qmc_batch_points(void *h) {
     u64 p = 0;
     for (auto &q : ((Plan *)h)->reqs) p += (u64)q.r * (1ull << q.m) * (u64)q.d;
@@ -897,9 +1044,7 @@ QMC_EXPORT u64 qmc_batch_slab_offset(void *h, int q, int l) {
 QMC_EXPORT int qmc_batch_run_dev(void *h, double *ring, qmc_stream_t s) {
     Plan *P = (Plan *)h;
     if (!P->total_tiles) return QMC_OK;
-    launch_batch(P, ring, s);
-    QMC_HIP_CHECK(hipGetLastError());
-    return QMC_OK;
+    return launch_batch(P, ring, s);
 }
 
 // The same launch replayed from a hipGraph (captured once per ring on a
@@ -908,7 +1053,7 @@ QMC_EXPORT int qmc_batch_run_dev(void *h, double *ring, qmc_stream_t s) {
 QMC_EXPORT int qmc_batch_graph_run_dev(void *h, double *ring, qmc_stream_t s) {
     Plan *P = (Plan *)h;
     if (!P->total_tiles) return QMC_OK;
-    if (!P->gexec || P->gring != ring) {
+    if (!P->gexec || P->gring != ring || P->gdigest != P->digest) {
         if (P->gexec) {
             QMC_HIP_CHECK(hipGraphExecDestroy(P->gexec));
             P->gexec = nullptr;
@@ -916,11 +1061,14 @@ QMC_EXPORT int qmc_batch_graph_run_dev(void *h, double *ring, qmc_stream_t s) {
         if (!P->cap) QMC_HIP_CHECK(hipStreamCreateWithFlags(&P->cap, hipStreamNonBlocking));
         hipGraph_t g = nullptr;
         QMC_HIP_CHECK(hipStreamBeginCapture(P->cap, hipStreamCaptureModeThreadLocal));
-        launch_batch(P, ring, P->cap);
-        QMC_HIP_CHECK(hipStreamEndCapture(P->cap, &g));
+        const int rc = launch_batch(P, ring, P->cap);
+        const hipError_t ec = hipStreamEndCapture(P->cap, &g);
+        if (rc) return rc;
+        QMC_HIP_CHECK(ec);
         QMC_HIP_CHECK(hipGraphInstantiate(&P->gexec, g, nullptr, nullptr, 0));
         QMC_HIP_CHECK(hipGraphDestroy(g));
         P->gring = ring;
+        P->gdigest = P->digest;
     }
     QMC_HIP_CHECK(hipGraphLaunch(P->gexec, s));
     return QMC_OK;
@@ -932,12 +1080,23 @@ QMC_EXPORT int qmc_batch_run_per_call_dev(void *h, double *ring, qmc_stream_t s)
     for (size_t i = 0; i < P->reqs.size(); i++) {
         const Req &q = P->reqs[i];
         const u64 tiles = (u64)q.r * q.tiles_per_l;
-        if (method_class(q.method) == 0)
-            hipLaunchKernelGGL(batch_single_kernel<0>, dim3((unsigned)tiles), dim3(kBlock), kLdsClass0, s,
-                               P->d_reqs + i, P->d_ws, ring);
-        else
-            hipLaunchKernelGGL(batch_single_kernel<1>, dim3((unsigned)tiles), dim3(kBlock), P->lds_bytes, s,
-                               P->d_reqs + i, P->d_ws, ring);
+        const bool c0 = method_class(q.method) == 0;
+        const unsigned lds = c0 ? kLdsClass0 : P->lds_bytes;
+        if (P->digest) {
+            if (c0)
+                hipLaunchKernelGGL((batch_single_kernel<0, true>), dim3((unsigned)tiles), dim3(kBlock), lds, s,
+                                   P->d_reqs + i, P->d_ws, ring, P->digest);
+            else
+                hipLaunchKernelGGL((batch_single_kernel<1, true>), dim3((unsigned)tiles), dim3(kBlock), lds, s,
+                                   P->d_reqs + i, P->d_ws, ring, P->digest);
+        } else {
+            if (c0)
+                hipLaunchKernelGGL((batch_single_kernel<0, false>), dim3((unsigned)tiles), dim3(kBlock), lds, s,
+                                   P->d_reqs + i, P->d_ws, ring, P->digest);
+            else
+                hipLaunchKernelGGL((batch_single_kernel<1, false>), dim3((unsigned)tiles), dim3(kBlock), lds, s,
+                                   P->d_reqs + i, P->d_ws, ring, P->digest);
+        }
     }
     QMC_HIP_CHECK(hipGetLastError());
     return QMC_OK;

@@ -206,6 +206,66 @@ __global__ __launch_bounds__(kBlock) void gdn_digitwise_kernel(Rows g, u64 r_x, 
     if (bad) flag(status, bad);
 }
 
+// Digit permutation with one column's tables in LDS: a block takes column j
+// of replication l for kPermPts consecutive points, so its gathers
+// perms[l][j][t][digit] hit a u32 copy of that column's tmax_new x bmax table
+// instead of L2.  Values of 2^32 or more, and digits >= bmax (which the
+// reference reads from the following rows), take the global table.
+constexpr int kPermPts = 1024;
+constexpr u64 kPermLdsEntries = 12288;  // 48 KiB of u32
+
+__global__ __launch_bounds__(kBlock) void gdn_perm_lds_kernel(u64 n, u64 d, u64 r_x, u64 tmax, u64 tmax_new,
+                                                              u64 bmax, const u64 *__restrict__ perms, u64 tab_total,
+                                                              const u64 *__restrict__ xdig,
+                                                              u64 *__restrict__ xnew, u32 *status) {
+    extern __shared__ u32 sperm[];
+    __shared__ int s_big;
+    const u64 l = blockIdx.y;
+    const u64 j = blockIdx.x % d;
+    const u64 i0 = (blockIdx.x / d) * (u64)kPermPts;
+    const u64 pbase = (l * d + j) * tmax_new * bmax;
+    const u32 entries = (u32)(tmax_new * bmax);
+    if (threadIdx.x == 0) s_big = 0;
+    __syncthreads();
+    int big = 0;
+    for (u32 e = threadIdx.x; e < entries; e += blockDim.x) {
+        const u64 v = perms[pbase + e];
+        sperm[e] = (u32)v;
+        big |= (v >> 32) != 0;
+    }
+    if (big) atomicOr(&s_big, 1);
+    __syncthreads();
+    const bool wide = s_big != 0;
+    const u32 T = (u32)tmax_new;
+    const u64 npts = (n - i0) < (u64)kPermPts ? (n - i0) : (u64)kPermPts;
+    const u32 total = (u32)npts * T;
+    const DivU32 tdiv = make_div(T);
+    const u64 *src = xdig + ((l % r_x) * n * d + j) * tmax;
+    u64 *out = xnew + (l * n * d + j) * tmax_new;
+    u32 bad = 0;
+#pragma unroll 4
+    for (u32 f = threadIdx.x; f < total; f += blockDim.x) {
+        const u32 ii = div_q(f, tdiv);
+        const u32 t = f - ii * T;
+        const u64 i = i0 + ii;
+        const u64 p = t < tmax ? src[i * d * tmax + t] : 0;  // gdn_digital_permutation.c:36-42
+        u64 v;
+        if (p < bmax && !wide) {
+            v = sperm[t * bmax + p];
+        } else {
+            const u64 pi = pbase + (u64)t * bmax + p;
+            if (pi < tab_total) {
+                v = perms[pi];
+            } else {
+                v = 0;
+                bad |= STATUS_OOB;
+            }
+        }
+        store_stream(out + i * d * tmax_new + t, v);
+    }
+    if (bad) flag(status, bad);
+}
+
 // -------------------------------------------------------------- to double
 
 // gdn_integer_to_float.c:32-40: v += recip*digit; recip /= b, sequential per
@@ -371,7 +431,13 @@ static int digitwise(bool permute, const char *who, u64 r, u64 n, u64 d, u64 r_x
     if (!status) return QMC_ERR_HIP;
     Rows g = make_rows(n, d, tmax_new);
     dim3 grid((unsigned)((g.nd + kRows - 1) / kRows), (unsigned)r);
-    if (permute)
+    if (permute && tmax_new * bmax <= kPermLdsEntries && bmax) {
+        const u64 chunks = (n + kPermPts - 1) / kPermPts;
+        if (chunks * d > 0x7fffffffull) return qmc_fail(QMC_ERR_ARG, "%s: n*d too large", who);
+        hipLaunchKernelGGL(gdn_perm_lds_kernel, dim3((unsigned)(chunks * d), (unsigned)r), dim3(kBlock),
+                           sizeof(u32) * tmax_new * bmax, s, n, d, r_x, tmax, tmax_new, bmax, tab,
+                           r * d * tmax_new * bmax, xdig, xnew, status);
+    } else if (permute)
         hipLaunchKernelGGL(gdn_digitwise_kernel<true>, grid, dim3(kBlock), 0, s, g, r_x, r_b, tmax, tmax_new, bmax,
                            bases, tab, r * d * tmax_new * bmax, xdig, xnew, status);
     else

@@ -43,7 +43,7 @@ struct DivU32 {
     u32 d;
 };
 
-static inline DivU32 make_div(u32 d) {
+__host__ __device__ static inline DivU32 make_div(u32 d) {
     DivU32 r;
     r.d = d;
     r.M = (d == 1) ? 0 : (~(u64)0) / d + 1;

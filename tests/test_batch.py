@@ -270,3 +270,22 @@ def test_owen_hash64_np_matches_library():
     xs = rng.integers(0, 2**63, size=50, dtype=np.uint64) * np.uint64(2)
     for x, s in zip(xs, rng.integers(0, 2**63, size=50, dtype=np.uint64)):
         assert int(owen_hash64_np(np.array([x]), s)[0]) == B.owen_hash64(int(x), int(s))
+
+
+def test_digests_match_outputs_and_per_call(torch, wide):
+    """Per-slab digests (order-independent sums of the stored bit patterns)
+    equal the same sums over the materialized outputs, and the batched,
+    graph and per-call paths give equal digests."""
+    plan, got = wide
+    ring = torch.empty(plan.ring_elems, dtype=torch.float64, device="cuda")
+    d_b = plan.digests(ring, graph=False)
+    d_g = plan.digests(ring, graph=True)
+    d_p = plan.digests(ring, per_call=True)
+    assert np.array_equal(d_b, d_g) and np.array_equal(d_b, d_p)
+    want = []
+    for q in range(len(plan.m)):
+        n, d = 1 << int(plan.m[q]), int(plan.d[q])
+        for l in range(int(plan.r[q])):
+            o = plan.slab_offset(q, l)
+            want.append(int(np.sum(got[o:o + n * d].view(np.uint64), dtype=np.uint64)))
+    assert np.array_equal(d_b, np.array(want, dtype=np.uint64))
