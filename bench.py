@@ -183,10 +183,12 @@ def section_gdn(torch, q, dev, timed, r=8):
            "chain_Gpts_s": round(pts / (ms_gen + ms_perm + ms_flt) / 1e6, 2)}
     del xperm, x, C_lms
     torch.cuda.empty_cache()
-    # nested uniform scrambling of the same net's first 3^8 points (host trees,
-    # numpy draws in the reference's order, GPU walk), 20 -> 52 digits
+    # nested uniform scrambling of the same net's first 3^7 points for r = 2
+    # (host trees, numpy draws in the reference's order, GPU walk), 20 -> 52
+    # digits.  Every new leaf draws tmax_new - t - 1 permutations of b values
+    # (rand_funcs.py:328-351), so the draws dominate and grow as n * b.
     from qmctoolscl_amd import nus
-    n8, tmax = 3 ** 8, mmax
+    n8, tmax, r = 3 ** 7, mmax, 2
     xd = xdig[:1, :n8, :, :tmax].contiguous()  # the unscrambled Halton digits are not needed: regenerate
     q.gdn_gen_natural(U(1), U(n8), U(d), U(1), U(mmax), U(tmax), U(0), Bd, Cd, xd)
     xr_nus = torch.empty((r, n8, d, tnew), dtype=torch.int64, device=dev)
@@ -203,7 +205,7 @@ def section_gdn(torch, q, dev, timed, r=8):
     nus.gdn_nested_uniform_scramble(r, n8, d, 1, 1, tmax, tnew, rngs, roots, Bd, xd, xr_nus)
     torch.cuda.synchronize()
     t_again = time.perf_counter() - t0
-    out["nus"] = {"workload": "gdn nested uniform scramble, first 3^8 points, d=32, r=8, 20 -> 52 digits",
+    out["nus"] = {"workload": "gdn nested uniform scramble, first 3^7 points, d=32, r=2, 20 -> 52 digits",
                   "points": r * n8 * d, "first_call_s": round(t_first, 3), "repeat_call_s": round(t_again, 4),
                   "first_call_Mpts_s": round(r * n8 * d / t_first / 1e6, 3)}
     del xdig, xd, xr_nus

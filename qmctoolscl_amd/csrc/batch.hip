@@ -25,8 +25,8 @@
 // each); a setup kernel builds every request's small tables (prefix-XOR
 // columns, shifts, LMS rows, permutation parameters) in a workspace, then
 // persistent grids drain tile counters: one kernel for the lattice and dnb2
-// tiles (store-bound, 28 registers, full occupancy) and one for the gdn tiles
-// (digit walks, ~110 registers), each compiled without the other's code and
+// tiles (store-bound, few registers) and one for the gdn tiles (digit walks,
+// ~110 registers, ALU-bound), each compiled without the other's code and
 // launched back to back (one graph).  Outputs go to a ring buffer:
 // every replication slab (n*d doubles) of every request is placed contiguously
 // at an offset chosen on the host, wrapping to the start when the ring is full,
@@ -242,11 +242,14 @@ std::vector<GdnGroup> gdn_groups(int d, u32 RT) {
     return g;
 }
 
+// Words of column j's block.  Base 2 (j = 0, read as u64) comes first and is
+// even-sized; the u32 blocks of odd bases are odd-sized, so the lanes of a wave,
+// each reading its own column's row at the same offset, fall on distinct LDS
+// banks.
 inline u32 gdn_words(const Req &q, int j, u32 W) {
     const u32 b = h_primes[j];
     const u32 mm = (u32)gdn_mm(b, q.m);
-    const u32 w = b == 2 ? 2 + 2 * mm : (mm + 1) * W;
-    return (w + 1) & ~1u;  // even: every block starts 8-byte aligned
+    return b == 2 ? 2 + 2 * mm : ((mm + 1) * W) | 1u;
 }
 
 u64 table_words(Req &q, const std::vector<GdnGroup> &grp) {
@@ -259,10 +262,10 @@ u64 table_words(Req &q, const std::vector<GdnGroup> &grp) {
             u32 w = 0;
             for (const auto &x : grp)
                 for (u32 j = x.j0; j < x.j0 + x.gc; j++) w += gdn_words(q, (int)j, x.w);
-            q.gtw = w;
+            q.gtw = (w + 1) & ~1u;  // even: every replication's block starts 8-byte aligned
             q.ggrp = (u64)kGJ * d;
             q.gtab = q.ggrp + 2 * grp.size();
-            return q.gtab + r * (u64)(w / 2);
+            return q.gtab + r * (u64)(q.gtw / 2);
         }
     }
 }
@@ -779,10 +782,7 @@ struct Plan {
     int grid = 0;
     int device = 0;
     u32 lds_bytes = 0;               // dynamic LDS of the gdn kernel (digit tables)
-    int cus = 256;
     hipStream_t cap = nullptr;       // private stream for graph capture
-    hipStream_t side = nullptr;      // the gdn kernel runs here, beside the store-bound one
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipGraphExec_t gexec = nullptr;  // [counter reset, persistent kernel]
     double *gring = nullptr;         // ring the graph was captured for
     unsigned long long *digest = nullptr;   // per-slab digests, or null
@@ -792,33 +792,19 @@ struct Plan {
 
 constexpr u32 kLdsClass0 = 256 * sizeof(u32);  // dnb2 first-row values
 
-// The two classes run concurrently when both have tiles: the store-bound
-// kernel on s with the full grid, the gdn kernel on the plan's side stream
-// with two workgroups per CU, so both stay resident together (28 + 2 x ~120
-// VGPRs per SIMD lane) and the digit walks fill the ALUs while the other
-// kernel waits on HBM.  Fork and join are events, so a graph capture of s
-// takes in both branches.
+// The two classes run back to back on s, each with the full grid.  (Running
+// them on two streams measured no faster, with either grid split: each
+// persistent grid holds its CUs until its queue drains.)
 template <bool DIG>
 int launch_batch_t(Plan *P, double *ring, hipStream_t s) {
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, P->d_counter);
-    const bool both = P->qtiles[0] && P->qtiles[1];
-    if (both) {
-        QMC_HIP_CHECK(hipEventRecord(P->ev_fork, s));
-        QMC_HIP_CHECK(hipStreamWaitEvent(P->side, P->ev_fork, 0));
-        const int g1 = P->grid < 2 * P->cus ? P->grid : 2 * P->cus;
-        hipLaunchKernelGGL((batch_persistent_kernel<1, DIG>), dim3(g1), dim3(kBlock), P->lds_bytes, P->side,
-                           P->d_qreqs + P->qn[0], P->qn[1], P->qtiles[1], P->d_counter + 1, P->d_ws, ring,
-                           P->digest);
-        QMC_HIP_CHECK(hipEventRecord(P->ev_join, P->side));
-    } else if (P->qtiles[1]) {
-        hipLaunchKernelGGL((batch_persistent_kernel<1, DIG>), dim3(P->grid), dim3(kBlock), P->lds_bytes, s,
-                           P->d_qreqs + P->qn[0], P->qn[1], P->qtiles[1], P->d_counter + 1, P->d_ws, ring,
-                           P->digest);
-    }
     if (P->qtiles[0])
         hipLaunchKernelGGL((batch_persistent_kernel<0, DIG>), dim3(P->grid), dim3(kBlock), kLdsClass0, s, P->d_qreqs,
                            P->qn[0], P->qtiles[0], P->d_counter, P->d_ws, ring, P->digest);
-    if (both) QMC_HIP_CHECK(hipStreamWaitEvent(s, P->ev_join, 0));
+    if (P->qtiles[1])
+        hipLaunchKernelGGL((batch_persistent_kernel<1, DIG>), dim3(P->grid), dim3(kBlock), P->lds_bytes, s,
+                           P->d_qreqs + P->qn[0], P->qn[1], P->qtiles[1], P->d_counter + 1, P->d_ws, ring,
+                           P->digest);
     QMC_HIP_CHECK(hipGetLastError());
     return QMC_OK;
 }
@@ -912,15 +898,7 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
     if (hipGetDevice(&P->device) != hipSuccess) { delete P; qmc_fail(QMC_ERR_HIP, "batch: no device"); return nullptr; }
     int cus = 256;
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, P->device);
-    P->cus = cus;
     P->grid = grid > 0 ? grid : 4 * cus;
-    if (hipStreamCreateWithFlags(&P->side, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&P->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&P->ev_join, hipEventDisableTiming) != hipSuccess) {
-        qmc_fail(QMC_ERR_HIP, "batch: stream or event creation failed");
-        delete P;
-        return nullptr;
-    }
     std::vector<u64> ws_host(words, 0);
     // gdn dimension info and lane groups (the digit tables are built on the device)
     for (size_t qi = 0; qi < P->reqs.size(); qi++) {
@@ -1002,9 +980,6 @@ QMC_EXPORT void qmc_batch_free(void *h) {
     if (!P) return;
     if (P->gexec) hipGraphExecDestroy(P->gexec);
     if (P->cap) hipStreamDestroy(P->cap);
-    if (P->side) hipStreamDestroy(P->side);
-    if (P->ev_fork) hipEventDestroy(P->ev_fork);
-    if (P->ev_join) hipEventDestroy(P->ev_join);
     hipFree(P->d_reqs);
     hipFree(P->d_ws);
     hipFree(P->d_counter);
