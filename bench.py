@@ -369,7 +369,7 @@ def section_transforms(torch, q, dev, timed):
         rng = np.random.Generator(np.random.SFC64(m))
         x0 = torch.from_numpy(rng.standard_normal((V, n))).to(dev)
         x = x0.clone()
-        ms64, _ = timed(lambda: q.fwht_1d_radix2(U(1), U(V), U(n // 2), x, wait=False), 5, 1)
+        ms64, _ = timed(lambda: q.fwht_1d_radix2(U(1), U(V), U(n // 2), x, wait=False), 20, 3)
         # accuracy: one transform of the same input each way
         ref = x0.clone()
         q.fwht_1d_radix2(U(1), U(V), U(n // 2), ref)
@@ -378,11 +378,11 @@ def section_transforms(torch, q, dev, timed):
         torch.cuda.synchronize()
         l2, mx = relative_errors(xm.cpu().numpy(), ref.cpu().numpy())
         work = torch.empty(V * n, dtype=torch.float32, device=dev)
-        msm, _ = timed(lambda: fwht_mfma(1, V, n // 2, x, work), 5, 1)
+        msm, _ = timed(lambda: fwht_mfma(1, V, n // 2, x, work), 20, 3)
         xr, xi = x0.clone(), x0.clone()
         twr = torch.empty(n, dtype=torch.float64, device=dev)
         twi = torch.empty(n, dtype=torch.float64, device=dev)
-        msf, _ = timed(lambda: q.fft_bro_1d_radix2(U(1), U(V), U(n // 2), twr, twi, xr, xi, wait=False), 5, 1)
+        msf, _ = timed(lambda: q.fft_bro_1d_radix2(U(1), U(V), U(n // 2), twr, twi, xr, xi, wait=False), 20, 3)
         # FFT accuracy and time on the matrix cores
         yr, yi = x0.clone(), x0.flip(-1).contiguous()
         q.fft_bro_1d_radix2(U(1), U(V), U(n // 2), twr, twi, yr, yi)
@@ -393,7 +393,7 @@ def section_transforms(torch, q, dev, timed):
                                    np.stack([yr.cpu().numpy(), yi.cpu().numpy()]))
         del yr, yi, zr, zi
         fwork = torch.empty(2 * V * n, dtype=torch.float32, device=dev)
-        msfm, _ = timed(lambda: fft_bro_mfma(1, V, n // 2, xr, xi, fwork), 5, 1)
+        msfm, _ = timed(lambda: fft_bro_mfma(1, V, n // 2, xr, xi, fwork), 20, 3)
         del fwork
         pts = V * n
         # MFMA work: per 16-point axis, two 16x16x32 products per 256 outputs
@@ -417,6 +417,23 @@ def section_transforms(torch, q, dev, timed):
     return out
 
 
+def section_halton(torch, dev, timed):
+    """Generalized Halton (halton_qrng): n = 2^20 points in the first 32 prime
+    bases, random digit shift, generalized factors; res is d x n float64."""
+    from qmctoolscl_amd.halton import halton_qrng
+    n, d = 1 << 20, 32
+    rng = np.random.Generator(np.random.SFC64(3))
+    randu = torch.from_numpy(rng.random((d, 32))).to(dev)
+    dvec = torch.arange(d, dtype=torch.int32, device=dev)
+    res = torch.empty((d, n), dtype=torch.float64, device=dev)
+    ms, _ = timed(lambda: halton_qrng(n, d, 0, 1, res, randu, dvec), 20, 3)
+    pts = n * d
+    del res
+    return {"workload": "generalized Halton, n=2^20, d=32 (first 32 primes), random digit shift, "
+                        "generalized factors", "points": pts, "ms": round(ms, 4),
+            "Gpts_s": round(pts / ms / 1e6, 2), "GBs_written": round(8 * pts / ms / 1e6, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -424,7 +441,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-rows", type=int, default=1 << 19)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--sections", default="main,dnb2,transforms,gdn,batch")
+    ap.add_argument("--sections", default="main,dnb2,transforms,gdn,halton,batch")
     ap.add_argument("--quick-batch", action="store_true", help="cap request sizes at n=2^16")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one plain 2 GiB device-to-device copy (a known-size kernel that scales "
@@ -511,6 +528,9 @@ def main():
     if "gdn" in sections:
         log("section: gdn")
         out["sections"]["gdn"] = section_gdn(torch, q, dev, timed)
+    if "halton" in sections:
+        log("section: halton")
+        out["sections"]["halton"] = section_halton(torch, dev, timed)
     if "batch" in sections:
         del xr
         torch.cuda.empty_cache()

@@ -135,6 +135,13 @@ QMC_EXPORT void fft_bro_1d_radix2(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, qmc_u6
 QMC_EXPORT void ifft_bro_1d_radix2(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, qmc_u64 bs_d1, qmc_u64 bs_d2,
                                    qmc_u64 bs_n_half, double *twiddler, double *twiddlei, double *xr, double *xi);
 
+/* ------------------------------------------- drop-in: generalized Halton
+ * halton_qrng.c:3-24 (bound by the reference's pdm_build.py:85-94 as
+ * halton_qrng_c, argtypes c_int x 4, double *, double *, int *).  res is
+ * d x n (res[j n + i] = point n0 + i in dimension j), randu_d_32 is d x 32,
+ * dvec holds d prime indices below 360.  No state is kept between calls. */
+QMC_EXPORT void halton_qrng(int n, int d, int n0, int generalized, double *res, double *randu_d_32, int *dvec);
+
 /* ----------------------------------------------- device entry points
  * Each replaces the drop-in of the same name (same citation) for callers
  * whose buffers already live on the GPU. */
@@ -191,6 +198,8 @@ QMC_EXPORT int fft_bro_1d_radix2_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, dou
                                      double *xr, double *xi, qmc_stream_t s);
 QMC_EXPORT int ifft_bro_1d_radix2_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, double *twiddler, double *twiddlei,
                                       double *xr, double *xi, qmc_stream_t s);
+QMC_EXPORT int qmc_halton_qrng_dev(int n, int d, int n0, int generalized, double *res, const double *randu_d_32,
+                                   const int *dvec, qmc_stream_t s);
 
 /* ------------------------------------------------------- fused pipelines
  * No single reference symbol: each equals a chain of reference calls as the

@@ -28,6 +28,7 @@
 #include <math.h>
 #include <stdint.h>
 #include "qmc_oracle.h"
+#include "../qmctoolscl_amd/csrc/halton_tables.h"  /* data only: the generalized Halton factors */
 
 /* index of the lowest set bit; k > 0 (the reference's `while(!((k>>b)&1)) b++`) */
 static u64 low_bit_index(u64 k) {
@@ -482,6 +483,55 @@ void oracle_ifft_bro_1d_radix2(u64 d1, u64 d2, u64 n_half, double *twr, double *
                 re[b] = (yr * c - yi * sn) / rt2;
                 im[b] = (yr * sn + yi * c) / rt2;
             }
+        }
+    }
+}
+
+/* -------------------------------------------------------- generalized Halton */
+
+/* halton_qrng.c:3-162.  res[j*n + i - n0] for the points i in [n0, n0 + n):
+ * u = 0; for k = 31..0: u += (f*c_k + s_k) % b; u /= b, with c_k the base-b
+ * digits of i (at most 32, halton_qrng.c:126-138), s_k = (int)(b * randu[32 j
+ * + k]) kept as unsigned int (halton_qrng.c:86-92), f = 1 or the generalized
+ * factor (halton_qrng.c:100-112).  With n0 == 0, point 0 comes from the set-up
+ * loop with the unreduced s_k (halton_qrng.c:88-95) and the main loop starts at
+ * 1 (halton_qrng.c:113-121).  Unlike the reference, nothing persists between
+ * calls (its perm[] is static) and n == 0 writes nothing. */
+void oracle_halton_qrng(int n, int d, int n0, int generalized, double *res,
+                        const double *randu_d_32, const int *dvec) {
+    unsigned primes[QMC_HALTON_DIMS];
+    int found = 0;
+    for (unsigned v = 2; found < QMC_HALTON_DIMS; v++) {
+        int prime = 1;
+        for (int t = 0; t < found && primes[t] * primes[t] <= v; t++)
+            if (v % primes[t] == 0) { prime = 0; break; }
+        if (prime) primes[found++] = v;
+    }
+    if (n <= 0) return;
+    for (int j = 0; j < d; j++) {
+        const int base = (int)primes[dvec[j]];
+        const unsigned f = generalized ? qmc_halton_factor[dvec[j]] : 1u;
+        unsigned sh[32];
+        double u = 0;
+        for (int k = 31; k >= 0; k--) {
+            sh[k] = (unsigned)(int)(base * randu_d_32[j * 32 + k]);
+            u += sh[k];
+            u /= base;
+        }
+        if (n0 == 0) res[(long)j * n] = u;
+        for (int i = (n0 == 0 ? 1 : n0); i < n + n0; i++) {
+            unsigned coeff[32] = {0};
+            unsigned tmp = (unsigned)i;
+            for (int k = 0; tmp > 0 && k < 32; k++) {
+                coeff[k] = tmp % (unsigned)base;
+                tmp /= (unsigned)base;
+            }
+            double v = 0.0;
+            for (int k = 31; k >= 0; k--) {
+                v += (f * coeff[k] + sh[k]) % (unsigned)base;
+                v /= base;
+            }
+            res[(long)j * n + (i - n0)] = v;
         }
     }
 }

@@ -71,4 +71,8 @@ void oracle_fft_bro_1d_radix2(u64 d1, u64 d2, u64 n_half, double *twr, double *t
 void oracle_ifft_bro_1d_radix2(u64 d1, u64 d2, u64 n_half, double *twr, double *twi,
                                double *xr, double *xi);
 
+/* generalized Halton (halton_qrng.c, int arguments as in the reference) */
+void oracle_halton_qrng(int n, int d, int n0, int generalized, double *res,
+                        const double *randu_d_32, const int *dvec);
+
 #endif

@@ -8,7 +8,8 @@ Drop-in for the `qmctoolscl` call surface on that path:
 * the host-side random draws (lat_get_shifts, dnb2_get_*, gdn_get_*,
   random_tbit_uint64s, random_uint64_permutations) -- rand_funcs.py;
 * nested uniform scrambling with persistent trees (NUSNode_dnb2,
-  NUSNode_gdn, dnb2_nested_uniform_scramble, gdn_nested_uniform_scramble).
+  NUSNode_gdn, dnb2_nested_uniform_scramble, gdn_nested_uniform_scramble);
+* the generalized Halton generator halton_qrng_c (halton.py).
 
 All compute runs in hand-written gfx950 HIP kernels (libqmctoolscl_amd.so,
 C ABI in include/qmctoolscl_amd.h); there is no CPU fallback.
@@ -24,5 +25,6 @@ from .kernels import *  # noqa: F401,F403
 from .nus import (  # noqa: F401
     NUSNode_dnb2, NUSNode_gdn, dnb2_nested_uniform_scramble, gdn_nested_uniform_scramble,
 )
+from .halton import halton_qrng, halton_qrng_c  # noqa: F401
 from ._api import SIGNATURES, KERNEL_NAMES  # noqa: F401
 from ._lib import QMCError  # noqa: F401

@@ -366,4 +366,17 @@ QMC_EXPORT void ifft_bro_1d_radix2(u64 d1, u64 d2, u64 n_half, u64, u64, u64, do
     DROPIN_RUN(ifft_bro_1d_radix2_dev(d1, d2, n_half, tr, ti, dr, di, st.stream()));
 }
 
+// generalized Halton -------------------------------------------------------
+QMC_EXPORT void halton_qrng(int n, int d, int n0, int generalized, double *res, double *randu_d_32, int *dvec) {
+    DROPIN_BEGIN
+    if (n <= 0 || d <= 0) {
+        if (n < 0 || d < 0 || n0 < 0) qmc_fail(QMC_ERR_ARG, "halton_qrng: negative size");
+        return;
+    }
+    auto du = st.in(randu_d_32, (u64)d * 32);
+    auto dv = st.in(dvec, (u64)d);
+    auto dr = st.out(res, (u64)d * (u64)n);
+    DROPIN_RUN(qmc_halton_qrng_dev(n, d, n0, generalized, dr, du, dv, st.stream()));
+}
+
 }  // extern "C"

@@ -20,7 +20,17 @@
 // fp32 directly.  Tiles of 2^13 complex values stay in LDS; longer vectors
 // take one pass per 8 index bits above the first 12, with an fp32
 // intermediate.  The 1/sqrt(N) scale is applied once, in float64.
+//
+// Twiddles.  The exponent brev(c) k of a level splits over the bits of c: the
+// bits inside the tile give a factor w_{2^q}^j (q <= 13) that depends only on
+// the position in the tile, read from two small LDS tables (w_{2^13}^J =
+// t1[J >> 7] t2[J & 127]); the bits fixed for the whole tile give one factor
+// per (level, k), computed once per tile from the FP64 tables of w_N.  All
+// table entries are FP64 values rounded once to fp32.
 #include "qmc_common.hpp"
+
+#include <algorithm>
+#include <type_traits>
 
 namespace {
 
@@ -36,6 +46,8 @@ constexpr int kThreads = 512;  // 8 waves; two workgroups (68 KiB of LDS, <= 128
 // the lowest level (addresses 16 apart) on distinct banks
 __device__ __forceinline__ int pad(int t) { return t + (t >> 4); }
 static inline size_t padded_bytes(int T) { return 2 * sizeof(float) * (((size_t)1 << T) + ((size_t)1 << T) / 16); }
+struct TwLds;
+static inline size_t shm_bytes(int T);
 
 __device__ __forceinline__ void split3f(float x, __bf16 &a, __bf16 &b, __bf16 &c) {
     a = (__bf16)x;
@@ -130,55 +142,77 @@ __device__ __forceinline__ void twiddle(const Twiddles &tw, u64 E, float &c, flo
     s = (float)(a.x * b.y + a.y * b.x);
 }
 
-// radix-16 level on tile bits [ta, ta+4) == global bits [ga, ga+4)
-__device__ void dft16_level(float *re, float *im, const PassGeom &G, u64 cb, int T, int ta, int ga,
-                            const Wtile &W, int wave, int lane, const Twiddles &tw) {
+// LDS twiddle tables after the two value tiles
+struct TwLds {
+    Wtile w[64];     // the DFT-16 operands of each lane (the same for every wave)
+    f32x2 t1[64];    // w_64^a = w_{2^13}^(128 a)
+    f32x2 t2[128];   // w_{2^13}^b
+    f32x2 f[4][16];  // per level of the pass: the tile's fixed factor for k = 0..15
+};
+
+static inline size_t shm_bytes(int T) { return padded_bytes(T) + sizeof(TwLds); }
+
+__device__ __forceinline__ f32x2 cmul(f32x2 a, f32x2 b) {
+    f32x2 r;
+    r.x = a.x * b.x - a.y * b.y;
+    r.y = a.x * b.y + a.y * b.x;
+    return r;
+}
+
+// radix-16 level on tile bits [ta, ta+4), level `lev` of the pass; with tw,
+// output k of the column with in-tile bits c above the level (b of them) is
+// multiplied by w_{2^(b+4)}^(brev_b(c) k) * f[lev][k]
+__device__ void dft16_level(float *re, float *im, int T, int ta, bool tw, int wave, int lane, const TwLds &tl,
+                            int lev, u32 salt) {
+    const Wtile W = tl.w[lane];
     const int nblk = 1 << (T - 8);
     const int n = lane & 15, g = lane >> 4;
     const int c0 = (g & 1) * 8;
     float *src = g < 2 ? re : im;
-    const int hi_bits = G.m - ga - 4;  // index bits above this level
+    const int b = T - ta - 4;  // tile bits above this level (<= 9)
     for (int blk = wave; blk < nblk; blk += (int)(blockDim.x >> 6)) {
         const int col = blk * 16 + n;
         const int lo = col & ((1 << ta) - 1);
         const int base = ((col >> ta) << (ta + 4)) | lo;
-        bf16x8 b[3];
+        const u32 sg = dither_sign((u32)col, salt);
+        bf16x8 x[3];
 #pragma unroll
         for (int i = 0; i < 8; i++) {
             __bf16 s0, s1, s2;
-            split3f(src[pad(base + ((c0 + i) << ta))], s0, s1, s2);
-            b[0][i] = s0;
-            b[1][i] = s1;
-            b[2][i] = s2;
+            split3f(flip(src[pad(base + ((c0 + i) << ta))], sg), s0, s1, s2);
+            x[0][i] = s0;
+            x[1][i] = s1;
+            x[2][i] = s2;
         }
         f32x4 yr = {0.f, 0.f, 0.f, 0.f}, yi = {0.f, 0.f, 0.f, 0.f};
         // sum over pieces p + q <= 2 of W_p * X_q, smallest terms first
-        yr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.Ar[2], b[0], yr, 0, 0, 0);
-        yr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.Ar[1], b[1], yr, 0, 0, 0);
-        yr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.Ar[0], b[2], yr, 0, 0, 0);
-        yr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.Ar[1], b[0], yr, 0, 0, 0);
-        yr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.Ar[0], b[1], yr, 0, 0, 0);
-        yr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.Ar[0], b[0], yr, 0, 0, 0);
-        yi = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.Ai[2], b[0], yi, 0, 0, 0);
-        yi = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.Ai[1], b[1], yi, 0, 0, 0);
-        yi = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.Ai[0], b[2], yi, 0, 0, 0);
-        yi = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.Ai[1], b[0], yi, 0, 0, 0);
-        yi = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.Ai[0], b[1], yi, 0, 0, 0);
-        yi = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.Ai[0], b[0], yi, 0, 0, 0);
-        // lane holds outputs k = 4g + i of column n; twiddle w_{Ns}^(brev(c_hi) * k)
+        yr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.Ar[2], x[0], yr, 0, 0, 0);
+        yr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.Ar[1], x[1], yr, 0, 0, 0);
+        yr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.Ar[0], x[2], yr, 0, 0, 0);
+        yr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.Ar[1], x[0], yr, 0, 0, 0);
+        yr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.Ar[0], x[1], yr, 0, 0, 0);
+        yr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.Ar[0], x[0], yr, 0, 0, 0);
+        yi = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.Ai[2], x[0], yi, 0, 0, 0);
+        yi = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.Ai[1], x[1], yi, 0, 0, 0);
+        yi = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.Ai[0], x[2], yi, 0, 0, 0);
+        yi = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.Ai[1], x[0], yi, 0, 0, 0);
+        yi = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.Ai[0], x[1], yi, 0, 0, 0);
+        yi = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.Ai[0], x[0], yi, 0, 0, 0);
+        // lane holds outputs k = 4g + i of column n
+        const int rb = b > 0 ? brev_bits(col >> ta, b) : 0;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const int k = 4 * g + i;
             const int t = base + (k << ta);
-            float vr = yr[i], vi = yi[i];
-            if (hi_bits > 0) {
-                const u64 p = gpos(G, cb, t);
-                const u64 chi = p >> (ga + 4);
-                const u64 e = (__brevll(chi) >> (64 - hi_bits)) * (u64)k;  // < Ns = 2^(hi_bits+4)
-                float s, c;
-                twiddle(tw, e << (G.m - hi_bits - 4), c, s);  // w_Ns^e = w_N^(e N/Ns)
-                const float tr = vr * c - vi * s;
-                vi = vr * s + vi * c;
+            float vr = flip(yr[i], sg), vi = flip(yi[i], sg);
+            if (tw) {
+                f32x2 w = tl.f[lev][k];
+                if (b > 0) {
+                    const int J = ((rb * k) & ((16 << b) - 1)) << (9 - b);  // w_{2^(b+4)}^j = w_{2^13}^(j 2^(9-b))
+                    w = cmul(cmul(tl.t1[J >> 7], tl.t2[J & 127]), w);
+                }
+                const float tr = vr * w.x - vi * w.y;
+                vi = vr * w.y + vi * w.x;
                 vr = tr;
             }
             re[pad(t)] = vr;
@@ -214,72 +248,123 @@ __device__ void dft_small_level(float *re, float *im, int T, int ta, int rb) {
     }
 }
 
-// levels on global bits [glo, ghi) of tiles; first/last pass move float64
-template <int IN64, int OUT64>
+// Levels on global bits [glo, ghi) of tiles; the first/last pass move
+// float64.  Workgroups are persistent: each walks tiles blockIdx.x,
+// +gridDim.x, ... with all NP value pairs (of re and of im) of a thread
+// (2^T = 2 NP blockDim.x) loaded at once.
+template <int IN64, int OUT64, int NP>
 __global__ __launch_bounds__(kThreads) void fft_mfma_pass(PassGeom G, int glo, int ghi, const void *__restrict__ sr,
                                                         const void *__restrict__ si, void *__restrict__ dr,
-                                                        void *__restrict__ di, double scale, Twiddles tw) {
+                                                        void *__restrict__ di, double scale, Twiddles tw, u64 ntiles) {
+    typedef typename std::conditional<IN64 != 0, f64x2, f32x2>::type In2;
     extern __shared__ float lds[];
     const int T = G.lr + G.lc;
+    const int P = (1 << T) + (1 << T) / 16;
     float *re = lds;
-    float *im = lds + (1 << T) + (1 << T) / 16;
-    const u64 cblocks = (1ull << G.m) >> T;
-    const u64 v = blockIdx.x / cblocks;
-    const u64 cb = blockIdx.x - v * cblocks;
-    const u64 vbase = v << G.m;
-    // two consecutive values per access (G.lc >= 1: gpos(t + 1) = gpos(t) + 1 for even t)
-#pragma unroll 4
-    for (int t = 2 * threadIdx.x; t < (1 << T); t += 2 * blockDim.x) {
-        const u64 gi = vbase + gpos(G, cb, t);
-        if (IN64) {
-            const f64x2 a = *(const f64x2 *)((const double *)sr + gi);
-            const f64x2 b = *(const f64x2 *)((const double *)si + gi);
-            re[pad(t)] = (float)a.x;
-            re[pad(t + 1)] = (float)a.y;
-            im[pad(t)] = (float)b.x;
-            im[pad(t + 1)] = (float)b.y;
-        } else {
-            const f32x2 a = *(const f32x2 *)((const float *)sr + gi);
-            const f32x2 b = *(const f32x2 *)((const float *)si + gi);
-            re[pad(t)] = a.x;
-            re[pad(t + 1)] = a.y;
-            im[pad(t)] = b.x;
-            im[pad(t + 1)] = b.y;
-        }
+    float *im = lds + P;
+    TwLds &tl = *(TwLds *)(lds + 2 * P);
+    const int tlog = G.m - T;
+    const u64 cmask_tile = (1ull << tlog) - 1;
+    if (blockIdx.x >= ntiles) return;
+    if (threadIdx.x < 64) tl.w[threadIdx.x] = make_w(threadIdx.x);
+    for (int i = threadIdx.x; i < 192; i += blockDim.x) {
+        double sn, cs;
+        if (i < 64)
+            sincospi(-2.0 * (double)i / 64.0, &sn, &cs);
+        else
+            sincospi(-2.0 * (double)(i - 64) / 8192.0, &sn, &cs);
+        f32x2 w;
+        w.x = (float)cs;
+        w.y = (float)sn;
+        if (i < 64)
+            tl.t1[i] = w;
+        else
+            tl.t2[i - 64] = w;
     }
-    __syncthreads();
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const Wtile W = make_w(lane);
-    for (int ga = glo; ga < ghi; ga += 4) {
-        // tile bit of global bit ga
-        const int ta = ga < G.lc ? ga : G.lc + (ga - G.lo);
-        if (ghi - ga >= 4) {
-            dft16_level(re, im, G, cb, T, ta, ga, W, wave, lane, tw);
-        } else {
-            dft_small_level(re, im, T, ta, ghi - ga);  // only the top level can be short
+    // value pair p of this thread: tile index 2 tid + 2 p blockDim.x (two consecutive values share a
+    // row as G.lc >= 1); as 2 blockDim.x and 2^lc are powers of two, its global position is that of
+    // pair 0 plus a per-p constant
+    auto g0 = [&](u64 ti, int tid) -> u64 { return ((ti >> tlog) << G.m) + gpos(G, ti & cmask_tile, 2 * tid); };
+    auto poff = [&](int p) -> u64 {
+        const u32 q = 2u * (u32)p * blockDim.x;
+        return ((u64)(q >> G.lc) << G.lo) + (q & ((1u << G.lc) - 1));
+    };
+    const int nlev = (ghi - glo + 3) / 4;
+    for (u64 tix = blockIdx.x; tix < ntiles; tix += gridDim.x) {
+        const u64 cb = tix & cmask_tile;
+        const int tid = opaque(threadIdx.x);
+        {
+            const u64 a0 = g0(tix, tid);
+            In2 qa[NP], qc[NP];
+#pragma unroll
+            for (int p = 0; p < NP; p++) {
+                qa[p] = __builtin_nontemporal_load((const In2 *)sr + (a0 + poff(p)) / 2);
+                qc[p] = __builtin_nontemporal_load((const In2 *)si + (a0 + poff(p)) / 2);
+            }
+#pragma unroll
+            for (int p = 0; p < NP; p++) {
+                const int t = 2 * (tid + p * blockDim.x);
+                re[pad(t)] = (float)qa[p].x;
+                re[pad(t + 1)] = (float)qa[p].y;
+                im[pad(t)] = (float)qc[p].x;
+                im[pad(t + 1)] = (float)qc[p].y;
+            }
+        }
+        // the tile's fixed twiddle factors: level lev, output k
+        for (int idx = tid; idx < 16 * nlev; idx += blockDim.x) {
+            const int lev = idx >> 4, k = idx & 15;
+            const int ga = glo + 4 * lev;
+            const int hi_bits = G.m - ga - 4;
+            if (ghi - ga < 4 || hi_bits <= 0) continue;  // no twiddle after this level
+            const int ta = ga < G.lc ? ga : G.lc + (ga - G.lo);
+            const int b = T - ta - 4;
+            const int nf = hi_bits - b;  // index bits above the level that the tile fixes
+            const u64 chi_f = gpos(G, cb, 0) >> (ga + 4 + b);
+            const u64 bf = nf > 0 ? __brevll(chi_f) >> (64 - nf) : 0;
+            const u64 e = (bf * (u64)k) & ((2ull << (hi_bits + 3)) - 1);  // mod 2^(hi_bits + 4)
+            float c, sn;
+            twiddle(tw, e << ga, c, sn);  // w_{2^(hi_bits+4)}^e = w_N^(e 2^ga)
+            f32x2 w;
+            w.x = c;
+            w.y = sn;
+            tl.f[lev][k] = w;
         }
         __syncthreads();
-    }
-#pragma unroll 4
-    for (int t = 2 * threadIdx.x; t < (1 << T); t += 2 * blockDim.x) {
-        const u64 gi = vbase + gpos(G, cb, t);
-        if (OUT64) {
-            f64x2 a, b;
-            a.x = (double)re[pad(t)] * scale;
-            a.y = (double)re[pad(t + 1)] * scale;
-            b.x = (double)im[pad(t)] * scale;
-            b.y = (double)im[pad(t + 1)] * scale;
-            __builtin_nontemporal_store(a, (f64x2 *)((double *)dr + gi));
-            __builtin_nontemporal_store(b, (f64x2 *)((double *)di + gi));
-        } else {
-            f32x2 a, b;
-            a.x = re[pad(t)];
-            a.y = re[pad(t + 1)];
-            b.x = im[pad(t)];
-            b.y = im[pad(t + 1)];
-            *(f32x2 *)((float *)dr + gi) = a;
-            *(f32x2 *)((float *)di + gi) = b;
+        for (int lev = 0, ga = glo; ga < ghi; ga += 4, lev++) {
+            const int ltid = opaque(threadIdx.x);
+            const int ta = ga < G.lc ? ga : G.lc + (ga - G.lo);  // tile bit of global bit ga
+            if (ghi - ga >= 4)
+                dft16_level(re, im, T, ta, G.m - ga - 4 > 0, ltid >> 6, ltid & 63, tl, lev,
+                            (u32)tix * 0x632BE5ABu + (u32)ga * 0x27D4EB2Fu);
+            else
+                dft_small_level(re, im, T, ta, ghi - ga);  // only the top level can be short
+            __syncthreads();
         }
+        const int stid = opaque(threadIdx.x);
+        const u64 a0 = g0(tix, stid);
+#pragma unroll
+        for (int p = 0; p < NP; p++) {
+            const int t = 2 * (stid + p * blockDim.x);
+            const u64 gi = a0 + poff(p);
+            if (OUT64) {
+                f64x2 a, c;
+                a.x = (double)re[pad(t)] * scale;
+                a.y = (double)re[pad(t + 1)] * scale;
+                c.x = (double)im[pad(t)] * scale;
+                c.y = (double)im[pad(t + 1)] * scale;
+                __builtin_nontemporal_store(a, (f64x2 *)((double *)dr + gi));
+                __builtin_nontemporal_store(c, (f64x2 *)((double *)di + gi));
+            } else {
+                f32x2 a, c;
+                a.x = re[pad(t)];
+                a.y = re[pad(t + 1)];
+                c.x = im[pad(t)];
+                c.y = im[pad(t + 1)];
+                *(f32x2 *)((float *)dr + gi) = a;
+                *(f32x2 *)((float *)di + gi) = c;
+            }
+        }
+        __syncthreads();  // every LDS read of this tile is done before the next one is staged
     }
 }
 
@@ -305,14 +390,14 @@ QMC_EXPORT int qmc_fft_bro_mfma_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, doub
     const double scale = (m % 2 == 0) ? ldexp(1.0, -m / 2) : ldexp(1.0, -(m - 1) / 2) / 1.4142135623730951;
     static bool attr_set = false;
     if (!attr_set) {  // padded tiles need more than the default 64 KiB of LDS
-        const int bytes = (int)padded_bytes(kT);
-        QMC_HIP_CHECK(hipFuncSetAttribute((const void *)fft_mfma_pass<1, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-        QMC_HIP_CHECK(hipFuncSetAttribute((const void *)fft_mfma_pass<1, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-        QMC_HIP_CHECK(hipFuncSetAttribute((const void *)fft_mfma_pass<0, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-        QMC_HIP_CHECK(hipFuncSetAttribute((const void *)fft_mfma_pass<0, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+        const int bytes = (int)shm_bytes(kT);
+        QMC_HIP_CHECK(hipFuncSetAttribute((const void *)fft_mfma_pass<1, 1, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+        QMC_HIP_CHECK(hipFuncSetAttribute((const void *)fft_mfma_pass<1, 0, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+        QMC_HIP_CHECK(hipFuncSetAttribute((const void *)fft_mfma_pass<0, 0, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+        QMC_HIP_CHECK(hipFuncSetAttribute((const void *)fft_mfma_pass<0, 1, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
         attr_set = true;
     }
-    const size_t shm = padded_bytes(kT);
+    const u64 cus = (u64)device_cus();
     // FP64 twiddle tables (one product per twiddle), 2^klo + 2^(m-klo) entries
     Twiddles tw;
     tw.klo = (m + 1) / 2;
@@ -323,21 +408,44 @@ QMC_EXPORT int qmc_fft_bro_mfma_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, doub
     tw.hi = tt + (1ull << tw.klo);
     hipLaunchKernelGGL(twiddle_tables_kernel, dim3((unsigned)((ntw + kThreads / 4 - 1) / (kThreads / 4))),
                        dim3(kThreads / 4), 0, s, m, tw.klo, tt, tt + (1ull << tw.klo));
+    // persistent grid: as many workgroups as stay resident, at most one per tile
+    auto grid_of = [&](const void *fn, int threads, size_t shm, u64 ntiles, unsigned &grid) -> int {
+        grid = (unsigned)std::min<u64>(ntiles, (u64)resident_per_cu(fn, threads, shm) * cus);
+        return QMC_OK;
+    };
     float *buf = nullptr;
+    unsigned grid = 0;
     if (m <= kT) {
         PassGeom G{m, 0, m, m, m};
-        // one vector per workgroup: half the threads keeps more workgroups per CU
-        hipLaunchKernelGGL((fft_mfma_pass<1, 1>), dim3((unsigned)V), dim3(kThreads / 2), padded_bytes(m), s, G, 0, m,
-                           (const void *)xr, (const void *)xi, (void *)xr, (void *)xi, scale, tw);
+        // one vector per tile, 8 value pairs per thread (m >= 10), fewer below
+        const int np = m >= 10 ? 8 : 1 << (m - 7);
+        const int threads = (1 << m) / (2 * np);
+        const size_t shm = shm_bytes(m);
+        if (np == 8) {
+            if (int rc = grid_of((const void *)fft_mfma_pass<1, 1, 8>, threads, shm, V, grid)) return rc;
+            hipLaunchKernelGGL((fft_mfma_pass<1, 1, 8>), dim3(grid), dim3(threads), shm, s, G, 0, m,
+                               (const void *)xr, (const void *)xi, (void *)xr, (void *)xi, scale, tw, V);
+        } else if (np == 4) {
+            if (int rc = grid_of((const void *)fft_mfma_pass<1, 1, 4>, threads, shm, V, grid)) return rc;
+            hipLaunchKernelGGL((fft_mfma_pass<1, 1, 4>), dim3(grid), dim3(threads), shm, s, G, 0, m,
+                               (const void *)xr, (const void *)xi, (void *)xr, (void *)xi, scale, tw, V);
+        } else {
+            if (int rc = grid_of((const void *)fft_mfma_pass<1, 1, 2>, threads, shm, V, grid)) return rc;
+            hipLaunchKernelGGL((fft_mfma_pass<1, 1, 2>), dim3(grid), dim3(threads), shm, s, G, 0, m,
+                               (const void *)xr, (const void *)xi, (void *)xr, (void *)xi, scale, tw, V);
+        }
     } else {
         buf = work;
         if (!buf) QMC_HIP_CHECK(hipMallocAsync((void **)&buf, 2 * sizeof(float) * V * n, s));
         float *br = buf, *bi = buf + V * n;
+        const size_t shm = shm_bytes(kT);
+        const u64 ntiles = V << (m - kT);
         // pass 1: global bits [0, 12) over contiguous tiles of 2^13
         {
             PassGeom G{m, 0, kT, kT, kT};
-            hipLaunchKernelGGL((fft_mfma_pass<1, 0>), dim3((unsigned)(V << (m - kT))), dim3(kThreads), shm, s, G, 0,
-                               12, (const void *)xr, (const void *)xi, (void *)br, (void *)bi, 1.0, tw);
+            if (int rc = grid_of((const void *)fft_mfma_pass<1, 0, 8>, kThreads, shm, ntiles, grid)) return rc;
+            hipLaunchKernelGGL((fft_mfma_pass<1, 0, 8>), dim3(grid), dim3(kThreads), shm, s, G, 0, 12,
+                               (const void *)xr, (const void *)xi, (void *)br, (void *)bi, 1.0, tw, ntiles);
         }
         // then 8 bits (two levels) per pass: tiles of 2^8 rows x 2^5 columns
         for (int lo = 12; lo < m; lo += 8) {
@@ -345,12 +453,15 @@ QMC_EXPORT int qmc_fft_bro_mfma_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, doub
             const int lr = hi - lo, lc = kT - lr;
             PassGeom G{m, lr, lc, lo, hi};
             const bool last = hi == m;
-            if (last)
-                hipLaunchKernelGGL((fft_mfma_pass<0, 1>), dim3((unsigned)(V << (m - kT))), dim3(kThreads), shm, s, G,
-                                   lo, hi, (const void *)br, (const void *)bi, (void *)xr, (void *)xi, scale, tw);
-            else
-                hipLaunchKernelGGL((fft_mfma_pass<0, 0>), dim3((unsigned)(V << (m - kT))), dim3(kThreads), shm, s, G,
-                                   lo, hi, (const void *)br, (const void *)bi, (void *)br, (void *)bi, 1.0, tw);
+            if (last) {
+                if (int rc = grid_of((const void *)fft_mfma_pass<0, 1, 8>, kThreads, shm, ntiles, grid)) return rc;
+                hipLaunchKernelGGL((fft_mfma_pass<0, 1, 8>), dim3(grid), dim3(kThreads), shm, s, G, lo, hi,
+                                   (const void *)br, (const void *)bi, (void *)xr, (void *)xi, scale, tw, ntiles);
+            } else {
+                if (int rc = grid_of((const void *)fft_mfma_pass<0, 0, 8>, kThreads, shm, ntiles, grid)) return rc;
+                hipLaunchKernelGGL((fft_mfma_pass<0, 0, 8>), dim3(grid), dim3(kThreads), shm, s, G, lo, hi,
+                                   (const void *)br, (const void *)bi, (void *)br, (void *)bi, 1.0, tw, ntiles);
+            }
         }
     }
     hipError_t e = hipGetLastError();

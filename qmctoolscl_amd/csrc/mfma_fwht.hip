@@ -19,6 +19,9 @@
 // error against the float64 transform (see tests/test_mfma.py).
 #include "qmc_common.hpp"
 
+#include <algorithm>
+#include <type_traits>
+
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -58,7 +61,7 @@ __device__ __forceinline__ bf16x8 hadamard_a(int lane) {
 // 16-point Hadamard along tile index bits [a, a+4) of a 2^T-value fp32 tile
 // in LDS.  Each wave owns whole 16-column blocks, so reads and writes of a
 // column never cross waves.
-__device__ void h16_axis(float *tile, int T, int a, int wave, int lane, bf16x8 A) {
+__device__ void h16_axis(float *tile, int T, int a, int wave, int lane, bf16x8 A, u32 salt) {
     const int nblk = 1 << (T - 8);
     const int n = lane & 15, g = lane >> 4;
     const int c0 = (g & 1) * 8;
@@ -66,11 +69,12 @@ __device__ void h16_axis(float *tile, int T, int a, int wave, int lane, bf16x8 A
         const int col = blk * 16 + n;
         const int lo = col & ((1 << a) - 1);
         const int base = ((col >> a) << (a + 4)) | lo;
+        const u32 sg = dither_sign((u32)col, salt);
         bf16x8 b1, b2;
 #pragma unroll
         for (int i = 0; i < 8; i++) {
             __bf16 s0, s1, s2;
-            split3(tile[pad(base + ((c0 + i) << a))], s0, s1, s2);
+            split3(flip(tile[pad(base + ((c0 + i) << a))], sg), s0, s1, s2);
             b1[i] = g < 2 ? s0 : s1;
             b2[i] = g < 2 ? s2 : (__bf16)0.0f;
         }
@@ -79,7 +83,7 @@ __device__ void h16_axis(float *tile, int T, int a, int wave, int lane, bf16x8 A
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, b2, acc, 0, 0, 0);
         // D layout: lane holds D[4g + i][n], i = 0..3
 #pragma unroll
-        for (int i = 0; i < 4; i++) tile[pad(base + ((g * 4 + i) << a))] = acc[i];
+        for (int i = 0; i < 4; i++) tile[pad(base + ((g * 4 + i) << a))] = flip(acc[i], sg);
     }
 }
 
@@ -97,62 +101,75 @@ __device__ void h2_axes(float *tile, int T, int a, int k) {
     }
 }
 
-// One pass: each workgroup owns a tile of 2^lr rows x 2^lc columns of one
-// vector (global index = v*2^m + row*row_stride + cb*2^lc + col) and applies
-// the Hadamard product along tile bits [tb0, tb0 + nb).  IN/OUT: 0 fp32, 1
-// float64 (the output is then scaled by `scale`).
-template <int IN64, int OUT64>
+// One pass over tiles of 2^lr rows x 2^lc columns of one vector (global
+// index = v*2^m + row*row_stride + cb*2^lc + col), applying the Hadamard
+// product along tile bits [tb0, tb0 + nb).  IN/OUT: 0 fp32, 1 float64 (the
+// output is then scaled by `scale`).  Workgroups are persistent: each walks
+// tiles blockIdx.x, +gridDim.x, ... with all NP value pairs of a thread
+// (2^T = 2 NP blockDim.x) loaded at once.
+template <int IN64, int OUT64, int NP>
 __global__ __launch_bounds__(kThreads) void fwht_mfma_pass(int m, int lr, int lc, u64 row_stride, int tb0, int nb,
                                                          const void *__restrict__ src, void *__restrict__ dst,
-                                                         double scale) {
+                                                         double scale, u64 ntiles) {
+    typedef typename std::conditional<IN64 != 0, f64x2, f32x2>::type In2;
     extern __shared__ float tile[];
     const int T = lr + lc;
-    const u64 cblocks = (1ull << m) >> (lr + lc);  // tiles per vector
-    const u64 v = blockIdx.x / cblocks;
-    const u64 cb = blockIdx.x - v * cblocks;
-    const u64 vbase = v << m;
+    const int tlog = m - T;  // log2 of tiles per vector
+    const u64 cmask_tile = (1ull << tlog) - 1;
     const int cmask = (1 << lc) - 1;
-    // two consecutive values per access (lc >= 1: they share a row), 8 accesses in flight per lane
-#pragma unroll 8
-    for (int t = 2 * threadIdx.x; t < (1 << T); t += 2 * blockDim.x) {
-        const u64 gidx = vbase + (u64)(t >> lc) * row_stride + (cb << lc) + (t & cmask);
-        float v0, v1;
-        if (IN64) {
-            const f64x2 w = *(const f64x2 *)((const double *)src + gidx);
-            v0 = (float)w.x;
-            v1 = (float)w.y;
-        } else {
-            const f32x2 w = *(const f32x2 *)((const float *)src + gidx);
-            v0 = w.x;
-            v1 = w.y;
+    // value pair p of this thread sits at tile index t = 2 tid + 2 p blockDim.x; as 2 blockDim.x and 2^lc
+    // are powers of two, its global index is that of pair 0 plus a per-p constant (no carries)
+    auto gidx = [&](u64 ti, int tid) -> u64 {
+        const int t = 2 * tid;
+        return ((ti >> tlog) << m) + (u64)(t >> lc) * row_stride + ((ti & cmask_tile) << lc) + (u64)(t & cmask);
+    };
+    auto poff = [&](int p) -> u64 {
+        const u32 q = 2u * (u32)p * blockDim.x;
+        return (u64)(q >> lc) * row_stride + (q & (u32)cmask);
+    };
+    for (u64 tix = blockIdx.x; tix < ntiles; tix += gridDim.x) {
+        {
+            const int tid = opaque(threadIdx.x);
+            const In2 *s0 = (const In2 *)src + gidx(tix, tid) / 2;
+            In2 r[NP];
+#pragma unroll
+            for (int p = 0; p < NP; p++) r[p] = __builtin_nontemporal_load(s0 + poff(p) / 2);
+#pragma unroll
+            for (int p = 0; p < NP; p++) {
+                const int t = 2 * (tid + p * blockDim.x);
+                tile[pad(t)] = (float)r[p].x;
+                tile[pad(t + 1)] = (float)r[p].y;
+            }
         }
-        tile[pad(t)] = v0;
-        tile[pad(t + 1)] = v1;
-    }
-    __syncthreads();
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const bf16x8 A = hadamard_a(lane);
-    int a = tb0;
-    for (; a + 4 <= tb0 + nb; a += 4) {
-        h16_axis(tile, T, a, wave, lane, A);
         __syncthreads();
-    }
-    if (a < tb0 + nb) h2_axes(tile, T, a, tb0 + nb - a);
-#pragma unroll 8
-    for (int t = 2 * threadIdx.x; t < (1 << T); t += 2 * blockDim.x) {
-        const u64 gidx = vbase + (u64)(t >> lc) * row_stride + (cb << lc) + (t & cmask);
-        const float v0 = tile[pad(t)], v1 = tile[pad(t + 1)];
-        if (OUT64) {
-            f64x2 w;
-            w.x = (double)v0 * scale;
-            w.y = (double)v1 * scale;
-            __builtin_nontemporal_store(w, (f64x2 *)((double *)dst + gidx));
-        } else {
-            f32x2 w;
-            w.x = v0;
-            w.y = v1;
-            *(f32x2 *)((float *)dst + gidx) = w;
+        int a = tb0;
+        for (; a + 4 <= tb0 + nb; a += 4) {
+            const int ltid = opaque(threadIdx.x);
+            h16_axis(tile, T, a, ltid >> 6, ltid & 63, hadamard_a(ltid & 63),
+                     (u32)tix * 0x632BE5ABu + (u32)a * 0x27D4EB2Fu);
+            __syncthreads();
         }
+        if (a < tb0 + nb) h2_axes(tile, T, a, tb0 + nb - a);
+        const int stid = opaque(threadIdx.x);
+        const u64 g0 = gidx(tix, stid);
+#pragma unroll
+        for (int p = 0; p < NP; p++) {
+            const int t = 2 * (stid + p * blockDim.x);
+            const float v0 = tile[pad(t)], v1 = tile[pad(t + 1)];
+            const u64 g = g0 + poff(p);
+            if (OUT64) {
+                f64x2 w;
+                w.x = (double)v0 * scale;
+                w.y = (double)v1 * scale;
+                __builtin_nontemporal_store(w, (f64x2 *)((double *)dst + g));
+            } else {
+                f32x2 w;
+                w.x = v0;
+                w.y = v1;
+                *(f32x2 *)((float *)dst + g) = w;
+            }
+        }
+        __syncthreads();  // every LDS read of this tile is done before the next one is staged
     }
 }
 
@@ -192,30 +209,53 @@ QMC_EXPORT int qmc_fwht_mfma_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, double 
     static bool attr_set = false;
     if (!attr_set) {  // tiles of 2^14 padded values need more than the default 64 KiB of LDS
         const int bytes = (int)padded_bytes(kTileLog2);
-        QMC_HIP_CHECK(hipFuncSetAttribute((const void *)fwht_mfma_pass<1, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-        QMC_HIP_CHECK(hipFuncSetAttribute((const void *)fwht_mfma_pass<1, 0>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-        QMC_HIP_CHECK(hipFuncSetAttribute((const void *)fwht_mfma_pass<0, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+        QMC_HIP_CHECK(hipFuncSetAttribute((const void *)fwht_mfma_pass<1, 1, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+        QMC_HIP_CHECK(hipFuncSetAttribute((const void *)fwht_mfma_pass<1, 0, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+        QMC_HIP_CHECK(hipFuncSetAttribute((const void *)fwht_mfma_pass<0, 1, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
         attr_set = true;
     }
+    const u64 cus = (u64)device_cus();
     if (m <= kTileLog2) {
+        // one vector per tile, 16 values per thread (m >= 10), fewer below
+        const int np = m >= 10 ? 8 : 1 << (m - 7);
+        const unsigned threads = (1u << m) / (2u * np);
         const size_t shm = padded_bytes(m);
-        // one vector per workgroup: a quarter of the threads keeps more workgroups per CU
-        hipLaunchKernelGGL((fwht_mfma_pass<1, 1>), dim3((unsigned)V), dim3(kThreads / 4), shm, s, m, 0, m, (u64)0, 0, m,
-                           (const void *)x, (void *)x, scale);
+        const void *fn = np == 8 ? (const void *)fwht_mfma_pass<1, 1, 8>
+                                 : np == 4 ? (const void *)fwht_mfma_pass<1, 1, 4>
+                                           : (const void *)fwht_mfma_pass<1, 1, 2>;
+        const unsigned grid = (unsigned)std::min<u64>(V, (u64)resident_per_cu(fn, (int)threads, shm) * cus);
+        switch (np) {
+            case 8:
+                hipLaunchKernelGGL((fwht_mfma_pass<1, 1, 8>), dim3(grid), dim3(threads), shm, s, m, 0, m,
+                                   (u64)0, 0, m, (const void *)x, (void *)x, scale, V);
+                break;
+            case 4:
+                hipLaunchKernelGGL((fwht_mfma_pass<1, 1, 4>), dim3(grid), dim3(threads), shm, s, m, 0, m,
+                                   (u64)0, 0, m, (const void *)x, (void *)x, scale, V);
+                break;
+            default:
+                hipLaunchKernelGGL((fwht_mfma_pass<1, 1, 2>), dim3(grid), dim3(threads), shm, s, m, 0, m,
+                                   (u64)0, 0, m, (const void *)x, (void *)x, scale, V);
+                break;
+        }
         QMC_HIP_CHECK(hipGetLastError());
         return QMC_OK;
     }
     float *buf = work;
     if (!buf) QMC_HIP_CHECK(hipMallocAsync((void **)&buf, sizeof(float) * V * n, s));
+    // tiles of 2^14 values, 1024 threads x 8 pairs, one workgroup per tile (measured faster here than a
+    // grid of resident workgroups looping over tiles)
+    const u64 ntiles = V << (m - kTileLog2);
+    const size_t shm = padded_bytes(kTileLog2);
+    const unsigned grid1 = (unsigned)std::min<u64>(ntiles, 1u << 30), grid2 = grid1;
     // pass 1: the low kTileLog2 bits over contiguous tiles
     const int m1 = kTileLog2, m2 = m - kTileLog2;
-    const size_t shm = padded_bytes(kTileLog2);
-    hipLaunchKernelGGL((fwht_mfma_pass<1, 0>), dim3((unsigned)(V << (m - m1))), dim3(kThreads), shm, s, m, 0, m1,
-                       (u64)0, 0, m1, (const void *)x, (void *)buf, 1.0);
+    hipLaunchKernelGGL((fwht_mfma_pass<1, 0, 8>), dim3(grid1), dim3(kThreads), shm, s, m, 0, m1, (u64)0, 0, m1,
+                       (const void *)x, (void *)buf, 1.0, ntiles);
     // pass 2: the high m2 bits; a tile is 2^m2 rows (stride 2^m1) x 2^(14-m2) columns
     const int lc = kTileLog2 - m2;
-    hipLaunchKernelGGL((fwht_mfma_pass<0, 1>), dim3((unsigned)(V << (m - kTileLog2))), dim3(kThreads), shm, s, m, m2,
-                       lc, (u64)1 << m1, lc, m2, (const void *)buf, (void *)x, scale);
+    hipLaunchKernelGGL((fwht_mfma_pass<0, 1, 8>), dim3(grid2), dim3(kThreads), shm, s, m, m2, lc, (u64)1 << m1, lc,
+                       m2, (const void *)buf, (void *)x, scale, ntiles);
     hipError_t e = hipGetLastError();
     if (!work) (void)hipFreeAsync(buf, s);
     if (e != hipSuccess) return qmc_fail_hip(e, "fwht_mfma_pass");

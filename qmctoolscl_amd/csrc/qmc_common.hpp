@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "../../include/qmctoolscl_amd.h"  // status codes and the exported prototypes
 
 typedef uint64_t u64;
@@ -26,6 +28,66 @@ static inline unsigned grid_for(u64 work, u64 per_block, unsigned cap = 1u << 20
     if (g > cap) g = cap;
     return (unsigned)g;
 }
+
+// Workgroups of `fn` that stay resident on one CU at this launch shape
+// (looked up once per kernel, shape and device; at least 1).
+static inline int resident_per_cu(const void *fn, int threads, size_t shm) {
+    struct Entry {
+        const void *fn;
+        int threads, dev;
+        size_t shm;
+        int per_cu;
+    };
+    static Entry cache[128];
+    static int used = 0;
+    static std::mutex mu;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lock(mu);
+    for (int i = 0; i < used; i++)
+        if (cache[i].fn == fn && cache[i].threads == threads && cache[i].dev == dev && cache[i].shm == shm)
+            return cache[i].per_cu;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, shm) != hipSuccess || per_cu < 1) per_cu = 1;
+    if (used < 128) cache[used++] = Entry{fn, threads, dev, shm, per_cu};
+    return per_cu;
+}
+
+// Compute units of the current device (looked up once per device).
+static inline int device_cus() {
+    static int cache[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    int c = __atomic_load_n(&cache[dev], __ATOMIC_RELAXED);
+    if (c <= 0) {
+        if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0) c = 256;
+        __atomic_store_n(&cache[dev], c, __ATOMIC_RELAXED);
+    }
+    return c;
+}
+
+// v, hidden from the optimizer.  Persistent loops read the thread index
+// through this once per iteration, so per-thread addresses derived from it are
+// recomputed each iteration instead of being hoisted out of the loop and kept
+// in registers for its whole length.
+__device__ __forceinline__ int opaque(int v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
+// Sign bit (0 or 0x80000000) of a pseudo-random +-1 for a (column, salt)
+// pair.  The matrix cores' fp32 accumulation rounds with a small bias; the
+// transforms flip the sign of each 16-point column's inputs and outputs by
+// this (an exact linear change), which turns that bias into noise instead of
+// letting it add up coherently in the lowest frequencies.
+__device__ __forceinline__ u32 dither_sign(u32 col, u32 salt) {
+    u32 h = (col ^ salt) * 0x9E3779B1u;
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    return h & 0x80000000u;
+}
+__device__ __forceinline__ float flip(float v, u32 sbit) { return __uint_as_float(__float_as_uint(v) ^ sbit); }
 
 // --------------------------------------------------------- exact small helpers
 

@@ -52,7 +52,8 @@ def _fwht64(x):
     return y / np.sqrt(n)
 
 
-@pytest.mark.parametrize("m,batch", [(8, 3), (12, 5), (14, 2), (15, 2), (17, 2), (20, 1), (22, 1)])
+@pytest.mark.parametrize("m,batch", [(8, 3), (9, 3), (10, 4), (12, 5), (14, 2), (15, 2), (17, 2), (20, 1), (22, 1),
+                                     (24, 1)])
 def test_fwht_mfma_accuracy(torch, m, batch):
     from qmctoolscl_amd.mfma import fwht_mfma, relative_errors
     rng = np.random.default_rng(m)
@@ -82,14 +83,25 @@ def test_fwht_mfma_smooth_integrand(torch):
     assert l2 <= L2_TOL and mx <= MAX_TOL, (l2, mx)
 
 
-@pytest.mark.parametrize("m,batch", [(8, 3), (11, 2), (13, 2), (14, 2), (16, 2), (19, 1), (20, 1), (21, 1)])
+def _bitrev(m):
+    i = np.arange(1 << m, dtype=np.int64)
+    r = np.zeros_like(i)
+    for b in range(m):
+        r |= ((i >> b) & 1) << (m - 1 - b)
+    return r
+
+
+@pytest.mark.parametrize("m,batch", [(8, 3), (9, 3), (10, 3), (11, 2), (12, 3), (13, 2), (14, 2), (16, 2), (19, 1),
+                                     (20, 1), (21, 1), (22, 1), (24, 1)])
 def test_fft_bro_mfma_accuracy(torch, m, batch):
+    """Every pass shape: one tile (2^8 .. 2^13), two passes, three passes
+    with a radix-2/4 or a radix-16 top level."""
     from qmctoolscl_amd.mfma import fft_bro_mfma, relative_errors
     rng = np.random.default_rng(100 + m)
     n = 1 << m
     xr = rng.standard_normal((batch, n))
     xi = rng.standard_normal((batch, n))
-    ir = np.array([int(format(i, "0%db" % m)[::-1], 2) for i in range(n)])
+    ir = _bitrev(m)
     want = np.fft.fft((xr + 1j * xi)[:, ir], norm="ortho")
     tr, ti = torch.from_numpy(xr.copy()).cuda(), torch.from_numpy(xi.copy()).cuda()
     fft_bro_mfma(1, batch, n // 2, tr, ti)
