@@ -417,6 +417,61 @@ def section_transforms(torch, q, dev, timed):
     return out
 
 
+def section_kernel_methods(torch, q, dev, timed):
+    """Gram-matrix transforms for fast QMC kernel methods: the first column of
+    a smoothness-1 shift-invariant kernel on lattice points (n = 2^20, d = 64,
+    4 generating vectors) evaluated from g in registers, then fft_bro; beside
+    it the chain that materializes the points (lat_gen_natural, the kernel in
+    torch, fft_bro).  Same for the order-1 digital kernel on a base-2 net
+    (32-bit, d = 64) with fwht."""
+    from qmctoolscl_amd import kernel_methods as km
+    U = np.uint64
+    r, m, d = 4, 20, 64
+    n = 1 << m
+    rng = np.random.Generator(np.random.SFC64(5))
+    g = 2 * rng.integers(0, 2 ** 19, size=(r, d), dtype=np.uint64) + U(1)
+    coef_row = np.array(km.lat_kernel_coef(1.0 / (1.0 + np.arange(d)) ** 2, 1))
+    gt = torch.from_numpy(g.view(np.int64)).to(dev)
+    ct = torch.from_numpy(np.tile(coef_row, (r, 1))).to(dev)
+    ms_f, _ = timed(lambda: km.lat_kernel_fft(1, gt, ct, n), 10, 2)
+    x = torch.empty((r, n, d), dtype=torch.float64, device=dev)
+    kr = torch.empty((r, n), dtype=torch.float64, device=dev)
+    ki = torch.zeros_like(kr)
+    twr = torch.empty(n, dtype=torch.float64, device=dev)
+    twi = torch.empty_like(twr)
+
+    def chain():
+        q.lat_gen_natural(U(r), U(n), U(d), U(0), gt, x, wait=False)
+        b = (x - 1.0) * x + 1.0 / 6.0
+        kr.copy_(torch.prod(1.0 + ct[:, None, :] * b, dim=-1))
+        ki.zero_()
+        q.fft_bro_1d_radix2(U(1), U(r), U(n // 2), twr, twi, kr, ki, wait=False)
+    ms_c, _ = timed(chain, 3, 1)
+    del x
+    torch.cuda.empty_cache()
+    # base-2 net, order-1 digital kernel, fwht
+    t = 32
+    C = np.empty((r, d, t), dtype=np.uint64)
+    for l in range(r):
+        for j in range(d):
+            above = rng.integers(0, 2 ** 31, size=t, dtype=np.uint64)
+            for k in range(t):
+                C[l, j, k] = (1 << (t - 1 - k)) | ((int(above[k]) & ((1 << k) - 1)) << (t - k))
+    Ct = torch.from_numpy(C.view(np.int64)).to(dev)
+    cn = torch.from_numpy(np.tile(0.9 / (1.0 + np.arange(d)), (r, 1))).to(dev)
+    ms_n, _ = timed(lambda: km.dnb2_kernel_fwht(Ct, cn, n, t), 10, 2)
+    ms_col, _ = timed(lambda: km.dnb2_kernel_column(Ct, cn, n, t), 10, 2)
+    ms_lcol, _ = timed(lambda: km.lat_kernel_column(1, gt, ct, n), 10, 2)
+    pts = r * n * d
+    return {"workload": "kernel-matrix transforms, r=4 point sets, n=2^20, d=64: lattice (smoothness-1 "
+                        "shift-invariant kernel, fft_bro) and 32-bit base-2 net (order-1 digital kernel, fwht)",
+            "lattice_fused_ms": round(ms_f, 4), "lattice_column_ms": round(ms_lcol, 4),
+            "lattice_chain_ms": round(ms_c, 4), "lattice_speedup_vs_chain": round(ms_c / ms_f, 2),
+            "lattice_Gevals_s": round(pts / ms_lcol / 1e6, 1),
+            "net_fused_ms": round(ms_n, 4), "net_column_ms": round(ms_col, 4),
+            "net_Gevals_s": round(pts / ms_col / 1e6, 1)}
+
+
 def section_halton(torch, dev, timed):
     """Generalized Halton (halton_qrng): n = 2^20 points in the first 32 prime
     bases, random digit shift, generalized factors; res is d x n float64."""
@@ -441,7 +496,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-rows", type=int, default=1 << 19)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--sections", default="main,dnb2,transforms,gdn,halton,batch")
+    ap.add_argument("--sections", default="main,dnb2,transforms,gdn,halton,kernels,batch")
     ap.add_argument("--quick-batch", action="store_true", help="cap request sizes at n=2^16")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one plain 2 GiB device-to-device copy (a known-size kernel that scales "
@@ -528,6 +583,9 @@ def main():
     if "gdn" in sections:
         log("section: gdn")
         out["sections"]["gdn"] = section_gdn(torch, q, dev, timed)
+    if "kernels" in sections:
+        log("section: kernel methods")
+        out["sections"]["kernel_methods"] = section_kernel_methods(torch, q, dev, timed)
     if "halton" in sections:
         log("section: halton")
         out["sections"]["halton"] = section_halton(torch, dev, timed)

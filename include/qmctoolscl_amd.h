@@ -201,6 +201,31 @@ QMC_EXPORT int ifft_bro_1d_radix2_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, do
 QMC_EXPORT int qmc_halton_qrng_dev(int n, int d, int n0, int generalized, double *res, const double *randu_d_32,
                                    const int *dvec, qmc_stream_t s);
 
+/* ------------------------------------------------- fast QMC kernel methods
+ * No single reference symbol: the reference's documentation chains its
+ * transforms on a kernel matrix's first column this way (docs/index.md:
+ * 1825-1883 with fft_bro/ifft_bro, 1929-1977 with fwht).  The column k of
+ * the Gram matrix of a product kernel on natural-order points (lattice:
+ * prod_j (1 + coef_j B_2alpha(x_ij)), alpha = 1, 2; base-2 net: prod_j
+ * (1 + coef_j w(X_ij)), w(X) = 1 - 3 2^(bitlen X - 1 - tmax), w(0) = 1) is
+ * evaluated from g or C without writing the points.  *_fft / *_fwht write the
+ * orthonormal transform kt of k (r vectors of n); *_apply overwrite y with
+ * K y (mode 0) or K^-1 y (mode 1) as ifft(fft(y) kt) sqrt(n), ifft(fft(y)/kt)
+ * / sqrt(n) (fwht for nets).  n is a power of two, r <= 65535. */
+QMC_EXPORT int qmc_lat_kernel_column_dev(int alpha, qmc_u64 r, qmc_u64 n, qmc_u64 d, const qmc_u64 *g,
+                                         const double *coef, double *k, qmc_stream_t s);
+QMC_EXPORT int qmc_dnb2_kernel_column_dev(qmc_u64 r, qmc_u64 n, qmc_u64 d, qmc_u64 mmax, int tmax, const qmc_u64 *C,
+                                          const double *coef, double *k, qmc_stream_t s);
+QMC_EXPORT int qmc_lat_kernel_fft_dev(int alpha, qmc_u64 r, qmc_u64 n, qmc_u64 d, const qmc_u64 *g,
+                                      const double *coef, double *twiddler, double *twiddlei, double *kr, double *ki,
+                                      qmc_stream_t s);
+QMC_EXPORT int qmc_dnb2_kernel_fwht_dev(qmc_u64 r, qmc_u64 n, qmc_u64 d, qmc_u64 mmax, int tmax, const qmc_u64 *C,
+                                        const double *coef, double *kt, qmc_stream_t s);
+QMC_EXPORT int qmc_lat_kernel_apply_dev(int mode, qmc_u64 r, qmc_u64 n, const double *kr, const double *ki,
+                                        double *twiddler, double *twiddlei, double *yr, double *yi, qmc_stream_t s);
+QMC_EXPORT int qmc_dnb2_kernel_apply_dev(int mode, qmc_u64 r, qmc_u64 n, const double *kt, double *y,
+                                         qmc_stream_t s);
+
 /* ------------------------------------------------------- fused pipelines
  * No single reference symbol: each equals a chain of reference calls as the
  * doctests chain them (docs/index.md:64-208, 419-677), in one HBM pass. */

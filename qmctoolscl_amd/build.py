@@ -20,7 +20,7 @@ LIB = os.path.join(HERE, "libqmctoolscl_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["capi.cpp", "lattice.hip", "dnb2.hip", "gdn.hip", "transforms.hip", "nus.hip", "mfma_fwht.hip", "mfma_fft.hip", "batch.hip", "halton.hip"]
+SOURCES = ["capi.cpp", "lattice.hip", "dnb2.hip", "gdn.hip", "transforms.hip", "nus.hip", "mfma_fwht.hip", "mfma_fft.hip", "batch.hip", "halton.hip", "kernel_methods.hip"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-fvisibility=hidden", "-ffp-contract=off",
           "-Wall", "-Wno-unused-function", "-Wno-unused-result"]
