@@ -26,8 +26,9 @@
 // columns, shifts, LMS rows, permutation parameters) in a workspace, then
 // persistent grids drain tile counters: one kernel for the lattice and dnb2
 // tiles (store-bound, few registers) and one for the gdn tiles (digit walks,
-// ~110 registers, ALU-bound), each compiled without the other's code and
-// launched back to back (one graph).  Outputs go to a ring buffer:
+// ~110 registers, ALU-bound), each compiled without the other's code; when a
+// batch has both, the gdn kernel runs on a second stream beside the other
+// (one graph either way).  Outputs go to a ring buffer:
 // every replication slab (n*d doubles) of every request is placed contiguously
 // at an offset chosen on the host, wrapping to the start when the ring is full,
 // so batches far larger than HBM can stream through a bounded buffer.
@@ -783,6 +784,8 @@ struct Plan {
     int device = 0;
     u32 lds_bytes = 0;               // dynamic LDS of the gdn kernel (digit tables)
     hipStream_t cap = nullptr;       // private stream for graph capture
+    hipStream_t side = nullptr;      // the gdn class runs here, beside the store-bound class
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipGraphExec_t gexec = nullptr;  // [counter reset, persistent kernel]
     double *gring = nullptr;         // ring the graph was captured for
     unsigned long long *digest = nullptr;   // per-slab digests, or null
@@ -792,12 +795,27 @@ struct Plan {
 
 constexpr u32 kLdsClass0 = 256 * sizeof(u32);  // dnb2 first-row values
 
-// The two classes run back to back on s, each with the full grid.  (Running
-// them on two streams measured no faster, with either grid split: each
-// persistent grid holds its CUs until its queue drains.)
+// With both classes present, the gdn kernel (the long tiles) is forked onto the
+// plan's side stream and joined back through events, so a graph captured from
+// s takes in both branches.  On large batches this changes nothing (each
+// persistent grid holds its CUs until its queue drains); on small ragged ones
+// the store-bound tiles finish under the gdn tiles (9% on a 48-request batch).
 template <bool DIG>
 int launch_batch_t(Plan *P, double *ring, hipStream_t s) {
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, P->d_counter);
+    if (P->qtiles[0] && P->qtiles[1]) {
+        QMC_HIP_CHECK(hipEventRecord(P->ev_fork, s));
+        QMC_HIP_CHECK(hipStreamWaitEvent(P->side, P->ev_fork, 0));
+        hipLaunchKernelGGL((batch_persistent_kernel<1, DIG>), dim3(P->grid), dim3(kBlock), P->lds_bytes, P->side,
+                           P->d_qreqs + P->qn[0], P->qn[1], P->qtiles[1], P->d_counter + 1, P->d_ws, ring,
+                           P->digest);
+        QMC_HIP_CHECK(hipEventRecord(P->ev_join, P->side));
+        hipLaunchKernelGGL((batch_persistent_kernel<0, DIG>), dim3(P->grid), dim3(kBlock), kLdsClass0, s, P->d_qreqs,
+                           P->qn[0], P->qtiles[0], P->d_counter, P->d_ws, ring, P->digest);
+        QMC_HIP_CHECK(hipStreamWaitEvent(s, P->ev_join, 0));
+        QMC_HIP_CHECK(hipGetLastError());
+        return QMC_OK;
+    }
     if (P->qtiles[0])
         hipLaunchKernelGGL((batch_persistent_kernel<0, DIG>), dim3(P->grid), dim3(kBlock), kLdsClass0, s, P->d_qreqs,
                            P->qn[0], P->qtiles[0], P->d_counter, P->d_ws, ring, P->digest);
@@ -816,6 +834,8 @@ int launch_batch(Plan *P, double *ring, hipStream_t s) {
 }  // namespace
 
 extern "C" {
+
+QMC_EXPORT void qmc_batch_free(void *h);
 
 // Build a plan for nreq requests: methods[q] in 0..3, m[q] = log2 n in [1, 31],
 // d[q] in [1, 256], r[q] >= 1, seeds[q]; `points_per_tile` (power of two) and
@@ -840,12 +860,12 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
             (q.method == M_GDN_LMS && q.m > kTB)) {
             qmc_fail(QMC_ERR_ARG, "batch: request %d has unsupported (method, m, d, r) = (%d, %d, %d, %d)", i,
                      q.method, q.m, q.d, q.r);
-            delete P;
+            qmc_batch_free(P);
             return nullptr;
         }
         const u64 n = 1ull << q.m;
         u64 rows = points_per_tile / (u64)q.d;
-        if (q.method == M_GDN_LMS && rows < 1024) rows = 1024;  // tables loaded once per tile; long chains
+        if (q.method == M_GDN_LMS && rows < 256) rows = 256;  // digit tables are loaded once per tile
         u64 p2 = 1;
         while (p2 * 2 <= rows) p2 *= 2;
         if (p2 > n) p2 = n;
@@ -866,7 +886,7 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
             if ((u64)q.gtw * 4 > 65536) {
                 qmc_fail(QMC_ERR_ARG, "batch: gdn request %d needs %u bytes of digit tables (max 65536)", i,
                          q.gtw * 4);
-                delete P;
+                qmc_batch_free(P);
                 return nullptr;
             }
             if (q.gtw * 4 > P->lds_bytes) P->lds_bytes = q.gtw * 4;
@@ -880,7 +900,7 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
         if (slab > ring_elems) {
             qmc_fail(QMC_ERR_ARG, "batch: a replication slab of request %d (%llu doubles) exceeds the ring", i,
                      (unsigned long long)slab);
-            delete P;
+            qmc_batch_free(P);
             return nullptr;
         }
     }
@@ -895,7 +915,11 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
             P->qreqs.push_back(q);
             P->qn[c]++;
         }
-    if (hipGetDevice(&P->device) != hipSuccess) { delete P; qmc_fail(QMC_ERR_HIP, "batch: no device"); return nullptr; }
+    if (hipGetDevice(&P->device) != hipSuccess) {
+        qmc_batch_free(P);
+        qmc_fail(QMC_ERR_HIP, "batch: no device");
+        return nullptr;
+    }
     int cus = 256;
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, P->device);
     P->grid = grid > 0 ? grid : 4 * cus;
@@ -912,7 +936,7 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
                                   (x.w == 3 && x.l == 2);
             if (!shape_ok) {
                 qmc_fail(QMC_ERR_ARG, "batch: gdn lane group with no kernel shape (w=%u, l=%u)", x.w, x.l);
-                delete P;
+                qmc_batch_free(P);
                 return nullptr;
             }
             gw[0] = x.j0 | (x.gc << 16) | (x.ch << 24) | ((u64)x.w << 32) | ((u64)x.l << 40);
@@ -926,7 +950,7 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
                                 std::pow((double)b * b, (double)(np > L ? np - L : 0)) < 0x1p53)) {
                     qmc_fail(QMC_ERR_ARG, "batch: gdn base %u does not split into exact halves (W=%u, L=%u)", b,
                              (u32)W, (u32)L);
-                    delete P;
+                    qmc_batch_free(P);
                     return nullptr;
                 }
                 double scale = 1.0;
@@ -958,7 +982,7 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
         hipMalloc((void **)&P->d_qreqs, sizeof(Req) * P->qreqs.size()) != hipSuccess ||
         hipMalloc((void **)&P->d_counter, 2 * sizeof(unsigned long long)) != hipSuccess) {
         qmc_fail(QMC_ERR_NOMEM, "batch: device allocation failed");
-        delete P;
+        qmc_batch_free(P);
         return nullptr;
     }
     hipMemcpy(P->d_reqs, P->reqs.data(), sizeof(Req) * P->reqs.size(), hipMemcpyHostToDevice);
@@ -969,7 +993,15 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
                        P->d_ws);
     if (hipDeviceSynchronize() != hipSuccess) {
         qmc_fail(QMC_ERR_HIP, "batch: setup kernel failed");
-        delete P;
+        qmc_batch_free(P);
+        return nullptr;
+    }
+    // the second class's stream and the events that fork it from, and join it to, the caller's stream
+    if (hipStreamCreateWithFlags(&P->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&P->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&P->ev_join, hipEventDisableTiming) != hipSuccess) {
+        qmc_fail(QMC_ERR_HIP, "batch: stream or event creation failed");
+        qmc_batch_free(P);
         return nullptr;
     }
     return P;
@@ -980,6 +1012,9 @@ QMC_EXPORT void qmc_batch_free(void *h) {
     if (!P) return;
     if (P->gexec) hipGraphExecDestroy(P->gexec);
     if (P->cap) hipStreamDestroy(P->cap);
+    if (P->side) hipStreamDestroy(P->side);
+    if (P->ev_fork) hipEventDestroy(P->ev_fork);
+    if (P->ev_join) hipEventDestroy(P->ev_join);
     hipFree(P->d_reqs);
     hipFree(P->d_ws);
     hipFree(P->d_counter);
