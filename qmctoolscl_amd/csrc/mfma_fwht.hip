@@ -104,9 +104,10 @@ __device__ void h2_axes(float *tile, int T, int a, int k) {
 // One pass over tiles of 2^lr rows x 2^lc columns of one vector (global
 // index = v*2^m + row*row_stride + cb*2^lc + col), applying the Hadamard
 // product along tile bits [tb0, tb0 + nb).  IN/OUT: 0 fp32, 1 float64 (the
-// output is then scaled by `scale`).  Workgroups are persistent: each walks
-// tiles blockIdx.x, +gridDim.x, ... with all NP value pairs of a thread
-// (2^T = 2 NP blockDim.x) loaded at once.
+// output is then scaled by `scale`).  A workgroup walks tiles blockIdx.x,
+// +gridDim.x, ... (the launches below use one workgroup per tile, which
+// measured faster than fewer resident ones), with all NP value pairs of a
+// thread (2^T = 2 NP blockDim.x) loaded at once.
 template <int IN64, int OUT64, int NP>
 __global__ __launch_bounds__(kThreads) void fwht_mfma_pass(int m, int lr, int lc, u64 row_stride, int tb0, int nb,
                                                          const void *__restrict__ src, void *__restrict__ dst,
@@ -133,7 +134,15 @@ __global__ __launch_bounds__(kThreads) void fwht_mfma_pass(int m, int lr, int lc
             const In2 *s0 = (const In2 *)src + gidx(tix, tid) / 2;
             In2 r[NP];
 #pragma unroll
-            for (int p = 0; p < NP; p++) r[p] = __builtin_nontemporal_load(s0 + poff(p) / 2);
+            for (int p = 0; p < NP; p++) {
+                // the caller's float64 input streams through; the fp32 intermediate is read with ordinary
+                // loads, so the L2 keeps whole lines for the neighbouring tile when a tile row is narrower
+                // than a line (2^24: 15% faster)
+                if (IN64)
+                    r[p] = __builtin_nontemporal_load(s0 + poff(p) / 2);
+                else
+                    r[p] = s0[poff(p) / 2];
+            }
 #pragma unroll
             for (int p = 0; p < NP; p++) {
                 const int t = 2 * (tid + p * blockDim.x);
@@ -214,16 +223,13 @@ QMC_EXPORT int qmc_fwht_mfma_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, double 
         QMC_HIP_CHECK(hipFuncSetAttribute((const void *)fwht_mfma_pass<0, 1, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
         attr_set = true;
     }
-    const u64 cus = (u64)device_cus();
     if (m <= kTileLog2) {
         // one vector per tile, 16 values per thread (m >= 10), fewer below
         const int np = m >= 10 ? 8 : 1 << (m - 7);
         const unsigned threads = (1u << m) / (2u * np);
         const size_t shm = padded_bytes(m);
-        const void *fn = np == 8 ? (const void *)fwht_mfma_pass<1, 1, 8>
-                                 : np == 4 ? (const void *)fwht_mfma_pass<1, 1, 4>
-                                           : (const void *)fwht_mfma_pass<1, 1, 2>;
-        const unsigned grid = (unsigned)std::min<u64>(V, (u64)resident_per_cu(fn, (int)threads, shm) * cus);
+        // one workgroup per vector (10% faster at 2^12 than resident workgroups looping over vectors)
+        const unsigned grid = (unsigned)std::min<u64>(V, 1u << 30);
         switch (np) {
             case 8:
                 hipLaunchKernelGGL((fwht_mfma_pass<1, 1, 8>), dim3(grid), dim3(threads), shm, s, m, 0, m,
@@ -243,19 +249,22 @@ QMC_EXPORT int qmc_fwht_mfma_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, double 
     }
     float *buf = work;
     if (!buf) QMC_HIP_CHECK(hipMallocAsync((void **)&buf, sizeof(float) * V * n, s));
-    // tiles of 2^14 values, 1024 threads x 8 pairs, one workgroup per tile (measured faster here than a
-    // grid of resident workgroups looping over tiles)
-    const u64 ntiles = V << (m - kTileLog2);
-    const size_t shm = padded_bytes(kTileLog2);
-    const unsigned grid1 = (unsigned)std::min<u64>(ntiles, 1u << 30), grid2 = grid1;
-    // pass 1: the low kTileLog2 bits over contiguous tiles
-    const int m1 = kTileLog2, m2 = m - kTileLog2;
-    hipLaunchKernelGGL((fwht_mfma_pass<1, 0, 8>), dim3(grid1), dim3(kThreads), shm, s, m, 0, m1, (u64)0, 0, m1,
-                       (const void *)x, (void *)buf, 1.0, ntiles);
-    // pass 2: the high m2 bits; a tile is 2^m2 rows (stride 2^m1) x 2^(14-m2) columns
+    // pass 1: the low m1 bits over contiguous tiles of 2^m1 values, 8 value pairs per thread; pass 2:
+    // the high m2 = m - m1 bits over tiles of 2^m2 rows (stride 2^m1) x 2^(14 - m2) columns, 1024
+    // threads.  One workgroup per tile (measured faster here than resident workgroups looping over
+    // tiles).  m1 = 13 (four 34 KiB workgroups per CU in pass 1, 4% faster than 14 at 2^20) unless
+    // pass 2 would then get rows of fewer than 32 floats.
+    int m1 = std::max(13, m - 9);
+    if (m1 > kTileLog2) m1 = kTileLog2;
+    const int m2 = m - m1;
+    const u64 ntiles1 = V << (m - m1), ntiles2 = V << (m - kTileLog2);
+    const unsigned grid1 = (unsigned)std::min<u64>(ntiles1, 1u << 30);
+    const unsigned grid2 = (unsigned)std::min<u64>(ntiles2, 1u << 30);
+    hipLaunchKernelGGL((fwht_mfma_pass<1, 0, 8>), dim3(grid1), dim3((1u << m1) / 16), padded_bytes(m1), s, m, 0, m1,
+                       (u64)0, 0, m1, (const void *)x, (void *)buf, 1.0, ntiles1);
     const int lc = kTileLog2 - m2;
-    hipLaunchKernelGGL((fwht_mfma_pass<0, 1, 8>), dim3(grid2), dim3(kThreads), shm, s, m, m2, lc, (u64)1 << m1, lc,
-                       m2, (const void *)buf, (void *)x, scale, ntiles);
+    hipLaunchKernelGGL((fwht_mfma_pass<0, 1, 8>), dim3(grid2), dim3(kThreads), padded_bytes(kTileLog2), s, m, m2, lc,
+                       (u64)1 << m1, lc, m2, (const void *)buf, (void *)x, scale, ntiles2);
     hipError_t e = hipGetLastError();
     if (!work) (void)hipFreeAsync(buf, s);
     if (e != hipSuccess) return qmc_fail_hip(e, "fwht_mfma_pass");
