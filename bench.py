@@ -361,7 +361,7 @@ def section_batch_vs_chain(torch, q, dev, timed):
 
 
 def section_transforms(torch, q, dev, timed0):
-    from qmctoolscl_amd.mfma import fft_bro_mfma, fwht_mfma, relative_errors
+    from qmctoolscl_amd.mfma import fft_bro_mfma, fwht_mfma, ifft_bro_mfma, relative_errors
 
     def timed(fn, steps, warmup):
         # the best of three rounds for every path alike (the FP64 and the matrix-core
@@ -399,7 +399,17 @@ def section_transforms(torch, q, dev, timed0):
         del yr, yi, zr, zi
         fwork = torch.empty(2 * V * n, dtype=torch.float32, device=dev)
         msfm, _ = timed(lambda: fft_bro_mfma(1, V, n // 2, xr, xi, fwork), 20, 3)
-        del fwork
+        # inverse: float64 drop-in and matrix cores, accuracy on the same input
+        msi, _ = timed(lambda: q.ifft_bro_1d_radix2(U(1), U(V), U(n // 2), twr, twi, xr, xi, wait=False), 20, 3)
+        msim, _ = timed(lambda: ifft_bro_mfma(1, V, n // 2, xr, xi, fwork), 20, 3)
+        yr, yi = x0.clone(), x0.flip(-1).contiguous()
+        q.ifft_bro_1d_radix2(U(1), U(V), U(n // 2), twr, twi, yr, yi)
+        zr, zi = x0.clone(), x0.flip(-1).contiguous()
+        ifft_bro_mfma(1, V, n // 2, zr, zi)
+        torch.cuda.synchronize()
+        il2, imx = relative_errors(np.stack([zr.cpu().numpy(), zi.cpu().numpy()]),
+                                   np.stack([yr.cpu().numpy(), yi.cpu().numpy()]))
+        del yr, yi, zr, zi, fwork
         pts = V * n
         # MFMA work: per 16-point axis, two 16x16x32 products per 256 outputs
         axes = m // 4
@@ -414,6 +424,9 @@ def section_transforms(torch, q, dev, timed0):
                     "fft_mfma_ms": round(msfm, 4), "fft_mfma_GBs": round(32 * pts / msfm / 1e6, 1),
                     "fft_mfma_speedup": round(msf / msfm, 3),
                     "fft_mfma_rel_l2": float("%.3g" % fl2), "fft_mfma_rel_max": float("%.3g" % fmx),
+                    "ifft_fp64_ms": round(msi, 4), "ifft_mfma_ms": round(msim, 4),
+                    "ifft_mfma_speedup": round(msi / msim, 3),
+                    "ifft_mfma_rel_l2": float("%.3g" % il2), "ifft_mfma_rel_max": float("%.3g" % imx),
                     # 12 products of 16x16x32 per 256 complex values per radix-16 level
                     "fft_mfma_util": float("%.3g" % (pts * (m // 4) * 768 / (msfm * 1e-3) /
                                                      (BF16_PEAK_TFLOPS * 1e12)))})

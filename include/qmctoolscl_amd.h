@@ -283,6 +283,8 @@ QMC_EXPORT int qmc_fwht_mfma_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, double 
  * may be NULL.  Writes no twiddle table. */
 QMC_EXPORT int qmc_fft_bro_mfma_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, double *xr, double *xi, float *work,
                                     qmc_stream_t s);
+QMC_EXPORT int qmc_ifft_bro_mfma_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, double *xr, double *xi, float *work,
+                                     qmc_stream_t s); /* ifft_bro_1d_radix2.c:3-17, reduced precision */
 /* one 16x16x32 bf16 MFMA on lane-major operands (layout check) */
 QMC_EXPORT int qmc_mfma_probe_dev(const float *a, const float *b, float *d, qmc_stream_t s);
 

@@ -70,18 +70,20 @@ struct Wtile {
     bf16x8 Ar[3], Ai[3];
 };
 
-__device__ Wtile make_w(int lane) {
+// forward: W[m][c] = w16^(m brev4(c)); inverse: the conjugate transpose,
+// W'[m][c] = w16^(-c brev4(m))
+__device__ Wtile make_w(int lane, bool inverse) {
     const int m = lane & 15, g = lane >> 4;
     Wtile w;
 #pragma unroll
     for (int i = 0; i < 8; i++) {
         const int k = g * 8 + i;
         const int c = k & 15;
-        // w16^(m * brev4(c)) = cos(2 pi e/16) - i sin(2 pi e/16), generated in FP64
-        const int e = (m * brev_bits(c, 4)) & 15;
+        // w16^e = cos(2 pi e/16) - i sin(2 pi e/16), generated in FP64
+        const int e = inverse ? (c * brev_bits(m, 4)) & 15 : (m * brev_bits(c, 4)) & 15;
         double s, co;
         sincospi(2.0 * e / 16.0, &s, &co);
-        const double wr = co, wi = -s;
+        const double wr = co, wi = inverse ? s : -s;
         const double vr = k < 16 ? wr : -wi;  // [Wr | -Wi]
         const double vi = k < 16 ? wi : wr;   // [Wi | Wr]
         __bf16 a0, a1, a2;
@@ -162,6 +164,7 @@ __device__ __forceinline__ f32x2 cmul(f32x2 a, f32x2 b) {
 // radix-16 level on tile bits [ta, ta+4), level `lev` of the pass; with tw,
 // output k of the column with in-tile bits c above the level (b of them) is
 // multiplied by w_{2^(b+4)}^(brev_b(c) k) * f[lev][k]
+template <bool INV>
 __device__ void dft16_level(float *re, float *im, int T, int ta, bool tw, int wave, int lane, const TwLds &tl,
                             int lev, u32 salt) {
     const Wtile W = tl.w[lane];
@@ -175,11 +178,36 @@ __device__ void dft16_level(float *re, float *im, int T, int ta, bool tw, int wa
         const int lo = col & ((1 << ta) - 1);
         const int base = ((col >> ta) << (ta + 4)) | lo;
         const u32 sg = dither_sign((u32)col, salt);
+        const int rb = b > 0 ? brev_bits(col >> ta, b) : 0;
+        // twiddle of input/output row k of this column: w_{2^(b+4)}^(brev_b(c_in) k) f[lev][k]
+        auto twid = [&](int k) -> f32x2 {
+            f32x2 w = tl.f[lev][k];
+            if (b > 0) {
+                const int J = ((rb * k) & ((16 << b) - 1)) << (9 - b);  // w_{2^(b+4)}^j = w_{2^13}^(j 2^(9-b))
+                w = cmul(cmul(tl.t1[J >> 7], tl.t2[J & 127]), w);
+            }
+            return w;
+        };
         bf16x8 x[3];
 #pragma unroll
         for (int i = 0; i < 8; i++) {
             __bf16 s0, s1, s2;
-            split3f(flip(src[pad(base + ((c0 + i) << ta))], sg), s0, s1, s2);
+            float v;
+            if (INV) {
+                // the inverse applies the conjugate twiddle to the inputs (the transposed network)
+                const int k = c0 + i;
+                float vr = re[pad(base + (k << ta))], vi = im[pad(base + (k << ta))];
+                if (tw) {
+                    const f32x2 w = twid(k);
+                    const float tr = vr * w.x + vi * w.y;
+                    vi = vi * w.x - vr * w.y;
+                    vr = tr;
+                }
+                v = g < 2 ? vr : vi;
+            } else {
+                v = src[pad(base + ((c0 + i) << ta))];
+            }
+            split3f(flip(v, sg), s0, s1, s2);
             x[0][i] = s0;
             x[1][i] = s1;
             x[2][i] = s2;
@@ -199,18 +227,13 @@ __device__ void dft16_level(float *re, float *im, int T, int ta, bool tw, int wa
         yi = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.Ai[0], x[1], yi, 0, 0, 0);
         yi = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.Ai[0], x[0], yi, 0, 0, 0);
         // lane holds outputs k = 4g + i of column n
-        const int rb = b > 0 ? brev_bits(col >> ta, b) : 0;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const int k = 4 * g + i;
             const int t = base + (k << ta);
             float vr = flip(yr[i], sg), vi = flip(yi[i], sg);
-            if (tw) {
-                f32x2 w = tl.f[lev][k];
-                if (b > 0) {
-                    const int J = ((rb * k) & ((16 << b) - 1)) << (9 - b);  // w_{2^(b+4)}^j = w_{2^13}^(j 2^(9-b))
-                    w = cmul(cmul(tl.t1[J >> 7], tl.t2[J & 127]), w);
-                }
+            if (!INV && tw) {
+                const f32x2 w = twid(k);
                 const float tr = vr * w.x - vi * w.y;
                 vi = vr * w.y + vi * w.x;
                 vr = tr;
@@ -222,6 +245,7 @@ __device__ void dft16_level(float *re, float *im, int T, int ta, bool tw, int wa
 }
 
 // leftover top level of R = 2^rb points (rb < 4) along tile bits [ta, ta+rb)
+template <bool INV>
 __device__ void dft_small_level(float *re, float *im, int T, int ta, int rb) {
     const int R = 1 << rb;
     const int groups = 1 << (T - rb);
@@ -236,9 +260,10 @@ __device__ void dft_small_level(float *re, float *im, int T, int ta, int rb) {
         for (int k = 0; k < R; k++) {
             float ar = 0.f, ai = 0.f;
             for (int j = 0; j < R; j++) {
-                const int e = (k * brev_bits(j, rb)) & (R - 1);
+                // forward W[k][j] = w_R^(k brev(j)); inverse W'[k][j] = w_R^(-j brev(k))
+                const int e = INV ? (j * brev_bits(k, rb)) & (R - 1) : (k * brev_bits(j, rb)) & (R - 1);
                 float s, c;
-                sincospif(-2.0f * (float)e / (float)R, &s, &c);
+                sincospif((INV ? 2.0f : -2.0f) * (float)e / (float)R, &s, &c);
                 ar += xr[j] * c - xi[j] * s;
                 ai += xr[j] * s + xi[j] * c;
             }
@@ -252,7 +277,7 @@ __device__ void dft_small_level(float *re, float *im, int T, int ta, int rb) {
 // float64.  Workgroups are persistent: each walks tiles blockIdx.x,
 // +gridDim.x, ... with all NP value pairs (of re and of im) of a thread
 // (2^T = 2 NP blockDim.x) loaded at once.
-template <int IN64, int OUT64, int NP>
+template <int IN64, int OUT64, int NP, bool INV>
 __global__ __launch_bounds__(kThreads) void fft_mfma_pass(PassGeom G, int glo, int ghi, const void *__restrict__ sr,
                                                         const void *__restrict__ si, void *__restrict__ dr,
                                                         void *__restrict__ di, double scale, Twiddles tw, u64 ntiles) {
@@ -266,7 +291,7 @@ __global__ __launch_bounds__(kThreads) void fft_mfma_pass(PassGeom G, int glo, i
     const int tlog = G.m - T;
     const u64 cmask_tile = (1ull << tlog) - 1;
     if (blockIdx.x >= ntiles) return;
-    if (threadIdx.x < 64) tl.w[threadIdx.x] = make_w(threadIdx.x);
+    if (threadIdx.x < 64) tl.w[threadIdx.x] = make_w(threadIdx.x, INV);
     for (int i = threadIdx.x; i < 192; i += blockDim.x) {
         double sn, cs;
         if (i < 64)
@@ -330,14 +355,17 @@ __global__ __launch_bounds__(kThreads) void fft_mfma_pass(PassGeom G, int glo, i
             tl.f[lev][k] = w;
         }
         __syncthreads();
-        for (int lev = 0, ga = glo; ga < ghi; ga += 4, lev++) {
+        // forward: levels upward; inverse (the transposed network): downward, twiddles on the inputs
+        for (int step = 0; step < nlev; step++) {
+            const int lev = INV ? nlev - 1 - step : step;
+            const int ga = glo + 4 * lev;
             const int ltid = opaque(threadIdx.x);
             const int ta = ga < G.lc ? ga : G.lc + (ga - G.lo);  // tile bit of global bit ga
             if (ghi - ga >= 4)
-                dft16_level(re, im, T, ta, G.m - ga - 4 > 0, ltid >> 6, ltid & 63, tl, lev,
-                            (u32)tix * 0x632BE5ABu + (u32)ga * 0x27D4EB2Fu);
+                dft16_level<INV>(re, im, T, ta, G.m - ga - 4 > 0, ltid >> 6, ltid & 63, tl, lev,
+                                 (u32)tix * 0x632BE5ABu + (u32)ga * 0x27D4EB2Fu);
             else
-                dft_small_level(re, im, T, ta, ghi - ga);  // only the top level can be short
+                dft_small_level<INV>(re, im, T, ta, ghi - ga);  // only the top level can be short
             __syncthreads();
         }
         const int stid = opaque(threadIdx.x);
@@ -368,33 +396,58 @@ __global__ __launch_bounds__(kThreads) void fft_mfma_pass(PassGeom G, int glo, i
     }
 }
 
-}  // namespace
+// launch one pass of direction INV with float64 input (in) / output (out) chosen at run time
+template <bool INV>
+int launch_pass(bool in, bool out, int threads, size_t shm, unsigned grid, hipStream_t s, const PassGeom &G, int glo,
+                int ghi, const void *sr, const void *si, void *dr, void *di, double scale, const Twiddles &tw,
+                u64 ntiles) {
+    if (in && out)
+        hipLaunchKernelGGL((fft_mfma_pass<1, 1, 8, INV>), dim3(grid), dim3(threads), shm, s, G, glo, ghi, sr, si, dr,
+                           di, scale, tw, ntiles);
+    else if (in)
+        hipLaunchKernelGGL((fft_mfma_pass<1, 0, 8, INV>), dim3(grid), dim3(threads), shm, s, G, glo, ghi, sr, si, dr,
+                           di, scale, tw, ntiles);
+    else if (out)
+        hipLaunchKernelGGL((fft_mfma_pass<0, 1, 8, INV>), dim3(grid), dim3(threads), shm, s, G, glo, ghi, sr, si, dr,
+                           di, scale, tw, ntiles);
+    else
+        hipLaunchKernelGGL((fft_mfma_pass<0, 0, 8, INV>), dim3(grid), dim3(threads), shm, s, G, glo, ghi, sr, si, dr,
+                           di, scale, tw, ntiles);
+    QMC_HIP_CHECK(hipGetLastError());
+    return QMC_OK;
+}
 
-extern "C" {
+template <bool INV>
+const void *pass_fn(bool in, bool out) {
+    return in && out ? (const void *)fft_mfma_pass<1, 1, 8, INV>
+           : in      ? (const void *)fft_mfma_pass<1, 0, 8, INV>
+           : out     ? (const void *)fft_mfma_pass<0, 1, 8, INV>
+                     : (const void *)fft_mfma_pass<0, 0, 8, INV>;
+}
 
-// In-place orthonormal FFT of d1*d2 split-complex vectors of length 2*n_half,
-// input bit-reversed, output natural (as fft_bro_1d_radix2), on the matrix
-// cores.  Lengths 2^8 .. 2^28.  `work` (2 * d1*d2*2*n_half floats) may be
-// NULL to let the library allocate it on the stream.
-QMC_EXPORT int qmc_fft_bro_mfma_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, double *xr, double *xi, float *work,
-                                    qmc_stream_t s) {
+// The forward transform (INV = false, fft_bro_1d_radix2's) runs its passes
+// from the lowest index bits up; the inverse (ifft_bro_1d_radix2's) runs the
+// conjugate transpose of the same network: the same passes in reverse order,
+// each level's conjugated twiddles applied to its inputs and the conjugate
+// transposed DFT-16 tiles.
+template <bool INV>
+int run_mfma_fft(const char *who, u64 d1, u64 d2, u64 n_half, double *xr, double *xi, float *work, hipStream_t s) {
     const u64 V = d1 * d2;
     if (V == 0 || n_half == 0) return QMC_OK;
     const u64 n = 2 * n_half;
     int m = 0;
     while ((1ull << m) < n) m++;
-    if ((1ull << m) != n) return qmc_fail(QMC_ERR_ARG, "fft_mfma: length is not a power of two");
-    if (m < 8 || m > 28) return qmc_fail(QMC_ERR_ARG, "fft_mfma: length 2^%d outside 2^8 .. 2^28", m);
+    if ((1ull << m) != n) return qmc_fail(QMC_ERR_ARG, "%s: length is not a power of two", who);
+    if (m < 8 || m > 28) return qmc_fail(QMC_ERR_ARG, "%s: length 2^%d outside 2^8 .. 2^28", who, m);
     if (((uintptr_t)xr | (uintptr_t)xi | (uintptr_t)work) % 16)
-        return qmc_fail(QMC_ERR_ARG, "fft_mfma: xr, xi and work must be 16-byte aligned");
+        return qmc_fail(QMC_ERR_ARG, "%s: xr, xi and work must be 16-byte aligned", who);
     const double scale = (m % 2 == 0) ? ldexp(1.0, -m / 2) : ldexp(1.0, -(m - 1) / 2) / 1.4142135623730951;
     static bool attr_set = false;
     if (!attr_set) {  // padded tiles need more than the default 64 KiB of LDS
         const int bytes = (int)shm_bytes(kT);
-        QMC_HIP_CHECK(hipFuncSetAttribute((const void *)fft_mfma_pass<1, 1, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-        QMC_HIP_CHECK(hipFuncSetAttribute((const void *)fft_mfma_pass<1, 0, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-        QMC_HIP_CHECK(hipFuncSetAttribute((const void *)fft_mfma_pass<0, 0, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-        QMC_HIP_CHECK(hipFuncSetAttribute((const void *)fft_mfma_pass<0, 1, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+        for (int k = 0; k < 4; k++)
+            QMC_HIP_CHECK(hipFuncSetAttribute(pass_fn<INV>(k & 1, k >> 1), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                              bytes));
         attr_set = true;
     }
     const u64 cus = (u64)device_cus();
@@ -409,66 +462,84 @@ QMC_EXPORT int qmc_fft_bro_mfma_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, doub
     hipLaunchKernelGGL(twiddle_tables_kernel, dim3((unsigned)((ntw + kThreads / 4 - 1) / (kThreads / 4))),
                        dim3(kThreads / 4), 0, s, m, tw.klo, tt, tt + (1ull << tw.klo));
     // persistent grid: as many workgroups as stay resident, at most one per tile
-    auto grid_of = [&](const void *fn, int threads, size_t shm, u64 ntiles, unsigned &grid) -> int {
-        grid = (unsigned)std::min<u64>(ntiles, (u64)resident_per_cu(fn, threads, shm) * cus);
-        return QMC_OK;
+    auto grid_of = [&](const void *fn, int threads, size_t shm, u64 ntiles) -> unsigned {
+        return (unsigned)std::min<u64>(ntiles, (u64)resident_per_cu(fn, threads, shm) * cus);
     };
     float *buf = nullptr;
-    unsigned grid = 0;
+    int rc = QMC_OK;
     if (m <= kT) {
         PassGeom G{m, 0, m, m, m};
         // one vector per tile, 8 value pairs per thread (m >= 10), fewer below
         const int np = m >= 10 ? 8 : 1 << (m - 7);
         const int threads = (1 << m) / (2 * np);
         const size_t shm = shm_bytes(m);
-        if (np == 8) {
-            if (int rc = grid_of((const void *)fft_mfma_pass<1, 1, 8>, threads, shm, V, grid)) return rc;
-            hipLaunchKernelGGL((fft_mfma_pass<1, 1, 8>), dim3(grid), dim3(threads), shm, s, G, 0, m,
+        const void *fn = np == 8 ? (const void *)fft_mfma_pass<1, 1, 8, INV>
+                         : np == 4 ? (const void *)fft_mfma_pass<1, 1, 4, INV>
+                                   : (const void *)fft_mfma_pass<1, 1, 2, INV>;
+        const unsigned grid = grid_of(fn, threads, shm, V);
+        if (np == 8)
+            hipLaunchKernelGGL((fft_mfma_pass<1, 1, 8, INV>), dim3(grid), dim3(threads), shm, s, G, 0, m,
                                (const void *)xr, (const void *)xi, (void *)xr, (void *)xi, scale, tw, V);
-        } else if (np == 4) {
-            if (int rc = grid_of((const void *)fft_mfma_pass<1, 1, 4>, threads, shm, V, grid)) return rc;
-            hipLaunchKernelGGL((fft_mfma_pass<1, 1, 4>), dim3(grid), dim3(threads), shm, s, G, 0, m,
+        else if (np == 4)
+            hipLaunchKernelGGL((fft_mfma_pass<1, 1, 4, INV>), dim3(grid), dim3(threads), shm, s, G, 0, m,
                                (const void *)xr, (const void *)xi, (void *)xr, (void *)xi, scale, tw, V);
-        } else {
-            if (int rc = grid_of((const void *)fft_mfma_pass<1, 1, 2>, threads, shm, V, grid)) return rc;
-            hipLaunchKernelGGL((fft_mfma_pass<1, 1, 2>), dim3(grid), dim3(threads), shm, s, G, 0, m,
+        else
+            hipLaunchKernelGGL((fft_mfma_pass<1, 1, 2, INV>), dim3(grid), dim3(threads), shm, s, G, 0, m,
                                (const void *)xr, (const void *)xi, (void *)xr, (void *)xi, scale, tw, V);
-        }
+        if (hipGetLastError() != hipSuccess) rc = qmc_fail(QMC_ERR_HIP, "%s: launch failed", who);
     } else {
         buf = work;
         if (!buf) QMC_HIP_CHECK(hipMallocAsync((void **)&buf, 2 * sizeof(float) * V * n, s));
         float *br = buf, *bi = buf + V * n;
         const size_t shm = shm_bytes(kT);
         const u64 ntiles = V << (m - kT);
-        // pass 1: global bits [0, 12) over contiguous tiles of 2^13
-        {
-            PassGeom G{m, 0, kT, kT, kT};
-            if (int rc = grid_of((const void *)fft_mfma_pass<1, 0, 8>, kThreads, shm, ntiles, grid)) return rc;
-            hipLaunchKernelGGL((fft_mfma_pass<1, 0, 8>), dim3(grid), dim3(kThreads), shm, s, G, 0, 12,
-                               (const void *)xr, (const void *)xi, (void *)br, (void *)bi, 1.0, tw, ntiles);
-        }
-        // then 8 bits (two levels) per pass: tiles of 2^8 rows x 2^5 columns
+        // passes of the forward order: global bits [0, 12) over contiguous tiles of 2^13, then 8 bits (two
+        // levels) per pass over tiles of 2^8 rows x 2^5 columns
+        struct P {
+            PassGeom G;
+            int glo, ghi;
+        };
+        P passes[4];
+        int np = 0;
+        passes[np++] = P{PassGeom{m, 0, kT, kT, kT}, 0, 12};
         for (int lo = 12; lo < m; lo += 8) {
             const int hi = lo + 8 < m ? lo + 8 : m;
             const int lr = hi - lo, lc = kT - lr;
-            PassGeom G{m, lr, lc, lo, hi};
-            const bool last = hi == m;
-            if (last) {
-                if (int rc = grid_of((const void *)fft_mfma_pass<0, 1, 8>, kThreads, shm, ntiles, grid)) return rc;
-                hipLaunchKernelGGL((fft_mfma_pass<0, 1, 8>), dim3(grid), dim3(kThreads), shm, s, G, lo, hi,
-                                   (const void *)br, (const void *)bi, (void *)xr, (void *)xi, scale, tw, ntiles);
-            } else {
-                if (int rc = grid_of((const void *)fft_mfma_pass<0, 0, 8>, kThreads, shm, ntiles, grid)) return rc;
-                hipLaunchKernelGGL((fft_mfma_pass<0, 0, 8>), dim3(grid), dim3(kThreads), shm, s, G, lo, hi,
-                                   (const void *)br, (const void *)bi, (void *)br, (void *)bi, 1.0, tw, ntiles);
-            }
+            passes[np++] = P{PassGeom{m, lr, lc, lo, hi}, lo, hi};
+        }
+        for (int step = 0; step < np && !rc; step++) {
+            const P &q = passes[INV ? np - 1 - step : step];
+            const bool in = step == 0, out = step == np - 1;
+            const unsigned grid = grid_of(pass_fn<INV>(in, out), kThreads, shm, ntiles);
+            rc = launch_pass<INV>(in, out, kThreads, shm, grid, s, q.G, q.glo, q.ghi,
+                                  in ? (const void *)xr : (const void *)br, in ? (const void *)xi : (const void *)bi,
+                                  out ? (void *)xr : (void *)br, out ? (void *)xi : (void *)bi, out ? scale : 1.0,
+                                  tw, ntiles);
         }
     }
-    hipError_t e = hipGetLastError();
     if (buf && !work) (void)hipFreeAsync(buf, s);
     (void)hipFreeAsync(tt, s);
-    if (e != hipSuccess) return qmc_fail_hip(e, "fft_mfma_pass");
-    return QMC_OK;
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+// In-place orthonormal FFT of d1*d2 split-complex vectors of length 2*n_half,
+// input bit-reversed, output natural (as fft_bro_1d_radix2), on the matrix
+// cores.  Lengths 2^8 .. 2^28.  `work` (2 * d1*d2*2*n_half floats) may be
+// NULL to let the library allocate it on the stream.
+QMC_EXPORT int qmc_fft_bro_mfma_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, double *xr, double *xi, float *work,
+                                    qmc_stream_t s) {
+    return run_mfma_fft<false>("fft_mfma", d1, d2, n_half, xr, xi, work, s);
+}
+
+// In-place orthonormal inverse FFT with natural-order input and bit-reversed
+// output (as ifft_bro_1d_radix2), on the matrix cores; same arguments.
+QMC_EXPORT int qmc_ifft_bro_mfma_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, double *xr, double *xi, float *work,
+                                     qmc_stream_t s) {
+    return run_mfma_fft<true>("ifft_mfma", d1, d2, n_half, xr, xi, work, s);
 }
 
 }  // extern "C"

@@ -9,6 +9,9 @@ fwht_1d_radix2 for the bit-exact result.  Lengths 2^8 .. 2^24.
 
 fft_bro_mfma(d1, d2, n_half, xr, xi): fft_bro_1d_radix2's transform (input
 bit-reversed, output natural, orthonormal) with radix-16 DFT tiles on MFMA.
+
+ifft_bro_mfma(d1, d2, n_half, xr, xi): ifft_bro_1d_radix2's transform
+(natural input, bit-reversed output), the same network transposed.
 """
 import ctypes
 
@@ -22,6 +25,8 @@ def _declare(L):
         return
     L.qmc_fwht_mfma_dev.restype = ctypes.c_int
     L.qmc_fwht_mfma_dev.argtypes = [ctypes.c_uint64] * 3 + [ctypes.c_void_p] * 3
+    L.qmc_ifft_bro_mfma_dev.restype = ctypes.c_int
+    L.qmc_ifft_bro_mfma_dev.argtypes = [ctypes.c_uint64] * 3 + [ctypes.c_void_p] * 4
     L.qmc_fft_bro_mfma_dev.restype = ctypes.c_int
     L.qmc_fft_bro_mfma_dev.argtypes = [ctypes.c_uint64] * 3 + [ctypes.c_void_p] * 4
     L.qmc_mfma_probe_dev.restype = ctypes.c_int
@@ -64,6 +69,20 @@ def fft_bro_mfma(d1, d2, n_half, xr, xi, work=None, stream=None):
         raise ctypes.ArgumentError("work must be a contiguous float32 GPU tensor")
     rc = _L().qmc_fft_bro_mfma_dev(int(d1), int(d2), int(n_half), xr.data_ptr(), xi.data_ptr(),
                                    work.data_ptr() if work is not None else None, _stream(xr, stream))
+    _lib.check(rc)
+
+
+def ifft_bro_mfma(d1, d2, n_half, xr, xi, work=None, stream=None):
+    """In-place orthonormal inverse FFT with natural-order input and
+    bit-reversed output (ifft_bro_1d_radix2's transform) on the matrix cores:
+    the conjugate transpose of fft_bro_mfma's network, same accuracy."""
+    for t in (xr, xi):
+        if not (t.is_cuda and t.is_contiguous()) or str(t.dtype) != "torch.float64":
+            raise ctypes.ArgumentError("xr, xi must be contiguous float64 GPU tensors")
+    if work is not None and not (work.is_cuda and work.is_contiguous() and str(work.dtype) == "torch.float32"):
+        raise ctypes.ArgumentError("work must be a contiguous float32 GPU tensor")
+    rc = _L().qmc_ifft_bro_mfma_dev(int(d1), int(d2), int(n_half), xr.data_ptr(), xi.data_ptr(),
+                                    work.data_ptr() if work is not None else None, _stream(xr, stream))
     _lib.check(rc)
 
 

@@ -108,3 +108,25 @@ def test_fft_bro_mfma_accuracy(torch, m, batch):
     got = tr.cpu().numpy() + 1j * ti.cpu().numpy()
     l2, mx = relative_errors(np.stack([got.real, got.imag]), np.stack([want.real, want.imag]))
     assert l2 <= L2_TOL and mx <= MAX_TOL, (m, l2, mx)
+
+
+@pytest.mark.parametrize("m,batch", [(8, 3), (9, 3), (11, 2), (12, 3), (13, 2), (14, 2), (16, 2), (20, 1), (21, 1),
+                                     (22, 1), (24, 1)])
+def test_ifft_bro_mfma_accuracy(torch, m, batch):
+    """ifft_bro_1d_radix2's transform (natural in, bit-reversed out) on the
+    matrix cores, every pass shape, and the round trip with fft_bro_mfma."""
+    from qmctoolscl_amd.mfma import fft_bro_mfma, ifft_bro_mfma, relative_errors
+    rng = np.random.default_rng(200 + m)
+    n = 1 << m
+    xr = rng.standard_normal((batch, n))
+    xi = rng.standard_normal((batch, n))
+    want = np.fft.ifft(xr + 1j * xi, norm="ortho")[:, _bitrev(m)]
+    tr, ti = torch.from_numpy(xr.copy()).cuda(), torch.from_numpy(xi.copy()).cuda()
+    ifft_bro_mfma(1, batch, n // 2, tr, ti)
+    got = tr.cpu().numpy() + 1j * ti.cpu().numpy()
+    l2, mx = relative_errors(np.stack([got.real, got.imag]), np.stack([want.real, want.imag]))
+    assert l2 <= L2_TOL and mx <= MAX_TOL, (m, l2, mx)
+    fft_bro_mfma(1, batch, n // 2, tr, ti)
+    back = tr.cpu().numpy() + 1j * ti.cpu().numpy()
+    l2, mx = relative_errors(np.stack([back.real, back.imag]), np.stack([xr, xi]))
+    assert l2 <= 2 * L2_TOL and mx <= 2 * MAX_TOL, (m, l2, mx)
