@@ -249,7 +249,9 @@ QMC_EXPORT int qmc_dnb2_gen_shift_float_dev(int natural, qmc_u64 r, qmc_u64 n, q
  * reference draws them (base 2: bit widths, for rng.integers(0, 2**w); base b:
  * row counts, for that many rng.permutation(b) rows); the caller appends the
  * draw values (the root's first) and *_apply_dev scrambles on the GPU.
- * Trees persist across calls until freed. */
+ * Base-b rows are 16-bit (b <= 2^16).  Trees persist across calls until
+ * freed; qmc_nusb_apply_dev keeps the packed device copy of the last set of
+ * trees it was given and uploads again only when a tree grew. */
 QMC_EXPORT void *qmc_nus2_tree_new(int tmax_new);
 QMC_EXPORT void qmc_nus2_tree_free(void *tree);
 QMC_EXPORT qmc_u64 qmc_nus2_tree_nodes(void *tree);
@@ -265,7 +267,7 @@ QMC_EXPORT qmc_u64 qmc_nusb_tree_rows(void *tree);
 QMC_EXPORT qmc_u64 qmc_nusb_tree_rows_needed(void *tree);
 QMC_EXPORT int64_t qmc_nusb_insert(void *tree, const qmc_u64 *v, qmc_u64 n, qmc_u64 stride_pt,
                                    qmc_u64 stride_dig, uint32_t *counts);
-QMC_EXPORT int qmc_nusb_add_rows(void *tree, const qmc_u64 *rows, qmc_u64 nrows);
+QMC_EXPORT int qmc_nusb_add_rows(void *tree, const uint16_t *rows, qmc_u64 nrows);
 QMC_EXPORT int qmc_nusb_apply_dev(void **trees, qmc_u64 r, qmc_u64 n, qmc_u64 d, qmc_u64 r_x, qmc_u64 r_b,
                                   int tmax, int tmax_new, const qmc_u64 *bases, const qmc_u64 *xdig,
                                   qmc_u64 *xrdig, qmc_stream_t s);

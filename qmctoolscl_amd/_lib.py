@@ -34,6 +34,14 @@ def lib():
     if not os.path.exists(LIB_PATH):
         raise ImportError("qmctoolscl_amd: the HIP library %s is not built; run "
                           "`python -m qmctoolscl_amd.build` (hipcc, gfx950)" % LIB_PATH)
+    # PyTorch-ROCm wheels bundle their own libamdhip64 under the same soname.
+    # Loaded after torch, the library binds to that runtime, so the streams and
+    # device pointers torch hands over are valid in it; loaded first, the
+    # process would hold two HIP runtimes and torch's streams would be foreign.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     L.qmc_last_error.restype = ctypes.c_char_p
     L.qmc_last_error_code.restype = ctypes.c_int

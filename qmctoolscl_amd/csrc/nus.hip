@@ -25,6 +25,9 @@
 #include <cstring>
 #include <vector>
 
+#include <atomic>
+#include <mutex>
+
 #include "qmc_common.hpp"
 
 namespace {
@@ -113,13 +116,16 @@ struct Block {
 
 struct TreeB {
     int tmax, tmax_new;
-    u64 b;
+    u64 b;                    // <= 65536, so a permuted digit fits 16 bits
+    u64 id, version;          // identity and mutation count: the device copy is reused while both match
     std::vector<NodeB> nodes;
     std::vector<int32_t> kids;
     std::vector<u64> digits;  // leaf digit pool
     std::vector<Block> blocks;
-    std::vector<u64> rows;    // permutation rows, b values each, in draw order
+    std::vector<uint16_t> rows;  // permutation rows, b values each, in draw order
 };
+
+std::atomic<u64> g_treeb_ids{1};
 
 int32_t new_leafB(TreeB *T, const u64 *rest, int len, int32_t owner) {
     NodeB nd;
@@ -237,9 +243,9 @@ struct PackedB {
     const int32_t *kids;       // child slots (tree-local node ids)
     const int64_t *blk_row0;   // per block: first row (tree-local)
     const int32_t *blk_depth0; // per block
-    const u64 *rows;           // b values per row (bmax stride)
-    const int64_t *node0, *kid0, *blk0, *row0;  // per tree offsets
-    const u64 *bases;
+    const uint16_t *rows;      // b values per row, each tree's rows packed
+    const int64_t *node0, *kid0, *blk0, *row0;  // per tree offsets (row0 in values)
+    const uint32_t *tb;        // per tree base
 };
 
 __global__ __launch_bounds__(kBlock) void nusb_apply_kernel(u64 r, u64 n, u64 d, u64 r_x, u64 r_b, int tmax,
@@ -256,20 +262,22 @@ __global__ __launch_bounds__(kBlock) void nusb_apply_kernel(u64 r, u64 n, u64 d,
         const u64 tree = l * d + j;
         const u64 *src = xdig + (((l % r_x) * n + i) * d + j) * tmax;
         u64 *out = xrdig + e * tmax_new;
-        const int64_t nb = P.node0[tree], kb = P.kid0[tree], bb = P.blk0[tree], rb = P.row0[tree];
+        const int64_t nb = P.node0[tree], kb = P.kid0[tree], bb = P.blk0[tree];
+        const uint16_t *rows = P.rows + P.row0[tree];
+        const int64_t b = P.tb[tree];
         int64_t node = 0;  // tree-local
         int t = 0;
         for (; t < tmax_new; t++) {
             const u64 dg = t < tmax ? src[t] : 0;
             const int32_t blk = P.owner[nb + node];
-            const int64_t row = rb + P.blk_row0[bb + blk] + (t - P.blk_depth0[bb + blk]);
-            out[t] = P.rows[row * bmax + dg];
+            const int64_t row = P.blk_row0[bb + blk] + (t - P.blk_depth0[bb + blk]);
+            out[t] = rows[row * b + (int64_t)dg];
             const int32_t ko = P.kids_off[nb + node];
             if (ko < 0) {  // leaf: its block's rows cover every deeper level
                 for (int tt = t + 1; tt < tmax_new; tt++) {
                     const u64 g = tt < tmax ? src[tt] : 0;
-                    const int64_t rw = rb + P.blk_row0[bb + blk] + (tt - P.blk_depth0[bb + blk]);
-                    out[tt] = P.rows[rw * bmax + g];
+                    const int64_t rw = P.blk_row0[bb + blk] + (tt - P.blk_depth0[bb + blk]);
+                    out[tt] = rows[rw * b + (int64_t)g];
                 }
                 break;
             }
@@ -277,6 +285,19 @@ __global__ __launch_bounds__(kBlock) void nusb_apply_kernel(u64 r, u64 n, u64 d,
         }
     }
 }
+
+// The packed device copy of the last set of trees applied, reused while the
+// trees (by identity and mutation count), r and d are unchanged: a repeat call
+// on the same points uploads nothing.  Calls are synchronous (the copy is
+// released or replaced only after the previous call's kernel finished).
+struct NusbDevice {
+    std::vector<u64> sig;
+    int device = -1;
+    char *buf = nullptr;
+    PackedB P{};
+};
+NusbDevice g_nusb;  // guarded by g_nusb_mu
+std::mutex g_nusb_mu;
 
 }  // namespace
 
@@ -385,14 +406,16 @@ QMC_EXPORT int qmc_nus2_apply_dev(void **trees, u64 r, u64 n, u64 d, u64 r_x, in
 }
 
 QMC_EXPORT void *qmc_nusb_tree_new(int tmax, int tmax_new, u64 b) {
-    if (tmax < 0 || tmax_new < 1 || tmax > tmax_new || b < 1 || b > (1u << 20)) {
-        qmc_fail(QMC_ERR_ARG, "nus: need 0 <= tmax <= tmax_new, 1 <= b <= 2^20");
+    if (tmax < 0 || tmax_new < 1 || tmax > tmax_new || b < 1 || b > (1u << 16)) {
+        qmc_fail(QMC_ERR_ARG, "nus: need 0 <= tmax <= tmax_new, 1 <= b <= 2^16");
         return nullptr;
     }
     TreeB *T = new TreeB();
     T->tmax = tmax;
     T->tmax_new = tmax_new;
     T->b = b;
+    T->id = g_treeb_ids++;
+    T->version = 0;
     std::vector<uint32_t> dummy;
     new_blockB(T, 0, tmax_new, dummy);  // block 0: the root's tmax_new rows, drawn by the caller
     std::vector<u64> zeros(tmax_new, 0);
@@ -423,18 +446,21 @@ QMC_EXPORT int64_t qmc_nusb_insert(void *h, const u64 *v, u64 n, u64 stride_pt, 
                          (unsigned long long)p[(u64)t * stride_dig], (unsigned long long)T->b);
                 return -1;
             }
+        const size_t nodes = T->nodes.size();
         insertB(T, p, stride_dig, c);
+        if (T->nodes.size() != nodes) T->version++;
     }
     if (!c.empty()) std::memcpy(counts, c.data(), c.size() * sizeof(uint32_t));
     return (int64_t)c.size();
 }
 
-// Append permutation rows (b values each) in draw order.
-QMC_EXPORT int qmc_nusb_add_rows(void *h, const u64 *rows, u64 nrows) {
+// Append permutation rows (b values each, below b) in draw order.
+QMC_EXPORT int qmc_nusb_add_rows(void *h, const uint16_t *rows, u64 nrows) {
     TreeB *T = (TreeB *)h;
     const u64 have = T->rows.size() / T->b;
     if (have + nrows > qmc_nusb_tree_rows_needed(h)) return qmc_fail(QMC_ERR_ARG, "nus: more rows than asked for");
     T->rows.insert(T->rows.end(), rows, rows + nrows * T->b);
+    T->version++;
     // assign row offsets to the blocks now fully covered
     u64 acc = 0;
     for (auto &bl : T->blocks) {
@@ -444,76 +470,104 @@ QMC_EXPORT int qmc_nusb_add_rows(void *h, const u64 *rows, u64 nrows) {
     return QMC_OK;
 }
 
+
 QMC_EXPORT int qmc_nusb_apply_dev(void **trees, u64 r, u64 n, u64 d, u64 r_x, u64 r_b, int tmax, int tmax_new,
                                   const u64 *bases, const u64 *xdig, u64 *xrdig, qmc_stream_t s) {
+    (void)bases;  // each tree carries its base (checked against the bases when it was grown)
     if (!(r && n && d)) return QMC_OK;
     const u64 nt = r * d;
-    u64 bmax = 1;
-    for (u64 k = 0; k < nt; k++) bmax = ((TreeB *)trees[k])->b > bmax ? ((TreeB *)trees[k])->b : bmax;
-    std::vector<int64_t> node0(nt), kid0(nt), blk0(nt), row0(nt);
-    u64 nn = 0, nk = 0, nb = 0, nr = 0;
+    std::vector<u64> sig;
+    sig.reserve(2 * nt + 2);
+    sig.push_back(r);
+    sig.push_back(d);
     for (u64 k = 0; k < nt; k++) {
         TreeB *T = (TreeB *)trees[k];
         if (T->tmax != tmax || T->tmax_new != tmax_new)
             return qmc_fail(QMC_ERR_ARG, "nus: tree %llu was built with other tmax/tmax_new", (unsigned long long)k);
         if (T->rows.size() / T->b != qmc_nusb_tree_rows_needed(T))
             return qmc_fail(QMC_ERR_ARG, "nus: tree %llu is missing permutation rows", (unsigned long long)k);
-        node0[k] = (int64_t)nn;
-        kid0[k] = (int64_t)nk;
-        blk0[k] = (int64_t)nb;
-        row0[k] = (int64_t)nr;
-        nn += T->nodes.size();
-        nk += T->kids.size();
-        nb += T->blocks.size();
-        nr += T->rows.size() / T->b;
+        sig.push_back(T->id);
+        sig.push_back(T->version);
     }
-    std::vector<int32_t> kids_off(nn), owner(nn), kids(nk > 0 ? nk : 1), bdepth(nb);
-    std::vector<int64_t> brow(nb);
-    std::vector<u64> rows(nr * bmax, 0);
-    for (u64 k = 0; k < nt; k++) {
-        TreeB *T = (TreeB *)trees[k];
-        for (size_t q = 0; q < T->nodes.size(); q++) {
-            kids_off[node0[k] + q] = T->nodes[q].kids;
-            owner[node0[k] + q] = T->nodes[q].owner;
+    int dev = 0;
+    QMC_HIP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_nusb_mu);
+    if (g_nusb.buf == nullptr || g_nusb.device != dev || g_nusb.sig != sig) {
+        std::vector<int64_t> node0(nt), kid0(nt), blk0(nt), row0(nt);
+        std::vector<uint32_t> tb(nt);
+        u64 nn = 0, nk = 0, nb = 0, nr = 0;
+        for (u64 k = 0; k < nt; k++) {
+            TreeB *T = (TreeB *)trees[k];
+            node0[k] = (int64_t)nn;
+            kid0[k] = (int64_t)nk;
+            blk0[k] = (int64_t)nb;
+            row0[k] = (int64_t)nr;
+            tb[k] = (uint32_t)T->b;
+            nn += T->nodes.size();
+            nk += T->kids.size();
+            nb += T->blocks.size();
+            nr += T->rows.size();
         }
-        for (size_t q = 0; q < T->kids.size(); q++) kids[kid0[k] + q] = T->kids[q] < 0 ? 0 : T->kids[q];
-        for (size_t q = 0; q < T->blocks.size(); q++) {
-            brow[blk0[k] + q] = T->blocks[q].row0;
-            bdepth[blk0[k] + q] = T->blocks[q].depth0;
+        std::vector<int32_t> kids_off(nn), owner(nn), kids(nk > 0 ? nk : 1), bdepth(nb);
+        std::vector<int64_t> brow(nb);
+        std::vector<uint16_t> rows(nr > 0 ? nr : 1);
+        for (u64 k = 0; k < nt; k++) {
+            TreeB *T = (TreeB *)trees[k];
+            for (size_t q = 0; q < T->nodes.size(); q++) {
+                kids_off[node0[k] + q] = T->nodes[q].kids;
+                owner[node0[k] + q] = T->nodes[q].owner;
+            }
+            for (size_t q = 0; q < T->kids.size(); q++) kids[kid0[k] + q] = T->kids[q] < 0 ? 0 : T->kids[q];
+            for (size_t q = 0; q < T->blocks.size(); q++) {
+                brow[blk0[k] + q] = T->blocks[q].row0;
+                bdepth[blk0[k] + q] = T->blocks[q].depth0;
+            }
+            if (!T->rows.empty()) std::memcpy(&rows[row0[k]], T->rows.data(), T->rows.size() * sizeof(uint16_t));
         }
-        const u64 R = T->rows.size() / T->b;
-        for (u64 q = 0; q < R; q++)
-            std::memcpy(&rows[(row0[k] + q) * bmax], &T->rows[q * T->b], T->b * sizeof(u64));
+        if (g_nusb.buf) {
+            (void)hipFree(g_nusb.buf);
+            g_nusb.buf = nullptr;
+        }
+        g_nusb.sig.clear();
+        const size_t total = kids_off.size() * 4 + owner.size() * 4 + kids.size() * 4 + brow.size() * 8 +
+                             bdepth.size() * 4 + rows.size() * 2 + 4 * nt * 8 + nt * 4 + 128;
+        char *buf = nullptr;
+        if (hipMalloc((void **)&buf, total) != hipSuccess)
+            return qmc_fail(QMC_ERR_NOMEM, "nus: device allocation of %zu bytes for the trees failed", total);
+        size_t o = 0;
+        hipError_t ce = hipSuccess;
+        auto put = [&](const void *src, size_t bytes) -> void * {
+            o = (o + 7) & ~(size_t)7;
+            void *dst = buf + o;
+            if (bytes && ce == hipSuccess) ce = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+            o += bytes;
+            return dst;
+        };
+        PackedB P;
+        P.kids_off = (const int32_t *)put(kids_off.data(), kids_off.size() * 4);
+        P.owner = (const int32_t *)put(owner.data(), owner.size() * 4);
+        P.kids = (const int32_t *)put(kids.data(), kids.size() * 4);
+        P.blk_row0 = (const int64_t *)put(brow.data(), brow.size() * 8);
+        P.blk_depth0 = (const int32_t *)put(bdepth.data(), bdepth.size() * 4);
+        P.rows = (const uint16_t *)put(rows.data(), rows.size() * 2);
+        P.node0 = (const int64_t *)put(node0.data(), nt * 8);
+        P.kid0 = (const int64_t *)put(kid0.data(), nt * 8);
+        P.blk0 = (const int64_t *)put(blk0.data(), nt * 8);
+        P.row0 = (const int64_t *)put(row0.data(), nt * 8);
+        P.tb = (const uint32_t *)put(tb.data(), nt * 4);
+        if (ce != hipSuccess) {
+            (void)hipFree(buf);
+            return qmc_fail_hip(ce, "nus: copying the trees");
+        }
+        g_nusb.buf = buf;
+        g_nusb.P = P;
+        g_nusb.device = dev;
+        g_nusb.sig = sig;
     }
-    char *buf = nullptr;
-    const size_t total = kids_off.size() * 4 + owner.size() * 4 + kids.size() * 4 + brow.size() * 8 +
-                         bdepth.size() * 4 + rows.size() * 8 + 4 * nt * 8 + 128;
-    QMC_HIP_CHECK(hipMallocAsync((void **)&buf, total, s));
-    size_t o = 0;
-    auto put = [&](const void *src, size_t bytes) -> void * {
-        o = (o + 7) & ~(size_t)7;
-        void *dst = buf + o;
-        if (bytes) (void)hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
-        o += bytes;
-        return dst;
-    };
-    PackedB P;
-    P.kids_off = (const int32_t *)put(kids_off.data(), kids_off.size() * 4);
-    P.owner = (const int32_t *)put(owner.data(), owner.size() * 4);
-    P.kids = (const int32_t *)put(kids.data(), kids.size() * 4);
-    P.blk_row0 = (const int64_t *)put(brow.data(), brow.size() * 8);
-    P.blk_depth0 = (const int32_t *)put(bdepth.data(), bdepth.size() * 4);
-    P.rows = (const u64 *)put(rows.data(), rows.size() * 8);
-    P.node0 = (const int64_t *)put(node0.data(), nt * 8);
-    P.kid0 = (const int64_t *)put(kid0.data(), nt * 8);
-    P.blk0 = (const int64_t *)put(blk0.data(), nt * 8);
-    P.row0 = (const int64_t *)put(row0.data(), nt * 8);
-    P.bases = bases;
     hipLaunchKernelGGL(nusb_apply_kernel, dim3(grid_for(r * n * d, kBlock, 2048 * 8)), dim3(kBlock), 0, s, r, n, d,
-                       r_x, r_b, tmax, tmax_new, bmax, P, xdig, xrdig);
+                       r_x, r_b, tmax, tmax_new, (u64)0, g_nusb.P, xdig, xrdig);
     hipError_t e = hipGetLastError();
-    (void)hipFreeAsync(buf, s);
-    QMC_HIP_CHECK(hipStreamSynchronize(s));
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return qmc_fail_hip(e, "nusb_apply_kernel");
     return QMC_OK;
 }

@@ -46,7 +46,7 @@ def _declare(L):
     L.qmc_nusb_insert.restype = ctypes.c_int64
     L.qmc_nusb_insert.argtypes = [ctypes.c_void_p, _u64p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, _u64p]
     L.qmc_nusb_add_rows.restype = ctypes.c_int
-    L.qmc_nusb_add_rows.argtypes = [ctypes.c_void_p, _u64p, ctypes.c_uint64]
+    L.qmc_nusb_add_rows.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]
     L.qmc_nusb_apply_dev.restype = ctypes.c_int
     L.qmc_nusb_apply_dev.argtypes = ([ctypes.c_void_p] + [ctypes.c_uint64] * 5 + [ctypes.c_int] * 2 +
                                      [_u64p, _u64p, _u64p, ctypes.c_void_p])
@@ -209,7 +209,8 @@ def gdn_nested_uniform_scramble(r, n, d, r_x, r_b, tmax, tmax_new, rngs, root_no
                 node._tree = _NativeTree(h, L.qmc_nusb_tree_free)
                 node._key = (tmax, tmax_new, int(b))
                 rows = random_uint64_permutations(rng, tmax_new, b)  # rand_funcs.py:316
-                _lib.check(L.qmc_nusb_add_rows(h, np.ascontiguousarray(rows).ctypes.data, tmax_new))
+                rows16 = np.ascontiguousarray(rows, dtype=np.uint16)  # b <= 2^16: every entry fits
+                _lib.check(L.qmc_nusb_add_rows(h, rows16.ctypes.data, tmax_new))
                 node.perm, node.xdig = rows, np.zeros(tmax_new, dtype=np.uint64)
             elif node._key != (tmax, tmax_new, int(b)):
                 raise ValueError("root node was grown with (tmax, tmax_new, b) = %s" % (node._key,))
@@ -221,8 +222,10 @@ def gdn_nested_uniform_scramble(r, n, d, r_x, r_b, tmax, tmax_new, rngs, root_no
             total = int(counts[:k].sum()) if k else 0
             if total:
                 bi = int(b)
-                rows = rng.permuted(np.tile(np.arange(b, dtype=np.uint64), (total, 1)), axis=1)
-                rows = np.ascontiguousarray(rows, dtype=np.uint64)
+                # Generator.permuted shuffles each row with the same draws whatever
+                # the dtype, so 16-bit rows equal the reference's 64-bit ones.
+                rows = rng.permuted(np.tile(np.arange(bi, dtype=np.uint16), (total, 1)), axis=1)
+                rows = np.ascontiguousarray(rows)
                 _lib.check(L.qmc_nusb_add_rows(h, rows.ctypes.data, total))
                 assert rows.shape == (total, bi)
             handles[l * d + j] = h

@@ -3,6 +3,8 @@ that include/qmctoolscl_amd.h declares, and the Python surface mirrors the
 reference's argument lists and error behaviour without touching a GPU."""
 import ctypes
 import os
+import subprocess
+import sys
 import re
 
 import numpy as np
@@ -31,11 +33,29 @@ def test_header_declares_every_reference_kernel():
 def test_library_loads_and_exports_all_declared_symbols():
     if not os.path.exists(_lib.LIB_PATH):
         pytest.skip("library not built (python -m qmctoolscl_amd.build)")
-    L = ctypes.CDLL(_lib.LIB_PATH)
+    L = _lib.lib()  # prototypes declare cleanly
     missing = [s for s in declared_symbols() if not hasattr(L, s)]
     assert missing == []
-    L = _lib.lib()  # prototypes declare cleanly
     assert b"gfx950" in L.qmc_version()
+
+
+def test_library_shares_torchs_hip_runtime():
+    """The package loads the library after torch, whose ROCm wheel bundles a
+    libamdhip64 of the same soname: the process then maps one HIP runtime, so
+    torch's streams are valid handles in the library.  Checked in a fresh
+    interpreter that touches the library before importing torch itself."""
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built (python -m qmctoolscl_amd.build)")
+    pytest.importorskip("torch")
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "from qmctoolscl_amd import _lib\n"
+            "_lib.lib()\n"
+            "import torch\n"
+            "maps = {l.split()[-1] for l in open('/proc/self/maps') if 'libamdhip64' in l}\n"
+            "print(len(maps), sorted(maps))\n") % os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.split()[0] == "1", out.stdout
 
 
 def test_signatures_match_reference_prototypes_order():

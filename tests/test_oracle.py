@@ -285,3 +285,11 @@ def test_vectorized_draws_match_sequential_draws():
     rows = [a.permutation(np.uint64(13)) for _ in range(9)]
     tiled = b.permuted(np.tile(np.arange(13, dtype=np.uint64), (9, 1)), axis=1)
     assert np.array_equal(np.array(rows), tiled)
+    # ... and the product's 16-bit rows take the same draws as 64-bit ones
+    for base, k in ((2, 40), (13, 9), (257, 5), (65536, 2)):
+        a = np.random.Generator(np.random.SFC64(base))
+        b = np.random.Generator(np.random.SFC64(base))
+        rows = [a.permutation(np.uint64(base)) for _ in range(k)]
+        tiled = b.permuted(np.tile(np.arange(base, dtype=np.uint16), (k, 1)), axis=1)
+        assert np.array_equal(np.array(rows, dtype=np.uint64), tiled.astype(np.uint64)), base
+        assert a.integers(0, 2**40) == b.integers(0, 2**40)
