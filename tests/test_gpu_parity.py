@@ -223,3 +223,30 @@ def test_large_transforms_match_oracle(m, V):
         _clib.call_oracle(name, want)
         got = run_product(name, cases.copy_case(case), "device")
         compare(name, got, want)
+
+
+@pytest.mark.parametrize("r,n,d,r_b,tmax,wide", [
+    (2, 1000, 7, 1, 52, False),   # reciprocal table in LDS, a partial last chunk
+    (3, 333, 5, 2, 33, True),     # odd tmax (odd word count per chunk), digits >= 2^32
+    (1, 257, 200, 1, 40, False),  # table too large for LDS: global reciprocals
+    (2, 130, 3, 1, 64, True),     # the largest staged row
+    (2, 100, 3, 1, 65, False),    # longer rows: one lane per row
+    (1, 64, 1, 1, 1, False),
+])
+def test_gdn_integer_to_float_shapes(r, n, d, r_b, tmax, wide):
+    """The staged conversion (rows through LDS) at the shapes that select its
+    variants, bit for bit against the oracle."""
+    rng = np.random.default_rng(tmax)
+    primes = np.array([2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53], dtype=np.uint64)
+    bases = rng.choice(primes, size=(r_b, d)).astype(np.uint64)
+    xdig = rng.integers(0, 131, size=(r, n, d, tmax), dtype=np.uint64)
+    if wide:
+        xdig[:, ::7, :, ::3] = rng.integers(2**32, 2**63, size=xdig[:, ::7, :, ::3].shape, dtype=np.uint64)
+    want = np.empty((r, n, d))
+    _clib.call_oracle("gdn_integer_to_float", dict(r=r, n=n, d=d, r_b=r_b, tmax=tmax, bases=bases, xdig=xdig,
+                                                   x=want))
+    for path in ("host", "device"):
+        case = dict(r=np.uint64(r), n=np.uint64(n), d=np.uint64(d), r_b=np.uint64(r_b), tmax=np.uint64(tmax),
+                    bases=bases.copy(), xdig=xdig.copy(), x=np.empty((r, n, d)))
+        got = run_product("gdn_integer_to_float", case, path)["x"]
+        assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), path
