@@ -174,13 +174,24 @@ def section_gdn(torch, q, dev, timed, r=8):
     ms_flt, _ = timed(lambda: q.gdn_integer_to_float(U(r), U(n), U(d), U(1), U(tnew), Bd, xperm, x, wait=False), 3, 1)
     pts = r * n * d
     dig_bytes = 8 * pts * tnew
+    # the same chain fused: digits walk in registers, only the doubles are written
+    from qmctoolscl_amd import fused
+    xf = torch.empty_like(x)
+    ms_fused, _ = timed(lambda: fused.gdn_gen_scramble_float(1, r, n, d, 1, mmax, tnew, tnew, 0, bmax, Bd, C_lms, Pd,
+                                                             xf), 3, 1)
+    fused_exact = bool(torch.equal(xf.view(torch.int64), x.view(torch.int64)))
+    del xf
     out = {"workload": "Halton-type net, bases = first 32 primes, n=3^12, d=32, r=%d, LMS to 52 digits, "
                        "digit permutations, float" % r,
            "points": pts, "digit_bytes_per_copy": dig_bytes,
            "gen_ms": round(ms_gen, 3), "gen_GBs_written": round(dig_bytes / ms_gen / 1e6, 1),
            "perm_ms": round(ms_perm, 3), "perm_GBs": round(2 * dig_bytes / ms_perm / 1e6, 1),
            "to_float_ms": round(ms_flt, 3), "to_float_GBs": round((dig_bytes + 8 * pts) / ms_flt / 1e6, 1),
-           "chain_Gpts_s": round(pts / (ms_gen + ms_perm + ms_flt) / 1e6, 2)}
+           "chain_Gpts_s": round(pts / (ms_gen + ms_perm + ms_flt) / 1e6, 2),
+           "fused_ms": round(ms_fused, 3), "fused_Gpts_s": round(pts / ms_fused / 1e6, 2),
+           "fused_GBs_written": round(8 * pts / ms_fused / 1e6, 1),
+           "fused_speedup_vs_chain": round((ms_gen + ms_perm + ms_flt) / ms_fused, 1),
+           "fused_bit_exact_vs_chain": fused_exact}
     del xperm, x, C_lms
     torch.cuda.empty_cache()
     # nested uniform scrambling of the same net's first 3^7 points for r = 2

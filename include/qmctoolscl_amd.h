@@ -241,6 +241,16 @@ QMC_EXPORT int qmc_dnb2_gen_shift_float_dev(int natural, qmc_u64 r, qmc_u64 n, q
                                             const qmc_u64 *lshifts, const qmc_u64 *shiftsb,
                                             const qmc_u64 *tmaxes, qmc_u64 *xrb, double *x, qmc_stream_t s);
 
+/* gdn_gen_natural(r, ...) -> mode 1: gdn_digital_permutation(r, ..., r_x = r, ...)
+ * or mode 2: gdn_digital_shift(r, ..., r_x = r, ...) -> gdn_integer_to_float,
+ * writing only x (r, n, d); mode 0 converts the generated digits directly
+ * (tmax_new == tmax).  tab: perms (r, d, tmax_new, bmax) or shifts
+ * (r, d, tmax_new).  Needs 1 <= tmax <= tmax_new <= 64 and mmax <= 64. */
+QMC_EXPORT int qmc_gdn_gen_scramble_float_dev(int mode, qmc_u64 r, qmc_u64 n, qmc_u64 d, qmc_u64 r_b,
+                                              qmc_u64 mmax, qmc_u64 tmax, qmc_u64 tmax_new, qmc_u64 n_start,
+                                              qmc_u64 bmax, const qmc_u64 *bases, const qmc_u64 *C,
+                                              const qmc_u64 *tab, double *x, qmc_stream_t s);
+
 /* ------------------------------------------- nested uniform scrambling
  * Replaces the pure-Python dnb2_nested_uniform_scramble (rand_funcs.py:162-266)
  * and gdn_nested_uniform_scramble (rand_funcs.py:274-356).  A tree per

@@ -5,15 +5,14 @@
 // Digit outputs are laid out [l][i][j][t] with t fastest, so for one
 // replication l the rows rho = i*d + j and their digits form one contiguous
 // array.  The row kernels give each block R consecutive rows: lane f of the
-// block produces digit (rho0 + f / T, f % T), so stores are fully coalesced,
-// and anything shared by a row (its base, the base-b digits of its index) is
-// computed once into LDS.
+// block produces digit (rho0 + f / T, f % T), so stores are fully coalesced.
+// Generation walks down consecutive points with the same row layout across
+// the lanes (see gdn_gen_chain_kernel).
 //
 // Arithmetic follows the reference's unsigned 64-bit semantics exactly,
 // including wrap-around: gdn_gen_natural.c:40-50 reduces after every digit,
-// x = (x + dig*C) % b.  When b < 2^16 and every C entry used is < 2^32, no
-// step can wrap, so the sum is accumulated exactly and reduced once; any
-// element that does not meet that bound takes the step-by-step path.
+// x = (x + dig*C) % b.  Where no step can wrap, the walks use exact modular
+// sums; every other element takes the step-by-step path.
 #include "qmc_common.hpp"
 
 #include <algorithm>
@@ -21,7 +20,6 @@
 namespace {
 
 constexpr int kRows = 64;        // rows per block in the row kernels
-constexpr int kMaxDigits = 64;   // base-b digits of a 64-bit index, b >= 2
 
 enum { STATUS_BAD_BASE = 1u, STATUS_OOB = 2u };
 
@@ -45,105 +43,145 @@ __device__ __forceinline__ u64 j_of(const Rows &g, u64 rho) {
 
 // ---------------------------------------------------------------- generation
 
-__global__ __launch_bounds__(kBlock) void gdn_gen_kernel(Rows g, u64 n_start, u64 r_b, u64 mmax, u64 tmax,
-                                                         const u64 *__restrict__ bases, u64 b_common,
-                                                         const u64 *__restrict__ C, u64 Ctotal,
-                                                         u64 *__restrict__ xdig, u32 *status) {
-    __shared__ u32 sdig[kRows][kMaxDigits];
-    __shared__ u64 sbase[kRows];
-    __shared__ u64 smagic[kRows];  // ceil(2^64 / b) for b < 2^16
-    __shared__ u32 sr32[kRows];    // 2^32 mod b
-    __shared__ u32 scount[kRows];
-    __shared__ u64 sj[kRows];
-    const u64 l = blockIdx.y;
-    const u64 rho0 = (u64)blockIdx.x * kRows;
-    const u64 nrows = (g.nd - rho0) < (u64)kRows ? (g.nd - rho0) : (u64)kRows;
-
-    // 1. base-b digits of k = i + n_start for each row of the block
-    if (threadIdx.x < nrows) {
-        const u64 rho = rho0 + threadIdx.x;
-        const u64 j = j_of(g, rho);
-        const u64 b = bases ? bases[(l % r_b) * g.d + j] : b_common;
-        u64 k = rho / g.d + n_start;
-        u32 cnt = 0;
-        if (b < 2) {
-            flag(status, STATUS_BAD_BASE);  // the reference loops forever (b=1) or divides by 0
-            k = 0;
-        }
+// One output digit as gdn_gen_natural.c:40-50 computes it: over the base-b
+// digits dig_m of k, x = (x + dig_m * C[cbase + m*tmax]) % b in wrapping u64
+// arithmetic.  Zero digits leave x unchanged and read nothing.
+__device__ u64 gdn_digit(u64 k, u64 b, const u64 *__restrict__ C, u64 cbase, u64 tmax, u64 Ctotal, u32 &bad) {
+    if (b < 2) {  // the reference loops forever (b = 1) or divides by zero
+        bad |= STATUS_BAD_BASE;
+        return 0;
+    }
+    u64 acc = 0;
+    for (u64 m = 0; k; m++) {
+        u64 q, dg;
         if ((k >> 32) == 0 && (b >> 32) == 0) {
-            u32 kk = (u32)k, bb = (u32)b;
-            while (kk) {
-                const u32 q = kk / bb;
-                sdig[threadIdx.x][cnt++] = kk - q * bb;
-                kk = q;
+            q = (u32)k / (u32)b;
+            dg = (u32)k - (u32)q * (u32)b;
+        } else {
+            q = k / b;
+            dg = k - q * b;
+        }
+        k = q;
+        if (!dg) continue;
+        const u64 ci = cbase + m * tmax;
+        u64 c = 0;
+        if (ci < Ctotal) c = C[ci];
+        else bad |= STATUS_OOB;
+        acc = (acc + dg * c) % b;
+    }
+    return acc;
+}
+
+// Point walks.  From k to k+1 the c trailing base-b digits equal to b-1 turn
+// to 0 and digit c grows by 1, so every output digit moves by
+//     Q_c[t] = (C_0[t] + ... + C_c[t]) mod b      (-(b-1) = 1 mod b),
+// provided the reference's sums are exact: b < 2^31 and C < 2^32 keep
+// x + dig*C below 2^64, and k < b^mmax keeps every digit inside the column's
+// mmax rows of C.  gdn_chain_setup_kernel writes Q (u32) and that condition
+// per (l, j); the walk kernel takes the row of one point, d*tmax contiguous
+// digits, across a block's lanes (every store instruction writes 512
+// consecutive bytes per wave) and moves down kChainH points, adding Q_0 (held
+// in a register) or, on a carry, Q_c.  Columns failing the condition take
+// gdn_digit per point.
+constexpr int kChainE = 8;                       // row entries per lane
+constexpr u64 kChainF = (u64)kBlock * kChainE;  // row entries per block
+constexpr u32 kChainH = 256;                     // points per block
+
+__global__ __launch_bounds__(kBlock) void gdn_chain_setup_kernel(u64 d, u64 mmax, u64 tmax, u64 kmax, u64 r_b,
+                                                                 const u64 *__restrict__ bases, u64 b_common,
+                                                                 const u64 *__restrict__ C, u32 *__restrict__ Q,
+                                                                 u32 *__restrict__ ok) {
+    const u64 lj = blockIdx.x;  // l * d + j
+    const u64 l = lj / d, j = lj - (lj / d) * d;
+    const u64 b = bases ? bases[(l % r_b) * d + j] : b_common;
+    const bool b_ok = b >= 2 && (b >> 31) == 0;  // z + Q < 2^32 in the walk
+    int big = 0;
+    for (u64 t = threadIdx.x; t < tmax; t += blockDim.x) {
+        u64 acc = 0;
+        for (u64 m = 0; m < mmax; m++) {
+            const u64 c = C[(lj * mmax + m) * tmax + t];
+            big |= (c >> 32) != 0;
+            if (b_ok) acc = (acc + (c & 0xffffffffull)) % b;
+            Q[(lj * mmax + m) * tmax + t] = (u32)acc;
+        }
+    }
+    big = __syncthreads_or(big);
+    if (threadIdx.x == 0) {
+        // every k <= kmax has at most mmax base-b digits
+        bool range = false;
+        if (b_ok) {
+            u64 p = 1;
+            for (u64 m = 0; m < mmax && !range; m++) {
+                if (p > kmax / b) range = true;  // p * b > kmax
+                p *= b;
+            }
+            range = range || p > kmax;
+        }
+        ok[lj] = b_ok && !big && range;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void gdn_gen_chain_kernel(u64 n, u64 d, u64 tmax, u64 n_start, u64 r_b,
+                                                               u64 mmax, const u64 *__restrict__ bases,
+                                                               u64 b_common, const u64 *__restrict__ C, u64 Ctotal,
+                                                               const u32 *__restrict__ Q,
+                                                               const u32 *__restrict__ okf,
+                                                               u64 *__restrict__ xdig, u32 *status) {
+    const u64 l = blockIdx.z;
+    const u64 i0 = (u64)blockIdx.x * kChainH;
+    const u32 H = (n - i0) < (u64)kChainH ? (u32)(n - i0) : kChainH;
+    const u64 row = d * tmax;
+    const u64 k0 = i0 + n_start;
+    u64 *out = xdig + (l * n + i0) * row;
+    u32 bad = 0;
+#pragma unroll 1
+    for (int e = 0; e < kChainE; e++) {
+        const u64 f = (u64)blockIdx.y * kChainF + (u64)e * kBlock + threadIdx.x;
+        if (f >= row) break;
+        const u64 j = f / tmax, t = f - (f / tmax) * tmax;
+        const u64 lj = l * d + j;
+        const u64 b = bases ? bases[(l % r_b) * d + j] : b_common;
+        const u64 cbase = lj * mmax * tmax + t;
+        u64 *o = out + f;
+        if (Q && okf[lj]) {
+            const u32 bb = (u32)b;
+            const DivU32 bdiv = make_div(bb);
+            const u32 *q = Q + lj * mmax * tmax + t;
+            const u32 q0 = q[0];
+            u32 z = (u32)gdn_digit(k0, b, C, cbase, tmax, Ctotal, bad);
+            u32 dlow = (k0 >> 32) ? (u32)(k0 % b) : (u32)k0 - div_q((u32)k0, bdiv) * bb;
+            u64 k = k0;
+            for (u32 h = 0;; h++) {
+                store_stream(o, (u64)z);
+                if (h + 1 == H) break;
+                u32 inc = q0;
+                if (dlow + 1 < bb) {
+                    dlow++;
+                } else {
+                    // k + 1 ends in c >= 1 zero digits: c = 1 + trailing zeros of (k + 1) / b
+                    dlow = 0;
+                    u64 x = (k + 1) / b;
+                    u32 c = 1;
+                    while (x != 0 && c + 1 < (u32)mmax) {  // c < mmax whenever the column qualifies
+                        u64 qq;
+                        if ((x >> 32) == 0) qq = div_q((u32)x, bdiv);
+                        else qq = x / b;
+                        if (x - qq * b != 0) break;
+                        x = qq;
+                        c++;
+                    }
+                    inc = q[(u64)(c < (u32)mmax ? c : (u32)mmax - 1) * tmax];
+                }
+                k++;
+                z += inc;
+                z = z >= bb ? z - bb : z;
+                o += row;
             }
         } else {
-            while (k) {
-                const u64 q = k / b;
-                sdig[threadIdx.x][cnt++] = (u32)(k - q * b);
-                k = q;
-            }
+            for (u32 h = 0; h < H; h++) store_stream(o + h * row, gdn_digit(k0 + h, b, C, cbase, tmax, Ctotal, bad));
         }
-        sbase[threadIdx.x] = b;
-        smagic[threadIdx.x] = (b >= 2 && b < (1u << 16)) ? (~0ull) / b + 1 : 0;
-        sr32[threadIdx.x] = (b >= 2 && b < (1u << 16)) ? (u32)(0x100000000ull % b) : 0;
-        scount[threadIdx.x] = cnt;
-        sj[threadIdx.x] = j;
     }
-    __syncthreads();
-
-    // 2. one output digit per lane
-    const u32 T = (u32)tmax;
-    const u32 total = (u32)nrows * T;
-    u64 *out = xdig + (l * g.nd + rho0) * tmax;
-    const u64 cl = l * g.d * mmax * tmax;
-    for (u32 f = threadIdx.x; f < total; f += blockDim.x) {
-        const u32 row = div_q(f, g.tdiv);
-        const u32 t = f - row * T;
-        const u64 b = sbase[row];
-        const u32 cnt = scount[row];
-        const u64 cbase = cl + sj[row] * mmax * tmax + t;
-        u64 acc = 0;
-        // fast path: b < 2^16 and every column entry < 2^32, so no step wraps;
-        // all loads of the row are independent (checked once, after the sum)
-        bool fast = b >= 2 && b < (1u << 16) && (cnt == 0 || cbase + (u64)(cnt - 1) * tmax < Ctotal);
-        if (fast) {
-            u64 orc = 0;
-            const u64 *Cp = C + cbase;
-#pragma unroll 4
-            for (u32 m = 0; m < cnt; m++) {
-                const u64 c = Cp[(u64)m * tmax];
-                orc |= c;
-                acc += (u64)sdig[row][m] * c;
-            }
-            fast = (orc >> 32) == 0;
-        }
-        if (fast) {
-            // acc mod b with 32-bit magic divisions: acc = h 2^32 + lo, b < 2^16
-            const u64 M = smagic[row];
-            const u32 bb = (u32)b;
-            const u32 h = (u32)(acc >> 32), lo = (u32)acc;
-            const u32 hm = h - (u32)__umul64hi(M, h) * bb;
-            const u32 r32 = sr32[row];
-            const u32 lm = lo - (u32)__umul64hi(M, lo) * bb;
-            const u32 y = hm * r32 + lm;  // < b^2 <= 2^32
-            acc = y - (u32)__umul64hi(M, y) * bb;
-        } else if (b < 2) {
-            acc = 0;
-        } else {  // step by step, as gdn_gen_natural.c:46
-            acc = 0;
-            for (u32 m = 0; m < cnt; m++) {
-                const u64 dg = sdig[row][m];
-                if (!dg) continue;
-                const u64 ci = cbase + (u64)m * tmax;
-                u64 c = 0;
-                if (ci < Ctotal) c = C[ci];
-                else flag(status, STATUS_OOB);
-                acc = (acc + dg * c) % b;
-            }
-        }
-        store_stream(out + f, acc);
-    }
+    if (bad) flag(status, bad);
 }
 
 // -------------------------------------------------- digit-wise shift / permute
@@ -453,6 +491,231 @@ __global__ __launch_bounds__(kBlock) void gdn_interlace_kernel(int undo, u64 r, 
     dst[e] = src[((l * d_in + ji) * mmax + k) * t_in + ti];
 }
 
+// ------------------------------------- fused generation, scrambling, float
+//
+// gdn_gen_natural -> gdn_digital_permutation or gdn_digital_shift (r_x = r)
+// -> gdn_integer_to_float in one pass that writes only the doubles.  A block
+// takes one column (l, j) and kBlock * kFuseH consecutive points; each lane
+// walks kFuseH of them with the column's tmax digits in registers (the walk
+// of gdn_gen_chain_kernel, its prefix rows Q in LDS), looks each digit up in
+// the column's permutation rows (LDS when they fit as u32) or adds its shift,
+// and sums v += recip_t * y_t in the reference's order with the same
+// reciprocals.  Columns that fail the walk's exactness condition compute
+// every digit from its index (gdn_digit).  The d blocks of one point range
+// are consecutive on one XCD (block b runs on XCD b % 8), so the 8-byte
+// stores of a point's row meet in that XCD's L2.
+constexpr u32 kFuseH = 64;  // consecutive points per lane
+
+struct FuseArgs {
+    u64 n, d, r_b, mmax, n_start, bmax, Ctotal, tab_total, ngroups, nchunks;
+    const u64 *bases, *C, *tab;
+    const u32 *Q, *okf;
+    double *x;
+    u32 *status;
+    u32 tmax, tnew, perm_lds, off_rt, off_tab;
+};
+
+template <int MODE>
+__device__ __forceinline__ u64 fuse_scramble(const FuseArgs &a, u32 t, u64 z, u64 b, u64 pbase, const u32 *sP,
+                                             const u64 *sS, bool plds, u32 &bad) {
+    if (MODE == 0) return z;
+    if (MODE == 1) {  // gdn_digital_permutation.c:36-42
+        if (plds && z < a.bmax) return sP[(u64)t * a.bmax + z];
+        const u64 pi = pbase + (u64)t * a.bmax + z;
+        if (pi < a.tab_total) return a.tab[pi];
+        bad |= STATUS_OOB;
+        return 0;
+    }
+    // gdn_digital_shift.c:37-42 (the sum wraps as the reference's u64 sum)
+    const u64 sh = sS[t];
+    if (b == 0) {
+        bad |= STATUS_BAD_BASE;
+        return 0;
+    }
+    const u64 xs = z + sh;
+    return xs < 2 * b && xs >= sh ? (xs >= b ? xs - b : xs) : umod(xs, b);
+}
+
+template <int MODE>
+__device__ __forceinline__ u64 fuse_tail(const FuseArgs &a, u32 t, u64 pbase, const u32 *sP, const u64 *sS,
+                                         bool plds, u32 &bad) {
+    if (MODE == 1) {  // perms[l, j, t, 0] past tmax
+        if (plds) return sP[(u64)t * a.bmax];
+        const u64 pi = pbase + (u64)t * a.bmax;
+        if (pi < a.tab_total) return a.tab[pi];
+        bad |= STATUS_OOB;
+        return 0;
+    }
+    return MODE == 2 ? sS[t] : 0;  // the raw shift past tmax
+}
+
+template <int TP, int MODE>
+__global__ __launch_bounds__(kBlock) void gdn_fused_kernel(FuseArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const u64 bid = blockIdx.x;
+    const u64 xcd = bid & 7, qd = bid >> 3;
+    const u64 j = qd % a.d, u = qd / a.d;
+    const u64 g = u * 8 + xcd;  // (l, point range), one per XCD lane of the grid
+    if (g >= a.ngroups) return;  // uniform over the block, before any barrier
+    const u64 l = g / a.nchunks, chunk = g - (g / a.nchunks) * a.nchunks;
+    const u64 lj = l * a.d + j;
+    const u64 b = a.bases[(l % a.r_b) * a.d + j];
+    const u32 tmax = a.tmax, tnew = a.tnew;
+    const u32 rows = (u32)TP > tnew ? (u32)TP : tnew;  // table rows in LDS, zero past tnew
+    const bool ok = a.Q != nullptr && a.okf[lj] != 0;
+    // LDS: Q rows (mmax x TP u32, zero past tmax) | reciprocals (tnew) | the
+    // same zeroed past tmax (TP) | perms (rows x bmax u32) or shifts (rows u64)
+    u32 *sQ = (u32 *)smem;
+    double *srt = (double *)(smem + a.off_rt);
+    double *srtm = srt + tnew;
+    u32 *sP = (u32 *)(smem + a.off_tab);
+    u64 *sS = (u64 *)(smem + a.off_tab);
+    const u64 pbase = lj * tnew * a.bmax;
+    if (ok)
+        for (u32 e = threadIdx.x; e < (u32)a.mmax * TP; e += blockDim.x) {
+            const u32 m = e / TP, t = e - m * TP;
+            sQ[e] = t < tmax ? a.Q[(lj * a.mmax + m) * tmax + t] : 0u;
+        }
+    if (threadIdx.x == 0) {  // gdn_integer_to_float.c:34-38
+        const double bd = (double)b;
+        double recip = 1 / bd;
+        for (u32 t = 0; t < tnew; t++) {
+            srt[t] = recip;
+            if (t < (u32)TP) srtm[t] = t < tmax ? recip : 0.;
+            recip /= bd;
+        }
+        for (u32 t = tnew; t < (u32)TP; t++) srtm[t] = 0.;
+    }
+    int flags = 0;  // bit 0: a permutation entry >= 2^32; bit 1: a shift >= b
+    if (MODE == 1 && a.perm_lds)
+        for (u32 e = threadIdx.x; e < rows * (u32)a.bmax; e += blockDim.x) {
+            const u64 v = e < tnew * (u32)a.bmax ? a.tab[pbase + e] : 0;
+            sP[e] = (u32)v;
+            flags |= (v >> 32) != 0;
+        }
+    if (MODE == 2)
+        for (u32 t = threadIdx.x; t < rows; t += blockDim.x) {
+            const u64 v = t < tnew ? a.tab[lj * tnew + t] : 0;
+            sS[t] = v;
+            flags |= (t < tmax && v >= b) ? 2 : 0;
+        }
+    flags = (__syncthreads_or(flags & 1) ? 1 : 0) | (__syncthreads_or(flags & 2) ? 2 : 0);
+    const bool plds = MODE == 1 && a.perm_lds && !(flags & 1);
+    // branch-free digit loop: every digit a walk produces is < b, so it needs
+    // a table in LDS and b <= bmax (permutations) or shifts < b (no wrap)
+    const bool fast = MODE == 0 || (MODE == 1 && plds && b <= a.bmax) || (MODE == 2 && !(flags & 2));
+
+    const u64 i_first = chunk * ((u64)kBlock * kFuseH) + (u64)threadIdx.x * kFuseH;
+    if (i_first >= a.n) return;
+    const u32 H = (a.n - i_first) < (u64)kFuseH ? (u32)(a.n - i_first) : kFuseH;
+    u64 k = i_first + a.n_start;
+    double *xo = a.x + (l * a.n + i_first) * a.d + j;
+    u32 bad = 0;
+    if (!ok) {
+        for (u32 h = 0; h < H; h++, k++) {
+            double v = 0.;
+            for (u32 t = 0; t < tnew; t++) {
+                u64 y;
+                if (t < tmax)
+                    y = fuse_scramble<MODE>(a, t, gdn_digit(k, b, a.C, lj * a.mmax * tmax + t, tmax, a.Ctotal, bad),
+                                            b, pbase, sP, sS, plds, bad);
+                else
+                    y = fuse_tail<MODE>(a, t, pbase, sP, sS, plds, bad);
+                v += srt[t] * (double)y;
+            }
+            xo[(u64)h * a.d] = v;
+        }
+        if (bad) flag(a.status, bad);
+        return;
+    }
+    // digits of the first point: z_t = sum_m (dig_m - dig_{m+1}) Q_m[t] mod b
+    // (the padded entries t >= tmax stay 0)
+    const u32 bb = (u32)b;
+    const DivU32 bdiv = make_div(bb);
+    u32 z[TP];
+#pragma unroll
+    for (int t = 0; t < TP; t++) z[t] = 0;
+    {
+        u64 kk = k;
+        u32 dig = (kk >> 32) ? (u32)(kk % b) : (u32)kk - div_q((u32)kk, bdiv) * bb;
+        kk = (kk - dig) / b;
+        for (u32 m = 0; dig != 0 || kk != 0; m++) {
+            const u32 dn = (kk >> 32) ? (u32)(kk % b) : (u32)kk - div_q((u32)kk, bdiv) * bb;
+            kk = (kk - dn) / b;
+            const u32 e = dig >= dn ? dig - dn : dig + bb - dn;
+            const u32 *qm = sQ + m * TP;
+            if (bb < (1u << 16)) {
+#pragma unroll
+                for (int t = 0; t < TP; t++) {
+                    const u32 p = e * qm[t];
+                    const u32 s2 = z[t] + (p - div_q(p, bdiv) * bb);
+                    z[t] = __builtin_elementwise_min(s2, s2 - bb);
+                }
+            } else {
+#pragma unroll
+                for (int t = 0; t < TP; t++) {
+                    const u32 s2 = z[t] + (u32)(((u64)e * qm[t]) % b);
+                    z[t] = __builtin_elementwise_min(s2, s2 - bb);
+                }
+            }
+            dig = dn;
+        }
+    }
+    u32 dlow = (k >> 32) ? (u32)(k % b) : (u32)k - div_q((u32)k, bdiv) * bb;
+    const u32 bm = (u32)a.bmax;
+    for (u32 h = 0;; h++) {
+        double v = 0.;
+        if (fast) {
+#pragma unroll
+            for (int t = 0; t < TP; t++) {
+                u32 y;
+                if (MODE == 0) {
+                    y = z[t];
+                } else if (MODE == 1) {
+                    y = sP[t * bm + z[t]];
+                } else {
+                    const u32 s2 = z[t] + (u32)sS[t];
+                    y = __builtin_elementwise_min(s2, s2 - bb);
+                }
+                v += srtm[t] * (double)y;  // + 0 for the padded digits
+            }
+        } else {
+            for (u32 t = 0; t < tmax; t++) {
+                u32 zt = 0;
+#pragma unroll
+                for (int tt = 0; tt < TP; tt++)
+                    if (tt == (int)t) zt = z[tt];
+                v += srt[t] * (double)fuse_scramble<MODE>(a, t, zt, b, pbase, sP, sS, plds, bad);
+            }
+        }
+        for (u32 t = tmax; t < tnew; t++) v += srt[t] * (double)fuse_tail<MODE>(a, t, pbase, sP, sS, plds, bad);
+        xo[(u64)h * a.d] = v;
+        if (h + 1 == H) break;
+        u32 c = 0;
+        if (dlow + 1 < bb) {
+            dlow++;
+        } else {
+            dlow = 0;
+            u64 xq = (k + 1) / b;
+            c = 1;
+            while (xq != 0 && c + 1 < (u32)a.mmax) {
+                const u64 qq = (xq >> 32) ? xq / b : (u64)div_q((u32)xq, bdiv);
+                if (xq - qq * b != 0) break;
+                xq = qq;
+                c++;
+            }
+        }
+        const u32 *qc = sQ + (c < (u32)a.mmax ? c : (u32)a.mmax - 1) * TP;
+#pragma unroll
+        for (int t = 0; t < TP; t++) {
+            const u32 s2 = z[t] + qc[t];
+            z[t] = __builtin_elementwise_min(s2, s2 - bb);
+        }
+        k++;
+    }
+    if (bad) flag(a.status, bad);
+}
+
 Rows make_rows(u64 n, u64 d, u64 tdigits) {
     Rows g;
     g.nd = n * d;
@@ -478,11 +741,28 @@ static int gen_common(const char *who, u64 r, u64 n, u64 d, u64 r_b, u64 mmax, u
     if (r > 65535) return qmc_fail(QMC_ERR_ARG, "%s: r above 65535 is not supported", who);
     u32 *status = qmc_status_ptr();
     if (!status) return QMC_ERR_HIP;
-    Rows g = make_rows(n, d, tmax);
-    dim3 grid((unsigned)((g.nd + kRows - 1) / kRows), (unsigned)r);
-    hipLaunchKernelGGL(gdn_gen_kernel, grid, dim3(kBlock), 0, s, g, n_start, r_b, mmax, tmax, bases, b_common, C,
-                       r * d * mmax * tmax, xdig, status);
-    QMC_HIP_CHECK(hipGetLastError());
+    // the prefix table Q (u32 per C entry) and one flag per column; past 1 GiB
+    // of table every digit is computed from its index
+    const u64 qwords = r * d * mmax * tmax;
+    u32 *ws = nullptr;
+    const u64 kmax = n_start + (n - 1);
+    const bool walk = qwords * 4 <= (1ull << 30) && mmax > 0 && r * d < (1ull << 31) &&
+                      kmax >= n_start;  // no index wrap-around
+    if (walk) QMC_HIP_CHECK(hipMallocAsync((void **)&ws, sizeof(u32) * (qwords + r * d), s));
+    u32 *okf = walk ? ws + qwords : nullptr;
+    if (walk)
+        hipLaunchKernelGGL(gdn_chain_setup_kernel, dim3((unsigned)(r * d)), dim3(64), 0, s, d, mmax, tmax, kmax,
+                           r_b, bases, b_common, C, ws, okf);
+    const u64 row = d * tmax;
+    dim3 grid((unsigned)((n + kChainH - 1) / kChainH), (unsigned)((row + kChainF - 1) / kChainF), (unsigned)r);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(gdn_gen_chain_kernel, grid, dim3(kBlock), 0, s, n, d, tmax, n_start, r_b, mmax, bases,
+                           b_common, C, r * d * mmax * tmax, walk ? ws : nullptr, okf, xdig, status);
+        e = hipGetLastError();
+    }
+    if (ws) (void)hipFreeAsync(ws, s);
+    if (e != hipSuccess) return qmc_fail_hip(e, who);
     return QMC_OK;
 }
 
@@ -536,6 +816,89 @@ static int digitwise(bool permute, const char *who, u64 r, u64 n, u64 d, u64 r_x
         hipLaunchKernelGGL(gdn_digitwise_kernel<false>, grid, dim3(kBlock), 0, s, g, r_x, r_b, tmax, tmax_new, bmax,
                            bases, tab, r * d * tmax_new, xdig, xnew, status);
     QMC_HIP_CHECK(hipGetLastError());
+    return QMC_OK;
+}
+
+// gdn_gen_natural -> (mode 1) gdn_digital_permutation or (mode 2)
+// gdn_digital_shift, each with r_x = r -> gdn_integer_to_float, writing only
+// x (r, n, d).  Mode 0 converts the generated digits directly (tmax_new must
+// equal tmax).  tab: perms (r, d, tmax_new, bmax) or shifts (r, d, tmax_new).
+QMC_EXPORT int qmc_gdn_gen_scramble_float_dev(int mode, u64 r, u64 n, u64 d, u64 r_b, u64 mmax, u64 tmax,
+                                              u64 tmax_new, u64 n_start, u64 bmax, const u64 *bases, const u64 *C,
+                                              const u64 *tab, double *x, hipStream_t s) {
+    const char *who = "gdn_gen_scramble_float";
+    if (!(r && n && d)) return QMC_OK;
+    if (mode < 0 || mode > 2) return qmc_fail(QMC_ERR_ARG, "%s: mode must be 0, 1 or 2", who);
+    if (r_b == 0) return qmc_fail(QMC_ERR_ARG, "%s: r_b must be positive", who);
+    if (mode == 0 && tmax_new != tmax) return qmc_fail(QMC_ERR_ARG, "%s: mode 0 needs tmax_new == tmax", who);
+    if (tmax < 1 || tmax > tmax_new || tmax_new > 64)
+        return qmc_fail(QMC_ERR_ARG, "%s: needs 1 <= tmax <= tmax_new <= 64", who);
+    if (mmax > 64) return qmc_fail(QMC_ERR_ARG, "%s: mmax above 64 is not supported", who);
+    if (mode == 1 && bmax == 0) return qmc_fail(QMC_ERR_ARG, "%s: bmax must be positive", who);
+    u32 *status = qmc_status_ptr();
+    if (!status) return QMC_ERR_HIP;
+    const u64 kmax = n_start + (n - 1);
+    const u64 qwords = r * d * mmax * tmax;
+    const bool walk = mmax > 0 && kmax >= n_start && qwords * 4 <= (1ull << 30) && r * d < (1ull << 31);
+    const int TP = tmax <= 16 ? 16 : tmax <= 32 ? 32 : tmax <= 56 ? 56 : 64;
+    const u64 rows = (u64)TP > tmax_new ? (u64)TP : tmax_new;
+    FuseArgs a;
+    a.n = n;
+    a.d = d;
+    a.r_b = r_b;
+    a.mmax = mmax;
+    a.n_start = n_start;
+    a.bmax = bmax;
+    a.Ctotal = r * d * mmax * tmax;
+    a.tab_total = mode == 1 ? r * d * tmax_new * bmax : r * d * tmax_new;
+    a.nchunks = (n + (u64)kBlock * kFuseH - 1) / ((u64)kBlock * kFuseH);
+    a.ngroups = r * a.nchunks;
+    a.bases = bases;
+    a.C = C;
+    a.tab = tab;
+    a.x = x;
+    a.status = status;
+    a.tmax = (u32)tmax;
+    a.tnew = (u32)tmax_new;
+    a.perm_lds = mode == 1 && rows * bmax * 4 <= 40960;
+    a.off_rt = (u32)((sizeof(u32) * mmax * TP + 15) & ~(u64)15);
+    a.off_tab = a.off_rt + (u32)(8 * (tmax_new + TP));
+    const size_t shm = a.off_tab + (mode == 1 ? (a.perm_lds ? 4 * rows * bmax : 0) : mode == 2 ? 8 * rows : 0);
+    const u64 nblocks = (a.ngroups + 7) / 8 * 8 * d;
+    if (nblocks > 0x7fffffffull) return qmc_fail(QMC_ERR_ARG, "%s: r*n*d too large", who);
+    u32 *ws = nullptr;
+    if (walk) QMC_HIP_CHECK(hipMallocAsync((void **)&ws, sizeof(u32) * (qwords + r * d), s));
+    a.Q = ws;
+    a.okf = walk ? ws + qwords : nullptr;
+    if (walk)
+        hipLaunchKernelGGL(gdn_chain_setup_kernel, dim3((unsigned)(r * d)), dim3(64), 0, s, d, mmax, tmax, kmax,
+                           r_b, bases, (u64)0, C, ws, ws + qwords);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+#define QMC_FUSE(tp, md)                                                                                    \
+    hipLaunchKernelGGL((gdn_fused_kernel<tp, md>), dim3((unsigned)nblocks), dim3(kBlock), shm, s, a)
+#define QMC_FUSE_TP(md)         \
+    if (TP == 16)               \
+        QMC_FUSE(16, md);       \
+    else if (TP == 32)          \
+        QMC_FUSE(32, md);       \
+    else if (TP == 56)          \
+        QMC_FUSE(56, md);       \
+    else                        \
+        QMC_FUSE(64, md)
+        if (mode == 0) {
+            QMC_FUSE_TP(0);
+        } else if (mode == 1) {
+            QMC_FUSE_TP(1);
+        } else {
+            QMC_FUSE_TP(2);
+        }
+#undef QMC_FUSE_TP
+#undef QMC_FUSE
+        e = hipGetLastError();
+    }
+    if (ws) (void)hipFreeAsync(ws, s);
+    if (e != hipSuccess) return qmc_fail_hip(e, who);
     return QMC_OK;
 }
 

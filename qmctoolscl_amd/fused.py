@@ -42,3 +42,20 @@ def dnb2_gen_shift_float(natural, r, n, d, r_x, n_start, mmax, C, lshifts, shift
                                                  int(n_start), int(mmax), _ptr(C), _ptr(lshifts), _ptr(shiftsb),
                                                  _ptr(tmaxes), _ptr(xrb), _ptr(x), _stream(stream, like))
     _lib.check(rc)
+
+
+def gdn_gen_scramble_float(mode, r, n, d, r_b, mmax, tmax, tmax_new, n_start, bmax, bases, C, tab, x, stream=None):
+    """x = gdn_integer_to_float(r, n, d, r_b, tmax_new, bases, S(gdn_gen_natural(r, n, d, r_b, mmax, tmax,
+    n_start, bases, C))) with S = the identity (mode 0, tmax_new == tmax),
+    gdn_digital_permutation(r, n, d, r, r_b, tmax, tmax_new, bmax, tab, .) (mode 1)
+    or gdn_digital_shift(r, n, d, r, r_b, tmax, tmax_new, bases, tab, .) (mode 2),
+    in one pass that materializes no digits.  bases: (r_b, d), C: (r, d, mmax,
+    tmax), tab: perms (r, d, tmax_new, bmax) or shifts (r, d, tmax_new),
+    x: (r, n, d) float64."""
+    L = _lib.lib()
+    fn = L.qmc_gdn_gen_scramble_float_dev
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_int] + [ctypes.c_uint64] * 9 + [ctypes.c_void_p] * 5
+    rc = fn(int(mode), int(r), int(n), int(d), int(r_b), int(mmax), int(tmax), int(tmax_new), int(n_start),
+            int(bmax), _ptr(bases), _ptr(C), _ptr(tab) if tab is not None else None, _ptr(x), _stream(stream, x))
+    _lib.check(rc)

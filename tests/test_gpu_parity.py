@@ -250,3 +250,136 @@ def test_gdn_integer_to_float_shapes(r, n, d, r_b, tmax, wide):
                     bases=bases.copy(), xdig=xdig.copy(), x=np.empty((r, n, d)))
         got = run_product("gdn_integer_to_float", case, path)["x"]
         assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), path
+
+
+def _gen_case(r, n, d, r_b, mmax, tmax, n_start, bases, seed, big_col=None):
+    rng = np.random.default_rng(seed)
+    C = np.empty((r, d, mmax, tmax), dtype=np.uint64)
+    for j in range(d):
+        for rb in range(r):
+            b = int(bases[rb % r_b, j])
+            C[rb, j] = rng.integers(0, 3 * b, size=(mmax, tmax), dtype=np.uint64)  # entries past b-1 too
+    if big_col is not None:
+        C[:, big_col, :, ::5] = rng.integers(2**32, 2**62, size=C[:, big_col, :, ::5].shape, dtype=np.uint64)
+    return dict(r=r, n=n, d=d, r_b=r_b, mmax=mmax, tmax=tmax, n_start=n_start, bases=bases, C=C,
+                xdig=np.empty((r, n, d, tmax), dtype=np.uint64))
+
+
+@pytest.mark.parametrize("shape", [
+    # r, n, d, r_b, mmax, tmax, n_start, big_col: blocks of 256 points, carries across them, two block columns
+    (2, 700, 50, 2, 12, 52, 1000, None),
+    # a column with C >= 2^32 and base-2 columns whose indices outgrow mmax (direct digits)
+    (2, 300, 6, 1, 5, 7, 3, 4),
+    # indices past 2^32 (64-bit digit and carry paths)
+    (1, 300, 3, 1, 40, 5, 2**32 - 100, None),
+    (3, 1, 4, 3, 3, 2, 0, None),
+])
+def test_gdn_gen_point_walks(shape):
+    """gdn_gen_natural's point walks (prefix table of C) and its per-digit
+    fallback, bit for bit against the oracle at shapes that reach each path."""
+    r, n, d, r_b, mmax, tmax, n_start, big_col = shape
+    rng = np.random.default_rng(n)
+    primes = np.array([2, 3, 5, 7, 11, 13, 17, 131, 257, 65537, 2**31 - 1], dtype=np.uint64)
+    bases = rng.choice(primes, size=(r_b, d)).astype(np.uint64)
+    if big_col is not None:
+        bases[:, 0] = 2  # j = 0 reads past its mmax rows into column 1 (inside C)
+        bases[:, -1] = 131  # the last column's reads stay inside C
+    want = _gen_case(r, n, d, r_b, mmax, tmax, n_start, bases, 7, big_col)
+    _clib.call_oracle("gdn_gen_natural", want)
+    for path in ("host", "device"):
+        got = run_product("gdn_gen_natural", _gen_case(r, n, d, r_b, mmax, tmax, n_start, bases, 7, big_col), path)
+        assert np.array_equal(got["xdig"], want["xdig"]), path
+    # one common base (where every index has at most mmax digits: past that the
+    # reference reads beyond C)
+    for b in (2, 3, 131):
+        if b ** mmax <= n_start + n - 1:
+            continue
+        case = _gen_case(r, n, d, 1, mmax, tmax, n_start, np.full((1, d), b, dtype=np.uint64), 8)
+        args = {k: case[k] for k in ("r", "n", "d", "mmax", "tmax", "n_start", "C", "xdig")}
+        args["b"] = b
+        want = cases.copy_case(args)
+        _clib.call_oracle("gdn_gen_natural_same_base", want)
+        got = run_product("gdn_gen_natural_same_base", cases.copy_case(args), "device")
+        assert np.array_equal(got["xdig"], want["xdig"]), b
+
+
+def _oracle_gdn_chain(mode, r, n, d, r_b, mmax, tmax, tnew, n_start, bmax, bases, C, tab):
+    xdig = np.empty((r, n, d, tmax), dtype=np.uint64)
+    _clib.call_oracle("gdn_gen_natural", dict(r=r, n=n, d=d, r_b=r_b, mmax=mmax, tmax=tmax, n_start=n_start,
+                                              bases=bases, C=C, xdig=xdig))
+    if mode == 0:
+        y = xdig
+    else:
+        y = np.empty((r, n, d, tnew), dtype=np.uint64)
+        if mode == 1:
+            _clib.call_oracle("gdn_digital_permutation", dict(r=r, n=n, d=d, r_x=r, r_b=r_b, tmax=tmax, tmax_new=tnew,
+                                                              bmax=bmax, perms=tab, xdig=xdig, xdig_new=y))
+        else:
+            _clib.call_oracle("gdn_digital_shift", dict(r=r, n=n, d=d, r_x=r, r_b=r_b, tmax=tmax, tmax_new=tnew,
+                                                        bases=bases, shifts=tab, xdig=xdig, xdig_new=y))
+    x = np.empty((r, n, d))
+    _clib.call_oracle("gdn_integer_to_float", dict(r=r, n=n, d=d, r_b=r_b, tmax=tnew, bases=bases, xdig=y, x=x))
+    return x
+
+
+@pytest.mark.parametrize("mode,r,n,d,r_b,mmax,tmax,tnew,n_start,kind", [
+    (1, 2, 20000, 7, 1, 10, 20, 20, 0, "halton"),     # two point ranges, LMS'd Halton matrices, permutations
+    (1, 3, 3000, 5, 3, 12, 9, 14, 123, "random"),     # tail rows (tmax < tmax_new), r_b = r, n_start
+    (2, 2, 2500, 6, 1, 12, 20, 26, 7, "random"),      # digital shifts (u64 sums that wrap)
+    (0, 1, 5000, 4, 1, 14, 33, 33, 0, "random"),      # no scramble, 64-register rows
+    (1, 2, 700, 4, 1, 6, 5, 8, 0, "wide"),            # C >= 2^32 column, perms >= 2^32, bases past b^mmax
+    (1, 1, 600, 3, 1, 3, 4, 6, 0, "bigb"),            # permutation rows too long for LDS (b = 1999)
+    (1, 2, 3000, 5, 1, 8, 52, 52, 0, "halton"),       # 56-register rows (the 52-digit layout)
+    (2, 2, 3000, 4, 2, 8, 60, 62, 5, "smallshift"),   # shifts < b (branch-free path), 64-register rows, tail
+])
+def test_fused_gdn_equals_chain(mode, r, n, d, r_b, mmax, tmax, tnew, n_start, kind):
+    """qmc_gdn_gen_scramble_float_dev against the oracle's three-call chain,
+    bit for bit."""
+    import torch
+    from qmctoolscl_amd import fused
+    rng = np.random.default_rng(mode * 100 + n)
+    primes = np.array([2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37], dtype=np.uint64)
+    if kind == "bigb":
+        bases = np.full((r_b, d), 1999, dtype=np.uint64)
+    else:
+        bases = np.stack([primes[:d]] * r_b) if kind == "halton" else rng.choice(primes, size=(r_b, d))
+        bases = bases.astype(np.uint64)
+    C = np.empty((r, d, mmax, tmax), dtype=np.uint64)
+    for l in range(r):
+        for j in range(d):
+            b = int(bases[l % r_b, j])
+            if kind == "halton":  # identity scrambled by a random lower-triangular S (values < b)
+                C[l, j] = 0
+                for m in range(min(mmax, tmax)):
+                    C[l, j, m, m] = rng.integers(1, b)
+                    C[l, j, m, m + 1:] = rng.integers(0, b, size=tmax - m - 1)
+            else:
+                C[l, j] = rng.integers(0, 3 * b, size=(mmax, tmax), dtype=np.uint64)
+    if kind == "wide":
+        C[:, 1, :, ::3] = rng.integers(2**32, 2**62, size=C[:, 1, :, ::3].shape, dtype=np.uint64)
+        bases[:, 0] = 2  # 2^6 < 700: column 0 reads C rows past mmax (inside C); last column keeps a big base
+        bases[:, -1] = 37
+    bmax = int(bases.max())
+    if mode == 1:
+        tab = np.empty((r, d, tnew, bmax), dtype=np.uint64)
+        for l in range(r):
+            for j in range(d):
+                for t in range(tnew):
+                    tab[l, j, t] = rng.permutation(bmax).astype(np.uint64)
+        if kind == "wide":
+            tab[:, :, ::2, 1] += np.uint64(2**40)
+    elif mode == 2 and kind == "smallshift":
+        tab = rng.integers(0, 2**64 - 1, size=(r, d, tnew), dtype=np.uint64)  # raw past tmax
+        for l in range(r):
+            for j in range(d):
+                tab[l, j, :tmax] = rng.integers(0, int(bases[l % r_b, j]), size=tmax, dtype=np.uint64)
+    elif mode == 2:
+        tab = rng.integers(0, 2**64 - 1, size=(r, d, tnew), dtype=np.uint64)
+    else:
+        tab = None
+    want = _oracle_gdn_chain(mode, r, n, d, r_b, mmax, tmax, tnew, n_start, bmax, bases, C, tab)
+    x = torch.empty((r, n, d), dtype=torch.float64, device="cuda")
+    fused.gdn_gen_scramble_float(mode, r, n, d, r_b, mmax, tmax, tnew, n_start, bmax, to_t(bases), to_t(C),
+                                 to_t(tab) if tab is not None else None, x)
+    got = x.cpu().numpy()
+    assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
