@@ -381,13 +381,20 @@ __device__ __forceinline__ u32 xor_cols(const u32 *__restrict__ cols, u64 q) {
 
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
-// Output of a tile: non-temporal stores, and with DIG a running wrap-around
-// sum of the stored bit patterns (an order-independent per-slab digest).
-template <bool DIG>
+// Output of a tile, and with DIG a running wrap-around sum of the stored bit
+// patterns (an order-independent per-slab digest).  STREAM: non-temporal
+// stores, for the store-bound class whose chains cover whole 128-byte lines;
+// the gdn class's rows (d doubles) share lines with the next row, which
+// cached stores complete in L2 (5% faster there, 3-5% slower for the
+// store-bound class).
+template <bool DIG, bool STREAM>
 struct Sink {
     u64 acc = 0;
     __device__ __forceinline__ void put(double *p, double v) {
-        store_stream(p, v);
+        if (STREAM)
+            store_stream(p, v);
+        else
+            *p = v;
         if (DIG) acc += (u64)__double_as_longlong(v);
     }
     // one atomic per wave: every lane of the block works on the same slab
@@ -419,9 +426,9 @@ __device__ __forceinline__ double dnb2_emit(u32 v, u64 xs, double lo) {
 // row index changes in bits clog..clog+c (c = ctz(g+1)), so the value changes by
 // pre[clog+c] ^ pre[clog-1] (pre = prefix XOR of the columns).  Steps come in
 // groups of 8 whose first 7 transitions use c = 0,1,0,2,0,1,0.
-template <bool LMS, bool DIG>
+template <bool LMS, bool DIG, class SinkT>
 __device__ __forceinline__ void dnb2_tile(const Req &q, u64 l, u64 row0, const u64 *__restrict__ tab,
-                                          double *__restrict__ slab, u32 *v0, Sink<DIG> &out) {
+                                          double *__restrict__ slab, u32 *v0, SinkT &out) {
     const u32 d = (u32)q.d;
     const u32 s = q.clog;
     const u32 R = 1u << s, H = q.rows_per_tile >> s;
@@ -514,9 +521,9 @@ __device__ __forceinline__ u32 pk_add_mod(u32 z, u32 q, u16x2 bb) {
 }
 
 // One gdn chain of a base-2 column: len consecutive points from k, row stride d.
-template <bool DIG>
+template <class SinkT>
 __device__ __forceinline__ void gdn_chain2(const u64 *__restrict__ Q, u32 k, u32 len, u32 d, double *dst,
-                                           Sink<DIG> &out) {
+                                           SinkT &out) {
     // Q = cmask, Q_0..Q_{mm-1}; point k's digits: cmask ^ XOR of columns S_i = Q_i ^ Q_{i-1}
     u64 z = Q[0];
     for (u32 kk = k; kk; kk &= kk - 1) {
@@ -536,9 +543,9 @@ __device__ __forceinline__ void gdn_chain2(const u64 *__restrict__ Q, u32 k, u32
 // row stride d.  T = the column's digit tables in LDS, W pairs per row (the
 // lane group's width, as a template so every pair lives in a named register);
 // L = pairs in the low part of the floating-point Horner split.
-template <int W, int L, bool DIG>
+template <int W, int L, class SinkT>
 __device__ __forceinline__ void gdn_chain(const u64 *__restrict__ gj, const u32 *T, u32 k, u32 len, u32 d,
-                                          double *dst, Sink<DIG> &out) {
+                                          double *dst, SinkT &out) {
     const u64 w1 = gj[1];
     const u32 b = (u32)gj[0] & 0xffffu;
     const u32 mm = (u32)((w1 >> 8) & 0xff);
@@ -638,7 +645,7 @@ __device__ void run_tile(const Req &q, u64 t, const u64 *__restrict__ ws, double
     const u64 row0 = (t - l * q.tiles_per_l) * (u64)q.rows_per_tile;
     const u64 *tab = ws + q.tab;
     double *slab = ring + ws[q.slabs + l];
-    Sink<DIG> out;
+    Sink<DIG, CLASS == 0> out;
     if (CLASS == 0 && q.method == M_LATTICE) {
         const u32 pts = q.rows_per_tile * (u32)d;
         const u64 *g = tab;
@@ -652,9 +659,9 @@ __device__ void run_tile(const Req &q, u64 t, const u64 *__restrict__ ws, double
             out.put(slab + row * d + j, frac_mod1(x + sh[j]));  // lat_shift_mod_1.c:31
         }
     } else if (CLASS == 0 && q.method == M_DNB2_LMS) {
-        dnb2_tile<true>(q, l, row0, tab, slab, lds, out);
+        dnb2_tile<true, DIG>(q, l, row0, tab, slab, lds, out);
     } else if (CLASS == 0) {
-        dnb2_tile<false>(q, l, row0, tab, slab, lds, out);
+        dnb2_tile<false, DIG>(q, l, row0, tab, slab, lds, out);
     } else {
         // this replication's digit tables into LDS
         const u32 *src = (const u32 *)(tab + q.gtab) + l * q.gtw;
@@ -800,6 +807,7 @@ struct Plan {
     unsigned long long *digest = nullptr;   // per-slab digests, or null
     unsigned long long *gdigest = nullptr;  // digest pointer the graph was captured with
     u64 nslabs = 0;
+    std::vector<u64> slab_pos;       // ring offset of every slab (host copy of the workspace's)
 };
 
 constexpr u32 kLdsClass0 = 256 * sizeof(u32);  // dnb2 first-row values
@@ -931,7 +939,7 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
         return nullptr;
     }
     int cus = 256;
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, P->device);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, P->device) != hipSuccess) cus = 256;
     P->grid = grid > 0 ? grid : 4 * cus;
     std::vector<u64> ws_host(words, 0);
     // gdn dimension info and lane groups (the digit tables are built on the device)
@@ -983,6 +991,7 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
         for (int l = 0; l < q.r; l++) {
             if (pos + slab > ring_elems) pos = 0;
             ws_host[q.slabs + l] = pos;
+            P->slab_pos.push_back(pos);
             pos += slab;
         }
     }
@@ -995,9 +1004,14 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
         qmc_batch_free(P);
         return nullptr;
     }
-    hipMemcpy(P->d_reqs, P->reqs.data(), sizeof(Req) * P->reqs.size(), hipMemcpyHostToDevice);
-    hipMemcpy(P->d_qreqs, P->qreqs.data(), sizeof(Req) * P->qreqs.size(), hipMemcpyHostToDevice);
-    hipMemcpy(P->d_ws, ws_host.data(), sizeof(u64) * words, hipMemcpyHostToDevice);
+    if (hipMemcpy(P->d_reqs, P->reqs.data(), sizeof(Req) * P->reqs.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(P->d_qreqs, P->qreqs.data(), sizeof(Req) * P->qreqs.size(), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemcpy(P->d_ws, ws_host.data(), sizeof(u64) * words, hipMemcpyHostToDevice) != hipSuccess) {
+        qmc_fail(QMC_ERR_HIP, "batch: copying the plan to the device failed");
+        qmc_batch_free(P);
+        return nullptr;
+    }
     // tables
     hipLaunchKernelGGL(batch_setup_kernel, dim3(64, (unsigned)P->reqs.size()), dim3(kBlock), 0, 0, P->d_reqs,
                        P->d_ws);
@@ -1020,15 +1034,15 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
 QMC_EXPORT void qmc_batch_free(void *h) {
     Plan *P = (Plan *)h;
     if (!P) return;
-    if (P->gexec) hipGraphExecDestroy(P->gexec);
-    if (P->cap) hipStreamDestroy(P->cap);
-    if (P->side) hipStreamDestroy(P->side);
-    if (P->ev_fork) hipEventDestroy(P->ev_fork);
-    if (P->ev_join) hipEventDestroy(P->ev_join);
-    hipFree(P->d_reqs);
-    hipFree(P->d_ws);
-    hipFree(P->d_counter);
-    hipFree(P->d_qreqs);
+    if (P->gexec) (void)hipGraphExecDestroy(P->gexec);
+    if (P->cap) (void)hipStreamDestroy(P->cap);
+    if (P->side) (void)hipStreamDestroy(P->side);
+    if (P->ev_fork) (void)hipEventDestroy(P->ev_fork);
+    if (P->ev_join) (void)hipEventDestroy(P->ev_join);
+    (void)hipFree(P->d_reqs);
+    (void)hipFree(P->d_ws);
+    (void)hipFree(P->d_counter);
+    (void)hipFree(P->d_qreqs);
     delete P;
 }
 
@@ -1053,10 +1067,10 @@ QMC_EXPORT u64 qmc_batch_points(void *h) {
 }
 // element offset in the ring of replication l of request q
 QMC_EXPORT u64 qmc_batch_slab_offset(void *h, int q, int l) {
-    Plan *P = (Plan *)h;
-    u64 v = 0;
-    hipMemcpy(&v, P->d_ws + P->reqs[q].slabs + l, sizeof(u64), hipMemcpyDeviceToHost);
-    return v;
+    const Plan *P = (const Plan *)h;
+    u64 s0 = 0;  // slabs are numbered request by request
+    for (int k = 0; k < q; k++) s0 += (u64)P->reqs[k].r;
+    return P->slab_pos[s0 + (u64)l];
 }
 
 // All requests in one persistent launch (capturable: a fill kernel resets the
