@@ -3,17 +3,19 @@
 // H_N = H16 (x) H16 (x) ... (x) H_{2^rem}: the transform of a length-2^m vector
 // is a 16-point Hadamard product along every 4-bit group of the index (and
 // radix-2 butterflies for the leftover bits).  A 16-point product over 16
-// columns is one 16x16 output tile of v_mfma_f32_16x16x32_bf16:
+// columns is one 16x16 output tile of four v_mfma_f32_16x16x4f32:
 //
-//     D = [H16 | H16] * [s0 ; s1]  +  [H16 | H16] * [s2 ; 0]
+//     D = sum over kb < 4 of H16[:, 4kb..4kb+3] * x[4kb..4kb+3, :]
 //
-// where x = s0 + s1 + s2 splits each fp32 value into three bf16 pieces (24
-// significant bits in all) and the +-1 Hadamard entries are exact in bf16, so
-// the only rounding is the fp32 accumulation.  Values stay in LDS (fp32)
-// between axes; a vector longer than one tile (2^14 values) takes two passes
-// over HBM with an fp32 intermediate, against up to five float64 passes of the
-// radix-2 path.  The orthonormal scale 1/sqrt(n) is applied once, in float64,
-// when the result is written.
+// with the fp32 values as they are (the +-1 entries are exact), so the only
+// rounding is the fp32 accumulation.  (A 3 x bf16 split on the bf16 matrix
+// cores needs about 100 VALU instructions per tile for the split and ran the
+// first pass at 2.9 TB/s, VALU-bound; four fp32 MFMAs take 4 LDS reads and 4
+// writes per lane and leave the passes memory-bound.)  Values stay in LDS
+// (fp32) between axes; a vector longer than one tile (2^14 values) takes two
+// passes over HBM with an fp32 intermediate, against up to five float64
+// passes of the radix-2 path.  The orthonormal scale 1/sqrt(n) is applied
+// once, in float64, when the result is written.
 //
 // Accuracy is not bit-exact: every result reports its relative L2 and max
 // error against the float64 transform (see tests/test_mfma.py).
@@ -37,51 +39,29 @@ constexpr int kThreads = 1024;  // 16 waves: two workgroups fill a CU, each with
 __device__ __forceinline__ int pad(int t) { return t + (t >> 4); }
 static inline size_t padded_bytes(int T) { return sizeof(float) * (((size_t)1 << T) + ((size_t)1 << T) / 16); }
 
-__device__ __forceinline__ void split3(float x, __bf16 &a, __bf16 &b, __bf16 &c) {
-    a = (__bf16)x;
-    const float r1 = x - (float)a;  // exact: a is x rounded to 8 significant bits
-    b = (__bf16)r1;
-    const float r2 = r1 - (float)b;
-    c = (__bf16)r2;
-}
-
-// Operand A of the 16x16x32 MFMA: lane (row m = lane%16, k-group g = lane/16)
-// holds A[m][8g .. 8g+7]; here A = [H16 | H16], H16[m][c] = (-1)^popcount(m&c).
-__device__ __forceinline__ bf16x8 hadamard_a(int lane) {
-    const int m = lane & 15, g = lane >> 4;
-    bf16x8 a;
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-        const int c = (g * 8 + i) & 15;
-        a[i] = (__popc(m & c) & 1) ? (__bf16)-1.0f : (__bf16)1.0f;
-    }
-    return a;
-}
-
 // 16-point Hadamard along tile index bits [a, a+4) of a 2^T-value fp32 tile
 // in LDS.  Each wave owns whole 16-column blocks, so reads and writes of a
-// column never cross waves.
-__device__ void h16_axis(float *tile, int T, int a, int wave, int lane, bf16x8 A, u32 salt) {
+// column never cross waves.  v_mfma_f32_16x16x4f32 operands: lane (n, g) =
+// (lane % 16, lane / 16) holds A[n][g] and B[g][n] of its k-block and gets
+// D[4g + i][n], i < 4.  The sign dither (one +-1 per column, undone on the
+// way out) keeps the accumulation's rounding bias from adding up over axes.
+__device__ void h16_axis(float *tile, int T, int a, int wave, int lane, u32 salt) {
     const int nblk = 1 << (T - 8);
     const int n = lane & 15, g = lane >> 4;
-    const int c0 = (g & 1) * 8;
+    float A[4];  // H16[n][4kb + g]
+#pragma unroll
+    for (int kb = 0; kb < 4; kb++) A[kb] = (__popc(n & (4 * kb + g)) & 1) ? -1.0f : 1.0f;
     for (int blk = wave; blk < nblk; blk += (int)(blockDim.x >> 6)) {
         const int col = blk * 16 + n;
         const int lo = col & ((1 << a) - 1);
         const int base = ((col >> a) << (a + 4)) | lo;
         const u32 sg = dither_sign((u32)col, salt);
-        bf16x8 b1, b2;
+        float b[4];
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
-            __bf16 s0, s1, s2;
-            split3(flip(tile[pad(base + ((c0 + i) << a))], sg), s0, s1, s2);
-            b1[i] = g < 2 ? s0 : s1;
-            b2[i] = g < 2 ? s2 : (__bf16)0.0f;
-        }
+        for (int kb = 0; kb < 4; kb++) b[kb] = flip(tile[pad(base + ((4 * kb + g) << a))], sg);
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, b1, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, b2, acc, 0, 0, 0);
-        // D layout: lane holds D[4g + i][n], i = 0..3
+#pragma unroll
+        for (int kb = 0; kb < 4; kb++) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(A[kb], b[kb], acc, 0, 0, 0);
 #pragma unroll
         for (int i = 0; i < 4; i++) tile[pad(base + ((g * 4 + i) << a))] = flip(acc[i], sg);
     }
@@ -154,8 +134,7 @@ __global__ __launch_bounds__(kThreads) void fwht_mfma_pass(int m, int lr, int lc
         int a = tb0;
         for (; a + 4 <= tb0 + nb; a += 4) {
             const int ltid = opaque(threadIdx.x);
-            h16_axis(tile, T, a, ltid >> 6, ltid & 63, hadamard_a(ltid & 63),
-                     (u32)tix * 0x632BE5ABu + (u32)a * 0x27D4EB2Fu);
+            h16_axis(tile, T, a, ltid >> 6, ltid & 63, (u32)tix * 0x632BE5ABu + (u32)a * 0x27D4EB2Fu);
             __syncthreads();
         }
         if (a < tb0 + nb) h2_axes(tile, T, a, tb0 + nb - a);
