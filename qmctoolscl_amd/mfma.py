@@ -2,9 +2,9 @@
 
 fwht_mfma(d1, d2, n_half, x): the orthonormal Walsh-Hadamard transform of
 fwht_1d_radix2 (same arguments, in place on a float64 tensor of d1*d2*2*n_half
-values), computed with 16-point Hadamard tiles on bf16 MFMA from a 3 x bf16
-split of each value with fp32 accumulation.  Not bit-exact: relative L2 error
-about 1e-7 against the float64 transform (see tests/test_mfma.py); use
+values), computed with 16-point Hadamard tiles on fp32 MFMA
+(v_mfma_f32_16x16x4f32) with fp32 accumulation.  Not bit-exact: relative L2
+error about 2e-7 against the float64 transform (see tests/test_mfma.py); use
 fwht_1d_radix2 for the bit-exact result.  Lengths 2^8 .. 2^24.
 
 fft_bro_mfma(d1, d2, n_half, xr, xi): fft_bro_1d_radix2's transform (input
