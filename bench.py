@@ -174,13 +174,21 @@ def section_gdn(torch, q, dev, timed, r=8):
     ms_flt, _ = timed(lambda: q.gdn_integer_to_float(U(r), U(n), U(d), U(1), U(tnew), Bd, xperm, x, wait=False), 3, 1)
     pts = r * n * d
     dig_bytes = 8 * pts * tnew
+    # the digital shift of the same digits (the other gdn scramble)
+    shifts = q.gdn_get_digital_shifts(rng, r, d, tnew, 1, bases)
+    Shd = t64(torch, shifts, dev)
+    ms_shift, _ = timed(lambda: q.gdn_digital_shift(U(r), U(n), U(d), U(r), U(1), U(tnew), U(tnew), Bd, Shd, xdig,
+                                                    xperm, wait=False), 3, 1)
+    xfs = torch.empty_like(x)
     # the same chain fused: digits walk in registers, only the doubles are written
     from qmctoolscl_amd import fused
     xf = torch.empty_like(x)
     ms_fused, _ = timed(lambda: fused.gdn_gen_scramble_float(1, r, n, d, 1, mmax, tnew, tnew, 0, bmax, Bd, C_lms, Pd,
                                                              xf), 3, 1)
     fused_exact = bool(torch.equal(xf.view(torch.int64), x.view(torch.int64)))
-    del xf
+    ms_fused_shift, _ = timed(lambda: fused.gdn_gen_scramble_float(2, r, n, d, 1, mmax, tnew, tnew, 0, bmax, Bd, C_lms,
+                                                                   Shd, xfs), 3, 1)
+    del xf, xfs, Shd
     out = {"workload": "Halton-type net, bases = first 32 primes, n=3^12, d=32, r=%d, LMS to 52 digits, "
                        "digit permutations, float" % r,
            "points": pts, "digit_bytes_per_copy": dig_bytes,
@@ -188,6 +196,8 @@ def section_gdn(torch, q, dev, timed, r=8):
            "perm_ms": round(ms_perm, 3), "perm_GBs": round(2 * dig_bytes / ms_perm / 1e6, 1),
            "to_float_ms": round(ms_flt, 3), "to_float_GBs": round((dig_bytes + 8 * pts) / ms_flt / 1e6, 1),
            "chain_Gpts_s": round(pts / (ms_gen + ms_perm + ms_flt) / 1e6, 2),
+           "shift_ms": round(ms_shift, 3), "shift_GBs": round(2 * dig_bytes / ms_shift / 1e6, 1),
+           "fused_shift_ms": round(ms_fused_shift, 3),
            "fused_ms": round(ms_fused, 3), "fused_Gpts_s": round(pts / ms_fused / 1e6, 2),
            "fused_GBs_written": round(8 * pts / ms_fused / 1e6, 1),
            "fused_speedup_vs_chain": round((ms_gen + ms_perm + ms_flt) / ms_fused, 1),
