@@ -131,14 +131,15 @@ QMC_EXPORT int qmc_halton_qrng_dev(int n, int d, int n0, int generalized, double
     if ((long long)n + (long long)n0 > 2147483647LL)
         return qmc_fail(QMC_ERR_RANGE, "halton_qrng: n0 + n exceeds the reference's int index range");
     if (n == 0 || d == 0) return QMC_OK;
-    static bool tables = false;
-    if (!tables) {  // the prime and factor tables, once per process
-        static unsigned p[QMC_HALTON_DIMS];
+    static PerDeviceOnce tables;  // the prime and factor tables live in each device's constant memory
+    const int trc = tables.run([]() -> int {
+        unsigned p[QMC_HALTON_DIMS];
         for (int i = 0; i < QMC_HALTON_DIMS; i++) p[i] = kPrimes[i];
         QMC_HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_prime), p, sizeof p));
         QMC_HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_factor), qmc_halton_factor, sizeof qmc_halton_factor));
-        tables = true;
-    }
+        return QMC_OK;
+    });
+    if (trc) return trc;
     u32 *status = qmc_status_ptr();
     if (!status) return QMC_ERR_HIP;
     Dim *dims = nullptr;

@@ -442,14 +442,15 @@ int run_mfma_fft(const char *who, u64 d1, u64 d2, u64 n_half, double *xr, double
     if (((uintptr_t)xr | (uintptr_t)xi | (uintptr_t)work) % 16)
         return qmc_fail(QMC_ERR_ARG, "%s: xr, xi and work must be 16-byte aligned", who);
     const double scale = (m % 2 == 0) ? ldexp(1.0, -m / 2) : ldexp(1.0, -(m - 1) / 2) / 1.4142135623730951;
-    static bool attr_set = false;
-    if (!attr_set) {  // padded tiles need more than the default 64 KiB of LDS
+    static PerDeviceOnce attrs;  // padded tiles need more than the default 64 KiB of LDS
+    const int arc = attrs.run([]() -> int {
         const int bytes = (int)shm_bytes(kT);
         for (int k = 0; k < 4; k++)
             QMC_HIP_CHECK(hipFuncSetAttribute(pass_fn<INV>(k & 1, k >> 1), hipFuncAttributeMaxDynamicSharedMemorySize,
                                               bytes));
-        attr_set = true;
-    }
+        return QMC_OK;
+    });
+    if (arc) return arc;
     const u64 cus = (u64)device_cus();
     // FP64 twiddle tables (one product per twiddle), 2^klo + 2^(m-klo) entries
     Twiddles tw;

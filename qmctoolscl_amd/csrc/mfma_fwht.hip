@@ -194,14 +194,15 @@ QMC_EXPORT int qmc_fwht_mfma_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, double 
         return qmc_fail(QMC_ERR_ARG, "fwht_mfma: x and work must be 16-byte aligned");
     // 1/sqrt(2^m), exact for even m
     const double scale = (m % 2 == 0) ? ldexp(1.0, -m / 2) : ldexp(1.0, -(m - 1) / 2) / 1.4142135623730951;
-    static bool attr_set = false;
-    if (!attr_set) {  // tiles of 2^14 padded values need more than the default 64 KiB of LDS
+    static PerDeviceOnce attrs;  // tiles of 2^14 padded values need more than the default 64 KiB of LDS
+    const int arc = attrs.run([]() -> int {
         const int bytes = (int)padded_bytes(kTileLog2);
         QMC_HIP_CHECK(hipFuncSetAttribute((const void *)fwht_mfma_pass<1, 1, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
         QMC_HIP_CHECK(hipFuncSetAttribute((const void *)fwht_mfma_pass<1, 0, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
         QMC_HIP_CHECK(hipFuncSetAttribute((const void *)fwht_mfma_pass<0, 1, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-        attr_set = true;
-    }
+        return QMC_OK;
+    });
+    if (arc) return arc;
     if (m <= kTileLog2) {
         // one vector per tile, 16 values per thread (m >= 10), fewer below
         const int np = m >= 10 ? 8 : 1 << (m - 7);

@@ -53,6 +53,24 @@ static inline int resident_per_cu(const void *fn, int threads, size_t shm) {
     return per_cu;
 }
 
+// Set-up that must happen once on every device a process uses (constant
+// tables, kernel attributes): run(f) calls f with the device current the first
+// time per device, under a mutex, and again after a failure.
+struct PerDeviceOnce {
+    std::mutex mu;
+    bool done[64] = {};
+    template <class F>
+    int run(F f) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return f();
+        std::lock_guard<std::mutex> lock(mu);
+        if (done[dev]) return QMC_OK;
+        const int rc = f();
+        if (rc == QMC_OK) done[dev] = true;
+        return rc;
+    }
+};
+
 // Compute units of the current device (looked up once per device).
 static inline int device_cus() {
     static int cache[64];

@@ -243,14 +243,15 @@ int launch_tile(u64 total, int m, int tlog2, double *xr, double *xi, const doubl
     const u64 tiles = (total + (1ull << tlog2) - 1) >> tlog2;
     const size_t words = ((size_t)1 << tlog2) + ((size_t)1 << tlog2) / 16;  // padded
     const size_t shm = (size_t)(KIND == KIND_FWHT ? 1 : 2) * sizeof(double) * words;
-    static bool attr_set = false;
-    if (!attr_set) {  // padded 64 KiB tiles need a little more than the default LDS limit
+    static PerDeviceOnce attrs;  // padded 64 KiB tiles need a little more than the default LDS limit
+    const int arc = attrs.run([]() -> int {
         const size_t most = (size_t)(KIND == KIND_FWHT ? 1 : 2) * sizeof(double) *
                             (((size_t)1 << (KIND == KIND_FWHT ? kTileLog2Real : kTileLog2Cplx)) * 17 / 16);
         QMC_HIP_CHECK(hipFuncSetAttribute((const void *)tile_kernel<KIND, ASC>,
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)most));
-        attr_set = true;
-    }
+        return QMC_OK;
+    });
+    if (arc) return arc;
     hipLaunchKernelGGL((tile_kernel<KIND, ASC>), dim3((unsigned)tiles), dim3(tile_threads<KIND>()), shm, s, total, m,
                        tlog2, tlog2,
                        xr, xi, tsr, tsi);

@@ -91,3 +91,42 @@ def test_dtype_checks_like_ndpointer():
     g = np.ones((1, 4), dtype=np.uint64)[:, ::2]  # not C-contiguous
     with pytest.raises(ctypes.ArgumentError):
         q.lat_gen_linear(np.uint64(1), np.uint64(3), np.uint64(2), g, x)
+
+
+def test_entry_points_reject_bad_arguments_before_any_gpu_call():
+    """Argument checks come first: these calls return their error codes (and
+    a message) on a machine with no GPU, as on one with."""
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built (python -m qmctoolscl_amd.build)")
+    L = _lib.lib()
+    u64, vp, ci = ctypes.c_uint64, ctypes.c_void_p, ctypes.c_int
+    fused = L.qmc_gdn_gen_scramble_float_dev
+    fused.restype = ci
+    fused.argtypes = [ci] + [u64] * 9 + [vp] * 5
+    ARG, RANGE = 1, 2
+    #            mode  r  n  d  r_b mmax tmax tnew n_start bmax
+    cases = [((3, 1, 8, 2, 1, 4, 4, 4, 0, 5), ARG),    # no such mode
+             ((1, 1, 8, 2, 1, 4, 5, 4, 0, 5), ARG),    # tmax > tmax_new
+             ((1, 1, 8, 2, 1, 4, 4, 65, 0, 5), ARG),   # tmax_new above 64
+             ((0, 1, 8, 2, 1, 4, 4, 6, 0, 5), ARG),    # mode 0 needs tmax_new == tmax
+             ((1, 1, 8, 2, 1, 65, 4, 4, 0, 5), ARG),   # mmax above 64
+             ((1, 1, 8, 2, 0, 4, 4, 4, 0, 5), ARG),    # r_b = 0
+             ((1, 1, 8, 2, 1, 4, 4, 4, 0, 0), ARG),    # bmax = 0 with permutations
+             ((1, 0, 8, 2, 1, 4, 4, 4, 0, 5), 0)]      # nothing to do
+    for args, want in cases:
+        assert fused(*args, None, None, None, None, None) == want, args
+        if want:
+            assert L.qmc_last_error().decode()
+    halton = L.qmc_halton_qrng_dev
+    halton.restype = ci
+    halton.argtypes = [ci, ci, ci, ci, vp, vp, vp, vp]
+    assert halton(-1, 2, 0, 1, None, None, None, None) == ARG
+    assert halton(10, 2, 2**31 - 5, 1, None, None, None, None) == RANGE
+    assert halton(0, 2, 0, 1, None, None, None, None) == 0
+    fw = L.qmc_fwht_mfma_dev
+    fw.restype = ci
+    fw.argtypes = [u64, u64, u64, vp, vp, vp]
+    assert fw(1, 1, 384, None, None, None) == ARG     # 768 is not a power of two
+    assert fw(1, 1, 64, None, None, None) == ARG      # 2^7 is below the minimum
+    assert fw(1, 1, 1 << 24, None, None, None) == ARG  # 2^25 is above the maximum
+    assert fw(1, 1, 512, 8, None, None) == ARG         # misaligned pointer
