@@ -47,10 +47,25 @@ __constant__ unsigned short c_factor[QMC_HALTON_DIMS];
 struct Dim {
     u32 base, factor;
     DivU32 div;
+    double rcp;                // RN(1 / base)
     u32 shift[kDigits];        // s_k as the reference's unsigned int
     double lead[kDigits + 1];  // u after the steps k = 31 .. K with digit 0 (lead[32] = 0)
     double first;              // point 0 of a call with n0 == 0
 };
+
+// x / b rounded as IEEE division rounds it, for an integer 2 <= b < 2^50 and
+// x >= 0 (no overflow or subnormals): q0 = RN(x y) with y = RN(1/b), then one
+// correction q1 = RN(q0 + r0 y), r0 = fma(-q0, b, x).  q0 + r0 y lies within
+// about 1.5 2^-53 ulp of x/b, while for odd b the quotient is never closer
+// than ulp/(2b) to a rounding midpoint (x = m b for a midpoint m would need
+// the odd 54-bit significand of m times b to fit in 53 bits), so q1 = RN(x/b);
+// for b = 2 every step is exact.  tests/test_div_small.py checks the sequence
+// against division for every odd divisor below 4096.
+__device__ __forceinline__ double div_small(double x, double b, double y) {
+    const double q0 = x * y;
+    const double r0 = __fma_rn(-q0, b, x);
+    return __fma_rn(r0, y, q0);
+}
 
 // one thread per dimension
 __global__ void halton_setup_kernel(int d, int generalized, const double *__restrict__ randu,
@@ -67,6 +82,7 @@ __global__ void halton_setup_kernel(int d, int generalized, const double *__rest
     D.base = b;
     D.factor = generalized ? c_factor[idx] : 1u;
     D.div = make_div(b);
+    D.rcp = 1.0 / (double)b;
     for (int k = 0; k < kDigits; k++) D.shift[k] = (u32)(int)((double)(int)b * randu[j * kDigits + k]);
     double u = 0.0, v = 0.0;
     D.lead[kDigits] = 0.0;
@@ -87,6 +103,10 @@ __global__ __launch_bounds__(kBlock) void halton_points_kernel(int n, int n0, co
     const Dim &D = dims[j];
     const u32 b = D.base, f = D.factor;
     const DivU32 dv = D.div;
+    const double bd = (double)b, y = D.rcp;
+    // digits of the largest index of the call: no index has more (uniform over the grid's rows)
+    int kall = 0;
+    for (u32 t = (u32)n0 + (u32)(n - 1); t != 0; t = div_q(t, dv)) kall++;
     double *out = res + (u64)j * (u64)n;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const u32 idx = (u32)n0 + (u32)i;
@@ -99,17 +119,21 @@ __global__ __launch_bounds__(kBlock) void halton_points_kernel(int n, int n0, co
             int K = 0;
 #pragma unroll
             for (int k = 0; k < kDigits; k++) {
-                const u32 q = div_q(t, dv);
-                c[k] = t - q * b;
-                if (t != 0) K = k + 1;
-                t = q;
+                if (k < kall) {
+                    const u32 q = div_q(t, dv);
+                    c[k] = t - q * b;
+                    if (t != 0) K = k + 1;
+                    t = q;
+                }
             }
             u = D.lead[K];
 #pragma unroll
             for (int k = kDigits - 1; k >= 0; k--) {
-                if (k < K) {
-                    u += (double)((f * c[k] + D.shift[k]) % b);
-                    u /= (double)b;
+                if (k < kall && k < K) {
+                    // halton_qrng.c: u = (u + (f c + s) % b) / b in unsigned int arithmetic;
+                    // the multiply-high division is exact for every 32-bit numerator
+                    const u32 num = f * c[k] + D.shift[k];
+                    u = div_small(u + (double)(num - div_q(num, dv) * b), bd, y);
                 }
             }
         }
