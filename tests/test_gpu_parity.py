@@ -331,6 +331,9 @@ def _oracle_gdn_chain(mode, r, n, d, r_b, mmax, tmax, tnew, n_start, bmax, bases
     (1, 1, 600, 3, 1, 3, 4, 6, 0, "bigb"),            # permutation rows too long for LDS (b = 1999)
     (1, 2, 3000, 5, 1, 8, 52, 52, 0, "halton"),       # 56-register rows (the 52-digit layout)
     (2, 2, 3000, 4, 2, 8, 60, 62, 5, "smallshift"),   # shifts < b (branch-free path), 64-register rows, tail
+    (1, 1, 1, 1, 1, 3, 1, 1, 0, "random"),            # one point, one digit
+    (0, 1, 700, 1, 1, 12, 12, 12, 0, "base2"),        # base 2 only
+    (1, 1, 500, 3, 1, 40, 6, 8, 2**32 - 250, "random"),  # indices across 2^32 (64-bit digit and carry paths)
 ])
 def test_fused_gdn_equals_chain(mode, r, n, d, r_b, mmax, tmax, tnew, n_start, kind):
     """qmc_gdn_gen_scramble_float_dev against the oracle's three-call chain,
@@ -341,6 +344,8 @@ def test_fused_gdn_equals_chain(mode, r, n, d, r_b, mmax, tmax, tnew, n_start, k
     primes = np.array([2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37], dtype=np.uint64)
     if kind == "bigb":
         bases = np.full((r_b, d), 1999, dtype=np.uint64)
+    elif kind == "base2":
+        bases = np.full((r_b, d), 2, dtype=np.uint64)
     else:
         bases = np.stack([primes[:d]] * r_b) if kind == "halton" else rng.choice(primes, size=(r_b, d))
         bases = bases.astype(np.uint64)
