@@ -141,10 +141,33 @@ def section_dnb2(torch, q, fused, timed, dev):
     ms_f, _ = timed(fused_step, 200, 10)
     ms_c, _ = timed(chain_step, 200, 10)
     pts = r * n * d
-    return {"workload": "dnb2 LMS + natural order + 64-bit digital shift + float, n=2^16, d=8, r=4, tmax=32",
-            "points": pts, "fused_ms": round(ms_f, 4), "fused_Gpts_s": round(pts / ms_f / 1e6, 2),
-            "fused_GBs_written": round(16 * pts / ms_f / 1e6, 1),
-            "per_call_ms": round(ms_c, 4), "per_call_Gpts_s": round(pts / ms_c / 1e6, 2)}
+    out = {"workload": "dnb2 LMS + natural order + 64-bit digital shift + float, n=2^16, d=8, r=4, tmax=32",
+           "points": pts, "fused_ms": round(ms_f, 4), "fused_Gpts_s": round(pts / ms_f / 1e6, 2),
+           "fused_GBs_written": round(16 * pts / ms_f / 1e6, 1),
+           "per_call_ms": round(ms_c, 4), "per_call_Gpts_s": round(pts / ms_c / 1e6, 2)}
+    # nested uniform scrambling of the same unscrambled net (host trees with numpy draws in the
+    # reference's order, GPU walk), 32 -> 64 bits; a repeat call on the grown trees draws nothing
+    from qmctoolscl_amd import nus
+    q.dnb2_gen_natural(U(1), U(n), U(d), U(0), U(t), Cd, xb[:1], wait=False)
+    torch.cuda.synchronize()
+    seqs = np.random.SeedSequence(7).spawn(r * d)
+    rngs = np.empty((r, d), dtype=object)
+    for k in range(r * d):
+        rngs[k // d, k % d] = np.random.Generator(np.random.SFC64(seqs[k]))
+    roots = np.array([[nus.NUSNode_dnb2() for _ in range(d)] for _ in range(r)], dtype=object)
+    xb1 = xb[:1].contiguous()
+    t0 = time.perf_counter()
+    nus.dnb2_nested_uniform_scramble(r, n, d, 1, t, 64, rngs, roots, xb1, xrb)
+    torch.cuda.synchronize()
+    t_first = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    nus.dnb2_nested_uniform_scramble(r, n, d, 1, t, 64, rngs, roots, xb1, xrb)
+    torch.cuda.synchronize()
+    t_again = time.perf_counter() - t0
+    out["nus"] = {"workload": "dnb2 nested uniform scramble, n=2^16, d=8, r=4, 32 -> 64 bits",
+                  "points": pts, "first_call_s": round(t_first, 3), "repeat_call_s": round(t_again, 4),
+                  "first_call_Mpts_s": round(pts / t_first / 1e6, 3)}
+    return out
 
 
 def section_gdn(torch, q, dev, timed, r=8):
