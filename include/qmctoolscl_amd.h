@@ -260,7 +260,7 @@ QMC_EXPORT int qmc_gdn_gen_scramble_float_dev(int mode, qmc_u64 r, qmc_u64 n, qm
  * row counts, for that many rng.permutation(b) rows); the caller appends the
  * draw values (the root's first) and *_apply_dev scrambles on the GPU.
  * Base-b rows are 16-bit (b <= 2^16).  Trees persist across calls until
- * freed; qmc_nusb_apply_dev keeps the packed device copy of the last set of
+ * freed; each *_apply_dev keeps the packed device copy of the last set of
  * trees it was given and uploads again only when a tree grew. */
 QMC_EXPORT void *qmc_nus2_tree_new(int tmax_new);
 QMC_EXPORT void qmc_nus2_tree_free(void *tree);

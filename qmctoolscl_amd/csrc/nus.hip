@@ -43,6 +43,7 @@ struct Node2 {
 
 struct Tree2 {
     int tmax_new;
+    u64 id = 0, version = 0;  // identity and mutation count: the device copy is reused while they match
     std::vector<Node2> nodes;
     std::vector<u64> draws;  // draw values, index = owner id
     u64 ndraws_scheduled;    // draws recorded so far (root included)
@@ -298,6 +299,15 @@ struct NusbDevice {
 };
 NusbDevice g_nusb;  // guarded by g_nusb_mu
 std::mutex g_nusb_mu;
+// the same for base-2 trees
+struct Nus2Device {
+    std::vector<u64> sig;
+    int device = -1;
+    char *buf = nullptr;
+    Packed2 P{};
+};
+Nus2Device g_nus2;  // guarded by g_nus2_mu
+std::mutex g_nus2_mu;
 
 }  // namespace
 
@@ -312,6 +322,7 @@ QMC_EXPORT void *qmc_nus2_tree_new(int tmax_new) {
     }
     Tree2 *T = new Tree2();
     T->tmax_new = tmax_new;
+    T->id = g_treeb_ids++;
     T->ndraws_scheduled = 1;  // draw 0: the root's tmax_new bits, drawn by the caller
     new_leaf2(T, 0, 0);       // the root starts as a leaf for the value 0 (rand_funcs.py:195-199)
     return T;
@@ -328,7 +339,9 @@ QMC_EXPORT int64_t qmc_nus2_insert(void *h, const u64 *v, u64 n, u64 stride, int
     Tree2 *T = (Tree2 *)h;
     std::vector<uint32_t> w;
     w.reserve(n);
+    const size_t before = T->nodes.size();
     for (u64 k = 0; k < n; k++) insert2(T, v[k * stride] << tdelta, w);
+    if (T->nodes.size() != before || !w.empty()) T->version++;
     if (!w.empty()) std::memcpy(widths, w.data(), w.size() * sizeof(uint32_t));
     return (int64_t)w.size();
 }
@@ -339,6 +352,7 @@ QMC_EXPORT int qmc_nus2_add_draws(void *h, const u64 *vals, u64 count) {
     if (T->draws.size() + count > T->ndraws_scheduled)
         return qmc_fail(QMC_ERR_ARG, "nus: more draws than the tree asked for");
     T->draws.insert(T->draws.end(), vals, vals + count);
+    if (count) T->version++;
     return QMC_OK;
 }
 
@@ -349,58 +363,84 @@ QMC_EXPORT int qmc_nus2_apply_dev(void **trees, u64 r, u64 n, u64 d, u64 r_x, in
                                   const u64 *xb, u64 *xrb, qmc_stream_t s) {
     if (!(r && n && d)) return QMC_OK;
     const u64 nt = r * d;
-    std::vector<int64_t> node0(nt), draw0(nt);
-    u64 nn = 0, ndr = 0;
+    std::vector<u64> sig;
+    sig.reserve(2 * nt + 2);
+    sig.push_back(r);
+    sig.push_back(d);
     for (u64 k = 0; k < nt; k++) {
         Tree2 *T = (Tree2 *)trees[k];
         if (T->draws.size() != T->ndraws_scheduled)
             return qmc_fail(QMC_ERR_ARG, "nus: tree %llu is missing draws", (unsigned long long)k);
-        node0[k] = (int64_t)nn;
-        draw0[k] = (int64_t)ndr;
-        nn += T->nodes.size();
-        ndr += T->draws.size();
+        sig.push_back(T->id);
+        sig.push_back(T->version);
     }
-    std::vector<int32_t> kid(2 * nn), owner(nn), leaf(nn);
-    std::vector<u64> draws(ndr);
-    for (u64 k = 0; k < nt; k++) {
-        Tree2 *T = (Tree2 *)trees[k];
-        for (size_t q = 0; q < T->nodes.size(); q++) {
-            const Node2 &N = T->nodes[q];
-            const u64 g = node0[k] + q;
-            kid[2 * g] = N.kid[0] < 0 ? 0 : N.kid[0];
-            kid[2 * g + 1] = N.kid[1] < 0 ? 0 : N.kid[1];
-            owner[g] = N.owner;
-            leaf[g] = N.leaf;
+    int dev = 0;
+    QMC_HIP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_nus2_mu);
+    if (g_nus2.buf == nullptr || g_nus2.device != dev || g_nus2.sig != sig) {
+        std::vector<int64_t> node0(nt), draw0(nt);
+        u64 nn = 0, ndr = 0;
+        for (u64 k = 0; k < nt; k++) {
+            Tree2 *T = (Tree2 *)trees[k];
+            node0[k] = (int64_t)nn;
+            draw0[k] = (int64_t)ndr;
+            nn += T->nodes.size();
+            ndr += T->draws.size();
         }
-        std::memcpy(draws.data() + draw0[k], T->draws.data(), T->draws.size() * sizeof(u64));
+        std::vector<int32_t> kid(2 * nn), owner(nn), leaf(nn);
+        std::vector<u64> draws(ndr > 0 ? ndr : 1);
+        for (u64 k = 0; k < nt; k++) {
+            Tree2 *T = (Tree2 *)trees[k];
+            for (size_t q = 0; q < T->nodes.size(); q++) {
+                const Node2 &N = T->nodes[q];
+                const u64 g = node0[k] + q;
+                kid[2 * g] = N.kid[0] < 0 ? 0 : N.kid[0];
+                kid[2 * g + 1] = N.kid[1] < 0 ? 0 : N.kid[1];
+                owner[g] = N.owner;
+                leaf[g] = N.leaf;
+            }
+            if (!T->draws.empty())
+                std::memcpy(draws.data() + draw0[k], T->draws.data(), T->draws.size() * sizeof(u64));
+        }
+        if (g_nus2.buf) {
+            (void)hipFree(g_nus2.buf);
+            g_nus2.buf = nullptr;
+        }
+        g_nus2.sig.clear();
+        const size_t total = kid.size() * 4 + owner.size() * 4 + leaf.size() * 4 + draws.size() * 8 + 2 * nt * 8 + 64;
+        char *buf = nullptr;
+        if (hipMalloc((void **)&buf, total) != hipSuccess)
+            return qmc_fail(QMC_ERR_NOMEM, "nus: device allocation of %zu bytes for the trees failed", total);
+        size_t o = 0;
+        hipError_t ce = hipSuccess;
+        auto put = [&](const void *src, size_t bytes) -> void * {
+            o = (o + 7) & ~(size_t)7;
+            void *dst = buf + o;
+            if (bytes && ce == hipSuccess) ce = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+            o += bytes;
+            return dst;
+        };
+        Packed2 P;
+        P.kid = (const int32_t *)put(kid.data(), kid.size() * 4);
+        P.owner = (const int32_t *)put(owner.data(), owner.size() * 4);
+        P.leaf = (const int32_t *)put(leaf.data(), leaf.size() * 4);
+        P.draws = (const u64 *)put(draws.data(), draws.size() * 8);
+        P.node0 = (const int64_t *)put(node0.data(), nt * 8);
+        P.draw0 = (const int64_t *)put(draw0.data(), nt * 8);
+        if (ce != hipSuccess) {
+            (void)hipFree(buf);
+            return qmc_fail_hip(ce, "nus: copying the trees");
+        }
+        g_nus2.buf = buf;
+        g_nus2.P = P;
+        g_nus2.device = dev;
+        g_nus2.sig = sig;
     }
-    char *buf = nullptr;
-    const size_t b_kid = kid.size() * 4, b_own = owner.size() * 4, b_leaf = leaf.size() * 4;
-    const size_t b_dr = draws.size() * 8, b_off = nt * 8;
-    const size_t total = b_kid + b_own + b_leaf + b_dr + 2 * b_off + 64;
-    QMC_HIP_CHECK(hipMallocAsync((void **)&buf, total, s));
-    size_t o = 0;
-    auto put = [&](const void *src, size_t bytes) -> void * {
-        o = (o + 7) & ~(size_t)7;
-        void *dst = buf + o;
-        (void)hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
-        o += bytes;
-        return dst;
-    };
-    Packed2 P;
-    P.kid = (const int32_t *)put(kid.data(), b_kid);
-    P.owner = (const int32_t *)put(owner.data(), b_own);
-    P.leaf = (const int32_t *)put(leaf.data(), b_leaf);
-    P.draws = (const u64 *)put(draws.data(), b_dr);
-    P.node0 = (const int64_t *)put(node0.data(), b_off);
-    P.draw0 = (const int64_t *)put(draw0.data(), b_off);
     const int tdelta = tmax_new - tmax;
     hipLaunchKernelGGL(nus2_apply_kernel, dim3(grid_for(r * n * d, kBlock, 2048 * 8)), dim3(kBlock), 0, s, r, n, d,
-                       r_x, tmax_new, tdelta, P, xb, xrb);
+                       r_x, tmax_new, tdelta, g_nus2.P, xb, xrb);
     hipError_t e = hipGetLastError();
-    (void)hipFreeAsync(buf, s);
-    // the host vectors must outlive the async copies
-    QMC_HIP_CHECK(hipStreamSynchronize(s));
+    if (e == hipSuccess) e = hipStreamSynchronize(s);  // the cached copy may be replaced by the next call
     if (e != hipSuccess) return qmc_fail_hip(e, "nus2_apply_kernel");
     return QMC_OK;
 }
