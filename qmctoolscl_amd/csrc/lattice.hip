@@ -202,8 +202,13 @@ int check_k_range(u64 n_start, u64 n, const char *who) {
     return QMC_OK;
 }
 
-// grid for `pairs` element pairs: one pair per thread up to 65536 workgroups
-unsigned pair_grid(u64 nd) { return grid_for((nd + 1) / 2, kBlock, 65536); }
+// grid for `pairs` element pairs: one pair per thread (up to 2^20 workgroups,
+// then the kernels' grid-stride loops take the rest).  A full grid writes
+// faster than a capped one looping: the waves in flight then cover one
+// contiguous window of the output, where a grid-stride loop sends each wave's
+// successive stores gridDim*blockDim pairs apart (n = 2^20, d = 64, r = 16:
+// 710 Gpts/s with 65536 workgroups, 745 with the full 131072).
+unsigned pair_grid(u64 nd) { return grid_for((nd + 1) / 2, kBlock); }
 
 bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
@@ -259,7 +264,7 @@ int lat_shift_mod_1_dev(u64 r, u64 n, u64 d, u64 r_x, const double *x, const dou
     if (r_x == 0) return qmc_fail(QMC_ERR_ARG, "lat_shift_mod_1: r_x must be positive");
     RowMap m = make_map(n, d, 0, 0);
     const u64 rx = r_x < r ? r_x : r;  // replications of x beyond r are never read
-    hipLaunchKernelGGL(lat_shift_kernel, dim3(grid_for(rx * ((m.nd + 1) / 2), kBlock, 65536)), dim3(kBlock), 0, s, m,
+    hipLaunchKernelGGL(lat_shift_kernel, dim3(grid_for(rx * ((m.nd + 1) / 2), kBlock)), dim3(kBlock), 0, s, m,
                        r, rx, use_vec(m.nd, x, xr), x, shifts, xr);
     QMC_HIP_CHECK(hipGetLastError());
     return QMC_OK;
