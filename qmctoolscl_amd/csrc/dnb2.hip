@@ -139,7 +139,7 @@ __global__ __launch_bounds__(kBlock) void shift2_kernel(u64 r, u64 r_x, u64 nd, 
                                                         u64 *__restrict__ xrb) {
     const u64 stride = (u64)gridDim.x * blockDim.x;
     for (u64 e2 = (u64)blockIdx.x * blockDim.x + threadIdx.x; e2 < nd; e2 += stride) {
-        const u64 j = e2 % d;
+        const u64 j = ((e2 | d) >> 32) ? e2 % d : (u64)((u32)e2 % (u32)d);
         for (u64 lx = 0; lx < r_x && lx < r; lx++) {
             const u64 v = xb[lx * nd + e2] << (lshifts[lx] & 63);
             for (u64 l = lx; l < r; l += r_x) store_stream(xrb + l * nd + e2, v ^ shiftsb[l * d + j]);
@@ -229,7 +229,8 @@ int launch_gen(bool natural, u64 r, u64 n, u64 d, u64 n_start, u64 mmax, const u
     return QMC_OK;
 }
 
-unsigned ew_grid(u64 work) { return grid_for(work, kBlock, 2048 * 8); }
+// one element per thread (up to 2^20 workgroups): full grids write faster than capped grid-stride ones
+unsigned ew_grid(u64 work) { return grid_for(work, kBlock); }
 
 }  // namespace
 
