@@ -596,12 +596,26 @@ def main():
         q.lat_gen_natural(U(1), U(n), U(D), U(0), g_t, x1, wait=False)
         q.lat_shift_mod_1(U(R), U(n), U(D), U(1), x1, s_t, xr, wait=False)
 
+    ceiling = None
     if args.calibrate_copy:
         a = torch.ones(1 << 28, dtype=torch.float64, device=dev)
         b = torch.empty_like(a)
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+        torch.cuda.empty_cache()
+        # write-rate references over the same 8 GiB the workload writes: torch's fill and the
+        # library's 16-byte full-grid fill
+        L = q._lib.lib()
+        L.qmc_fill_dev.restype = ctypes.c_int
+        L.qmc_fill_dev.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_double, ctypes.c_void_p]
+        buf = torch.empty(pts, dtype=torch.float64, device=dev)
+        ms_tf, _ = timed(lambda: buf.fill_(0.5), 5, 1)
+        ms_qf, _ = timed(lambda: q._lib.check(L.qmc_fill_dev(buf.data_ptr(), pts, 0.5, ctypes.c_void_p(
+            torch.cuda.current_stream(dev).cuda_stream))), 5, 1)
+        ceiling = {"bytes": 8 * pts, "torch_fill_GBs": round(8 * pts / ms_tf / 1e6, 1),
+                   "fill16_GBs": round(8 * pts / ms_qf / 1e6, 1)}
+        del buf
         torch.cuda.empty_cache()
     log("main: lattice n=2^20 d=64 r=16")
     ms_fused, wall = timed(step_fused, args.steps, args.warmup)
@@ -634,6 +648,8 @@ def main():
         "wall_s_timed": round(wall, 3),
         "sections": {},
     }
+    if ceiling is not None:
+        out["write_ceiling"] = ceiling
     del x1
     torch.cuda.empty_cache()
     if "dnb2" in sections:

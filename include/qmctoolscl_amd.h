@@ -230,6 +230,11 @@ QMC_EXPORT int qmc_dnb2_kernel_apply_dev(int mode, qmc_u64 r, qmc_u64 n, const d
  * No single reference symbol: each equals a chain of reference calls as the
  * doctests chain them (docs/index.md:64-208, 419-677), in one HBM pass. */
 
+/* x[0 .. n) = v with one 16-byte store per thread over a full grid (x 16-byte
+ * aligned, n even): no reference counterpart; the plain-store write rate the
+ * generators are measured against. */
+QMC_EXPORT int qmc_fill_dev(double *x, qmc_u64 n, double v, qmc_stream_t s);
+
 /* lat_gen_<order>(r_g, ...) then lat_shift_mod_1(r, ..., r_x = r_g, ...);
  * order 0 linear, 1 natural, 2 gray */
 QMC_EXPORT int qmc_lat_gen_shift_dev(int order, qmc_u64 r, qmc_u64 n, qmc_u64 d, qmc_u64 r_g, qmc_u64 n_start,

@@ -136,6 +136,16 @@ __global__ __launch_bounds__(kBlock) void lat_gen_shift_kernel(RowMap m, u64 r, 
     }
 }
 
+// x[i] = v, one 16-byte store per thread over a full grid: the store pattern
+// of the generators at its simplest, a reference for their write rate
+__global__ __launch_bounds__(kBlock) void fill_pairs_kernel(u64 pairs, double v, double *__restrict__ x) {
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 p = (u64)blockIdx.x * blockDim.x + threadIdx.x; p < pairs; p += stride) {
+        double2_t w = {v, v};
+        *(double2_t *)(x + 2 * p) = w;
+    }
+}
+
 // xr[l, i, j] = fmod(x[l % r_x, i, j] + shifts[l, j], 1): each x element is read
 // once and feeds the r / r_x replications that share it.
 __global__ __launch_bounds__(kBlock) void lat_shift_kernel(RowMap m, u64 r, u64 r_x, bool vec,
@@ -238,6 +248,16 @@ int launch_gen_shift(u64 r, u64 n, u64 d, u64 rg, u64 n_start, const u64 *g, con
 }  // namespace
 
 extern "C" {
+
+// x[0 .. n) = v (x 16-byte aligned, n even): the plain-store write rate the
+// generators are measured against
+QMC_EXPORT int qmc_fill_dev(double *x, qmc_u64 n, double v, qmc_stream_t s) {
+    if (n == 0) return QMC_OK;
+    if ((n % 2) || !aligned16(x)) return qmc_fail(QMC_ERR_ARG, "fill: x must be 16-byte aligned and n even");
+    hipLaunchKernelGGL(fill_pairs_kernel, dim3(grid_for(n / 2, kBlock, 0x7fffffffu)), dim3(kBlock), 0, s, n / 2, v, x);
+    QMC_HIP_CHECK(hipGetLastError());
+    return QMC_OK;
+}
 
 int lat_gen_linear_dev(u64 r, u64 n, u64 d, const u64 *g, double *x, hipStream_t s) {
     if (r == 0 || n == 0 || d == 0) return QMC_OK;
