@@ -150,3 +150,44 @@ def test_gdn_nus_matches_oracle_gpu(bases, tmax, tmax_new):
         nus_oracle.gdn_nus(r, part.shape[1], d, r_x, 1, tmax, tmax_new, rngs_o, roots_o, b, part, want)
         q.gdn_nested_uniform_scramble(r, part.shape[1], d, r_x, 1, tmax, tmax_new, rngs_p, roots_p, b, part, got)
         assert np.array_equal(got, want)
+
+
+@pytest.mark.gpu
+def test_nus_device_copies_follow_the_trees_passed():
+    """The apply calls keep the packed device copy of the last trees they were
+    given; calls alternating between two sets of trees (A, B, A grown, B
+    unchanged) must each use their own trees."""
+    rng = np.random.default_rng(21)
+    r, d, tmax, tnew = 2, 3, 6, 10
+    # base 2
+    xb = rng.integers(0, 2**tmax, size=(1, 80, d), dtype=np.uint64)
+    sets = []
+    for seed in (3, 4):
+        sets.append(dict(ro=np.array([nus_oracle.Node2() for _ in range(r * d)]).reshape(r, d),
+                         rp=np.array([q.NUSNode_dnb2() for _ in range(r * d)]).reshape(r, d),
+                         go=_rngs(r, d, seed), gp=_rngs(r, d, seed)))
+    for k, part in ((0, xb[:, :40]), (1, xb[:, :40]), (0, xb[:, 40:]), (1, xb[:, :40])):
+        part = np.ascontiguousarray(part)
+        S = sets[k]
+        want = np.zeros((r, part.shape[1], d), dtype=np.uint64)
+        got = np.zeros_like(want)
+        nus_oracle.dnb2_nus(r, part.shape[1], d, 1, tmax, tnew, S["go"], S["ro"], part, want)
+        q.dnb2_nested_uniform_scramble(r, part.shape[1], d, 1, tmax, tnew, S["gp"], S["rp"], part, got)
+        assert np.array_equal(got, want), k
+    # base b
+    bases = [2, 3, 7]
+    b = np.array([bases], dtype=np.uint64)
+    xdig = np.stack([rng.integers(0, bb, size=(1, 50, tmax), dtype=np.uint64) for bb in bases], axis=2)
+    sets = []
+    for seed in (5, 6):
+        sets.append(dict(ro=np.array([nus_oracle.NodeB() for _ in range(r * d)]).reshape(r, d),
+                         rp=np.array([q.NUSNode_gdn() for _ in range(r * d)]).reshape(r, d),
+                         go=_rngs(r, d, seed), gp=_rngs(r, d, seed)))
+    for k, part in ((0, xdig[:, :25]), (1, xdig[:, :25]), (0, xdig[:, 25:]), (1, xdig[:, :25])):
+        part = np.ascontiguousarray(part)
+        S = sets[k]
+        want = np.zeros((r, part.shape[1], d, tnew), dtype=np.uint64)
+        got = np.zeros_like(want)
+        nus_oracle.gdn_nus(r, part.shape[1], d, 1, 1, tmax, tnew, S["go"], S["ro"], b, part, want)
+        q.gdn_nested_uniform_scramble(r, part.shape[1], d, 1, 1, tmax, tnew, S["gp"], S["rp"], b, part, got)
+        assert np.array_equal(got, want), k
