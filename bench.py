@@ -25,7 +25,7 @@ CPU baseline: the reference C backend built from its own sources (oracle/_ref)
 -- or this project's C restatement of it where absent -- single-threaded as
 the reference C backend is, on a bounded sample of the headline workload.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--sections main,dnb2,batch,transforms]
+    python bench.py [--gpus 1] [--steps 50] [--warmup 20] [--sections main,dnb2,batch,transforms]
 """
 import argparse
 import ctypes
@@ -554,8 +554,8 @@ def section_halton(torch, dev, timed):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-rows", type=int, default=1 << 19)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sections", default="main,dnb2,transforms,gdn,halton,kernels,batch")

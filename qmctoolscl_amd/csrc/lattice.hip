@@ -98,40 +98,84 @@ __device__ __forceinline__ void store_pair(double *p, bool vec, bool ok0, bool o
     }
 }
 
+// Replication groups.  A thread computes its element pair's index fraction
+// once and writes the pair into kRepGroup consecutive replication slabs;
+// blockIdx.y selects the group and x-fastest dispatch order makes the grid
+// group-major, so the waves in flight write kRepGroup slabs, each over one
+// contiguous window.  Each replication reads its g and shift entries again
+// (L1 hits) just before its store.  Measured for n = 2^20, d = 64, r = 16:
+//   all 16 slabs per thread (one group)                     742 Gpts/s
+//   groups of 8                                             795
+//   groups of 4                                             791
+//   one slab per thread                                     650
+//   groups of 8 with g read once per thread                 758
+//   all 16 shifts loaded before the first store             565
+//   g and shifts staged in LDS per workgroup                655-713
+// against 868 for a plain 16-byte fill of the same 8 GiB.  Fewer windows
+// open at once and fewer stores in flight per wave both write faster.
+constexpr u32 kRepGroup = 8;
+
+struct RepGrid {
+    dim3 grid;
+    u64 groups;
+};
+
+// x: one pair per thread (up to 2^20 workgroups, then the kernels'
+// grid-stride loops take the rest).  A full grid writes faster than a capped
+// one looping: the waves in flight then cover one contiguous window of each
+// slab, where a grid-stride loop sends each wave's successive stores
+// gridDim*blockDim pairs apart (710 Gpts/s with 65536 workgroups, 745 with
+// the full 131072, before the groups).  y: the replication groups, looped
+// beyond 65535.
+RepGrid rep_grid(u64 pairs, u64 reps) {
+    RepGrid G;
+    G.groups = (reps + kRepGroup - 1) / kRepGroup;
+    G.grid = dim3(grid_for(pairs, kBlock), (unsigned)(G.groups < 65535 ? G.groups : 65535));
+    return G;
+}
+
 // x[l, row, j] = fmod((double)g[l, j] * frac(row), 1) for all r generating vectors
 template <int ORDER>
-__global__ __launch_bounds__(kBlock) void lat_gen_kernel(RowMap m, u64 r, bool vec,
+__global__ __launch_bounds__(kBlock) void lat_gen_kernel(RowMap m, u64 r, u64 groups, bool vec,
                                                          const u64 *__restrict__ g,
                                                          double *__restrict__ x) {
     const u64 stride = (u64)gridDim.x * blockDim.x;
     const u64 pairs = (m.nd + 1) / 2;
-    for (u64 p = (u64)blockIdx.x * blockDim.x + threadIdx.x; p < pairs; p += stride) {
-        const u64 e2 = 2 * p;
-        const Pair P = pair_at<ORDER>(m, e2);
-        for (u64 l = 0; l < r; l++)
-            store_pair(x + l * m.nd + e2, vec, P.ok0, P.ok1, frac_mod1((double)g[l * m.d + P.j0] * P.f0),
-                       P.ok1 ? frac_mod1((double)g[l * m.d + P.j1] * P.f1) : 0.0);
+    for (u64 grp = blockIdx.y; grp < groups; grp += gridDim.y) {
+        const u64 l0 = grp * kRepGroup, l1 = l0 + kRepGroup < r ? l0 + kRepGroup : r;
+        for (u64 p = (u64)blockIdx.x * blockDim.x + threadIdx.x; p < pairs; p += stride) {
+            const u64 e2 = 2 * p;
+            const Pair P = pair_at<ORDER>(m, e2);
+            for (u64 l = l0; l < l1; l++)
+                store_pair(x + l * m.nd + e2, vec, P.ok0, P.ok1, frac_mod1((double)g[l * m.d + P.j0] * P.f0),
+                           P.ok1 ? frac_mod1((double)g[l * m.d + P.j1] * P.f1) : 0.0);
+        }
     }
 }
 
 // xr[l, row, j] = fmod(fmod((double)g[l % r_g, j] * frac(row), 1) + shifts[l, j], 1):
 // lat_gen_* followed by lat_shift_mod_1 with r_x = r_g, one pass over HBM.
 template <int ORDER>
-__global__ __launch_bounds__(kBlock) void lat_gen_shift_kernel(RowMap m, u64 r, u64 r_g, bool vec,
+__global__ __launch_bounds__(kBlock) void lat_gen_shift_kernel(RowMap m, u64 r, u64 r_g, u64 groups, bool vec,
                                                                const u64 *__restrict__ g,
                                                                const double *__restrict__ shifts,
                                                                double *__restrict__ xr) {
     const u64 stride = (u64)gridDim.x * blockDim.x;
     const u64 pairs = (m.nd + 1) / 2;
-    for (u64 p = (u64)blockIdx.x * blockDim.x + threadIdx.x; p < pairs; p += stride) {
-        const u64 e2 = 2 * p;
-        const Pair P = pair_at<ORDER>(m, e2);
-        for (u64 lg = 0; lg < r_g; lg++) {
-            const double x0 = frac_mod1((double)g[lg * m.d + P.j0] * P.f0);
-            const double x1 = P.ok1 ? frac_mod1((double)g[lg * m.d + P.j1] * P.f1) : 0.0;
-            for (u64 l = lg; l < r; l += r_g)
+    for (u64 grp = blockIdx.y; grp < groups; grp += gridDim.y) {
+        const u64 l0 = grp * kRepGroup, l1 = l0 + kRepGroup < r ? l0 + kRepGroup : r;
+        const u64 lg0 = l0 % r_g;
+        for (u64 p = (u64)blockIdx.x * blockDim.x + threadIdx.x; p < pairs; p += stride) {
+            const u64 e2 = 2 * p;
+            const Pair P = pair_at<ORDER>(m, e2);
+            u64 lg = lg0;
+            for (u64 l = l0; l < l1; l++) {
+                const double x0 = frac_mod1((double)g[lg * m.d + P.j0] * P.f0);
+                const double x1 = P.ok1 ? frac_mod1((double)g[lg * m.d + P.j1] * P.f1) : 0.0;
                 store_pair(xr + l * m.nd + e2, vec, P.ok0, P.ok1, frac_mod1(x0 + shifts[l * m.d + P.j0]),
                            frac_mod1(x1 + shifts[l * m.d + P.j1]));
+                if (++lg == r_g) lg = 0;
+            }
         }
     }
 }
@@ -212,14 +256,6 @@ int check_k_range(u64 n_start, u64 n, const char *who) {
     return QMC_OK;
 }
 
-// grid for `pairs` element pairs: one pair per thread (up to 2^20 workgroups,
-// then the kernels' grid-stride loops take the rest).  A full grid writes
-// faster than a capped one looping: the waves in flight then cover one
-// contiguous window of the output, where a grid-stride loop sends each wave's
-// successive stores gridDim*blockDim pairs apart (n = 2^20, d = 64, r = 16:
-// 710 Gpts/s with 65536 workgroups, 745 with the full 131072).
-unsigned pair_grid(u64 nd) { return grid_for((nd + 1) / 2, kBlock); }
-
 bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
 // 16-byte stores need even slabs (pairs never straddle replications) and aligned bases
@@ -230,7 +266,8 @@ bool use_vec(u64 nd, const void *a, const void *b = nullptr) {
 template <int ORDER>
 int launch_gen(u64 r, u64 n, u64 d, u64 n_start, const u64 *g, double *x, hipStream_t s, int check) {
     RowMap m = make_map(n, d, n_start, check);
-    hipLaunchKernelGGL(lat_gen_kernel<ORDER>, dim3(pair_grid(m.nd)), dim3(kBlock), 0, s, m, r, use_vec(m.nd, x), g, x);
+    const RepGrid G = rep_grid((m.nd + 1) / 2, r);
+    hipLaunchKernelGGL(lat_gen_kernel<ORDER>, G.grid, dim3(kBlock), 0, s, m, r, G.groups, use_vec(m.nd, x), g, x);
     QMC_HIP_CHECK(hipGetLastError());
     return QMC_OK;
 }
@@ -239,7 +276,8 @@ template <int ORDER>
 int launch_gen_shift(u64 r, u64 n, u64 d, u64 rg, u64 n_start, const u64 *g, const double *shifts, double *xr,
                      hipStream_t s, int check) {
     RowMap m = make_map(n, d, n_start, check);
-    hipLaunchKernelGGL(lat_gen_shift_kernel<ORDER>, dim3(pair_grid(m.nd)), dim3(kBlock), 0, s, m, r, rg,
+    const RepGrid G = rep_grid((m.nd + 1) / 2, r);
+    hipLaunchKernelGGL(lat_gen_shift_kernel<ORDER>, G.grid, dim3(kBlock), 0, s, m, r, rg, G.groups,
                        use_vec(m.nd, xr), g, shifts, xr);
     QMC_HIP_CHECK(hipGetLastError());
     return QMC_OK;

@@ -106,22 +106,32 @@ def test_kernels_match_reference_digests():
             assert _clib.digest(got[o]) == dg, key + ":" + o
 
 
-def test_fused_lattice_equals_chain():
+# (n, d, r_g, r): the generators write replications in groups of 8 (grid y,
+# looped past 65535 groups); the cases cover one group, partial groups, r_g
+# not dividing the group start, and 75000 groups
+@pytest.mark.parametrize("n,d,r_g,r", [(1 << 12, 7, 2, 6), (1 << 10, 13, 3, 19), (1 << 9, 6, 11, 20),
+                                       (2, 1, 600000, 600000)])
+def test_fused_lattice_equals_chain(n, d, r_g, r):
     import torch
     from qmctoolscl_amd import fused
+    from oracle_api import OracleAPI
+    orc = OracleAPI()
     rng = np.random.default_rng(1)
-    n, d, r_g, r = 1 << 12, 7, 2, 6
     g = (2 * rng.integers(0, 2**19, size=(r_g, d), dtype=np.uint64) + 1)
     shifts = rng.random((r, d))
     for order, fn in ((0, "lat_gen_linear"), (1, "lat_gen_natural"), (2, "lat_gen_gray")):
-        x = np.empty((r_g, n, d))
         args = (np.uint64(r_g), np.uint64(n), np.uint64(d)) + (() if order == 0 else (np.uint64(0),))
-        getattr(q, fn)(*args, g, x)
-        xr = np.empty((r, n, d))
-        q.lat_shift_mod_1(np.uint64(r), np.uint64(n), np.uint64(d), np.uint64(r_g), x, shifts, xr)
+        chain = []
+        for api in (q, orc):
+            x = np.empty((r_g, n, d))
+            getattr(api, fn)(*args, g, x)
+            xr = np.empty((r, n, d))
+            api.lat_shift_mod_1(np.uint64(r), np.uint64(n), np.uint64(d), np.uint64(r_g), x, shifts, xr)
+            chain.append(xr.view(np.uint64))
+        assert np.array_equal(chain[0], chain[1]), order
         out = torch.empty((r, n, d), dtype=torch.float64, device="cuda")
         fused.lat_gen_shift(order, r, n, d, r_g, 0, to_t(g), to_t(shifts), out)
-        assert np.array_equal(out.cpu().numpy().view(np.uint64), xr.view(np.uint64)), order
+        assert np.array_equal(out.cpu().numpy().view(np.uint64), chain[1]), order
 
 
 def test_fused_dnb2_equals_chain():
