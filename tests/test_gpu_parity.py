@@ -108,9 +108,10 @@ def test_kernels_match_reference_digests():
 
 # (n, d, r_g, r): the generators write replications in groups of 8 (grid y,
 # looped past 65535 groups); the cases cover one group, partial groups, r_g
-# not dividing the group start, and 75000 groups
+# not dividing the group start, shifts writing more than 8 replications from
+# one x slab, and 75000 groups
 @pytest.mark.parametrize("n,d,r_g,r", [(1 << 12, 7, 2, 6), (1 << 10, 13, 3, 19), (1 << 9, 6, 11, 20),
-                                       (2, 1, 600000, 600000)])
+                                       (1 << 9, 5, 1, 20), (1 << 8, 3, 2, 37), (2, 1, 600000, 600000)])
 def test_fused_lattice_equals_chain(n, d, r_g, r):
     import torch
     from qmctoolscl_amd import fused
