@@ -520,8 +520,45 @@ struct FuseArgs {
     const u32 *Q, *okf;
     double *x;
     u32 *status;
-    u32 tmax, tnew, perm_lds, off_rt, off_tab;
+    u32 tmax, tnew, perm_lds, off_q, off_qs, off_rt, off_tab;
 };
+
+// Scrambled modes with a value table: V[z][t] = RN(recip_t * (double)y_t(z))
+// for z below the table's rows, t < TP (0 past tmax_new), where y_t is the
+// permutation perm_t(z) (mode 1) or the shifted digit (z + s_t) mod b, the
+// raw s_t past tmax (mode 2): the product the reference rounds before each
+// add (gdn_integer_to_float.c:36-38), so a digit costs one LDS load and one
+// add.  Mode 1 uses it when bmax rows of TP + 1 doubles fit in
+// kFuseTabBytes, mode 2 (bases unknown to the host) with as many rows as fit
+// there, a column with a larger base taking the general path; the table sits at LDS address 0 and the walk keeps each
+// digit as the byte offset z (TP + 1) 8 of its row, so the load's immediate
+// offset is 8 t.  The odd row length in doubles spreads the lanes' random z
+// over all LDS bank pairs (rows of TP doubles put every z on one of four).
+// Workgroups of kFuseTabBlock threads keep four waves per SIMD with two
+// tables per CU.
+constexpr u32 kFuseTabBytes = 65536;
+constexpr u32 kFuseTabBlock = 512;
+
+// carries of k -> k+1 in the point walk: returns the row of Q to add (the
+// number c of trailing digits of k equal to b - 1, capped at mmax - 1) and
+// advances the lowest digit dlow
+__device__ __forceinline__ u32 walk_carry(u32 &dlow, u64 k, u64 b, u32 bb, const DivU32 &bdiv, u32 mmax) {
+    u32 c = 0;
+    if (dlow + 1 < bb) {
+        dlow++;
+    } else {
+        dlow = 0;
+        u64 xq = (k + 1) / b;
+        c = 1;
+        while (xq != 0 && c + 1 < mmax) {
+            const u64 qq = (xq >> 32) ? xq / b : (u64)div_q((u32)xq, bdiv);
+            if (xq - qq * b != 0) break;
+            xq = qq;
+            c++;
+        }
+    }
+    return c < mmax ? c : mmax - 1;
+}
 
 template <int MODE>
 __device__ __forceinline__ u64 fuse_scramble(const FuseArgs &a, u32 t, u64 z, u64 b, u64 pbase, const u32 *sP,
@@ -557,8 +594,8 @@ __device__ __forceinline__ u64 fuse_tail(const FuseArgs &a, u32 t, u64 pbase, co
     return MODE == 2 ? sS[t] : 0;  // the raw shift past tmax
 }
 
-template <int TP, int MODE>
-__global__ __launch_bounds__(kBlock) void gdn_fused_kernel(FuseArgs a) {
+template <int TP, int MODE, bool TAB>
+__global__ __launch_bounds__(TAB ? kFuseTabBlock : kBlock) void gdn_fused_kernel(FuseArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const u64 bid = blockIdx.x;
     const u64 xcd = bid & 7, qd = bid >> 3;
@@ -571,9 +608,11 @@ __global__ __launch_bounds__(kBlock) void gdn_fused_kernel(FuseArgs a) {
     const u32 tmax = a.tmax, tnew = a.tnew;
     const u32 rows = (u32)TP > tnew ? (u32)TP : tnew;  // table rows in LDS, zero past tnew
     const bool ok = a.Q != nullptr && a.okf[lj] != 0;
-    // LDS: Q rows (mmax x TP u32, zero past tmax) | reciprocals (tnew) | the
-    // same zeroed past tmax (TP) | perms (rows x bmax u32) or shifts (rows u64)
-    u32 *sQ = (u32 *)smem;
+    // LDS: [value table V (bmax x TP f64) | Q rows scaled by TP * 8] when TAB |
+    // Q rows (mmax x TP u32, zero past tmax) | reciprocals (tnew) | the same
+    // zeroed past tmax (TP) | perms (rows x bmax u32) or shifts (rows u64)
+    u32 *sQ = (u32 *)(smem + a.off_q);
+    u32 *sQs = (u32 *)(smem + a.off_qs);
     double *srt = (double *)(smem + a.off_rt);
     double *srtm = srt + tnew;
     u32 *sP = (u32 *)(smem + a.off_tab);
@@ -582,7 +621,9 @@ __global__ __launch_bounds__(kBlock) void gdn_fused_kernel(FuseArgs a) {
     if (ok)
         for (u32 e = threadIdx.x; e < (u32)a.mmax * TP; e += blockDim.x) {
             const u32 m = e / TP, t = e - m * TP;
-            sQ[e] = t < tmax ? a.Q[(lj * a.mmax + m) * tmax + t] : 0u;
+            const u32 qv = t < tmax ? a.Q[(lj * a.mmax + m) * tmax + t] : 0u;
+            sQ[e] = qv;
+            if (TAB) sQs[e] = qv * (u32)((TP + 1) * 8);
         }
     if (threadIdx.x == 0) {  // gdn_integer_to_float.c:34-38
         const double bd = (double)b;
@@ -607,13 +648,38 @@ __global__ __launch_bounds__(kBlock) void gdn_fused_kernel(FuseArgs a) {
             sS[t] = v;
             flags |= (t < tmax && v >= b) ? 2 : 0;
         }
-    flags = (__syncthreads_or(flags & 1) ? 1 : 0) | (__syncthreads_or(flags & 2) ? 2 : 0);
+    if (TAB)
+        __syncthreads();  // no flags in this mode; the or-barriers would place LDS before the table
+    else
+        flags = (__syncthreads_or(flags & 1) ? 1 : 0) | (__syncthreads_or(flags & 2) ? 2 : 0);
+    if (TAB) {
+        double *V = (double *)smem;
+        const u32 vrows = (u32)(b < a.bmax ? b : a.bmax);  // digits z < b
+        for (u32 e = threadIdx.x; e < vrows * TP; e += blockDim.x) {
+            const u32 zz = e / TP, t = e - zz * TP;
+            double y = 0.0;
+            if (t < tnew) {
+                if (MODE == 1) {
+                    y = (double)a.tab[pbase + (u64)t * a.bmax + zz];
+                } else {  // gdn_digital_shift.c:37-42, the sum wrapping as the reference's u64 sum
+                    const u64 sh = sS[t], xs = (u64)zz + sh;
+                    y = (double)(t >= tmax ? sh : xs >= sh && xs < 2 * b ? (xs >= b ? xs - b : xs) : umod(xs, b));
+                }
+            }
+            V[zz * (TP + 1) + t] = t < tnew ? srt[t] * y : 0.0;
+        }
+        __syncthreads();
+    }
     const bool plds = MODE == 1 && a.perm_lds && !(flags & 1);
     // branch-free digit loop: every digit a walk produces is < b, so it needs
     // a table in LDS and b <= bmax (permutations) or shifts < b (no wrap)
-    const bool fast = MODE == 0 || (MODE == 1 && plds && b <= a.bmax) || (MODE == 2 && !(flags & 2));
+    // the table path addresses V by byte offset from LDS address 0, where
+    // the dynamic LDS starts when the kernel has no static LDS
+    const bool tab0 = TAB && (u32)(uintptr_t)(__attribute__((address_space(3))) const unsigned char *)smem == 0;
+    const bool fast = TAB ? tab0 && b <= a.bmax
+                          : MODE == 0 || (MODE == 1 && plds && b <= a.bmax) || (MODE == 2 && !(flags & 2));
 
-    const u64 i_first = chunk * ((u64)kBlock * kFuseH) + (u64)threadIdx.x * kFuseH;
+    const u64 i_first = chunk * ((u64)blockDim.x * kFuseH) + (u64)threadIdx.x * kFuseH;
     if (i_first >= a.n) return;
     const u32 H = (a.n - i_first) < (u64)kFuseH ? (u32)(a.n - i_first) : kFuseH;
     u64 k = i_first + a.n_start;
@@ -670,6 +736,30 @@ __global__ __launch_bounds__(kBlock) void gdn_fused_kernel(FuseArgs a) {
         }
     }
     u32 dlow = (k >> 32) ? (u32)(k % b) : (u32)k - div_q((u32)k, bdiv) * bb;
+    if (TAB && fast) {
+        constexpr u32 S = (TP + 1) * 8;
+        const u32 bS = bb * S;
+#pragma unroll
+        for (int t = 0; t < TP; t++) z[t] *= S;
+        for (u32 h = 0;; h++) {
+            double v = 0.;
+#pragma unroll
+            for (int t = 0; t < TP; t++)  // + 0 past tmax_new
+                v += *(__attribute__((address_space(3))) const double *)(uintptr_t)(z[t] + 8u * t);
+            for (u32 t = TP; t < tnew; t++) v += srt[t] * (double)fuse_tail<MODE>(a, t, pbase, sP, sS, false, bad);
+            xo[(u64)h * a.d] = v;
+            if (h + 1 == H) break;
+            const u32 *qc = sQs + walk_carry(dlow, k, b, bb, bdiv, (u32)a.mmax) * TP;
+#pragma unroll
+            for (int t = 0; t < TP; t++) {
+                const u32 s2 = z[t] + qc[t];
+                z[t] = __builtin_elementwise_min(s2, s2 - bS);
+            }
+            k++;
+        }
+        if (bad) flag(a.status, bad);
+        return;
+    }
     const u32 bm = (u32)a.bmax;
     for (u32 h = 0;; h++) {
         double v = 0.;
@@ -699,21 +789,7 @@ __global__ __launch_bounds__(kBlock) void gdn_fused_kernel(FuseArgs a) {
         for (u32 t = tmax; t < tnew; t++) v += srt[t] * (double)fuse_tail<MODE>(a, t, pbase, sP, sS, plds, bad);
         xo[(u64)h * a.d] = v;
         if (h + 1 == H) break;
-        u32 c = 0;
-        if (dlow + 1 < bb) {
-            dlow++;
-        } else {
-            dlow = 0;
-            u64 xq = (k + 1) / b;
-            c = 1;
-            while (xq != 0 && c + 1 < (u32)a.mmax) {
-                const u64 qq = (xq >> 32) ? xq / b : (u64)div_q((u32)xq, bdiv);
-                if (xq - qq * b != 0) break;
-                xq = qq;
-                c++;
-            }
-        }
-        const u32 *qc = sQ + (c < (u32)a.mmax ? c : (u32)a.mmax - 1) * TP;
+        const u32 *qc = sQ + walk_carry(dlow, k, b, bb, bdiv, (u32)a.mmax) * TP;
 #pragma unroll
         for (int t = 0; t < TP; t++) {
             const u32 s2 = z[t] + qc[t];
@@ -850,16 +926,19 @@ QMC_EXPORT int qmc_gdn_gen_scramble_float_dev(int mode, u64 r, u64 n, u64 d, u64
     const bool walk = mmax > 0 && kmax >= n_start && qwords * 4 <= (1ull << 30) && r * d < (1ull << 31);
     const int TP = tmax <= 16 ? 16 : tmax <= 32 ? 32 : tmax <= 56 ? 56 : 64;
     const u64 rows = (u64)TP > tmax_new ? (u64)TP : tmax_new;
+    const bool tabm = walk && ((mode == 1 && bmax * (TP + 1) * 8 <= kFuseTabBytes) || mode == 2);
+    const u64 vrows = mode == 1 ? bmax : kFuseTabBytes / ((TP + 1) * 8);  // rows of the value table
+    const u64 bs = tabm ? kFuseTabBlock : kBlock;
     FuseArgs a;
     a.n = n;
     a.d = d;
     a.r_b = r_b;
     a.mmax = mmax;
     a.n_start = n_start;
-    a.bmax = bmax;
+    a.bmax = tabm ? vrows : bmax;
     a.Ctotal = r * d * mmax * tmax;
     a.tab_total = mode == 1 ? r * d * tmax_new * bmax : r * d * tmax_new;
-    a.nchunks = (n + (u64)kBlock * kFuseH - 1) / ((u64)kBlock * kFuseH);
+    a.nchunks = (n + bs * kFuseH - 1) / (bs * kFuseH);
     a.ngroups = r * a.nchunks;
     a.bases = bases;
     a.C = C;
@@ -868,8 +947,11 @@ QMC_EXPORT int qmc_gdn_gen_scramble_float_dev(int mode, u64 r, u64 n, u64 d, u64
     a.status = status;
     a.tmax = (u32)tmax;
     a.tnew = (u32)tmax_new;
-    a.perm_lds = mode == 1 && rows * bmax * 4 <= 40960;
-    a.off_rt = (u32)((sizeof(u32) * mmax * TP + 15) & ~(u64)15);
+    a.perm_lds = mode == 1 && !tabm && rows * bmax * 4 <= 40960;
+    const u32 qbytes = (u32)((sizeof(u32) * mmax * TP + 15) & ~(u64)15);
+    a.off_q = tabm ? (u32)((vrows * (TP + 1) * 8 + 15) & ~(u64)15) : 0;
+    a.off_qs = a.off_q + qbytes;
+    a.off_rt = a.off_qs + (tabm ? qbytes : 0);
     a.off_tab = a.off_rt + (u32)(8 * (tmax_new + TP));
     const size_t shm = a.off_tab + (mode == 1 ? (a.perm_lds ? 4 * rows * bmax : 0) : mode == 2 ? 8 * rows : 0);
     const u64 nblocks = (a.ngroups + 7) / 8 * 8 * d;
@@ -881,25 +963,57 @@ QMC_EXPORT int qmc_gdn_gen_scramble_float_dev(int mode, u64 r, u64 n, u64 d, u64
     if (walk)
         hipLaunchKernelGGL(gdn_chain_setup_kernel, dim3((unsigned)(r * d)), dim3(64), 0, s, d, mmax, tmax, kmax,
                            r_b, bases, (u64)0, C, ws, ws + qwords);
+    if (tabm) {
+        // the value table and two Q tables exceed the default 64 KiB of LDS
+        static PerDeviceOnce attrs;
+        const int most = (int)(kFuseTabBytes + 2 * sizeof(u32) * 64 * 64 + 8 * 128 + 8 * 64);
+        const int rc = attrs.run([&]() -> int {
+            QMC_HIP_CHECK(hipFuncSetAttribute((const void *)gdn_fused_kernel<16, 1, true>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, most));
+            QMC_HIP_CHECK(hipFuncSetAttribute((const void *)gdn_fused_kernel<32, 1, true>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, most));
+            QMC_HIP_CHECK(hipFuncSetAttribute((const void *)gdn_fused_kernel<56, 1, true>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, most));
+            QMC_HIP_CHECK(hipFuncSetAttribute((const void *)gdn_fused_kernel<64, 1, true>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, most));
+            QMC_HIP_CHECK(hipFuncSetAttribute((const void *)gdn_fused_kernel<16, 2, true>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, most));
+            QMC_HIP_CHECK(hipFuncSetAttribute((const void *)gdn_fused_kernel<32, 2, true>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, most));
+            QMC_HIP_CHECK(hipFuncSetAttribute((const void *)gdn_fused_kernel<56, 2, true>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, most));
+            QMC_HIP_CHECK(hipFuncSetAttribute((const void *)gdn_fused_kernel<64, 2, true>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, most));
+            return QMC_OK;
+        });
+        if (rc != QMC_OK) {
+            if (ws) (void)hipFreeAsync(ws, s);
+            return rc;
+        }
+    }
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) {
-#define QMC_FUSE(tp, md)                                                                                    \
-    hipLaunchKernelGGL((gdn_fused_kernel<tp, md>), dim3((unsigned)nblocks), dim3(kBlock), shm, s, a)
-#define QMC_FUSE_TP(md)         \
-    if (TP == 16)               \
-        QMC_FUSE(16, md);       \
-    else if (TP == 32)          \
-        QMC_FUSE(32, md);       \
-    else if (TP == 56)          \
-        QMC_FUSE(56, md);       \
-    else                        \
-        QMC_FUSE(64, md)
+#define QMC_FUSE(tp, md, tb)                                                                                   \
+    hipLaunchKernelGGL((gdn_fused_kernel<tp, md, tb>), dim3((unsigned)nblocks), dim3((unsigned)bs), shm, s, a)
+#define QMC_FUSE_TP(md, tb)         \
+    if (TP == 16)                   \
+        QMC_FUSE(16, md, tb);       \
+    else if (TP == 32)              \
+        QMC_FUSE(32, md, tb);       \
+    else if (TP == 56)              \
+        QMC_FUSE(56, md, tb);       \
+    else                            \
+        QMC_FUSE(64, md, tb)
         if (mode == 0) {
-            QMC_FUSE_TP(0);
+            QMC_FUSE_TP(0, false);
+        } else if (mode == 1 && tabm) {
+            QMC_FUSE_TP(1, true);
         } else if (mode == 1) {
-            QMC_FUSE_TP(1);
+            QMC_FUSE_TP(1, false);
+        } else if (tabm) {
+            QMC_FUSE_TP(2, true);
         } else {
-            QMC_FUSE_TP(2);
+            QMC_FUSE_TP(2, false);
         }
 #undef QMC_FUSE_TP
 #undef QMC_FUSE

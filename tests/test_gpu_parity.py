@@ -345,6 +345,10 @@ def _oracle_gdn_chain(mode, r, n, d, r_b, mmax, tmax, tnew, n_start, bmax, bases
     (1, 1, 1, 1, 1, 3, 1, 1, 0, "random"),            # one point, one digit
     (0, 1, 700, 1, 1, 12, 12, 12, 0, "base2"),        # base 2 only
     (1, 1, 500, 3, 1, 40, 6, 8, 2**32 - 250, "random"),  # indices across 2^32 (64-bit digit and carry paths)
+    (1, 2, 1500, 4, 1, 8, 10, 30, 0, "random"),       # value table of 16 rows, tail rows 16..29 past it
+    (1, 2, 1200, 2, 1, 6, 6, 8, 0, "b593"),           # value table too large (b = 593): u32 rows in LDS
+    (2, 1, 900, 3, 1, 8, 12, 40, 0, "random"),        # shift table of 16 rows, raw shifts 16..39 past it
+    (2, 1, 800, 2, 1, 6, 60, 60, 0, "b131"),          # b = 131 above the 126 rows a 64-digit shift table holds
 ])
 def test_fused_gdn_equals_chain(mode, r, n, d, r_b, mmax, tmax, tnew, n_start, kind):
     """qmc_gdn_gen_scramble_float_dev against the oracle's three-call chain,
@@ -357,6 +361,8 @@ def test_fused_gdn_equals_chain(mode, r, n, d, r_b, mmax, tmax, tnew, n_start, k
         bases = np.full((r_b, d), 1999, dtype=np.uint64)
     elif kind == "base2":
         bases = np.full((r_b, d), 2, dtype=np.uint64)
+    elif kind in ("b593", "b131"):
+        bases = np.full((r_b, d), int(kind[1:]), dtype=np.uint64)
     else:
         bases = np.stack([primes[:d]] * r_b) if kind == "halton" else rng.choice(primes, size=(r_b, d))
         bases = bases.astype(np.uint64)
