@@ -655,8 +655,11 @@ __global__ __launch_bounds__(TAB ? kFuseTabBlock : kBlock) void gdn_fused_kernel
     if (TAB) {
         double *V = (double *)smem;
         const u32 vrows = (u32)(b < a.bmax ? b : a.bmax);  // digits z < b
+        const DivU32 vdiv = make_div(vrows > 0 ? vrows : 1u);
+        // z fastest: a permutation row's entries are contiguous in memory
+#pragma unroll 4
         for (u32 e = threadIdx.x; e < vrows * TP; e += blockDim.x) {
-            const u32 zz = e / TP, t = e - zz * TP;
+            const u32 t = div_q(e, vdiv), zz = e - t * vrows;
             double y = 0.0;
             if (t < tnew) {
                 if (MODE == 1) {
@@ -924,7 +927,9 @@ QMC_EXPORT int qmc_gdn_gen_scramble_float_dev(int mode, u64 r, u64 n, u64 d, u64
     const u64 kmax = n_start + (n - 1);
     const u64 qwords = r * d * mmax * tmax;
     const bool walk = mmax > 0 && kmax >= n_start && qwords * 4 <= (1ull << 30) && r * d < (1ull << 31);
-    const int TP = tmax <= 16 ? 16 : tmax <= 32 ? 32 : tmax <= 56 ? 56 : 64;
+    // 52 digits (the double-precision digit count of the bases up to 131
+    // with permutations) get rows of their own rather than 56 padded ones
+    const int TP = tmax <= 16 ? 16 : tmax <= 32 ? 32 : tmax <= 52 ? 52 : tmax <= 56 ? 56 : 64;
     const u64 rows = (u64)TP > tmax_new ? (u64)TP : tmax_new;
     const bool tabm = walk && ((mode == 1 && bmax * (TP + 1) * 8 <= kFuseTabBytes) || mode == 2);
     const u64 vrows = mode == 1 ? bmax : kFuseTabBytes / ((TP + 1) * 8);  // rows of the value table
@@ -972,6 +977,8 @@ QMC_EXPORT int qmc_gdn_gen_scramble_float_dev(int mode, u64 r, u64 n, u64 d, u64
                                               hipFuncAttributeMaxDynamicSharedMemorySize, most));
             QMC_HIP_CHECK(hipFuncSetAttribute((const void *)gdn_fused_kernel<32, 1, true>,
                                               hipFuncAttributeMaxDynamicSharedMemorySize, most));
+            QMC_HIP_CHECK(hipFuncSetAttribute((const void *)gdn_fused_kernel<52, 1, true>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, most));
             QMC_HIP_CHECK(hipFuncSetAttribute((const void *)gdn_fused_kernel<56, 1, true>,
                                               hipFuncAttributeMaxDynamicSharedMemorySize, most));
             QMC_HIP_CHECK(hipFuncSetAttribute((const void *)gdn_fused_kernel<64, 1, true>,
@@ -979,6 +986,8 @@ QMC_EXPORT int qmc_gdn_gen_scramble_float_dev(int mode, u64 r, u64 n, u64 d, u64
             QMC_HIP_CHECK(hipFuncSetAttribute((const void *)gdn_fused_kernel<16, 2, true>,
                                               hipFuncAttributeMaxDynamicSharedMemorySize, most));
             QMC_HIP_CHECK(hipFuncSetAttribute((const void *)gdn_fused_kernel<32, 2, true>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, most));
+            QMC_HIP_CHECK(hipFuncSetAttribute((const void *)gdn_fused_kernel<52, 2, true>,
                                               hipFuncAttributeMaxDynamicSharedMemorySize, most));
             QMC_HIP_CHECK(hipFuncSetAttribute((const void *)gdn_fused_kernel<56, 2, true>,
                                               hipFuncAttributeMaxDynamicSharedMemorySize, most));
@@ -1000,6 +1009,8 @@ QMC_EXPORT int qmc_gdn_gen_scramble_float_dev(int mode, u64 r, u64 n, u64 d, u64
         QMC_FUSE(16, md, tb);       \
     else if (TP == 32)              \
         QMC_FUSE(32, md, tb);       \
+    else if (TP == 52)              \
+        QMC_FUSE(52, md, tb);       \
     else if (TP == 56)              \
         QMC_FUSE(56, md, tb);       \
     else                            \

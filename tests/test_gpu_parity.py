@@ -340,7 +340,9 @@ def _oracle_gdn_chain(mode, r, n, d, r_b, mmax, tmax, tnew, n_start, bmax, bases
     (0, 1, 5000, 4, 1, 14, 33, 33, 0, "random"),      # no scramble, 64-register rows
     (1, 2, 700, 4, 1, 6, 5, 8, 0, "wide"),            # C >= 2^32 column, perms >= 2^32, bases past b^mmax
     (1, 1, 600, 3, 1, 3, 4, 6, 0, "bigb"),            # permutation rows too long for LDS (b = 1999)
-    (1, 2, 3000, 5, 1, 8, 52, 52, 0, "halton"),       # 56-register rows (the 52-digit layout)
+    (1, 2, 3000, 5, 1, 8, 52, 52, 0, "halton"),       # 52-register rows (the 52-digit layout)
+    (1, 1, 1000, 3, 1, 8, 55, 56, 0, "halton"),       # 56-register rows
+    (2, 1, 700, 2, 1, 10, 54, 54, 3, "random"),       # 56-register rows, shifts (2^10 > every index)
     (2, 2, 3000, 4, 2, 8, 60, 62, 5, "smallshift"),   # shifts < b (branch-free path), 64-register rows, tail
     (1, 1, 1, 1, 1, 3, 1, 1, 0, "random"),            # one point, one digit
     (0, 1, 700, 1, 1, 12, 12, 12, 0, "base2"),        # base 2 only
