@@ -106,6 +106,32 @@ def test_kernels_match_reference_digests():
             assert _clib.digest(got[o]) == dg, key + ":" + o
 
 
+# (tmax, tmax_new, wide): the LDS permutation kernel at even and odd digit
+# counts, with tail rows past tmax; wide tables (entries of 2^32 or more)
+# read the global table
+@pytest.mark.parametrize("tmax,tnew,wide", [(52, 52, False), (6, 8, False), (6, 8, True), (5, 8, False),
+                                            (6, 7, False)])
+def test_gdn_permutation_lds_forms(tmax, tnew, wide):
+    rng = np.random.default_rng(tmax * 100 + tnew)
+    r, n, d, r_x = 2, 3000, 5, 1
+    bases = np.array([[3, 5, 7, 11, 13]], dtype=np.uint64)
+    bmax = 13
+    perms = np.zeros((r, d, tnew, bmax), dtype=np.uint64)
+    for l in range(r):
+        for j in range(d):
+            for t in range(tnew):
+                perms[l, j, t] = rng.permutation(bmax)
+    if wide:
+        perms[:, 2, ::2, :] += np.uint64(2**40)
+    xdig = (rng.random((r_x, n, d, tmax)) * bases[0][None, None, :, None]).astype(np.uint64)
+    case = dict(r=r, n=n, d=d, r_x=r_x, r_b=1, tmax=tmax, tmax_new=tnew, bmax=bmax, perms=perms, xdig=xdig,
+                xdig_new=np.zeros((r, n, d, tnew), dtype=np.uint64))
+    want = dict(case, xdig_new=case["xdig_new"].copy())
+    _clib.call_oracle("gdn_digital_permutation", want)
+    got = run_product("gdn_digital_permutation", dict(case, xdig_new=case["xdig_new"].copy()), "device")
+    assert np.array_equal(got["xdig_new"], want["xdig_new"])
+
+
 # (n, d, r_g, r): the generators write replications in groups of 8 (grid y,
 # looped past 65535 groups); the cases cover one group, partial groups, r_g
 # not dividing the group start, shifts writing more than 8 replications from
