@@ -88,10 +88,14 @@ __device__ __forceinline__ Pair pair_at(const RowMap &m, u64 e2) {
     return P;
 }
 
+template <bool NT = false>
 __device__ __forceinline__ void store_pair(double *p, bool vec, bool ok0, bool ok1, double v0, double v1) {
     if (vec && ok0 && ok1) {
         double2_t v = {v0, v1};
-        *(double2_t *)p = v;
+        if (NT)
+            __builtin_nontemporal_store(v, (double2_t *)p);
+        else
+            *(double2_t *)p = v;
     } else {
         if (ok0) p[0] = v0;
         if (ok1) p[1] = v1;
@@ -112,7 +116,10 @@ __device__ __forceinline__ void store_pair(double *p, bool vec, bool ok0, bool o
 //   all 16 shifts loaded before the first store             565
 //   g and shifts staged in LDS per workgroup                655-713
 // against 868 for a plain 16-byte fill of the same 8 GiB.  Fewer windows
-// open at once and fewer stores in flight per wave both write faster.
+// open at once and fewer stores in flight per wave both write faster.  The
+// fused kernel's 16-byte stores are streaming ones (786 against 768-774
+// with cached stores in the same run; workgroups of 128 or 512 threads
+// change nothing).
 constexpr u32 kRepGroup = 8;
 
 struct RepGrid {
@@ -172,8 +179,8 @@ __global__ __launch_bounds__(kBlock) void lat_gen_shift_kernel(RowMap m, u64 r, 
             for (u64 l = l0; l < l1; l++) {
                 const double x0 = frac_mod1((double)g[lg * m.d + P.j0] * P.f0);
                 const double x1 = P.ok1 ? frac_mod1((double)g[lg * m.d + P.j1] * P.f1) : 0.0;
-                store_pair(xr + l * m.nd + e2, vec, P.ok0, P.ok1, frac_mod1(x0 + shifts[l * m.d + P.j0]),
-                           frac_mod1(x1 + shifts[l * m.d + P.j1]));
+                store_pair<true>(xr + l * m.nd + e2, vec, P.ok0, P.ok1, frac_mod1(x0 + shifts[l * m.d + P.j0]),
+                                 frac_mod1(x1 + shifts[l * m.d + P.j1]));
                 if (++lg == r_g) lg = 0;
             }
         }
