@@ -119,7 +119,8 @@ __device__ __forceinline__ void store_pair(double *p, bool vec, bool ok0, bool o
 // open at once and fewer stores in flight per wave both write faster.  The
 // fused kernel's 16-byte stores are streaming ones (786 against 768-774
 // with cached stores in the same run; workgroups of 128 or 512 threads
-// change nothing).
+// change nothing).  The generator alone keeps cached stores: streaming ones
+// made it slower (737 against 762 Gpts/s, r = 16, same run).
 constexpr u32 kRepGroup = 8;
 
 struct RepGrid {
