@@ -309,13 +309,16 @@ QMC_EXPORT int qmc_mfma_probe_dev(const float *a, const float *b, float *d, qmc_
  * No reference counterpart: the reference runs one call per kernel per
  * request (util.py:99-144).  A plan holds nreq requests (method 0 lattice +
  * shifts, 1 dnb2 LMS + digital shift, 2 dnb2 + hash-based nested scramble,
- * 3 Halton-type base-b LMS + digit permutation; r x 2^m x d float64 points,
- * natural order, counter-based random inputs from seeds[q]).  Outputs go to a
- * ring of ring_elems doubles, each replication slab contiguous at
- * qmc_batch_slab_offset(plan, q, l).  Calls on one plan must not overlap. */
+ * 3 Halton-type base-b LMS to 52 digits + uniform digit permutations; r x 2^m
+ * x d float64 points, natural order, counter-based random inputs from
+ * seeds[q]).  Outputs go to a ring of ring_elems doubles, each replication
+ * slab contiguous at qmc_batch_slab_offset(plan, q, l).  gdn_level_cap > 0
+ * limits the permutation levels gdn tiles read from their tables (the output
+ * does not change; a test hook for the other path).  Calls on one plan must
+ * not overlap. */
 QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t *m, const int32_t *d,
                                 const int32_t *r, const qmc_u64 *seeds, qmc_u64 points_per_tile, int grid,
-                                qmc_u64 ring_elems);
+                                qmc_u64 ring_elems, int gdn_level_cap);
 QMC_EXPORT void qmc_batch_free(void *plan);
 QMC_EXPORT qmc_u64 qmc_batch_total_tiles(void *plan);
 QMC_EXPORT qmc_u64 qmc_batch_points(void *plan);
@@ -333,8 +336,14 @@ QMC_EXPORT int qmc_batch_set_digest(void *plan, unsigned long long *digest);
 QMC_EXPORT void qmc_batch_lattice_inputs(qmc_u64 seed, int m, int d, int r, qmc_u64 *g, double *shifts);
 QMC_EXPORT void qmc_batch_dnb2_inputs(qmc_u64 seed, int method, int d, int r, qmc_u64 *C, qmc_u64 *S,
                                       qmc_u64 *shifts);
-QMC_EXPORT int qmc_batch_gdn_inputs(qmc_u64 seed, int m, int l, int j, uint32_t *b, int *tn, int *mm, uint32_t *S,
-                                    uint32_t *perm_a, uint32_t *perm_c);
+/* gdn request inputs for columns [j0, j0 + dj) in the reference's layouts:
+ * bases[dj], S[r][dj][52][m] (gdn_get_linear_scramble_matrix, tmax = m),
+ * perms[r][dj][52][bmax] (gdn_get_digital_permutations); identity C. */
+QMC_EXPORT int qmc_batch_gdn_inputs(qmc_u64 seed, int m, int j0, int dj, int r, qmc_u64 *bases, qmc_u64 *S,
+                                    qmc_u64 *perms);
+/* per-base constants of the gdn sums: rec[52], early-exit thresholds thr[52],
+ * the levels kept in LDS and the first level that can end a sum */
+QMC_EXPORT int qmc_batch_gdn_base(int j, double *rec, double *thr, int *levels, int *tmin);
 QMC_EXPORT qmc_u64 qmc_owen_hash64(qmc_u64 x, qmc_u64 seed);
 
 #ifdef __cplusplus

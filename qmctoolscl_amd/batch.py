@@ -8,7 +8,12 @@ order, methods
                                          dnb2_gen_natural -> dnb2_digital_shift
                                          -> dnb2_integer_to_float)
     2 dnb2 + hash-based nested uniform scramble
-    3 Halton-type base-b net + LMS + per-level digit permutation
+    3 Halton-type base-b net (identity generating matrices), LMS to 52 digits,
+      a uniform random permutation of the digit values per level, float
+                                        (gdn_linear_matrix_scramble ->
+                                         gdn_gen_natural ->
+                                         gdn_digital_permutation ->
+                                         gdn_integer_to_float)
 with all random inputs drawn counter-based from the seed on the device (see
 csrc/batch.hip).  Output goes to a caller-provided float64 ring buffer; every
 replication slab lands contiguously at offset `slab_offset(q, l)`.
@@ -33,7 +38,8 @@ def _declare(L):
     if getattr(L, "_batch_declared", False):
         return
     L.qmc_batch_plan.restype = ctypes.c_void_p
-    L.qmc_batch_plan.argtypes = [ctypes.c_int] + [_vp] * 5 + [ctypes.c_uint64, ctypes.c_int, ctypes.c_uint64]
+    L.qmc_batch_plan.argtypes = [ctypes.c_int] + [_vp] * 5 + [ctypes.c_uint64, ctypes.c_int, ctypes.c_uint64,
+                                                               ctypes.c_int]
     L.qmc_batch_free.argtypes = [_vp]
     L.qmc_batch_total_tiles.restype = ctypes.c_uint64
     L.qmc_batch_total_tiles.argtypes = [_vp]
@@ -50,7 +56,9 @@ def _declare(L):
     L.qmc_batch_lattice_inputs.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp]
     L.qmc_batch_dnb2_inputs.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]
     L.qmc_batch_gdn_inputs.restype = ctypes.c_int
-    L.qmc_batch_gdn_inputs.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [_vp] * 6
+    L.qmc_batch_gdn_inputs.argtypes = [ctypes.c_uint64] + [ctypes.c_int] * 4 + [_vp] * 3
+    L.qmc_batch_gdn_base.restype = ctypes.c_int
+    L.qmc_batch_gdn_base.argtypes = [ctypes.c_int] + [_vp] * 4
     L.qmc_owen_hash64.restype = ctypes.c_uint64
     L.qmc_owen_hash64.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
     L._batch_declared = True
@@ -76,7 +84,8 @@ def sample_requests(count, seed, m_range=(8, 22), d_range=(1, 256), r_range=(1, 
 
 
 class Batch:
-    def __init__(self, methods, m, d, r, seeds, points_per_tile=1 << 16, grid=0, ring_elems=None):
+    def __init__(self, methods, m, d, r, seeds, points_per_tile=1 << 16, grid=0, ring_elems=None,
+                 gdn_level_cap=0):
         self.methods = np.ascontiguousarray(methods, dtype=np.int32)
         self.m = np.ascontiguousarray(m, dtype=np.int32)
         self.d = np.ascontiguousarray(d, dtype=np.int32)
@@ -90,7 +99,7 @@ class Batch:
         L = _L()
         h = L.qmc_batch_plan(len(self.m), self.methods.ctypes.data, self.m.ctypes.data, self.d.ctypes.data,
                              self.r.ctypes.data, self.seeds.ctypes.data, self.points_per_tile, self.grid,
-                             self.ring_elems)
+                             self.ring_elems, int(gdn_level_cap))
         if not h:
             _lib.check()
         self._h = h
@@ -170,16 +179,39 @@ def dnb2_inputs(seed, method, d, r):
     return C, S, sh
 
 
-def gdn_inputs(seed, m, l, j):
-    b = ctypes.c_uint32()
-    tn = ctypes.c_int()
-    mm = ctypes.c_int()
-    S = np.zeros(52 * 32, dtype=np.uint32)
-    pa = np.zeros(52, dtype=np.uint32)
-    pc = np.zeros(52, dtype=np.uint32)
-    _L().qmc_batch_gdn_inputs(int(seed), int(m), int(l), int(j), ctypes.byref(b), ctypes.byref(tn), ctypes.byref(mm),
-                              S.ctypes.data, pa.ctypes.data, pc.ctypes.data)
-    return b.value, tn.value, mm.value, S[:tn.value * mm.value].reshape(tn.value, mm.value), pa[:tn.value], pc[:tn.value]
+def first_primes(k):
+    """The first k primes (the bases of a gdn request's columns)."""
+    out, c = [], 2
+    while len(out) < k:
+        if all(c % p for p in out if p * p <= c):
+            out.append(c)
+        c += 1
+    return out
+
+
+def gdn_inputs(seed, m, r, j0=0, dj=1):
+    """Inputs of a gdn request for columns [j0, j0 + dj) in the reference's
+    layouts: bases (1, dj), S (r, dj, 52, m) -- the LMS matrices, tmax = m --
+    and perms (r, dj, 52, bmax); the generating matrices are m x m identities."""
+    bmax = first_primes(j0 + dj)[-1]
+    bases = np.empty((1, dj), dtype=np.uint64)
+    S = np.empty((r, dj, 52, m), dtype=np.uint64)
+    perms = np.empty((r, dj, 52, bmax), dtype=np.uint64)
+    _lib.check(_L().qmc_batch_gdn_inputs(int(seed), int(m), int(j0), int(dj), int(r), bases.ctypes.data,
+                                          S.ctypes.data, perms.ctypes.data))
+    return bases, S, perms
+
+
+def gdn_base(j):
+    """(rec, thr, levels, tmin) of column j's base: the reciprocals of the
+    reference's sum, the early-exit thresholds, the levels kept in LDS and the
+    first level that can end a sum."""
+    rec = np.empty(52)
+    thr = np.empty(52)
+    lv = ctypes.c_int()
+    tm = ctypes.c_int()
+    _lib.check(_L().qmc_batch_gdn_base(int(j), rec.ctypes.data, thr.ctypes.data, ctypes.byref(lv), ctypes.byref(tm)))
+    return rec, thr, lv.value, tm.value
 
 
 def owen_hash64(x, seed):

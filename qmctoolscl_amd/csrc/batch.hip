@@ -9,26 +9,28 @@
 //                 shift, float (dnb2_linear_matrix_scramble -> dnb2_gen_natural
 //                 -> dnb2_digital_shift -> dnb2_integer_to_float)
 //   2 DNB2_NUS   base-2 net with a hash-based nested uniform scramble
-//   3 GDN_LMS    Halton-type base-b net (base = j-th prime), random
-//                 lower-triangular LMS, random digit permutation per level,
-//                 float
+//   3 GDN_LMS    Halton-type base-b net (base = j-th prime, identity
+//                 generating matrices), random lower-triangular LMS to 52
+//                 digits, a uniform random permutation of the digit values per
+//                 level, float (gdn_linear_matrix_scramble -> gdn_gen_natural
+//                 -> gdn_digital_permutation -> gdn_integer_to_float)
 //
 // Random inputs are counter-based: every generating vector, matrix, shift and
 // permutation is a hash of (request seed, role, indices), so a batch needs no
-// host-drawn arrays and any tile can be produced independently.  This is a
-// stream of its own -- the points equal this library's one-launch-per-request
-// path bit for bit, and for methods 0 and 1 they equal the reference kernels
-// applied to the same inputs (qmc_batch_*_inputs export them), but they are not
-// numpy's draws.
+// host-drawn arrays.  This is a stream of its own -- the points equal this
+// library's one-launch-per-request path bit for bit, and for methods 0, 1 and
+// 3 they equal the reference kernels applied to the same inputs
+// (qmc_batch_*_inputs export them), but they are not numpy's draws.
 //
-// Scheduling: each request is cut into tiles of whole rows (about P points
-// each); a setup kernel builds every request's small tables (prefix-XOR
-// columns, shifts, LMS rows, permutation parameters) in a workspace, then
-// persistent grids drain tile counters: one kernel for the lattice and dnb2
-// tiles (store-bound, few registers) and one for the gdn tiles (digit walks,
-// ~110 registers, ALU-bound), each compiled without the other's code; when a
-// batch has both, the gdn kernel runs on a second stream beside the other
-// (one graph either way).  Outputs go to a ring buffer:
+// Scheduling: each request is cut into tiles (rows of all columns for the
+// lattice and dnb2 methods, rows of one column for gdn); setup kernels build
+// every request's small tables (prefix-XOR columns, shifts, LMS rows; for gdn
+// the permutations of the levels a point usually needs and the walk tables) in
+// a workspace, then persistent grids drain tile counters: one kernel for the
+// lattice and dnb2 tiles (store-bound, few registers) and one for the gdn
+// tiles (digit walks and sequential sums, ALU-bound), each compiled without
+// the other's code; when a batch has both, the gdn kernel runs on a second
+// stream beside the other (one graph either way).  Outputs go to a ring buffer:
 // every replication slab (n*d doubles) of every request is placed contiguously
 // at an offset chosen on the host, wrapping to the start when the ring is full,
 // so batches far larger than HBM can stream through a bounded buffer.
@@ -53,15 +55,12 @@ struct Req {
     u64 seed;
     u64 tile0;           // first global tile
     u32 rows_per_tile;   // power of two, <= n
-    u32 tiles_per_l;     // n / rows_per_tile
+    u32 tiles_per_l;     // tiles per replication: n / rows_per_tile (times d for gdn)
     u64 tab;             // word offset of the request's tables in the workspace
     u64 slabs;           // word offset of r slab bases (output element offsets)
     u32 clog;            // dnb2: log2 of the row stride between a chain's points
-    u32 gng;             // gdn: lane groups (ranges of consecutive columns)
-    u32 gunits;          // gdn: units of work per tile (sum over groups)
-    u64 ggrp;            // gdn: word offset (in the request's tables) of the group table
-    u32 gtw;             // gdn: u32 words of one replication's digit tables
-    u64 gtab;            // gdn: word offset (in the request's tables) of the replication tables
+    u32 gcap;            // gdn: most permutation levels a tile reads from its tables
+    u64 lw;              // gdn: words of one replication's column tables
     u32 slab0;           // index of the request's first replication among all slabs (digests)
     DivU32 ddiv;         // division by d
 };
@@ -146,139 +145,151 @@ void fill_primes() {
     }
 }
 
-// digits of base b for 2^-52 resolution (b^tn >= 2^52), and for indices < 2^m
-__host__ __device__ __forceinline__ int gdn_tn(u32 b) {
-    int t = 0;
-    double v = 1.0;
-    while (v < 4503599627370496.0) { v *= b; t++; }
-    return t;
-}
+// ---------------------------------------------------------------- gdn inputs
+//
+// A gdn request is the reference's Halton-type chain with one tmax_new for all
+// columns: bases b_j = the j-th prime; generating matrices the m x m identity
+// (gdn_get_halton_generating_matrix, mmax = tmax = m); per (l, j) an LMS matrix
+// S of kTNmax x m entries (gdn_get_linear_scramble_matrix: below the diagonal
+// uniform in [0, b), the diagonal uniform in [1, b), zero above) and one
+// uniform random permutation of the b digit values per level t < kTNmax
+// (gdn_get_digital_permutations); then gdn_linear_matrix_scramble ->
+// gdn_gen_natural -> gdn_digital_permutation -> gdn_integer_to_float.  With the
+// identity C, C_lms[k][t] = S[t][k] (entries are below b, so the reference's
+// sum without a final mod is exact), and digit t of point k is
+//     z_t(k) = sum_i dig_i(k) S[t][i] mod b,   y_t = pi_t(z_t),
+//     x = ((p_0 + p_1) + ...) + p_51,   p_t = RN(rec_t * y_t),
+// rec_0 = RN(1/b), rec_{t+1} = RN(rec_t / b)  (gdn_integer_to_float.c:34-38).
+
+// base-b digits of indices below 2^m (at least 1)
 __host__ __device__ __forceinline__ int gdn_mm(u32 b, int m) {
     int t = 0;
     u64 v = 1;
     while (v < (1ull << m)) { v *= b; t++; }
     return t < 1 ? 1 : t;
 }
-// LMS entry S[t][k] (k <= t): diagonal in [1, b), below in [0, b)
-__host__ __device__ __forceinline__ u32 gdn_s(u64 seed, u64 l, u64 j, int t, int k, u32 b) {
-    const u64 h = key(seed, 7, (l << 32) | j, ((u64)t << 8) | k);
-    return t == k ? (u32)(1 + h % (b - 1)) : (u32)(h % b);
+
+// LMS entry S[t][i] of replication l, column j
+__host__ __device__ __forceinline__ u32 gdn_s(u64 seed, u64 l, u64 j, u32 t, u32 i, u32 b) {
+    if (i > t) return 0;
+    const u64 h = key(seed, 7, (l << 32) | j, ((u64)t << 8) | i);
+    return t == i ? (u32)(1 + h % (b - 1)) : (u32)(h % b);
 }
-// digit permutation at level t: y = (a*x + c) mod b, a in [1, b)
-__host__ __device__ __forceinline__ void gdn_perm(u64 seed, u64 l, u64 j, int t, u32 b, u32 &a, u32 &c) {
-    const u64 h = key(seed, 8, (l << 32) | j, t);
-    a = (u32)(1 + (h >> 32) % (b - 1));
-    c = (u32)((h & 0xffffffffull) % b);
+
+// Permutation pi_t of replication l, column j at level t: pi_t(z) = the rank of
+// key(z) among key(0), ..., key(b-1), key(z) = 53 hash bits above the 11 bits
+// of z.  Ranking i.i.d. uniform keys gives a uniform random permutation; the
+// index bits only break ties, and two keys share their 53 bits with
+// probability below b^2 2^-54 < 1.5e-10 per level (b <= 1619).
+__host__ __device__ __forceinline__ u64 gdn_pkey(u64 seed, u64 l, u64 j, u32 t, u32 z) {
+    return (key(seed, 8, (l << 32) | j, ((u64)t << 16) | z) & ~0x7ffull) | z;
 }
+
+// pi_t(z) by counting: b keys (the tables hold the same ranks, by sorting)
+__host__ __device__ inline u32 gdn_perm_at(u64 seed, u64 l, u64 j, u32 t, u32 b, u32 z) {
+    const u64 kz = gdn_pkey(seed, l, j, t, z);
+    u32 c = 0;
+    for (u32 w = 0; w < b; w++) c += gdn_pkey(seed, l, j, t, w) < kz ? 1u : 0u;
+    return c;
+}
+
+// Exact early exit of the sum.  B_t = RN(rec_t (b - 1)) bounds every later
+// term: p_s = RN(rec_s y_s) <= RN(rec_t (b - 1)) for s >= t, since y_s <= b - 1,
+// rec_s <= rec_t and rounding is monotone.  With thr_t = 2^(ilogb(B_t) + 54),
+// a partial sum v >= thr_t has ulp(v) / 2 >= 2^(ilogb(B_t) + 1) > B_t, so
+// v + p_s rounds back to v for every later term and v is final.  A wave stops
+// adding once all its lanes' sums pass thr_t; lanes already past it would only
+// have added terms that leave them unchanged.
+//
+// Levels kept in LDS per column: TB, the smallest of 8, 12, 16, 24, 32, 48 at
+// or above the first t with b^t >= 2^74.  A point needs a level t >= TB only
+// if its sum stays below thr_TB <= 2^55 b^-TB <= 2^-19 (a chance of about
+// 2^-19 for these uniformly spread points); such levels are computed from the
+// point's index and the permutation's keys (gdn_tail).  Base 2 keeps all 52
+// levels as bits of one word.
+__host__ __device__ inline int gdn_levels(u32 b) {
+    if (b == 2) return kTNmax;
+    int t = 0;
+    double v = 1.0;
+    while (v < 0x1p74) { v *= (double)b; t++; }
+    return t <= 8 ? 8 : t <= 12 ? 12 : t <= 16 ? 16 : t <= 24 ? 24 : t <= 32 ? 32 : 48;
+}
+
+// u16 entries per LDS row of a permutation table: TB rounded up so that a row
+// spans an odd number of 4-byte banks (random digits of a wave's lanes spread
+// over the banks)
+__host__ __device__ inline u32 gdn_row_u16(u32 TB) {
+    u32 r = (TB + 1) & ~1u;
+    if (((r / 2) & 1) == 0) r += 2;
+    return r;
+}
+
+// Per-base constants at the start of the workspace: rec[256][kTNmax], then
+// thr[256][kTNmax] (as doubles).
+constexpr u64 kBaseWords = 2ull * 256 * kTNmax;
+
+void gdn_base_tables(u32 b, double *rec, double *thr) {
+    double rt = 1.0 / (double)b;
+    for (int t = 0; t < kTNmax; t++) {
+        rec[t] = rt;
+        const double B = rt * (double)(b - 1);
+        thr[t] = std::ldexp(1.0, std::ilogb(B) + 54);
+        rt /= (double)b;
+    }
+}
+
+// first level whose threshold is below 1 (sums stay below 1, so no earlier
+// level can end one)
+inline u32 gdn_tmin(const double *thr) {
+    u32 t = 0;
+    while (t < (u32)kTNmax && !(thr[t] < 1.0)) t++;
+    return t;
+}
+
+// x mod b and x / b for x < 2^32 with M = ceil(2^64 / b) (exact, Lemire et al. 2019)
+__host__ __device__ __forceinline__ u64 recip_magic(u32 b) { return b == 1 ? 0 : (~0ull) / b + 1; }
+__device__ __forceinline__ u32 fast_div(u32 x, u64 M) { return (u32)__umul64hi(M, (u64)x); }
 
 // ----------------------------------------------------------- table layouts
 // lattice : g[d] | shift[r*d] (double bits)
 // dnb2    : per column set (l, j) -- r*d sets for LMS, d for NUS -- 64 u32:
 //           pre[32] (prefix XOR of the columns) then cols[32] | shift or NUS seed[r*d]
-// gdn     : per j kGJ words of dimension info | per replication l, gtw u32 of
-//           digit tables (layout below), one block per j at u32 offset qoff_j
-//
-// gdn digit tables.  Point k of dimension j has base-b digits dig_i(k); the
-// scrambled digit t is z_t(k) = (a_t * sum_i S[t][i] dig_i(k) + c_t) mod b.
-// Going from k to k+1 turns the c trailing digits equal to b-1 into 0 and adds
-// 1 to digit c, so every z_t moves by Q_c[t] = a_t * sum_{i<=c} S[t][i] mod b
-// (-(b-1) = 1 mod b).  A chain of consecutive points is then one table row
-// added per point, digitwise mod b.
-//   b >= 3: rows 0..mm of W u32 each; row 0 holds c_t, row 1+c holds Q_c.
-//           Digits are u16 halves, two per u32: padded digit u = t + (tn & 1)
-//           (odd tn gets a leading zero digit) sits in pair u/2, the more
-//           significant digit (even u) in the low half.  Rows are right-aligned
-//           to the width W of the widest expansion among the dimension's lane
-//           group (np = ceil(tn/2) <= W pairs, zero pairs in front), so every
-//           lane of a wave walks the same register pairs.
-//   b == 2: u64 words (a == 1): cmask, then Q_0..Q_{mm-1}, digit t at bit 51-t.
+// gdn     : info[d] | r replication blocks of lw words, column j's block at
+//           word colo_j of each:
+//             odd b: u16 perm[b][TB] (entry z*TB + t = pi_t(z)), then
+//                    u16 Q[mm][TB] with Q_c[t] = sum_{i <= c} S[t][i] mod b
+//                    (going from point k to k + 1 turns the c trailing digits
+//                    equal to b - 1 into 0 and adds 1 to digit c, so every
+//                    z_t moves by Q_c[t], as -(b - 1) = 1 mod b)
+//             b = 2: u64 cmask (bit 51 - t = pi_t(0)), Q_0 .. Q_{m-1} (bits
+//                    51 - t, mod 2)
+//           info_j = colo_j | TB << 32 | mm << 40 | tmin << 48
 
-// words of dimension info: b | L<<16 | qoff<<32,  tn | mm<<8 | np<<16 | W<<24 | b^2<<32,  ceil(2^64/b),  b^tn
-constexpr int kGJ = 4;
-constexpr int kNPmax = 17;   // u32 digit pairs of the longest b >= 3 expansion (b = 3: 33 digits + 1 pad)
-
-__host__ __device__ __forceinline__ u64 recip_magic(u32 b) { return b == 1 ? 0 : (~0ull) / b + 1; }
-
-// x mod b and x / b for x < 2^32 with M = ceil(2^64 / b) (exact, Lemire et al. 2019)
-__device__ __forceinline__ u32 fast_div(u32 x, u64 M) { return (u32)__umul64hi(M, (u64)x); }
-
-inline u32 gdn_np(u32 b) { return b == 2 ? 0 : ((u32)gdn_tn(b) + 1) / 2; }
-
-// Lane groups of a gdn tile: ranges [j0, j0 + gc) of consecutive columns,
-// [0, 8) (the small bases, up to 17 digit pairs for base 3) apart from the
-// rest, which go in groups of 64 from column 8; a wave works on ch = 64/gc
-// chains of one group, every lane walking w digit pairs (the group's widest
-// expansion: its smallest odd base).  Chains are k rows long, k a power of two
-// giving about 8 units of work per group.  (With one group for the first 64
-// columns every one of them walked base 3's 17 pairs; splitting off [0, 8)
-// made the gdn class 10% faster and a small ragged batch 29%.  Row segments of
-// 8 columns end mid-line, which the class's cached stores complete in L2.)
-struct GdnGroup {
-    u32 j0, gc, ch, w, l, k, cb0, nu;
-};
-
-std::vector<GdnGroup> gdn_groups(int d, u32 RT) {
-    std::vector<std::pair<u32, u32>> ranges;  // (j0, gc)
-    if (d <= 8) {
-        ranges.push_back({0u, (u32)d});
-    } else {
-        ranges.push_back({0u, 8u});
-        for (u32 j = 8; j < (u32)d; j += 64) ranges.push_back({j, std::min<u32>(64, (u32)d - j)});
-    }
-    const u32 ng = (u32)ranges.size();
-    double inv = 0;  // sum over groups of 1/ch: a tile has RT * inv / K units of K rows
-    for (auto &rg : ranges) inv += 1.0 / (double)(64 / rg.second);
-    const u64 want = (u64)((double)RT * inv / (8.0 * ng));
-    u32 K = 1;
-    while (K * 2 <= want && K < 256) K *= 2;
-    if (K > RT) K = RT;
-    std::vector<GdnGroup> g(ng);
-    u32 cb0 = 0;
-    for (u32 i = 0; i < ng; i++) {
-        GdnGroup &x = g[i];
-        x.j0 = ranges[i].first;
-        x.gc = ranges[i].second;
-        const u32 CH = 64 / x.gc;
-        x.ch = CH;
-        u32 bmax = 0;
-        for (u32 j = x.j0; j < x.j0 + x.gc; j++) {
-            x.w = std::max(x.w, gdn_np(h_primes[j]));
-            bmax = std::max(bmax, h_primes[j]);
-        }
-        const double bm = (double)bmax;
-        x.l = bm * bm * bm * bm * bm * bm < 0x1p53 ? 3 : 2;
-        x.k = K;
-        x.nu = (RT + CH * K - 1) / (CH * K);
-        x.cb0 = cb0;
-        cb0 += x.nu;
-    }
-    return g;
+inline u64 gdn_col_words(u32 b, int m) {
+    if (b == 2) return 1 + (u64)m;
+    const u64 TB = (u64)gdn_levels(b), mm = (u64)gdn_mm(b, m);
+    return ((b + mm) * TB * 2 + 7) / 8;
 }
 
-// Words of column j's block.  Base 2 (j = 0, read as u64) comes first and is
-// even-sized; the u32 blocks of odd bases are odd-sized, so the lanes of a wave,
-// each reading its own column's row at the same offset, fall on distinct LDS
-// banks.
-inline u32 gdn_words(const Req &q, int j, u32 W) {
-    const u32 b = h_primes[j];
-    const u32 mm = (u32)gdn_mm(b, q.m);
-    return b == 2 ? 2 + 2 * mm : ((mm + 1) * W) | 1u;
+// LDS bytes of a gdn tile of column j: the permutation rows, then the walk
+// table twice (scaled to row offsets, and plain)
+inline u32 gdn_col_lds(u32 b, int m) {
+    if (b == 2) return 8 * (1 + (u32)m);
+    const u32 TB = (u32)gdn_levels(b), mm = (u32)gdn_mm(b, m);
+    return ((b * gdn_row_u16(TB) * 2 + 15) & ~15u) + 8 * mm * TB;
 }
 
-u64 table_words(Req &q, const std::vector<GdnGroup> &grp) {
+u64 table_words(Req &q) {
     const u64 d = q.d, r = q.r;
     switch (q.method) {
         case M_LATTICE: return d + r * d;
         case M_DNB2_LMS: return 32 * r * d + r * d;
         case M_DNB2_NUS: return 32 * d + r * d;
         default: {
-            u32 w = 0;
-            for (const auto &x : grp)
-                for (u32 j = x.j0; j < x.j0 + x.gc; j++) w += gdn_words(q, (int)j, x.w);
-            q.gtw = (w + 1) & ~1u;  // even: every replication's block starts 8-byte aligned
-            q.ggrp = (u64)kGJ * d;
-            q.gtab = q.ggrp + 2 * grp.size();
-            return q.gtab + r * (u64)(q.gtw / 2);
+            u64 w = 0;
+            for (u64 j = 0; j < d; j++) w += gdn_col_words(h_primes[j], q.m);
+            q.lw = w;
+            return d + r * w;
         }
     }
 }
@@ -327,71 +338,83 @@ __global__ __launch_bounds__(kBlock) void batch_setup_kernel(const Req *__restri
                                                            : nus_seed(q.seed, f / d, f % d);
             }
         }
-    } else {
-        // gdn digit tables, one thread per (l, j, t); the dimension info was
-        // written by the host
-        u32 *T = (u32 *)(tab + q.gtab);
-        for (u64 e = tid; e < r * d * kTNmax; e += step) {
-            const int t = (int)(e % kTNmax);
-            const u64 lj = e / kTNmax, l = lj / d, j = lj % d;
-            const u64 w0 = tab[kGJ * j], w1 = tab[kGJ * j + 1];
-            const u32 b = (u32)w0 & 0xffffu;
-            const int tn = (int)(w1 & 0xff), mm = (int)((w1 >> 8) & 0xff), np = (int)((w1 >> 16) & 0xff);
-            const int W = (int)((w1 >> 24) & 0xff);
-            const u32 qoff = (u32)(w0 >> 32);
-            const u32 qend = j + 1 < d ? (u32)(tab[kGJ * (j + 1)] >> 32) : q.gtw;
-            const u32 need = b == 2 ? 2 * (u32)(mm + 1) : (u32)((mm + 1) * W);
-            if (qoff + need > qend) continue;  // never: the host sizes every block (keeps writes in it)
-            u32 *Tj = T + l * q.gtw + qoff;
-            if (b == 2) {
-                if (t) continue;
-                u64 *Q2 = (u64 *)Tj;
-                for (int i = 0; i <= mm; i++) Q2[i] = 0;
-                u64 cm = 0;
-                for (int tt = 0; tt < tn; tt++) {
-                    u32 a, cc;
-                    gdn_perm(q.seed, l, j, tt, 2, a, cc);
-                    cm |= (u64)cc << (51 - tt);
-                    u32 run = 0;
-                    for (int i = 0; i < mm; i++) {
-                        if (i <= tt) run ^= gdn_s(q.seed, l, j, tt, i, 2) & 1;
-                        Q2[1 + i] |= (u64)run << (51 - tt);
+    }
+    // gdn: batch_gdn_setup_kernel
+}
+
+// gdn tables, one block per column (request, l, j) of every gdn request: the
+// permutations of levels t < TB by sorting the b keys in LDS (bitonic, padded
+// to a power of two with keys above every real one), and the walk table.
+__global__ __launch_bounds__(kBlock) void batch_gdn_setup_kernel(const Req *__restrict__ reqs,
+                                                                 const u32 *__restrict__ greq,
+                                                                 const u64 *__restrict__ gcol0, int ng,
+                                                                 u64 *__restrict__ ws) {
+    __shared__ u64 keys[2048];
+    const u64 col = blockIdx.x;
+    int lo = 0, hi = ng - 1;  // last gdn request with gcol0 <= col
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (gcol0[mid] <= col) lo = mid;
+        else hi = mid - 1;
+    }
+    const Req q = reqs[greq[lo]];
+    const u64 c = col - gcol0[lo];
+    const u64 d = q.d, l = c / d, j = c - (c / d) * d;
+    u64 *tab = ws + q.tab;
+    const u64 info = tab[j];
+    const u32 TB = (u32)((info >> 32) & 0xff), mm = (u32)((info >> 40) & 0xff);
+    u64 *blk = tab + d + l * q.lw + (u32)info;
+    const u32 b = c_primes[j];
+    if (b == 2) {
+        if (threadIdx.x == 0) {
+            u64 cm = 0;  // pi_t(0) = 1 iff key(1) < key(0)
+            for (u32 t = 0; t < (u32)kTNmax; t++)
+                cm |= (u64)(gdn_pkey(q.seed, l, j, t, 1) < gdn_pkey(q.seed, l, j, t, 0)) << (kTNmax - 1 - t);
+            blk[0] = cm;
+        }
+        for (u32 cc = threadIdx.x; cc < (u32)q.m; cc += blockDim.x) {
+            u64 w = 0;
+            for (u32 t = 0; t < (u32)kTNmax; t++) {
+                u32 run = 0;
+                for (u32 i = 0; i <= cc && i <= t; i++) run ^= gdn_s(q.seed, l, j, t, i, 2) & 1u;
+                w |= (u64)run << (kTNmax - 1 - t);
+            }
+            blk[1 + cc] = w;
+        }
+        return;
+    }
+    unsigned short *P = (unsigned short *)blk;
+    unsigned short *Qt = P + (u64)b * TB;
+    u32 N = 1;
+    while (N < b) N *= 2;
+    for (u32 t = 0; t < TB; t++) {
+        for (u32 z = threadIdx.x; z < N; z += blockDim.x) keys[z] = z < b ? gdn_pkey(q.seed, l, j, t, z) : ~0ull;
+        __syncthreads();
+        for (u32 size = 2; size <= N; size <<= 1)
+            for (u32 stride = size >> 1; stride > 0; stride >>= 1) {
+                for (u32 i = threadIdx.x; i < N / 2; i += blockDim.x) {
+                    const u32 a = 2 * i - (i & (stride - 1)), e = a + stride;
+                    const u64 ka = keys[a], ke = keys[e];
+                    if ((ka > ke) == ((a & size) == 0)) {
+                        keys[a] = ke;
+                        keys[e] = ka;
                     }
                 }
-                Q2[0] = cm;
-            } else {
-                if (t >= tn) continue;
-                // digit u of row w at H[w * 2W + 2(W - np) + u]; zero pairs in front
-                unsigned short *H = (unsigned short *)Tj + 2 * (W - np);
-                const int u = t + (tn & 1);
-                if (t == 0)
-                    for (int w = 0; w <= mm; w++)
-                        for (int h = -2 * (W - np); h < (tn & 1); h++) H[w * 2 * W + h] = 0;
-                u32 a, cc;
-                gdn_perm(q.seed, l, j, t, b, a, cc);
-                H[u] = (unsigned short)cc;
-                u32 run = 0;
-                for (int i = 0; i < mm; i++) {
-                    if (i <= t) run = (run + a * gdn_s(q.seed, l, j, t, i, b)) % b;  // a, S < b <= 2^11
-                    H[(1 + i) * 2 * W + u] = (unsigned short)run;
-                }
+                __syncthreads();
             }
-        }
+        for (u32 pos = threadIdx.x; pos < b; pos += blockDim.x)
+            P[(u64)(keys[pos] & 0x7ff) * TB + t] = (unsigned short)pos;
+        __syncthreads();
+    }
+    for (u32 e = threadIdx.x; e < mm * TB; e += blockDim.x) {
+        const u32 cc = e / TB, t = e - cc * TB;
+        u32 run = 0;
+        for (u32 i = 0; i <= cc && i <= t; i++) run = (run + gdn_s(q.seed, l, j, t, i, b)) % b;
+        Qt[e] = (unsigned short)run;
     }
 }
 
 // ------------------------------------------------------------- one tile
-
-__device__ __forceinline__ u32 xor_cols(const u32 *__restrict__ cols, u64 q) {
-    u32 v = 0;
-    while (q) {
-        v ^= cols[__ffsll((long long)q) - 1];
-        q &= q - 1;
-    }
-    return v;
-}
-
-typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
 // Output of a tile, and with DIG a running wrap-around sum of the stored bit
 // patterns (an order-independent per-slab digest).  STREAM: non-temporal
@@ -419,8 +442,6 @@ struct Sink {
     }
 };
 
-__device__ __forceinline__ u16x2 as_pair(u32 w) { return __builtin_bit_cast(u16x2, w); }
-__device__ __forceinline__ u32 as_word(u16x2 p) { return __builtin_bit_cast(u32, p); }
 
 // dnb2 output of column value v (the top 32 of 64 bits): LMS adds the 64-bit
 // digital shift xs, NUS hashes; then x = y * 2^-64 (dnb2_integer_to_float.c:29).
@@ -525,214 +546,295 @@ __device__ __forceinline__ void dnb2_tile(const Req &q, u64 l, u64 row0, const u
     }
 }
 
-// z + q digitwise mod b for two u16 digits per word (both < b <= 2^11):
-// min(z + q, z + q - b) with 16-bit wrap-around (v_pk_add/sub/min_u16)
-__device__ __forceinline__ u32 pk_add_mod(u32 z, u32 q, u16x2 bb) {
-    const u16x2 sum = as_pair(z) + as_pair(q);
-    return as_word(__builtin_elementwise_min(sum, sum - bb));
-}
-
-// One gdn chain of a base-2 column: len consecutive points from k, row stride d.
+// One gdn column of base 2 (j = 0): H consecutive points from k, row stride d.
+// Q = cmask, Q_0 .. Q_{m-1}.  The permuted digits of point k are cmask ^ (XOR
+// of S_i = Q_i ^ Q_{i-1} over the set bits i of k), and each step k -> k + 1
+// adds Q_c, c = ctz(k + 1).  In base 2 every term rec_t y_t is 0 or 2^(-t-1)
+// and every partial sum of the reference's loop is a multiple of 2^-52 below
+// 1, so all of them are exact and x = Y 2^-52 for the 52-bit word Y.
 template <class SinkT>
-__device__ __forceinline__ void gdn_chain2(const u64 *__restrict__ Q, u32 k, u32 len, u32 d, double *dst,
-                                           SinkT &out) {
-    // Q = cmask, Q_0..Q_{mm-1}; point k's digits: cmask ^ XOR of columns S_i = Q_i ^ Q_{i-1}
+__device__ __forceinline__ void gdn_walk2(const u64 *__restrict__ Q, u32 k, u32 H, u32 d, double *dst,
+                                          SinkT &out) {
     u64 z = Q[0];
     for (u32 kk = k; kk; kk &= kk - 1) {
         const int i = __ffs(kk) - 1;
         z ^= Q[1 + i] ^ (i ? Q[i] : 0ull);
     }
     for (u32 h = 0;; h++) {
-        out.put(dst, (double)z * 0x1p-52);  // z < 2^52: exact, = z / 2^tn
-        if (h + 1 == len) break;
-        z ^= Q[__ffs(k + 1)];  // Q_c at Q[1 + c], c = ctz(k + 1)
+        out.put(dst, (double)z * 0x1p-52);
+        if (h + 1 == H) break;
+        z ^= Q[__ffs(k + 1)];  // Q_c at Q[1 + c]
         k++;
         dst += d;
     }
 }
 
-// One gdn chain of a column with odd base b: len consecutive points from k,
-// row stride d.  T = the column's digit tables in LDS, W pairs per row (the
-// lane group's width, as a template so every pair lives in a named register);
-// L = pairs in the low part of the floating-point Horner split.
-template <int W, int L, class SinkT>
-__device__ __forceinline__ void gdn_chain(const u64 *__restrict__ gj, const u32 *T, u32 k, u32 len, u32 d,
-                                          double *dst, SinkT &out) {
-    const u64 w1 = gj[1];
-    const u32 b = (u32)gj[0] & 0xffffu;
-    const u32 mm = (u32)((w1 >> 8) & 0xff);
-    const u32 b2 = (u32)(w1 >> 32);
-    const u64 M = gj[2];
-    const double scale = __longlong_as_double((long long)gj[3]);
-    // start: z = c + sum_i dig_i * S'_i  with  S'_i = Q_i - Q_{i-1}
-    //      = c + sum_i Q_i * e_i,  e_i = dig_i - dig_{i+1} mod b  (dig_mm = 0)
-    u32 acc_lo[W], acc_hi[W];
-#pragma unroll
-    for (int P = 0; P < W; P++) {
-        const u32 w = T[P];
-        acc_lo[P] = w & 0xffffu;
-        acc_hi[P] = w >> 16;
-    }
-    u32 qd = fast_div(k, M);
-    u32 dig = k - qd * b;
-    const u32 dlow0 = dig;
-    for (u32 i = 0; i < mm; i++) {
-        const u32 qn = fast_div(qd, M);
-        const u32 dn = qd - qn * b;
-        const u32 e = dig >= dn ? dig - dn : dig + b - dn;
-        const u32 *Qi = T + (1 + i) * W;
-#pragma unroll
-        for (int P = 0; P < W; P++) {
-            const u32 w = Qi[P];
-            acc_lo[P] += e * (w & 0xffffu);  // < (1 + mm (b-1)) b < 2^32
-            acc_hi[P] += e * (w >> 16);
+// A column of odd base b, as one lane walks it.
+struct GdnCol {
+    const double *rec, *thr;  // the base's constants (kTNmax each)
+    const u32 *Qs, *Qu;       // walk table in LDS: scaled to permutation-row offsets, and plain
+    u64 seed, l, j, M;
+    u32 b, mm, ts, tmin, rowb;
+};
+
+// u16 at byte address a of the dynamic LDS, which starts at address 0 in the
+// kernels that use this (they declare no static LDS)
+__device__ __forceinline__ u32 lds_u16(u32 a) {
+    return *(__attribute__((address_space(3))) const unsigned short *)(uintptr_t)a;
+}
+
+__device__ __forceinline__ double uniform_f64(const double *p) {
+    const u64 w = (u64) * (const long long *)p;
+    const u32 lo = __builtin_amdgcn_readfirstlane((u32)w), hi = __builtin_amdgcn_readfirstlane((u32)(w >> 32));
+    return __longlong_as_double((long long)(((u64)hi << 32) | lo));
+}
+
+// Levels t .. kTNmax-1 of the sum for the lanes still below thr_t: digit t from
+// the point's index digits, pi_t by counting keys.  Rare (see gdn_levels).
+__device__ __noinline__ double gdn_tail(GdnCol c, u32 k, u32 t, double v) {
+    for (; t < (u32)kTNmax; t++) {
+        const bool act = v < c.thr[t];
+        if (__builtin_amdgcn_ballot_w64(act) == 0) break;
+        if (act) {
+            u32 z = 0, kk = k;
+            for (u32 i = 0; i < c.mm && i <= t; i++) {
+                const u32 qn = fast_div(kk, c.M);
+                z = (z + (kk - qn * c.b) * gdn_s(c.seed, c.l, c.j, t, i, c.b)) % c.b;  // < b^2 + b
+                kk = qn;
+            }
+            v += c.rec[t] * (double)gdn_perm_at(c.seed, c.l, c.j, t, c.b, z);
         }
-        qd = qn;
-        dig = dn;
     }
-    u32 z[W];
+    return v;
+}
+
+// H consecutive points from k of a column of odd base b, TB levels in LDS.
+// Each lane keeps the digits z_t of its point as byte offsets z_t * rowb of
+// the permutation rows, so pi_t(z_t) is one u16 load with t in the immediate
+// offset, and a walk step adds a scaled table row modulo b * rowb.
+template <int TB, class SinkT>
+__device__ __forceinline__ void gdn_walk_odd(const GdnCol &c, u32 k, u32 H, u32 d, double *dst, SinkT &out) {
+    const u32 b = c.b, rowb = c.rowb;
+    const u64 M = c.M;
+    double rr[TB], th[TB];
 #pragma unroll
-    for (int P = 0; P < W; P++) {
-        const u32 lo = acc_lo[P] - fast_div(acc_lo[P], M) * b;
-        const u32 hi = acc_hi[P] - fast_div(acc_hi[P], M) * b;
-        z[P] = lo | (hi << 16);
+    for (int t = 0; t < TB; t++) {
+        rr[t] = uniform_f64(c.rec + t);
+        th[t] = uniform_f64(c.thr + t);
     }
-    const u16x2 bb = as_pair(b | (b << 16));
-    const u16x2 b1 = as_pair(b | (1u << 16));  // pair value = lo * b + hi (v_dot2_u32_u16)
-    const double bd2 = (double)b2;
-    double BL = bd2;
+    // first point: z_t = sum_i (dig_i - dig_{i+1}) Q_i[t] mod b (dig_mm = 0;
+    // Abel summation of S[t][i] = Q_i[t] - Q_{i-1}[t])
+    u32 z[TB];
+    {
+        u32 acc[TB];
 #pragma unroll
-    for (int i = 1; i < L; i++) BL *= bd2;  // (b^2)^L < 2^53: exact
-    u32 dlow = dlow0;
+        for (int t = 0; t < TB; t++) acc[t] = 0;
+        u32 qd = fast_div(k, M);
+        u32 dig = k - qd * b;
+        for (u32 i = 0; i < c.mm; i++) {
+            const u32 qn = fast_div(qd, M);
+            const u32 dn = qd - qn * b;
+            const u32 e = dig >= dn ? dig - dn : dig + b - dn;
+            const u32 *Qi = c.Qu + i * TB;
+#pragma unroll
+            for (int t = 0; t < TB; t++) acc[t] += e * Qi[t];  // < mm b^2 < 2^32
+            qd = qn;
+            dig = dn;
+        }
+#pragma unroll
+        for (int t = 0; t < TB; t++) z[t] = (acc[t] - fast_div(acc[t], M) * b) * rowb;
+    }
+    u32 dlow = k - fast_div(k, M) * b;
+    const u32 bR = b * rowb;
+    const int ts = (int)c.ts, tmin = (int)c.tmin;
     for (u32 h = 0;; h++) {
-        // Y = sum of digit pairs in base b^2 as Yhi * (b^2)^L + Ylo, both parts
-        // exact doubles (< 2^53); one fma rounds Y once, as u64 -> f64 does
-        double Yhi = 0.0, Ylo = 0.0;
+        // levels in LDS while the wave has a lane below the level's threshold
+        // (a flag rather than a break keeps the loop fully unrolled)
+        double v = 0.0;
+        int t = 0;
+        bool go = true, fin = false;
 #pragma unroll
-        for (int P = 0; P < W; P++) {
-            const double pv = (double)__builtin_amdgcn_udot2(as_pair(z[P]), b1, 0u, false);
-            if (P < W - L)
-                Yhi = __fma_rn(Yhi, bd2, pv);
-            else
-                Ylo = __fma_rn(Ylo, bd2, pv);
+        for (int tt = 0; tt < TB; tt++) {
+            if (go) {
+                if (tt >= ts) {
+                    go = false;
+                } else if (tt >= tmin && __builtin_amdgcn_ballot_w64(v < th[tt]) == 0) {
+                    go = false;
+                    fin = true;
+                } else {
+                    v += rr[tt] * (double)lds_u16(z[tt] + 2 * tt);  // gdn_integer_to_float.c:36-37
+                    t = tt + 1;
+                }
+            }
         }
-        out.put(dst, __fma_rn(Yhi, BL, Ylo) / scale);
-        if (h + 1 == len) break;
+        if (!fin && t < kTNmax) v = gdn_tail(c, k, (u32)t, v);
+        out.put(dst, v);
+        if (h + 1 == H) break;
         // carries of k -> k+1: c trailing digits equal to b-1
-        u32 c = 0;
+        u32 cc = 0;
         if (dlow == b - 1) {
             dlow = 0;
-            c = 1;
+            cc = 1;
             u32 u = fast_div(k, M);
             for (;;) {
                 const u32 un = fast_div(u, M);
                 if (u - un * b != b - 1) break;
-                c++;
+                cc++;
                 u = un;
             }
         } else {
             dlow++;
         }
         k++;
-        const u32 *Qc = T + (1 + c) * W;
-        u32 inc[W];
+        const u32 *Qc = c.Qs + cc * TB;
+        u32 inc[TB];
 #pragma unroll
-        for (int P = 0; P < W; P++) inc[P] = Qc[P];
+        for (int t = 0; t < TB; t++) inc[t] = Qc[t];
 #pragma unroll
-        for (int P = 0; P < W; P++) z[P] = pk_add_mod(z[P], inc[P], bb);
+        for (int t = 0; t < TB; t++) {
+            const u32 s2 = z[t] + inc[t];
+            z[t] = __builtin_elementwise_min(s2, s2 - bR);
+        }
         dst += d;
     }
 }
 
+// LDS fill of an odd-base column: permutation rows (pairs of levels as u32
+// words; TB and the row length are even) and the walk table twice
+template <int TB>
+__device__ __forceinline__ void gdn_fill_odd(const u64 *__restrict__ blk, u32 b, u32 mm, u32 rowu,
+                                             unsigned char *lds, u32 *Qs, u32 *Qu) {
+    const u32 *Pg = (const u32 *)blk;
+    u32 *Pl = (u32 *)lds;
+    constexpr u32 TW = TB / 2;
+    for (u32 e = threadIdx.x; e < b * TW; e += blockDim.x) {
+        const u32 z = e / TW;
+        Pl[z * (rowu / 2) + (e - z * TW)] = Pg[e];
+    }
+    const unsigned short *Qg = (const unsigned short *)blk + (u64)b * TB;
+    for (u32 e = threadIdx.x; e < mm * TB; e += blockDim.x) {
+        const u32 v = Qg[e];
+        Qs[e] = v * 2 * rowu;
+        Qu[e] = v;
+    }
+}
+
 // Tiles are drained by two persistent kernels: class 0 = lattice and dnb2
-// (HBM-write bound, few registers, full occupancy), class 1 = gdn (register
-// heavy digit walks).  Each class is compiled without the other's code.
+// (HBM-write bound, few registers, full occupancy), class 1 = gdn (digit walks
+// and the sequential sums).  Each class is compiled without the other's code.
 __host__ __device__ constexpr int method_class(int method) { return method == M_GDN_LMS ? 1 : 0; }
 
+// A gdn tile: rows [chunk RT, (chunk + 1) RT) of column j of replication l,
+// rem = chunk * d + j (the columns of one row range are consecutive tiles).
+// Lane i walks rows i H .. i H + H - 1 of the tile.
+template <class SinkT>
+__device__ __forceinline__ void gdn_tile(const Req &q, u64 l, u32 rem, const u64 *__restrict__ ws,
+                                         const u64 *__restrict__ tab, double *__restrict__ slab,
+                                         unsigned char *lds, SinkT &out) {
+    const u32 d = (u32)q.d;
+    const u32 chunk = div_q(rem, q.ddiv);
+    const u32 j = __builtin_amdgcn_readfirstlane(rem - chunk * d);
+    const u32 RT = q.rows_per_tile;
+    const u64 info = tab[j];
+    const u32 colo = __builtin_amdgcn_readfirstlane((u32)info);
+    const u32 TB = __builtin_amdgcn_readfirstlane((u32)((info >> 32) & 0xff));
+    const u32 mm = __builtin_amdgcn_readfirstlane((u32)((info >> 40) & 0xff));
+    const u32 tmin = __builtin_amdgcn_readfirstlane((u32)((info >> 48) & 0xff));
+    const u64 *blk = tab + q.d + l * q.lw + colo;
+    const u32 b = c_primes[j];
+    const u32 H = RT >= (u32)kBlock ? RT / (u32)kBlock : 1u;
+    const u32 i0 = threadIdx.x * H;
+    const u32 k = chunk * RT + i0;
+    double *dst = slab + (u64)k * d + j;
+    if (b == 2) {
+        u64 *Q = (u64 *)lds;
+        for (u32 e = threadIdx.x; e <= (u32)q.m; e += blockDim.x) Q[e] = blk[e];
+        __syncthreads();
+        if (i0 < RT) gdn_walk2(Q, k, H, d, dst, out);
+        return;
+    }
+    const u32 rowu = gdn_row_u16(TB);
+    u32 *Qs = (u32 *)(lds + ((b * rowu * 2 + 15) & ~15u));
+    u32 *Qu = Qs + mm * TB;
+    GdnCol c;
+    c.rec = (const double *)ws + (u64)j * kTNmax;
+    c.thr = (const double *)ws + (u64)(256 + j) * kTNmax;
+    c.Qs = Qs;
+    c.Qu = Qu;
+    c.seed = q.seed;
+    c.l = l;
+    c.j = j;
+    c.M = recip_magic(b);
+    c.b = b;
+    c.mm = mm;
+    c.ts = TB < q.gcap ? TB : q.gcap;
+    c.tmin = tmin;
+    c.rowb = 2 * rowu;
+#define QMC_GDN_TB(T_)                                   \
+    case T_:                                             \
+        gdn_fill_odd<T_>(blk, b, mm, rowu, lds, Qs, Qu); \
+        __syncthreads();                                 \
+        if (i0 < RT) gdn_walk_odd<T_>(c, k, H, d, dst, out); \
+        break;
+    switch (TB) {
+        QMC_GDN_TB(8) QMC_GDN_TB(12) QMC_GDN_TB(16) QMC_GDN_TB(24) QMC_GDN_TB(32) QMC_GDN_TB(48)
+        default: break;  // never: gdn_levels gives one of these
+    }
+#undef QMC_GDN_TB
+}
+
 template <int CLASS, bool DIG>
-__device__ void run_tile(const Req &q, u64 t, const u64 *__restrict__ ws, double *__restrict__ ring, u32 *lds,
-                         unsigned long long *digest) {
+__device__ void run_tile(const Req &q, u64 t, const u64 *__restrict__ ws, double *__restrict__ ring,
+                         unsigned char *lds, unsigned long long *digest) {
     const u64 d = q.d;
     const u64 l = t / q.tiles_per_l;
-    const u64 row0 = (t - l * q.tiles_per_l) * (u64)q.rows_per_tile;
+    const u64 rem = t - l * q.tiles_per_l;
     const u64 *tab = ws + q.tab;
     double *slab = ring + ws[q.slabs + l];
     Sink<DIG, CLASS == 0> out;
-    if (CLASS == 0 && q.method == M_LATTICE) {
-        const u32 pts = q.rows_per_tile * (u32)d;
-        const u64 *g = tab;
-        const double *sh = (const double *)(tab + d) + l * d;
-        for (u32 e = threadIdx.x; e < pts; e += blockDim.x) {
-            const u32 rr = div_q(e, q.ddiv);
-            const u32 j = e - rr * q.d;
-            const u64 row = row0 + rr;
-            const double phi = (double)__brevll(row) * 0x1p-64;  // phi2(row), exact
-            const double x = frac_mod1((double)g[j] * phi);      // lat_gen_natural.c:45
-            out.put(slab + row * d + j, frac_mod1(x + sh[j]));  // lat_shift_mod_1.c:31
-        }
-    } else if (CLASS == 0 && q.method == M_DNB2_LMS) {
-        dnb2_tile<true, DIG>(q, l, row0, tab, slab, lds, out);
-    } else if (CLASS == 0) {
-        dnb2_tile<false, DIG>(q, l, row0, tab, slab, lds, out);
-    } else {
-        // this replication's digit tables into LDS
-        const u32 *src = (const u32 *)(tab + q.gtab) + l * q.gtw;
-        for (u32 e = threadIdx.x; e < q.gtw; e += blockDim.x) lds[e] = src[e];
-        __syncthreads();
-        // units = (lane group, block of ch chains of k rows); lanes = (chain, column)
-        const u64 *grp = tab + q.ggrp;  // per group: j0 | gc<<16 | ch<<24 | w<<32 | l<<40,  k | cb0<<32 | nu<<48
-        const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
-        for (u32 u = wave; u < q.gunits; u += nwaves) {
-            u32 g = 0;
-            u64 g1 = grp[1];
-            while (u >= ((g1 >> 32) & 0xffff) + (g1 >> 48)) g1 = grp[2 * (++g) + 1];
-            const u64 g0 = grp[2 * g];
-            const u32 j0 = __builtin_amdgcn_readfirstlane((u32)(g0 & 0xffff));
-            const u32 gc = __builtin_amdgcn_readfirstlane((u32)((g0 >> 16) & 0xff));
-            const u32 CH = __builtin_amdgcn_readfirstlane((u32)((g0 >> 24) & 0xff));
-            const u32 Ww = __builtin_amdgcn_readfirstlane((u32)((g0 >> 32) & 0xff));
-            const u32 Lw = __builtin_amdgcn_readfirstlane((u32)((g0 >> 40) & 0xff));
-            const u32 K = __builtin_amdgcn_readfirstlane((u32)g1);
-            const u32 cb = u - __builtin_amdgcn_readfirstlane((u32)((g1 >> 32) & 0xffff));
-            const u32 ch = lane / gc, jj = lane - ch * gc;
-            const u32 j = j0 + jj;
-            const u32 rs = cb * CH * K + ch * K;
-            if (ch >= CH || rs >= q.rows_per_tile) continue;
-            const u32 len = q.rows_per_tile - rs < K ? q.rows_per_tile - rs : K;
-            const u64 k = row0 + rs;
-            const u64 *gj = tab + kGJ * j;
-            const u32 *T = lds + (u32)(gj[0] >> 32);
-            double *dst = slab + k * d + j;
-            if (j == 0) {
-                gdn_chain2((const u64 *)T, (u32)k, len, (u32)d, dst, out);
-                continue;
+    if (CLASS == 0) {
+        const u64 row0 = rem * (u64)q.rows_per_tile;
+        if (q.method == M_LATTICE) {
+            const u32 pts = q.rows_per_tile * (u32)d;
+            const u64 *g = tab;
+            const double *sh = (const double *)(tab + d) + l * d;
+            for (u32 e = threadIdx.x; e < pts; e += blockDim.x) {
+                const u32 rr = div_q(e, q.ddiv);
+                const u32 j = e - rr * q.d;
+                const u64 row = row0 + rr;
+                const double phi = (double)__brevll(row) * 0x1p-64;  // phi2(row), exact
+                const double x = frac_mod1((double)g[j] * phi);      // lat_gen_natural.c:45
+                out.put(slab + row * d + j, frac_mod1(x + sh[j]));  // lat_shift_mod_1.c:31
             }
-#define QMC_GDN_SHAPE(W_, L_) \
-    if (Ww == W_ && Lw == L_) { gdn_chain<W_, L_>(gj, T, (u32)k, len, (u32)d, dst, out); continue; }
-            // every (w, l) a group can have (host-checked): 17 for columns [0, 8),
-            // 6 for [8, 72), 4 or 3 for the groups of larger bases
-            QMC_GDN_SHAPE(17, 3) QMC_GDN_SHAPE(6, 3) QMC_GDN_SHAPE(4, 3) QMC_GDN_SHAPE(4, 2) QMC_GDN_SHAPE(3, 2)
-#undef QMC_GDN_SHAPE
+        } else if (q.method == M_DNB2_LMS) {
+            dnb2_tile<true, DIG>(q, l, row0, tab, slab, (u32 *)lds, out);
+        } else {
+            dnb2_tile<false, DIG>(q, l, row0, tab, slab, (u32 *)lds, out);
         }
+    } else {
+        gdn_tile(q, l, (u32)rem, ws, tab, slab, lds, out);
     }
     out.flush(digest + q.slab0 + l);
 }
 
 // ------------------------------------------------------------ the kernels
 
+// Persistent workgroups take tiles from a counter until it passes the class's
+// tile count.  The queue slots sit at byte `slot` of the dynamic LDS, after the
+// tile's tables (the kernels declare no static LDS).
 template <int CLASS, bool DIG>
 __global__ __launch_bounds__(kBlock) void batch_persistent_kernel(const Req *__restrict__ reqs, int nreq,
                                                                   u64 total_tiles, unsigned long long *counter,
                                                                   const u64 *__restrict__ ws,
                                                                   double *__restrict__ out,
-                                                                  unsigned long long *digest) {
-    __shared__ u64 s_tile;
-    __shared__ int s_req;
-    extern __shared__ u32 lds_tables[];  // dnb2 first-row values or gdn digit tables of the current tile
+                                                                  unsigned long long *digest, u32 slot) {
+    extern __shared__ __align__(16) unsigned char lds_dyn[];
+    u64 *s_tile = (u64 *)(lds_dyn + slot);
+    int *s_req = (int *)(lds_dyn + slot + 8);
     for (;;) {
         if (threadIdx.x == 0) {
             const u64 t = atomicAdd(counter, 1ull);
-            s_tile = t;
+            *s_tile = t;
             int lo = 0, hi = nreq - 1;  // last request with tile0 <= t
             if (t < total_tiles) {
                 while (lo < hi) {
@@ -741,15 +843,15 @@ __global__ __launch_bounds__(kBlock) void batch_persistent_kernel(const Req *__r
                     else hi = mid - 1;
                 }
             }
-            s_req = lo;
+            *s_req = lo;
         }
         __syncthreads();
-        const u64 t = s_tile;
-        const int qi = s_req;
+        const u64 t = *s_tile;
+        const int qi = *s_req;
         __syncthreads();
         if (t >= total_tiles) break;  // every wave of every block reaches this
         const Req q = reqs[qi];
-        run_tile<CLASS, DIG>(q, t - q.tile0, ws, out, lds_tables, digest);
+        run_tile<CLASS, DIG>(q, t - q.tile0, ws, out, lds_dyn, digest);
     }
 }
 
@@ -762,9 +864,9 @@ __global__ void reset_counter_kernel(unsigned long long *counter) {
 template <int CLASS, bool DIG>
 __global__ __launch_bounds__(kBlock) void batch_single_kernel(const Req *__restrict__ req, const u64 *__restrict__ ws,
                                                               double *__restrict__ out, unsigned long long *digest) {
-    extern __shared__ u32 lds_tables[];
+    extern __shared__ __align__(16) unsigned char lds_dyn[];
     const Req q = *req;
-    run_tile<CLASS, DIG>(q, blockIdx.x, ws, out, lds_tables, digest);
+    run_tile<CLASS, DIG>(q, blockIdx.x, ws, out, lds_dyn, digest);
 }
 
 // ----------------------------------------------------------- host plan
@@ -796,6 +898,8 @@ u32 dnb2_chain_log(u32 RT, u32 d) {
     return best_al != ~0u ? best_al : best;
 }
 
+constexpr u32 kLdsClass0 = 256 * sizeof(u32);  // dnb2 first-row values
+
 struct Plan {
     std::vector<Req> reqs;
     u64 total_tiles = 0;
@@ -810,7 +914,7 @@ struct Plan {
     unsigned long long *d_counter = nullptr;
     int grid = 0;
     int device = 0;
-    u32 lds_bytes = 0;               // dynamic LDS of the gdn kernel (digit tables)
+    u32 lds_bytes[2] = {kLdsClass0, 0};  // tile tables per class; the queue slots follow them
     hipStream_t cap = nullptr;       // private stream for graph capture
     hipStream_t side = nullptr;      // the gdn class runs here, beside the store-bound class
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -822,7 +926,15 @@ struct Plan {
     std::vector<u64> slab_pos;       // ring offset of every slab (host copy of the workspace's)
 };
 
-constexpr u32 kLdsClass0 = 256 * sizeof(u32);  // dnb2 first-row values
+inline u32 lds_total(u32 bytes) { return ((bytes + 15) & ~15u) + 16; }
+
+template <int CLASS, bool DIG>
+void launch_persistent(Plan *P, double *ring, hipStream_t s) {
+    const u32 slot = (P->lds_bytes[CLASS] + 15) & ~15u;
+    hipLaunchKernelGGL((batch_persistent_kernel<CLASS, DIG>), dim3(P->grid), dim3(kBlock), lds_total(P->lds_bytes[CLASS]),
+                       s, P->d_qreqs + (CLASS ? P->qn[0] : 0), P->qn[CLASS], P->qtiles[CLASS], P->d_counter + CLASS,
+                       P->d_ws, ring, P->digest, slot);
+}
 
 // With both classes present, the gdn kernel (the long tiles) is forked onto the
 // plan's side stream and joined back through events, so a graph captured from
@@ -835,23 +947,15 @@ int launch_batch_t(Plan *P, double *ring, hipStream_t s) {
     if (P->qtiles[0] && P->qtiles[1]) {
         QMC_HIP_CHECK(hipEventRecord(P->ev_fork, s));
         QMC_HIP_CHECK(hipStreamWaitEvent(P->side, P->ev_fork, 0));
-        hipLaunchKernelGGL((batch_persistent_kernel<1, DIG>), dim3(P->grid), dim3(kBlock), P->lds_bytes, P->side,
-                           P->d_qreqs + P->qn[0], P->qn[1], P->qtiles[1], P->d_counter + 1, P->d_ws, ring,
-                           P->digest);
+        launch_persistent<1, DIG>(P, ring, P->side);
         QMC_HIP_CHECK(hipEventRecord(P->ev_join, P->side));
-        hipLaunchKernelGGL((batch_persistent_kernel<0, DIG>), dim3(P->grid), dim3(kBlock), kLdsClass0, s, P->d_qreqs,
-                           P->qn[0], P->qtiles[0], P->d_counter, P->d_ws, ring, P->digest);
+        launch_persistent<0, DIG>(P, ring, s);
         QMC_HIP_CHECK(hipStreamWaitEvent(s, P->ev_join, 0));
         QMC_HIP_CHECK(hipGetLastError());
         return QMC_OK;
     }
-    if (P->qtiles[0])
-        hipLaunchKernelGGL((batch_persistent_kernel<0, DIG>), dim3(P->grid), dim3(kBlock), kLdsClass0, s, P->d_qreqs,
-                           P->qn[0], P->qtiles[0], P->d_counter, P->d_ws, ring, P->digest);
-    if (P->qtiles[1])
-        hipLaunchKernelGGL((batch_persistent_kernel<1, DIG>), dim3(P->grid), dim3(kBlock), P->lds_bytes, s,
-                           P->d_qreqs + P->qn[0], P->qn[1], P->qtiles[1], P->d_counter + 1, P->d_ws, ring,
-                           P->digest);
+    if (P->qtiles[0]) launch_persistent<0, DIG>(P, ring, s);
+    if (P->qtiles[1]) launch_persistent<1, DIG>(P, ring, s);
     QMC_HIP_CHECK(hipGetLastError());
     return QMC_OK;
 }
@@ -866,17 +970,20 @@ extern "C" {
 
 QMC_EXPORT void qmc_batch_free(void *h);
 
-// Build a plan for nreq requests: methods[q] in 0..3, m[q] = log2 n in [1, 31],
-// d[q] in [1, 256], r[q] >= 1, seeds[q]; `points_per_tile` (power of two) and
-// `grid` (persistent workgroups; 0 = 4 per CU) are the tunables; `ring_elems`
-// is the output ring size in doubles (every replication slab must fit).
+// Build a plan for nreq requests: methods[q] in 0..3, m[q] = log2 n in [1, 32]
+// (1..31 for lattices), d[q] in [1, 256], r[q] >= 1, seeds[q]; `points_per_tile`
+// (power of two) and `grid` (persistent workgroups; 0 = 4 per CU) are the
+// tunables; `ring_elems` is the output ring size in doubles (every replication
+// slab must fit); `gdn_level_cap` (0 = none) limits the permutation levels a
+// gdn tile reads from its tables, the rest taking the index-and-key path
+// (a test hook for that path; the output does not change).
 QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t *m, const int32_t *d,
-                                const int32_t *r, const u64 *seeds, u64 points_per_tile, int grid, u64 ring_elems) {
+                                const int32_t *r, const u64 *seeds, u64 points_per_tile, int grid, u64 ring_elems,
+                                int gdn_level_cap) {
     fill_primes();
     Plan *P = new Plan();
     P->ring_elems = ring_elems;
-    u64 tile = 0, words = 0;
-    std::vector<std::vector<GdnGroup>> groups(nreq);
+    u64 tile = 0, words = kBaseWords;
     for (int i = 0; i < nreq; i++) {
         Req q{};
         q.method = methods[i];
@@ -884,43 +991,35 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
         q.d = d[i];
         q.r = r[i];
         q.seed = seeds[i];
-        if (q.method < 0 || q.method > 3 || q.m < 1 || q.m > 31 || q.d < 1 || q.d > 256 || q.r < 1 ||
-            (q.method == M_LATTICE && q.m > 53) || ((q.method == 1 || q.method == 2) && q.m > kTB) ||
-            (q.method == M_GDN_LMS && q.m > kTB)) {
+        if (q.method < 0 || q.method > 3 || q.m < 1 || q.m > 32 || q.d < 1 || q.d > 256 || q.r < 1 ||
+            (q.method == M_LATTICE && q.m > 31)) {
             qmc_fail(QMC_ERR_ARG, "batch: request %d has unsupported (method, m, d, r) = (%d, %d, %d, %d)", i,
                      q.method, q.m, q.d, q.r);
             qmc_batch_free(P);
             return nullptr;
         }
         const u64 n = 1ull << q.m;
-        u64 rows = points_per_tile / (u64)q.d;
-        if (q.method == M_GDN_LMS && rows < 256) rows = 256;  // digit tables are loaded once per tile
-        if ((q.method == M_DNB2_LMS || q.method == M_DNB2_NUS) && rows < 1024) rows = 1024;  // 64-step chains at 2^s = 16
+        u64 rows;
+        if (q.method == M_GDN_LMS) {
+            rows = points_per_tile < (u64)kBlock ? (u64)kBlock : points_per_tile;  // a gdn tile is one column
+        } else {
+            rows = points_per_tile / (u64)q.d;
+            if ((q.method == M_DNB2_LMS || q.method == M_DNB2_NUS) && rows < 1024) rows = 1024;  // 64-step chains at 2^s = 16
+        }
         u64 p2 = 1;
         while (p2 * 2 <= rows) p2 *= 2;
         if (p2 > n) p2 = n;
         q.rows_per_tile = (u32)p2;
-        q.tiles_per_l = (u32)(n / p2);
+        q.tiles_per_l = (u32)(n / p2) * (q.method == M_GDN_LMS ? (u32)q.d : 1u);
         q.tile0 = tile;
         tile += (u64)q.r * q.tiles_per_l;
         q.ddiv = make_div((u32)q.d);
+        q.gcap = gdn_level_cap > 0 ? (u32)gdn_level_cap : 64u;
         if (q.method == M_DNB2_LMS || q.method == M_DNB2_NUS) q.clog = dnb2_chain_log(q.rows_per_tile, (u32)q.d);
-        if (q.method == M_GDN_LMS) {
-            groups[i] = gdn_groups(q.d, q.rows_per_tile);
-            q.gng = (u32)groups[i].size();
-            q.gunits = groups[i].back().cb0 + groups[i].back().nu;
-        }
         q.tab = words;
-        words += table_words(q, groups[i]);
-        if (q.method == M_GDN_LMS) {
-            if ((u64)q.gtw * 4 > 65536) {
-                qmc_fail(QMC_ERR_ARG, "batch: gdn request %d needs %u bytes of digit tables (max 65536)", i,
-                         q.gtw * 4);
-                qmc_batch_free(P);
-                return nullptr;
-            }
-            if (q.gtw * 4 > P->lds_bytes) P->lds_bytes = q.gtw * 4;
-        }
+        words += table_words(q);
+        if (q.method == M_GDN_LMS)
+            for (int j = 0; j < q.d; j++) P->lds_bytes[1] = std::max(P->lds_bytes[1], gdn_col_lds(h_primes[j], q.m));
         q.slabs = words;
         words += q.r;
         q.slab0 = (u32)P->nslabs;
@@ -954,47 +1053,32 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, P->device) != hipSuccess) cus = 256;
     P->grid = grid > 0 ? grid : 4 * cus;
     std::vector<u64> ws_host(words, 0);
-    // gdn dimension info and lane groups (the digit tables are built on the device)
+    // per-base constants, then each gdn request's column info (its tables are built on the device)
+    for (int j = 0; j < 256; j++) {
+        gdn_base_tables(h_primes[j], (double *)ws_host.data() + (u64)j * kTNmax,
+                        (double *)ws_host.data() + (u64)(256 + j) * kTNmax);
+    }
+    std::vector<u32> greq;
+    std::vector<u64> gcol0;
+    u64 gcols = 0;
     for (size_t qi = 0; qi < P->reqs.size(); qi++) {
         const Req &q = P->reqs[qi];
         if (q.method != M_GDN_LMS) continue;
-        u32 qoff = 0;
-        for (size_t gi = 0; gi < groups[qi].size(); gi++) {
-            const GdnGroup &x = groups[qi][gi];
-            u64 *gw = ws_host.data() + q.tab + q.ggrp + 2 * gi;
-            const bool shape_ok = x.w == 0 || (x.w == 17 && x.l == 3) || (x.w == 6 && x.l == 3) ||
-                                  (x.w == 4 && (x.l == 3 || x.l == 2)) || (x.w == 3 && x.l == 2);
-            if (!shape_ok) {
-                qmc_fail(QMC_ERR_ARG, "batch: gdn lane group with no kernel shape (w=%u, l=%u)", x.w, x.l);
-                qmc_batch_free(P);
-                return nullptr;
-            }
-            gw[0] = x.j0 | (x.gc << 16) | (x.ch << 24) | ((u64)x.w << 32) | ((u64)x.l << 40);
-            gw[1] = x.k | ((u64)x.cb0 << 32) | ((u64)x.nu << 48);
-            for (u32 j = x.j0; j < x.j0 + x.gc; j++) {
-                const u32 b = h_primes[j];
-                const u64 tn = (u64)gdn_tn(b), mm = (u64)gdn_mm(b, q.m), np = gdn_np(b);
-                const u64 W = x.w, L = x.l;
-                // Horner halves exact: (b^2)^L and (b^2)^(np-L) below 2^53
-                if (b != 2 && !((double)b * b * b * b < 0x1p53 && (L == 2 || std::pow((double)b, 6.0) < 0x1p53) &&
-                                std::pow((double)b * b, (double)(np > L ? np - L : 0)) < 0x1p53)) {
-                    qmc_fail(QMC_ERR_ARG, "batch: gdn base %u does not split into exact halves (W=%u, L=%u)", b,
-                             (u32)W, (u32)L);
-                    qmc_batch_free(P);
-                    return nullptr;
-                }
-                double scale = 1.0;
-                for (u64 t = 0; t < tn; t++) scale *= (double)b;
-                u64 sbits;
-                std::memcpy(&sbits, &scale, sizeof sbits);
-                u64 *ji = ws_host.data() + q.tab + (u64)kGJ * j;
-                ji[0] = b | (L << 16) | ((u64)qoff << 32);
-                ji[1] = tn | (mm << 8) | (np << 16) | (W << 24) | ((u64)b * b << 32);
-                ji[2] = recip_magic(b);
-                ji[3] = sbits;
-                qoff += gdn_words(q, (int)j, x.w);
-            }
+        u64 colo = 0;
+        for (int j = 0; j < q.d; j++) {
+            const u32 b = h_primes[j];
+            const u64 tmin = gdn_tmin((const double *)ws_host.data() + (u64)(256 + j) * kTNmax);
+            ws_host[q.tab + j] = colo | ((u64)gdn_levels(b) << 32) | ((u64)gdn_mm(b, q.m) << 40) | (tmin << 48);
+            colo += gdn_col_words(b, q.m);
         }
+        greq.push_back((u32)qi);
+        gcol0.push_back(gcols);
+        gcols += (u64)q.r * (u64)q.d;
+    }
+    if (gcols > 0x7fffffffull) {
+        qmc_fail(QMC_ERR_ARG, "batch: too many gdn columns (%llu)", (unsigned long long)gcols);
+        qmc_batch_free(P);
+        return nullptr;
     }
     // slab bases in the ring: consecutive, wrapping when a slab would run past the end
     u64 pos = 0;
@@ -1007,19 +1091,31 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
             pos += slab;
         }
     }
+    u32 *d_greq = nullptr;
+    u64 *d_gcol0 = nullptr;
     if (hipMemcpyToSymbol(HIP_SYMBOL(c_primes), h_primes, sizeof(h_primes)) != hipSuccess ||
         hipMalloc((void **)&P->d_reqs, sizeof(Req) * P->reqs.size()) != hipSuccess ||
-        hipMalloc((void **)&P->d_ws, sizeof(u64) * (words ? words : 1)) != hipSuccess ||
+        hipMalloc((void **)&P->d_ws, sizeof(u64) * words) != hipSuccess ||
         hipMalloc((void **)&P->d_qreqs, sizeof(Req) * P->qreqs.size()) != hipSuccess ||
-        hipMalloc((void **)&P->d_counter, 2 * sizeof(unsigned long long)) != hipSuccess) {
+        hipMalloc((void **)&P->d_counter, 2 * sizeof(unsigned long long)) != hipSuccess ||
+        (!greq.empty() && (hipMalloc((void **)&d_greq, sizeof(u32) * greq.size()) != hipSuccess ||
+                           hipMalloc((void **)&d_gcol0, sizeof(u64) * gcol0.size()) != hipSuccess))) {
+        (void)hipFree(d_greq);
         qmc_fail(QMC_ERR_NOMEM, "batch: device allocation failed");
         qmc_batch_free(P);
         return nullptr;
     }
-    if (hipMemcpy(P->d_reqs, P->reqs.data(), sizeof(Req) * P->reqs.size(), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(P->d_qreqs, P->qreqs.data(), sizeof(Req) * P->qreqs.size(), hipMemcpyHostToDevice) !=
-            hipSuccess ||
-        hipMemcpy(P->d_ws, ws_host.data(), sizeof(u64) * words, hipMemcpyHostToDevice) != hipSuccess) {
+    bool ok = hipMemcpy(P->d_reqs, P->reqs.data(), sizeof(Req) * P->reqs.size(), hipMemcpyHostToDevice) ==
+                  hipSuccess &&
+              hipMemcpy(P->d_qreqs, P->qreqs.data(), sizeof(Req) * P->qreqs.size(), hipMemcpyHostToDevice) ==
+                  hipSuccess &&
+              hipMemcpy(P->d_ws, ws_host.data(), sizeof(u64) * words, hipMemcpyHostToDevice) == hipSuccess;
+    if (ok && !greq.empty())
+        ok = hipMemcpy(d_greq, greq.data(), sizeof(u32) * greq.size(), hipMemcpyHostToDevice) == hipSuccess &&
+             hipMemcpy(d_gcol0, gcol0.data(), sizeof(u64) * gcol0.size(), hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) {
+        (void)hipFree(d_greq);
+        (void)hipFree(d_gcol0);
         qmc_fail(QMC_ERR_HIP, "batch: copying the plan to the device failed");
         qmc_batch_free(P);
         return nullptr;
@@ -1027,8 +1123,14 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
     // tables
     hipLaunchKernelGGL(batch_setup_kernel, dim3(64, (unsigned)P->reqs.size()), dim3(kBlock), 0, 0, P->d_reqs,
                        P->d_ws);
-    if (hipDeviceSynchronize() != hipSuccess) {
-        qmc_fail(QMC_ERR_HIP, "batch: setup kernel failed");
+    if (gcols)
+        hipLaunchKernelGGL(batch_gdn_setup_kernel, dim3((unsigned)gcols), dim3(kBlock), 0, 0, P->d_reqs, d_greq,
+                           d_gcol0, (int)greq.size(), P->d_ws);
+    const hipError_t se = hipDeviceSynchronize();
+    (void)hipFree(d_greq);
+    (void)hipFree(d_gcol0);
+    if (se != hipSuccess) {
+        qmc_fail(QMC_ERR_HIP, "batch: setup kernels failed");
         qmc_batch_free(P);
         return nullptr;
     }
@@ -1127,7 +1229,7 @@ QMC_EXPORT int qmc_batch_run_per_call_dev(void *h, double *ring, qmc_stream_t s)
         const Req &q = P->reqs[i];
         const u64 tiles = (u64)q.r * q.tiles_per_l;
         const bool c0 = method_class(q.method) == 0;
-        const unsigned lds = c0 ? kLdsClass0 : P->lds_bytes;
+        const unsigned lds = lds_total(P->lds_bytes[c0 ? 0 : 1]);
         if (P->digest) {
             if (c0)
                 hipLaunchKernelGGL((batch_single_kernel<0, true>), dim3((unsigned)tiles), dim3(kBlock), lds, s,
@@ -1167,17 +1269,45 @@ QMC_EXPORT void qmc_batch_dnb2_inputs(u64 seed, int method, int d, int r, u64 *C
         }
 }
 
-// gdn inputs for dimension j: base, digit counts, S (tn x mm), permutation (a, c) per level
-QMC_EXPORT int qmc_batch_gdn_inputs(u64 seed, int m, int l, int j, u32 *b, int *tn, int *mm, u32 *S, u32 *pa,
-                                    u32 *pc) {
+// gdn inputs of a request (seed, m, d, r) in the reference's layouts, for
+// columns j in [j0, j0 + dj): bases[dj] (the primes), S[r][dj][kTNmax][m]
+// (gdn_get_linear_scramble_matrix with tmax = m, tmax_new = kTNmax) and
+// perms[r][dj][kTNmax][bmax] (gdn_get_digital_permutations, zero past b),
+// bmax = the largest of the bases.  The generating matrices are identities.
+QMC_EXPORT int qmc_batch_gdn_inputs(u64 seed, int m, int j0, int dj, int r, u64 *bases, u64 *S, u64 *perms) {
     fill_primes();
-    *b = h_primes[j];
-    *tn = gdn_tn(*b);
-    *mm = gdn_mm(*b, m);
-    for (int t = 0; t < *tn; t++) {
-        for (int k = 0; k < *mm; k++) S[t * *mm + k] = k <= t ? gdn_s(seed, l, j, t, k, *b) : 0;
-        gdn_perm(seed, l, j, t, *b, pa[t], pc[t]);
-    }
+    if (m < 1 || m > 32 || j0 < 0 || dj < 1 || j0 + dj > 256 || r < 1)
+        return qmc_fail(QMC_ERR_ARG, "qmc_batch_gdn_inputs: bad (m, j0, dj, r) = (%d, %d, %d, %d)", m, j0, dj, r);
+    const u64 bmax = h_primes[j0 + dj - 1];
+    std::vector<u64> keys;
+    for (int jj = 0; jj < dj; jj++) bases[jj] = h_primes[j0 + jj];
+    for (int l = 0; l < r; l++)
+        for (int jj = 0; jj < dj; jj++) {
+            const u64 j = (u64)(j0 + jj);
+            const u32 b = h_primes[j];
+            u64 *Slj = S + ((u64)l * dj + jj) * kTNmax * m;
+            u64 *Plj = perms + ((u64)l * dj + jj) * kTNmax * bmax;
+            for (u32 t = 0; t < (u32)kTNmax; t++) {
+                for (u32 i = 0; i < (u32)m; i++) Slj[t * m + i] = gdn_s(seed, l, j, t, i, b);
+                keys.resize(b);
+                for (u32 z = 0; z < b; z++) keys[z] = gdn_pkey(seed, l, j, t, z);
+                std::sort(keys.begin(), keys.end());
+                for (u64 z = 0; z < bmax; z++) Plj[t * bmax + z] = 0;
+                for (u32 pos = 0; pos < b; pos++) Plj[t * bmax + (keys[pos] & 0x7ff)] = pos;
+            }
+        }
+    return QMC_OK;
+}
+
+// The per-base constants of the gdn sums for column j (base = the j-th prime):
+// rec[kTNmax], thr[kTNmax] (the early-exit thresholds), the levels kept in LDS
+// and the first level that can end a sum.
+QMC_EXPORT int qmc_batch_gdn_base(int j, double *rec, double *thr, int *levels, int *tmin) {
+    fill_primes();
+    if (j < 0 || j >= 256) return qmc_fail(QMC_ERR_ARG, "qmc_batch_gdn_base: j = %d outside [0, 256)", j);
+    gdn_base_tables(h_primes[j], rec, thr);
+    *levels = gdn_levels(h_primes[j]);
+    *tmin = (int)gdn_tmin(thr);
     return QMC_OK;
 }
 

@@ -2,7 +2,9 @@
 path and, on the same counter-based inputs, against the reference kernels
 chained as the reference's doctests chain them (lattice: lat_gen_natural ->
 lat_shift_mod_1; dnb2: dnb2_linear_matrix_scramble -> dnb2_gen_natural ->
-dnb2_digital_shift -> dnb2_integer_to_float), run on the CPU oracle."""
+dnb2_digital_shift -> dnb2_integer_to_float; gdn: gdn_linear_matrix_scramble
+-> gdn_gen_natural -> gdn_digital_permutation -> gdn_integer_to_float), run on
+the CPU oracle."""
 import numpy as np
 import pytest
 
@@ -103,33 +105,84 @@ def test_dnb2_nus_requests_are_nested_scrambles(small):
                 assert sorted(top.tolist()) == list(range(n)), (q, l, j)
 
 
-def test_gdn_requests_match_definition_and_stratify(small):
+def gdn_reference_chain(seed, m, r, cols, n=None):
+    """The reference's call sequence on a gdn request's exported inputs, for
+    the columns `cols` (consecutive or not; columns are independent):
+    gdn_linear_matrix_scramble (identity C, tmax = m, tmax_new = 52) ->
+    gdn_gen_natural -> gdn_digital_permutation -> gdn_integer_to_float, on the
+    CPU oracle.  Returns x (r, n, len(cols))."""
+    n = (1 << m) if n is None else n
+    parts = [B.gdn_inputs(seed, m, r, j0=j, dj=1) for j in cols]
+    dj = len(cols)
+    bases = np.array([[int(p[0][0, 0]) for p in parts]], dtype=np.uint64)
+    bmax = int(bases.max())
+    S = np.ascontiguousarray(np.concatenate([p[1] for p in parts], axis=1))
+    perms = np.zeros((r, dj, 52, bmax), dtype=np.uint64)
+    for k, p in enumerate(parts):
+        perms[:, k, :, :p[2].shape[3]] = p[2][:, 0]
+    C = np.ascontiguousarray(np.tile(np.eye(m, dtype=np.uint64)[None, None], (1, dj, 1, 1)))
+    C_lms = np.empty((r, dj, m, 52), dtype=np.uint64)
+    _clib.call_oracle("gdn_linear_matrix_scramble", dict(r=r, d=dj, mmax=m, r_C=1, r_b=1, tmax=m, tmax_new=52,
+                                                         bases=bases, S=S, C=C, C_lms=C_lms))
+    xdig = np.empty((r, n, dj, 52), dtype=np.uint64)
+    _clib.call_oracle("gdn_gen_natural", dict(r=r, n=n, d=dj, r_b=1, mmax=m, tmax=52, n_start=0, bases=bases,
+                                              C=C_lms, xdig=xdig))
+    xperm = np.empty_like(xdig)
+    _clib.call_oracle("gdn_digital_permutation", dict(r=r, n=n, d=dj, r_x=r, r_b=1, tmax=52, tmax_new=52, bmax=bmax,
+                                                      perms=perms, xdig=xdig, xdig_new=xperm))
+    del xdig
+    x = np.empty((r, n, dj))
+    _clib.call_oracle("gdn_integer_to_float", dict(r=r, n=n, d=dj, r_b=1, tmax=52, bases=bases, xdig=xperm, x=x))
+    return x
+
+
+def _stratified(x, b, n):
+    """The first b^k points of a column fall in distinct b-adic cells of size
+    b^-k (b^k <= n): lower-triangular LMS and per-level permutations keep the
+    Halton net's stratification."""
+    k = 0
+    while b ** (k + 1) <= n:
+        k += 1
+    cells = np.floor(x[:b ** k] * b ** k).astype(np.int64)
+    return sorted(cells.tolist()) == list(range(b ** k))
+
+
+def test_gdn_requests_match_reference_chain(small):
     plan, got = small
+    checked = 0
     for q in np.flatnonzero(plan.methods == 3):
         x_gpu, n, d, r = _req(plan, got, q)
         m = int(plan.m[q])
+        want = gdn_reference_chain(int(plan.seeds[q]), m, r, list(range(d)))
+        bad = np.argwhere(x_gpu.view(np.uint64) != want.view(np.uint64))
+        assert bad.size == 0, (q, m, d, r, bad[:5].tolist())
+        primes = B.first_primes(d)
         for l in range(r):
             for j in range(min(d, 6)):
-                b, tn, mm, S, pa, pc = B.gdn_inputs(plan.seeds[q], m, l, j)
-                want = np.empty(n)
-                for i in range(n):
-                    dig, k = [], i
-                    for _ in range(mm):
-                        dig.append(k % b)
-                        k //= b
-                    Y, scale = 0, 1.0
-                    for t in range(tn):
-                        acc = sum(int(S[t, kk]) * dig[kk] for kk in range(min(t + 1, mm)))
-                        y = (int(pa[t]) * (acc % b) + int(pc[t])) % b
-                        Y = Y * b + y
-                        scale *= float(b)
-                    want[i] = float(Y) / scale
-                assert np.array_equal(x_gpu[l, :, j], want), (q, l, j)
-                k = 0
-                while b ** (k + 1) <= n:
-                    k += 1
-                cells = np.floor(x_gpu[l, :b ** k, j] * b ** k).astype(int)
-                assert sorted(cells.tolist()) == list(range(b ** k)), (q, l, j)
+                assert _stratified(x_gpu[l, :, j], primes[j], n), (q, l, j)
+        checked += 1
+    assert checked > 0
+
+
+@pytest.mark.parametrize("cap", [1, 3])
+def test_gdn_levels_past_the_tables_match_reference_chain(torch, cap):
+    """With the tables' levels capped, most sums continue on the path that
+    computes digits from the index and permutation values from the keys; the
+    output stays the reference chain's."""
+    meth = np.full(4, 3, dtype=np.int32)
+    m = np.array([6, 9, 5, 8], dtype=np.int32)
+    d = np.array([5, 3, 40, 1], dtype=np.int32)
+    r = np.array([2, 1, 1, 3], dtype=np.int32)
+    seeds = np.array([11, 12, 13, 14], dtype=np.uint64)
+    plan = B.Batch(meth, m, d, r, seeds, points_per_tile=1 << 10, gdn_level_cap=cap)
+    ring = torch.full((plan.ring_elems,), float("nan"), dtype=torch.float64, device="cuda")
+    plan.run(ring)
+    torch.cuda.synchronize()
+    got = ring.cpu().numpy()
+    for q in range(len(m)):
+        x_gpu, n, dd, rr = _req(plan, got, q)
+        want = gdn_reference_chain(int(seeds[q]), int(m[q]), rr, list(range(dd)))
+        assert np.array_equal(x_gpu.view(np.uint64), want.view(np.uint64)), q
 
 
 def test_ring_wraps_slabs_contiguously(torch):
@@ -204,37 +257,16 @@ def owen_hash64_np(x, seed):
     return _brev64(x)
 
 
-def gdn_definition(seed, m, d, r):
-    """The gdn request on its exported inputs, every column: digits of i, LMS,
-    digit permutation, Y = sum of digits as an integer, float(Y) / b^tn."""
-    n = 1 << m
-    out = np.empty((r, n, d))
-    idx = np.arange(n, dtype=np.int64)
-    for l in range(r):
-        for j in range(d):
-            b, tn, mm, S, pa, pc = B.gdn_inputs(seed, m, l, j)
-            dig = np.empty((n, mm), dtype=np.int64)
-            k = idx.copy()
-            for i in range(mm):
-                dig[:, i] = k % b
-                k //= b
-            x = (dig @ S.astype(np.int64).T) % b
-            y = (pa.astype(np.int64) * x + pc.astype(np.int64)) % b
-            Y = np.zeros(n, dtype=np.uint64)
-            scale = 1.0
-            for t in range(tn):
-                Y = Y * np.uint64(b) + y[:, t].astype(np.uint64)
-                scale *= float(b)
-            out[l, :, j] = Y.astype(np.float64) / scale
-    return out
-
-
-def test_wide_gdn_requests_match_definition(wide):
+def test_wide_gdn_requests_match_reference_chain(wide):
+    """Columns at the kernels' shape edges (base 2, the small bases, the first
+    column of each level bucket, the last column) of every gdn request of the
+    wide set, at full n, against the reference chain."""
     plan, got = wide
     for q in np.flatnonzero(plan.methods == 3):
         x_gpu, n, d, r = _req(plan, got, q)
-        want = gdn_definition(int(plan.seeds[q]), int(plan.m[q]), d, r)
-        bad = np.argwhere(x_gpu.view(np.uint64) != want.view(np.uint64))
+        cols = sorted({j for j in (0, 1, 2, 3, 8, 9, 20, 63, 64, 121, d - 1) if j < d})
+        want = gdn_reference_chain(int(plan.seeds[q]), int(plan.m[q]), r, cols)
+        bad = np.argwhere(x_gpu[:, :, cols].view(np.uint64) != want.view(np.uint64))
         assert bad.size == 0, (q, int(plan.m[q]), d, r, bad[:5].tolist())
 
 
