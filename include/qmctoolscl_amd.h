@@ -323,6 +323,8 @@ QMC_EXPORT void qmc_batch_free(void *plan);
 QMC_EXPORT qmc_u64 qmc_batch_total_tiles(void *plan);
 QMC_EXPORT qmc_u64 qmc_batch_points(void *plan);
 QMC_EXPORT qmc_u64 qmc_batch_slab_offset(void *plan, int q, int l);
+/* CUs given to the gdn class while the store-bound class runs beside it */
+QMC_EXPORT int qmc_batch_set_gdn_cus(void *plan, int n);
 QMC_EXPORT int qmc_batch_run_dev(void *plan, double *ring, qmc_stream_t s);          /* persistent launch */
 QMC_EXPORT int qmc_batch_graph_run_dev(void *plan, double *ring, qmc_stream_t s);    /* same, hipGraph replay */
 QMC_EXPORT int qmc_batch_run_per_call_dev(void *plan, double *ring, qmc_stream_t s); /* one launch per request */

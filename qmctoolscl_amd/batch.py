@@ -48,6 +48,8 @@ def _declare(L):
     L.qmc_batch_slabs.restype = ctypes.c_uint64
     L.qmc_batch_slabs.argtypes = [_vp]
     L.qmc_batch_set_digest.argtypes = [_vp, _vp]
+    L.qmc_batch_set_gdn_cus.restype = ctypes.c_int
+    L.qmc_batch_set_gdn_cus.argtypes = [_vp, ctypes.c_int]
     L.qmc_batch_slab_offset.restype = ctypes.c_uint64
     L.qmc_batch_slab_offset.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
     for f in ("qmc_batch_run_dev", "qmc_batch_graph_run_dev", "qmc_batch_run_per_call_dev"):
@@ -133,6 +135,12 @@ class Batch:
         finally:
             _lib.check(_L().qmc_batch_set_digest(self._h, None))
         return dg[:self.slabs].cpu().numpy().view(np.uint64)
+
+    def set_gdn_cus(self, n):
+        """CUs (persistent workgroups) of the gdn class; the lattice and dnb2
+        class runs on the others beside it."""
+        _lib.check(_L().qmc_batch_set_gdn_cus(self._h, int(n)))
+        self.gdn_cus = int(n)
 
     def slab_offset(self, q, l):
         return int(_L().qmc_batch_slab_offset(self._h, int(q), int(l)))

@@ -60,7 +60,11 @@ struct Req {
     u64 slabs;           // word offset of r slab bases (output element offsets)
     u32 clog;            // dnb2: log2 of the row stride between a chain's points
     u32 gcap;            // gdn: most permutation levels a tile reads from its tables
+    u32 gng;             // gdn: column groups (tiles per row range)
+    u32 gslot;           // gdn: LDS bytes per column of a tile (the plan's largest)
+    u64 gblk;            // gdn: word offset of the replication blocks in the request's tables
     u64 lw;              // gdn: words of one replication's column tables
+    DivU32 gdiv;         // gdn: division by gng
     u32 slab0;           // index of the request's first replication among all slabs (digests)
     DivU32 ddiv;         // division by d
 };
@@ -201,8 +205,9 @@ __host__ __device__ inline u32 gdn_perm_at(u64 seed, u64 l, u64 j, u32 t, u32 b,
 // adding once all its lanes' sums pass thr_t; lanes already past it would only
 // have added terms that leave them unchanged.
 //
-// Levels kept in LDS per column: TB, the smallest of 8, 12, 16, 24, 32, 48 at
-// or above the first t with b^t >= 2^74.  A point needs a level t >= TB only
+// Levels kept in LDS per column: TB, the smallest of 8, 10, 12, 16, 24, 48 at
+// or above the first t with b^t >= 2^74 (one kernel instance per count; the
+// exact counts, ten instances, were no faster).  A point needs a level t >= TB only
 // if its sum stays below thr_TB <= 2^55 b^-TB <= 2^-19 (a chance of about
 // 2^-19 for these uniformly spread points); such levels are computed from the
 // point's index and the permutation's keys (gdn_tail).  Base 2 keeps all 52
@@ -212,7 +217,7 @@ __host__ __device__ inline int gdn_levels(u32 b) {
     int t = 0;
     double v = 1.0;
     while (v < 0x1p74) { v *= (double)b; t++; }
-    return t <= 8 ? 8 : t <= 12 ? 12 : t <= 16 ? 16 : t <= 24 ? 24 : t <= 32 ? 32 : 48;
+    return t <= 8 ? 8 : t <= 10 ? 10 : t <= 12 ? 12 : t <= 16 ? 16 : t <= 24 ? 24 : 48;
 }
 
 // u16 entries per LDS row of a permutation table: TB rounded up so that a row
@@ -256,27 +261,49 @@ __device__ __forceinline__ u32 fast_div(u32 x, u64 M) { return (u32)__umul64hi(M
 //           pre[32] (prefix XOR of the columns) then cols[32] | shift or NUS seed[r*d]
 // gdn     : info[d] | r replication blocks of lw words, column j's block at
 //           word colo_j of each:
-//             odd b: u16 perm[b][TB] (entry z*TB + t = pi_t(z)), then
-//                    u16 Q[mm][TB] with Q_c[t] = sum_{i <= c} S[t][i] mod b
+//             odd b: u16 perm[b][TE] (entry z*TE + t = pi_t(z), TE = TB
+//                    rounded up to even), then u16 Q[mm][TE] with
+//                    Q_c[t] = sum_{i <= c} S[t][i] mod b
 //                    (going from point k to k + 1 turns the c trailing digits
 //                    equal to b - 1 into 0 and adds 1 to digit c, so every
 //                    z_t moves by Q_c[t], as -(b - 1) = 1 mod b)
 //             b = 2: u64 cmask (bit 51 - t = pi_t(0)), Q_0 .. Q_{m-1} (bits
 //                    51 - t, mod 2)
-//           info_j = colo_j | TB << 32 | mm << 40 | tmin << 48
+//           info_j = colo_j | TB << 32 | mm << 40 | tmin << 48;
+//           the info words are followed by the column groups of the tiles
+//           (j0 | ncols << 16: up to kGdnCols adjacent columns with one TB),
+//           then the replication blocks
+
+// column groups of a gdn request: runs of up to 4 adjacent columns with the
+// same levels in LDS (j0, ncols)
+inline std::vector<std::pair<u32, u32>> gdn_groups(int d) {
+    std::vector<std::pair<u32, u32>> g;
+    for (int j = 0; j < d; j++) {
+        if (!g.empty() && g.back().second < 4 && j > 1 &&
+            gdn_levels(h_primes[j]) == gdn_levels(h_primes[g.back().first]))
+            g.back().second++;
+        else
+            g.push_back({(u32)j, 1u});
+    }
+    return g;
+}
+
+// u16 entries per row of a column's tables in the workspace
+__host__ __device__ inline u32 gdn_row_even(u32 TB) { return (TB + 1) & ~1u; }
 
 inline u64 gdn_col_words(u32 b, int m) {
     if (b == 2) return 1 + (u64)m;
-    const u64 TB = (u64)gdn_levels(b), mm = (u64)gdn_mm(b, m);
-    return ((b + mm) * TB * 2 + 7) / 8;
+    const u64 TE = gdn_row_even((u32)gdn_levels(b)), mm = (u64)gdn_mm(b, m);
+    return ((b + mm) * TE * 2 + 7) / 8;
 }
 
-// LDS bytes of a gdn tile of column j: the permutation rows, then the walk
-// table twice (scaled to row offsets, and plain)
+// LDS bytes of a gdn tile of column j: the permutation rows, the walk table
+// twice (scaled to row offsets, and plain), the reciprocals
 inline u32 gdn_col_lds(u32 b, int m) {
     if (b == 2) return 8 * (1 + (u32)m);
     const u32 TB = (u32)gdn_levels(b), mm = (u32)gdn_mm(b, m);
-    return ((b * gdn_row_u16(TB) * 2 + 15) & ~15u) + 8 * mm * TB;
+    const u32 TP = (TB + 1) / 2;
+    return ((b * gdn_row_u16(TB) * 2 + 15) & ~15u) + 4 * mm * TP + 8 * mm * TP + 8 * (TB + 2);
 }
 
 u64 table_words(Req &q) {
@@ -289,7 +316,9 @@ u64 table_words(Req &q) {
             u64 w = 0;
             for (u64 j = 0; j < d; j++) w += gdn_col_words(h_primes[j], q.m);
             q.lw = w;
-            return d + r * w;
+            q.gng = (u32)gdn_groups((int)d).size();
+            q.gblk = d + q.gng;
+            return q.gblk + r * w;
         }
     }
 }
@@ -363,7 +392,7 @@ __global__ __launch_bounds__(kBlock) void batch_gdn_setup_kernel(const Req *__re
     u64 *tab = ws + q.tab;
     const u64 info = tab[j];
     const u32 TB = (u32)((info >> 32) & 0xff), mm = (u32)((info >> 40) & 0xff);
-    u64 *blk = tab + d + l * q.lw + (u32)info;
+    u64 *blk = tab + q.gblk + l * q.lw + (u32)info;
     const u32 b = c_primes[j];
     if (b == 2) {
         if (threadIdx.x == 0) {
@@ -383,8 +412,9 @@ __global__ __launch_bounds__(kBlock) void batch_gdn_setup_kernel(const Req *__re
         }
         return;
     }
+    const u32 TE = gdn_row_even(TB);
     unsigned short *P = (unsigned short *)blk;
-    unsigned short *Qt = P + (u64)b * TB;
+    unsigned short *Qt = P + (u64)b * TE;
     u32 N = 1;
     while (N < b) N *= 2;
     for (u32 t = 0; t < TB; t++) {
@@ -403,15 +433,18 @@ __global__ __launch_bounds__(kBlock) void batch_gdn_setup_kernel(const Req *__re
                 __syncthreads();
             }
         for (u32 pos = threadIdx.x; pos < b; pos += blockDim.x)
-            P[(u64)(keys[pos] & 0x7ff) * TB + t] = (unsigned short)pos;
+            P[(u64)(keys[pos] & 0x7ff) * TE + t] = (unsigned short)pos;
         __syncthreads();
     }
-    for (u32 e = threadIdx.x; e < mm * TB; e += blockDim.x) {
-        const u32 cc = e / TB, t = e - cc * TB;
+    for (u32 e = threadIdx.x; e < mm * TE; e += blockDim.x) {
+        const u32 cc = e / TE, t = e - cc * TE;
         u32 run = 0;
-        for (u32 i = 0; i <= cc && i <= t; i++) run = (run + gdn_s(q.seed, l, j, t, i, b)) % b;
+        if (t < TB)
+            for (u32 i = 0; i <= cc && i <= t; i++) run = (run + gdn_s(q.seed, l, j, t, i, b)) % b;
         Qt[e] = (unsigned short)run;
     }
+    if (TE > TB)  // the padding level of the permutation rows
+        for (u32 z = threadIdx.x; z < b; z += blockDim.x) P[(u64)z * TE + TB] = 0;
 }
 
 // ------------------------------------------------------------- one tile
@@ -424,6 +457,7 @@ __global__ __launch_bounds__(kBlock) void batch_gdn_setup_kernel(const Req *__re
 // store-bound class).
 template <bool DIG, bool STREAM>
 struct Sink {
+    static constexpr bool kDig = DIG;
     u64 acc = 0;
     __device__ __forceinline__ void put(double *p, double v) {
         if (STREAM)
@@ -546,35 +580,53 @@ __device__ __forceinline__ void dnb2_tile(const Req &q, u64 l, u64 row0, const u
     }
 }
 
-// One gdn column of base 2 (j = 0): H consecutive points from k, row stride d.
-// Q = cmask, Q_0 .. Q_{m-1}.  The permuted digits of point k are cmask ^ (XOR
-// of S_i = Q_i ^ Q_{i-1} over the set bits i of k), and each step k -> k + 1
-// adds Q_c, c = ctz(k + 1).  In base 2 every term rec_t y_t is 0 or 2^(-t-1)
-// and every partial sum of the reference's loop is a multiple of 2^-52 below
-// 1, so all of them are exact and x = Y 2^-52 for the 52-bit word Y.
-template <class SinkT>
-__device__ __forceinline__ void gdn_walk2(const u64 *__restrict__ Q, u32 k, u32 H, u32 d, double *dst,
-                                          SinkT &out) {
-    u64 z = Q[0];
-    for (u32 kk = k; kk; kk &= kk - 1) {
-        const int i = __ffs(kk) - 1;
-        z ^= Q[1 + i] ^ (i ? Q[i] : 0ull);
+// A gdn tile is kGdnCols adjacent columns (with the same number of levels in
+// LDS) times RT rows; each column has kGdnLanes lanes of the workgroup, lane i
+// walking rows i H .. i H + H - 1.  After every step the columns' values meet
+// in LDS and are stored row by row, so one store instruction writes 16 rows x
+// 32 bytes instead of 64 rows x 8 bytes (the L2 takes one write request per
+// line an instruction touches; with one column per instruction those requests
+// were the gdn class's bound).
+constexpr int kGdnCols = 4;
+#ifndef QMC_GDN_LANES
+#define QMC_GDN_LANES 256
+#endif
+constexpr int kGdnLanes = QMC_GDN_LANES;        // lanes (row chains) per column
+constexpr int kBlock1 = kGdnCols * kGdnLanes;   // threads of a gdn workgroup
+
+// Base 2 (j = 0).  Q = cmask, Q_0 .. Q_{m-1}.  The permuted digits of point k
+// are cmask ^ (XOR of S_i = Q_i ^ Q_{i-1} over the set bits i of k), and a
+// step k -> k + 1 adds Q_c, c = ctz(k + 1).  In base 2 every term rec_t y_t
+// is 0 or 2^(-t-1) and every partial sum of the reference's loop is a
+// multiple of 2^-52 below 1, so all of them are exact and x = Y 2^-52 for the
+// 52-bit word Y.
+struct Walk2 {
+    const u64 *Q;
+    u64 z;
+    u32 k;
+    __device__ __forceinline__ void init(const u64 *Q_, u32 k0) {
+        Q = Q_;
+        k = k0;
+        z = Q[0];
+        for (u32 kk = k0; kk; kk &= kk - 1) {
+            const int i = __ffs(kk) - 1;
+            z ^= Q[1 + i] ^ (i ? Q[i] : 0ull);
+        }
     }
-    for (u32 h = 0;; h++) {
-        out.put(dst, (double)z * 0x1p-52);
-        if (h + 1 == H) break;
+    __device__ __forceinline__ double value() const { return (double)z * 0x1p-52; }
+    __device__ __forceinline__ void step() {
         z ^= Q[__ffs(k + 1)];  // Q_c at Q[1 + c]
         k++;
-        dst += d;
     }
-}
+};
 
-// A column of odd base b, as one lane walks it.
+// A column of odd base b, as its lanes walk it.
 struct GdnCol {
     const double *rec, *thr;  // the base's constants (kTNmax each)
-    const u32 *Qs, *Qu;       // walk table in LDS: scaled to permutation-row offsets, and plain
+    const u32 *Qs, *Qu;       // walk table in LDS: packed pairs of row offsets, and plain
+    const double *srec;       // the reciprocals of the walked levels in LDS (read as broadcasts)
     u64 seed, l, j, M;
-    u32 b, mm, ts, tmin, rowb;
+    u32 b, mm, rowb, base;    // base: LDS byte address of the permutation rows
 };
 
 // u16 at byte address a of the dynamic LDS, which starts at address 0 in the
@@ -608,76 +660,77 @@ __device__ __noinline__ double gdn_tail(GdnCol c, u32 k, u32 t, double v) {
     return v;
 }
 
-// H consecutive points from k of a column of odd base b, TB levels in LDS.
-// Each lane keeps the digits z_t of its point as byte offsets z_t * rowb of
-// the permutation rows, so pi_t(z_t) is one u16 load with t in the immediate
-// offset, and a walk step adds a scaled table row modulo b * rowb.
-template <int TB, class SinkT>
-__device__ __forceinline__ void gdn_walk_odd(const GdnCol &c, u32 k, u32 H, u32 d, double *dst, SinkT &out) {
-    const u32 b = c.b, rowb = c.rowb;
-    const u64 M = c.M;
-    double rr[TB], th[TB];
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ u16x2 as_pair(u32 w) { return __builtin_bit_cast(u16x2, w); }
+__device__ __forceinline__ u32 as_word(u16x2 p) { return __builtin_bit_cast(u32, p); }
+
+// Consecutive points of a column of odd base b, the first TW levels from LDS.
+// A lane keeps the digits z_t of its point as byte offsets z_t * rowb into the
+// permutation rows, two levels per register (u16 halves: b * rowb < 2^15,
+// host-checked), so pi_t(z_t) is one u16 load with t in the immediate offset,
+// and a step adds a table row of packed offsets modulo b * rowb.  Every point
+// adds all TW levels: a lane whose sum has passed a level's threshold adds
+// terms that leave it unchanged (gdn_levels), so only the check at level TW,
+// once per point, decides whether the rare tail runs.
+template <int TW>
+struct WalkOdd {
+    static constexpr int TP = (TW + 1) / 2;  // level pairs (the last half is padding when TW is odd)
+    u32 z[TP];
+    u32 k, dlow;
+    __device__ __forceinline__ void init(const GdnCol &c, u32 k0) {
+        const u32 b = c.b;
+        const u64 M = c.M;
+        k = k0;
+        // z_t = sum_i (dig_i - dig_{i+1}) Q_i[t] mod b (dig_mm = 0; Abel
+        // summation of S[t][i] = Q_i[t] - Q_{i-1}[t])
+        u32 acc[2 * TP];
 #pragma unroll
-    for (int t = 0; t < TB; t++) {
-        rr[t] = uniform_f64(c.rec + t);
-        th[t] = uniform_f64(c.thr + t);
-    }
-    // first point: z_t = sum_i (dig_i - dig_{i+1}) Q_i[t] mod b (dig_mm = 0;
-    // Abel summation of S[t][i] = Q_i[t] - Q_{i-1}[t])
-    u32 z[TB];
-    {
-        u32 acc[TB];
-#pragma unroll
-        for (int t = 0; t < TB; t++) acc[t] = 0;
-        u32 qd = fast_div(k, M);
-        u32 dig = k - qd * b;
+        for (int t = 0; t < 2 * TP; t++) acc[t] = 0;
+        u32 qd = fast_div(k0, M);
+        u32 dig = k0 - qd * b;
         for (u32 i = 0; i < c.mm; i++) {
             const u32 qn = fast_div(qd, M);
             const u32 dn = qd - qn * b;
             const u32 e = dig >= dn ? dig - dn : dig + b - dn;
-            const u32 *Qi = c.Qu + i * TB;
+            const u32 *Qi = c.Qu + i * 2 * TP;
 #pragma unroll
-            for (int t = 0; t < TB; t++) acc[t] += e * Qi[t];  // < mm b^2 < 2^32
+            for (int t = 0; t < 2 * TP; t++) acc[t] += e * Qi[t];  // < mm b^2 < 2^32
             qd = qn;
             dig = dn;
         }
 #pragma unroll
-        for (int t = 0; t < TB; t++) z[t] = (acc[t] - fast_div(acc[t], M) * b) * rowb;
+        for (int p = 0; p < TP; p++)
+            z[p] = (acc[2 * p] - fast_div(acc[2 * p], M) * b) * c.rowb |
+                   ((acc[2 * p + 1] - fast_div(acc[2 * p + 1], M) * b) * c.rowb) << 16;
+        dlow = k0 - fast_div(k0, M) * b;
     }
-    u32 dlow = k - fast_div(k, M) * b;
-    const u32 bR = b * rowb;
-    const int ts = (int)c.ts, tmin = (int)c.tmin;
-    for (u32 h = 0;; h++) {
-        // levels in LDS while the wave has a lane below the level's threshold
-        // (a flag rather than a break keeps the loop fully unrolled)
-        double v = 0.0;
-        int t = 0;
-        bool go = true, fin = false;
+    __device__ __forceinline__ double value(const GdnCol &c, double thw) const {
+        // gdn_integer_to_float.c:36-37; v starts as the first term (0.0 + p =
+        // p).  The reciprocals are read from LDS on every point (as
+        // broadcasts) rather than held in registers: the pointer passes
+        // through opaque().
+        const double *sr = c.srec + opaque(0);
+        const u32 base = c.base;
+        double v = sr[0] * (double)lds_u16((z[0] & 0xffffu) + base);
+        v += sr[1] * (double)lds_u16((z[0] >> 16) + base + 2);
 #pragma unroll
-        for (int tt = 0; tt < TB; tt++) {
-            if (go) {
-                if (tt >= ts) {
-                    go = false;
-                } else if (tt >= tmin && __builtin_amdgcn_ballot_w64(v < th[tt]) == 0) {
-                    go = false;
-                    fin = true;
-                } else {
-                    v += rr[tt] * (double)lds_u16(z[tt] + 2 * tt);  // gdn_integer_to_float.c:36-37
-                    t = tt + 1;
-                }
-            }
+        for (int p = 1; p < TP; p++) {
+            v += sr[2 * p] * (double)lds_u16((z[p] & 0xffffu) + base + 4 * p);
+            if (2 * p + 1 < TW) v += sr[2 * p + 1] * (double)lds_u16((z[p] >> 16) + base + 4 * p + 2);
         }
-        if (!fin && t < kTNmax) v = gdn_tail(c, k, (u32)t, v);
-        out.put(dst, v);
-        if (h + 1 == H) break;
-        // carries of k -> k+1: c trailing digits equal to b-1
+        if (TW < kTNmax && __builtin_amdgcn_ballot_w64(v < thw) != 0) v = gdn_tail(c, k, TW, v);
+        return v;
+    }
+    __device__ __forceinline__ void step(const GdnCol &c) {
+        const u32 b = c.b;
+        // carries of k -> k+1: cc trailing digits equal to b-1
         u32 cc = 0;
         if (dlow == b - 1) {
             dlow = 0;
             cc = 1;
-            u32 u = fast_div(k, M);
+            u32 u = fast_div(k, c.M);
             for (;;) {
-                const u32 un = fast_div(u, M);
+                const u32 un = fast_div(u, c.M);
                 if (u - un * b != b - 1) break;
                 cc++;
                 u = un;
@@ -686,37 +739,58 @@ __device__ __forceinline__ void gdn_walk_odd(const GdnCol &c, u32 k, u32 H, u32 
             dlow++;
         }
         k++;
-        const u32 *Qc = c.Qs + cc * TB;
-        u32 inc[TB];
+        const u32 *Qc = c.Qs + cc * TP;
+        u32 inc[TP];
 #pragma unroll
-        for (int t = 0; t < TB; t++) inc[t] = Qc[t];
+        for (int p = 0; p < TP; p++) inc[p] = Qc[p];
+        const u16x2 bR = as_pair((b * c.rowb) * 0x10001u);
 #pragma unroll
-        for (int t = 0; t < TB; t++) {
-            const u32 s2 = z[t] + inc[t];
-            z[t] = __builtin_elementwise_min(s2, s2 - bR);
+        for (int p = 0; p < TP; p++) {
+            const u16x2 s2 = as_pair(z[p]) + as_pair(inc[p]);  // < 2 b rowb < 2^16
+            z[p] = as_word(__builtin_elementwise_min(s2, s2 - bR));
         }
-        dst += d;
     }
-}
+};
 
-// LDS fill of an odd-base column: permutation rows (pairs of levels as u32
-// words; TB and the row length are even) and the walk table twice
-template <int TB>
-__device__ __forceinline__ void gdn_fill_odd(const u64 *__restrict__ blk, u32 b, u32 mm, u32 rowu,
-                                             unsigned char *lds, u32 *Qs, u32 *Qu) {
+// LDS fill of an odd-base column for TW walked levels out of TB stored, by the
+// column's kGdnLanes threads (t): the permutation rows (level pairs as u32
+// words; TW, TB and the row length are even), the walk table as packed pairs
+// of row offsets and plain, and the reciprocals.  Eight loads in flight per
+// thread before their LDS stores.
+template <int TW>
+__device__ __forceinline__ void gdn_fill_odd(u32 t, const u64 *__restrict__ blk, u32 b, u32 mm, u32 TB, u32 rowu,
+                                             unsigned char *Pl8, u32 *Qs, u32 *Qu, const double *__restrict__ rec,
+                                             double *srec) {
     const u32 *Pg = (const u32 *)blk;
-    u32 *Pl = (u32 *)lds;
-    constexpr u32 TW = TB / 2;
-    for (u32 e = threadIdx.x; e < b * TW; e += blockDim.x) {
-        const u32 z = e / TW;
-        Pl[z * (rowu / 2) + (e - z * TW)] = Pg[e];
+    u32 *Pl = (u32 *)Pl8;
+    constexpr u32 TP = (TW + 1) / 2;
+    const u32 TE = gdn_row_even(TB), TBp = TE / 2;
+    constexpr u32 kF = 8;
+    const u32 nw = b * TP;
+    for (u32 e0 = 0; e0 < nw; e0 += kF * kGdnLanes) {
+        u32 w[kF];
+#pragma unroll
+        for (u32 f = 0; f < kF; f++) {
+            const u32 e = e0 + f * kGdnLanes + t;
+            const u32 z = e / TP, tp = e - z * TP;
+            w[f] = e < nw ? Pg[z * TBp + tp] : 0u;
+        }
+#pragma unroll
+        for (u32 f = 0; f < kF; f++) {
+            const u32 e = e0 + f * kGdnLanes + t;
+            const u32 z = e / TP, tp = e - z * TP;
+            if (e < nw) Pl[z * (rowu / 2) + tp] = w[f];
+        }
     }
-    const unsigned short *Qg = (const unsigned short *)blk + (u64)b * TB;
-    for (u32 e = threadIdx.x; e < mm * TB; e += blockDim.x) {
-        const u32 v = Qg[e];
-        Qs[e] = v * 2 * rowu;
-        Qu[e] = v;
+    const unsigned short *Qg = (const unsigned short *)blk + (u64)b * TE;
+    for (u32 e = t; e < mm * TP; e += kGdnLanes) {
+        const u32 cc = e / TP, tp = e - cc * TP;
+        const u32 lo = Qg[cc * TE + 2 * tp], hi = 2 * tp + 1 < TW ? Qg[cc * TE + 2 * tp + 1] : 0u;
+        Qs[e] = (lo * 2 * rowu) | ((hi * 2 * rowu) << 16);
+        Qu[2 * e] = lo;
+        Qu[2 * e + 1] = hi;
     }
+    if (t < TW) srec[t] = rec[t];
 }
 
 // Tiles are drained by two persistent kernels: class 0 = lattice and dnb2
@@ -724,63 +798,109 @@ __device__ __forceinline__ void gdn_fill_odd(const u64 *__restrict__ blk, u32 b,
 // and the sequential sums).  Each class is compiled without the other's code.
 __host__ __device__ constexpr int method_class(int method) { return method == M_GDN_LMS ? 1 : 0; }
 
-// A gdn tile: rows [chunk RT, (chunk + 1) RT) of column j of replication l,
-// rem = chunk * d + j (the columns of one row range are consecutive tiles).
-// Lane i walks rows i H .. i H + H - 1 of the tile.
+// The step loop of a gdn tile: every lane of an active column computes its
+// point, the values meet in LDS, and the workgroup stores them row by row
+// (thread e: row e / 4 of the step, column e % 4).  H barriers for every
+// workgroup, whatever its columns.
+template <class W, class SinkT, class Value, class Step>
+__device__ __forceinline__ void gdn_steps(W &w, bool act, u32 H, u32 i, u32 cg, u32 ncols, u32 rows, u64 row0,
+                                          u32 d, u32 j0, double *__restrict__ slab, double *stage, SinkT &out,
+                                          Value value, Step step) {
+    const u32 rr = threadIdx.x >> 2, cc = threadIdx.x & 3;
+    const bool st = cc < ncols && rr < rows;
+    double *dst = slab + (row0 + (u64)rr * H) * d + j0 + cc;
+    for (u32 h = 0; h < H; h++) {
+        double *sb = stage + (h & 1) * kBlock1;
+        if (act) sb[cg * kGdnLanes + i] = value(w);
+        __syncthreads();
+        if (st) out.put(dst, sb[cc * kGdnLanes + rr]);
+        dst += d;
+        if (act && h + 1 < H) step(w);
+    }
+}
+
+// A gdn tile: rows [chunk RT, (chunk + 1) RT) of the column group g of
+// replication l, rem = chunk * G + g (the column groups of one row range are
+// consecutive tiles).  The group's columns j0 .. j0 + ncols - 1 share their
+// level count; workgroup threads [c kGdnLanes, (c + 1) kGdnLanes) take column
+// j0 + c.
 template <class SinkT>
 __device__ __forceinline__ void gdn_tile(const Req &q, u64 l, u32 rem, const u64 *__restrict__ ws,
                                          const u64 *__restrict__ tab, double *__restrict__ slab,
                                          unsigned char *lds, SinkT &out) {
     const u32 d = (u32)q.d;
-    const u32 chunk = div_q(rem, q.ddiv);
-    const u32 j = __builtin_amdgcn_readfirstlane(rem - chunk * d);
+    const u32 chunk = div_q(rem, q.gdiv);
+    const u32 g = rem - chunk * q.gng;
+    const u64 gw = tab[d + g];
+    const u32 j0 = __builtin_amdgcn_readfirstlane((u32)(gw & 0xffff));
+    const u32 ncols = __builtin_amdgcn_readfirstlane((u32)((gw >> 16) & 0xff));
+    const u32 cg = __builtin_amdgcn_readfirstlane(threadIdx.x / kGdnLanes);
+    const u32 i = threadIdx.x - cg * kGdnLanes;
+    const bool colact = cg < ncols;
+    const u32 j = j0 + (colact ? cg : 0u);
     const u32 RT = q.rows_per_tile;
     const u64 info = tab[j];
     const u32 colo = __builtin_amdgcn_readfirstlane((u32)info);
     const u32 TB = __builtin_amdgcn_readfirstlane((u32)((info >> 32) & 0xff));
     const u32 mm = __builtin_amdgcn_readfirstlane((u32)((info >> 40) & 0xff));
-    const u32 tmin = __builtin_amdgcn_readfirstlane((u32)((info >> 48) & 0xff));
-    const u64 *blk = tab + q.d + l * q.lw + colo;
+    const u64 *blk = tab + q.gblk + l * q.lw + colo;
     const u32 b = c_primes[j];
-    const u32 H = RT >= (u32)kBlock ? RT / (u32)kBlock : 1u;
-    const u32 i0 = threadIdx.x * H;
-    const u32 k = chunk * RT + i0;
-    double *dst = slab + (u64)k * d + j;
-    if (b == 2) {
-        u64 *Q = (u64 *)lds;
-        for (u32 e = threadIdx.x; e <= (u32)q.m; e += blockDim.x) Q[e] = blk[e];
+    const u32 H = RT >= (u32)kGdnLanes ? RT / (u32)kGdnLanes : 1u;
+    const u32 rows = RT >= (u32)kGdnLanes ? (u32)kGdnLanes : RT;  // lanes with points
+    const bool act = colact && i < rows;
+    const u64 row0 = (u64)chunk * RT;
+    const u32 k = (u32)row0 + i * H;
+    unsigned char *slot = lds + cg * q.gslot;
+    double *stage = (double *)(lds + kGdnCols * q.gslot);
+    if (b == 2) {  // column 0, alone in its group
+        u64 *Q = (u64 *)slot;
+        if (colact)
+            for (u32 e = i; e <= (u32)q.m; e += kGdnLanes) Q[e] = blk[e];
         __syncthreads();
-        if (i0 < RT) gdn_walk2(Q, k, H, d, dst, out);
+        Walk2 w;
+        if (act) w.init(Q, k);
+        gdn_steps(w, act, H, i, cg, ncols, rows, row0, d, j0, slab, stage, out,
+                  [](Walk2 &x) { return x.value(); }, [](Walk2 &x) { x.step(); });
         return;
     }
-    const u32 rowu = gdn_row_u16(TB);
-    u32 *Qs = (u32 *)(lds + ((b * rowu * 2 + 15) & ~15u));
-    u32 *Qu = Qs + mm * TB;
+    // levels walked: the stored TB, or a smaller bucket under the level cap
+    const u32 TW = q.gcap >= TB ? TB : q.gcap >= 24 ? 24 : q.gcap >= 16 ? 16 : q.gcap >= 12 ? 12 : q.gcap >= 10 ? 10 : 8;
+    const u32 rowu = gdn_row_u16(TW);
+    const u32 TP = (TW + 1) / 2;
+    // byte offsets in the slot (pointer arithmetic keeps the LDS address space)
+    const u32 qoff = (b * rowu * 2 + 15) & ~15u;
+    u32 *Qs = (u32 *)(slot + qoff);
+    u32 *Qu = Qs + mm * TP;
+    double *srec = (double *)(slot + ((qoff + 12 * mm * TP + 7) & ~7u));
     GdnCol c;
     c.rec = (const double *)ws + (u64)j * kTNmax;
     c.thr = (const double *)ws + (u64)(256 + j) * kTNmax;
     c.Qs = Qs;
     c.Qu = Qu;
+    c.srec = srec;
     c.seed = q.seed;
     c.l = l;
     c.j = j;
     c.M = recip_magic(b);
     c.b = b;
     c.mm = mm;
-    c.ts = TB < q.gcap ? TB : q.gcap;
-    c.tmin = tmin;
     c.rowb = 2 * rowu;
-#define QMC_GDN_TB(T_)                                   \
-    case T_:                                             \
-        gdn_fill_odd<T_>(blk, b, mm, rowu, lds, Qs, Qu); \
-        __syncthreads();                                 \
-        if (i0 < RT) gdn_walk_odd<T_>(c, k, H, d, dst, out); \
-        break;
-    switch (TB) {
-        QMC_GDN_TB(8) QMC_GDN_TB(12) QMC_GDN_TB(16) QMC_GDN_TB(24) QMC_GDN_TB(32) QMC_GDN_TB(48)
+    c.base = (u32)(uintptr_t)(__attribute__((address_space(3))) unsigned char *)slot;
+    const double thw = uniform_f64(c.thr + (TW < (u32)kTNmax ? TW : 0u));
+#define QMC_GDN_TW(T_)                                                                                      \
+    case T_: {                                                                                              \
+        if (colact) gdn_fill_odd<T_>(i, blk, b, mm, TB, rowu, slot, Qs, Qu, c.rec, srec);                  \
+        __syncthreads();                                                                                    \
+        WalkOdd<T_> w;                                                                                      \
+        if (act) w.init(c, k);                                                                              \
+        gdn_steps(w, act, H, i, cg, ncols, rows, row0, d, j0, slab, stage, out,                            \
+                  [&](WalkOdd<T_> &x) { return x.value(c, thw); }, [&](WalkOdd<T_> &x) { x.step(c); });   \
+    } break;
+    switch (TW) {
+        QMC_GDN_TW(8) QMC_GDN_TW(10) QMC_GDN_TW(12) QMC_GDN_TW(16) QMC_GDN_TW(24) QMC_GDN_TW(48)
         default: break;  // never: gdn_levels gives one of these
     }
-#undef QMC_GDN_TB
+#undef QMC_GDN_TW
 }
 
 template <int CLASS, bool DIG>
@@ -822,18 +942,51 @@ __device__ void run_tile(const Req &q, u64 t, const u64 *__restrict__ ws, double
 // Persistent workgroups take tiles from a counter until it passes the class's
 // tile count.  The queue slots sit at byte `slot` of the dynamic LDS, after the
 // tile's tables (the kernels declare no static LDS).
+//
+// The gdn class (CLASS 1) takes its tiles from eight queues, one per XCD: its
+// tiles come in groups (request, l, row range) of the d column tiles that
+// write the same rows, each group on one XCD's queue, columns in order.  A
+// workgroup drains the queue of the XCD it runs on (the hardware XCC_ID
+// register), so the column tiles that write the 8-byte pieces of a 128-byte
+// line run at the same time in the same L2 and the line leaves it whole; then
+// it helps with the other queues.  xq = [group offsets per XCD (9) | tiles per
+// XCD (8) | per group: tiles before it in its XCD's queue | per group: its
+// first tile].
 template <int CLASS, bool DIG>
-__global__ __launch_bounds__(kBlock) void batch_persistent_kernel(const Req *__restrict__ reqs, int nreq,
+__global__ __launch_bounds__(CLASS == 1 ? kBlock1 : kBlock) void batch_persistent_kernel(const Req *__restrict__ reqs, int nreq,
                                                                   u64 total_tiles, unsigned long long *counter,
+                                                                  const u64 *__restrict__ xq,
                                                                   const u64 *__restrict__ ws,
                                                                   double *__restrict__ out,
                                                                   unsigned long long *digest, u32 slot) {
     extern __shared__ __align__(16) unsigned char lds_dyn[];
     u64 *s_tile = (u64 *)(lds_dyn + slot);
     int *s_req = (int *)(lds_dyn + slot + 8);
+    u32 qx = 0;  // queues this workgroup found empty (thread 0)
+    const u32 x0 = CLASS == 1 ? (__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 7u) : 0u;  // HW_REG_XCC_ID
     for (;;) {
         if (threadIdx.x == 0) {
-            const u64 t = atomicAdd(counter, 1ull);
+            u64 t = total_tiles;
+            if (CLASS == 0) {
+                t = atomicAdd(counter, 1ull);
+            } else {
+                const u64 G = xq[8];
+                while (qx < 8) {
+                    const u32 x = (x0 + qx) & 7u;
+                    const u64 u = atomicAdd(counter + x, 1ull);
+                    if (u < xq[9 + x]) {
+                        u64 lo = xq[x], hi = xq[x + 1] - 1;  // last group of queue x with tiles-before <= u
+                        while (lo < hi) {
+                            const u64 mid = (lo + hi + 1) / 2;
+                            if (xq[17 + mid] <= u) lo = mid;
+                            else hi = mid - 1;
+                        }
+                        t = xq[17 + G + lo] + (u - xq[17 + lo]);
+                        break;
+                    }
+                    qx++;
+                }
+            }
             *s_tile = t;
             int lo = 0, hi = nreq - 1;  // last request with tile0 <= t
             if (t < total_tiles) {
@@ -855,14 +1008,15 @@ __global__ __launch_bounds__(kBlock) void batch_persistent_kernel(const Req *__r
     }
 }
 
+// the tile counters of both classes (1 + 8 queues)
+constexpr int kCounters = 9;
 __global__ void reset_counter_kernel(unsigned long long *counter) {
-    counter[0] = 0;
-    counter[1] = 0;
+    if (threadIdx.x < kCounters) counter[threadIdx.x] = 0;
 }
 
 // one request, one tile per block (the per-call form)
 template <int CLASS, bool DIG>
-__global__ __launch_bounds__(kBlock) void batch_single_kernel(const Req *__restrict__ req, const u64 *__restrict__ ws,
+__global__ __launch_bounds__(CLASS == 1 ? kBlock1 : kBlock) void batch_single_kernel(const Req *__restrict__ req, const u64 *__restrict__ ws,
                                                               double *__restrict__ out, unsigned long long *digest) {
     extern __shared__ __align__(16) unsigned char lds_dyn[];
     const Req q = *req;
@@ -912,7 +1066,10 @@ struct Plan {
     Req *d_reqs = nullptr;
     u64 *d_ws = nullptr;
     unsigned long long *d_counter = nullptr;
-    int grid = 0;
+    u64 *d_xq = nullptr;             // the gdn class's per-XCD queues (see batch_persistent_kernel)
+    int grid = 0;                    // persistent workgroups of the lattice and dnb2 class
+    int grid1 = 0;                   // persistent workgroups of the gdn class (kBlock1 threads each)
+    int cus = 0;
     int device = 0;
     u32 lds_bytes[2] = {kLdsClass0, 0};  // tile tables per class; the queue slots follow them
     hipStream_t cap = nullptr;       // private stream for graph capture
@@ -931,9 +1088,10 @@ inline u32 lds_total(u32 bytes) { return ((bytes + 15) & ~15u) + 16; }
 template <int CLASS, bool DIG>
 void launch_persistent(Plan *P, double *ring, hipStream_t s) {
     const u32 slot = (P->lds_bytes[CLASS] + 15) & ~15u;
-    hipLaunchKernelGGL((batch_persistent_kernel<CLASS, DIG>), dim3(P->grid), dim3(kBlock), lds_total(P->lds_bytes[CLASS]),
+    hipLaunchKernelGGL((batch_persistent_kernel<CLASS, DIG>), dim3(CLASS ? P->grid1 : P->grid),
+                       dim3(CLASS ? kBlock1 : kBlock), lds_total(P->lds_bytes[CLASS]),
                        s, P->d_qreqs + (CLASS ? P->qn[0] : 0), P->qn[CLASS], P->qtiles[CLASS], P->d_counter + CLASS,
-                       P->d_ws, ring, P->digest, slot);
+                       P->d_xq, P->d_ws, ring, P->digest, slot);
 }
 
 // With both classes present, the gdn kernel (the long tiles) is forked onto the
@@ -943,7 +1101,7 @@ void launch_persistent(Plan *P, double *ring, hipStream_t s) {
 // the store-bound tiles finish under the gdn tiles (9% on a 48-request batch).
 template <bool DIG>
 int launch_batch_t(Plan *P, double *ring, hipStream_t s) {
-    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, P->d_counter);
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, s, P->d_counter);
     if (P->qtiles[0] && P->qtiles[1]) {
         QMC_HIP_CHECK(hipEventRecord(P->ev_fork, s));
         QMC_HIP_CHECK(hipStreamWaitEvent(P->side, P->ev_fork, 0));
@@ -1001,7 +1159,8 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
         const u64 n = 1ull << q.m;
         u64 rows;
         if (q.method == M_GDN_LMS) {
-            rows = points_per_tile < (u64)kBlock ? (u64)kBlock : points_per_tile;  // a gdn tile is one column
+            rows = points_per_tile / kGdnCols;  // a gdn tile is kGdnCols columns
+            if (rows < (u64)kGdnLanes) rows = kGdnLanes;
         } else {
             rows = points_per_tile / (u64)q.d;
             if ((q.method == M_DNB2_LMS || q.method == M_DNB2_NUS) && rows < 1024) rows = 1024;  // 64-step chains at 2^s = 16
@@ -1010,16 +1169,25 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
         while (p2 * 2 <= rows) p2 *= 2;
         if (p2 > n) p2 = n;
         q.rows_per_tile = (u32)p2;
-        q.tiles_per_l = (u32)(n / p2) * (q.method == M_GDN_LMS ? (u32)q.d : 1u);
+        q.tab = words;
+        words += table_words(q);
+        q.tiles_per_l = (u32)(n / p2) * (q.method == M_GDN_LMS ? q.gng : 1u);
         q.tile0 = tile;
         tile += (u64)q.r * q.tiles_per_l;
         q.ddiv = make_div((u32)q.d);
+        q.gdiv = make_div(q.method == M_GDN_LMS ? q.gng : 1u);
         q.gcap = gdn_level_cap > 0 ? (u32)gdn_level_cap : 64u;
         if (q.method == M_DNB2_LMS || q.method == M_DNB2_NUS) q.clog = dnb2_chain_log(q.rows_per_tile, (u32)q.d);
-        q.tab = words;
-        words += table_words(q);
         if (q.method == M_GDN_LMS)
-            for (int j = 0; j < q.d; j++) P->lds_bytes[1] = std::max(P->lds_bytes[1], gdn_col_lds(h_primes[j], q.m));
+            for (int j = 0; j < q.d; j++) {
+                P->lds_bytes[1] = std::max(P->lds_bytes[1], (gdn_col_lds(h_primes[j], q.m) + 15) & ~15u);
+                // the walk's packed row offsets: two of them add below 2^16
+                if (h_primes[j] > 2 && h_primes[j] * 2 * gdn_row_u16((u32)gdn_levels(h_primes[j])) >= (1u << 15)) {
+                    qmc_fail(QMC_ERR_ARG, "batch: gdn column %d has too large a permutation table", j);
+                    qmc_batch_free(P);
+                    return nullptr;
+                }
+            }
         q.slabs = words;
         words += q.r;
         q.slab0 = (u32)P->nslabs;
@@ -1035,6 +1203,11 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
     }
     P->total_tiles = tile;
     P->ws_words = words;
+    // a gdn workgroup's LDS: kGdnCols column slots of the plan's largest
+    // column tables, then the two stages of a step's values
+    const u32 gslot = P->lds_bytes[1];
+    for (Req &q : P->reqs) q.gslot = gslot;
+    if (gslot) P->lds_bytes[1] = kGdnCols * gslot + 2 * kBlock1 * (u32)sizeof(double);
     for (int c = 0; c < 2; c++)
         for (const Req &q0 : P->reqs) {
             if (method_class(q0.method) != c) continue;
@@ -1052,6 +1225,32 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
     int cus = 256;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, P->device) != hipSuccess) cus = 256;
     P->grid = grid > 0 ? grid : 4 * cus;
+    if (P->qtiles[1]) {
+        // gdn workgroups can hold up to 160 KiB of LDS (more than the default 64 KiB)
+        static PerDeviceOnce attrs;
+        const int rc = attrs.run([&]() -> int {
+            const int most = 160 * 1024;
+            QMC_HIP_CHECK(hipFuncSetAttribute((const void *)batch_persistent_kernel<1, false>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, most));
+            QMC_HIP_CHECK(hipFuncSetAttribute((const void *)batch_persistent_kernel<1, true>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, most));
+            QMC_HIP_CHECK(hipFuncSetAttribute((const void *)batch_single_kernel<1, false>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, most));
+            QMC_HIP_CHECK(hipFuncSetAttribute((const void *)batch_single_kernel<1, true>,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, most));
+            return QMC_OK;
+        });
+        if (rc != QMC_OK || lds_total(P->lds_bytes[1]) > 160u * 1024) {
+            if (rc == QMC_OK) qmc_fail(QMC_ERR_ARG, "batch: gdn tiles need %u bytes of LDS", lds_total(P->lds_bytes[1]));
+            qmc_batch_free(P);
+            return nullptr;
+        }
+    }
+    // persistent gdn workgroups: one per CU (their registers fill it); with
+    // both classes in a batch, the store-bound kernel runs on the CUs left
+    // (qmc_batch_set_gdn_cus), by default a quarter of them
+    P->cus = cus;
+    P->grid1 = P->qtiles[0] ? (3 * cus) / 4 : cus;  // 192 of 256: 182 ms against 192 ms at 256 (512 requests, m <= 16)
     std::vector<u64> ws_host(words, 0);
     // per-base constants, then each gdn request's column info (its tables are built on the device)
     for (int j = 0; j < 256; j++) {
@@ -1071,6 +1270,8 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
             ws_host[q.tab + j] = colo | ((u64)gdn_levels(b) << 32) | ((u64)gdn_mm(b, q.m) << 40) | (tmin << 48);
             colo += gdn_col_words(b, q.m);
         }
+        const auto grp = gdn_groups(q.d);
+        for (size_t g = 0; g < grp.size(); g++) ws_host[q.tab + q.d + g] = grp[g].first | ((u64)grp[g].second << 16);
         greq.push_back((u32)qi);
         gcol0.push_back(gcols);
         gcols += (u64)q.r * (u64)q.d;
@@ -1091,13 +1292,41 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
             pos += slab;
         }
     }
+    // the gdn class's tile groups (request, l, row range), dealt to the XCD queues in turn
+    std::vector<u64> xq(17, 0);
+    {
+        std::vector<std::vector<std::pair<u64, u64>>> lists(8);  // (first tile, tiles) per XCD
+        u64 gi = 0;
+        for (int i = P->qn[0]; i < P->qn[0] + P->qn[1]; i++) {
+            const Req &q = P->qreqs[i];
+            const u64 nch = q.tiles_per_l / (u64)q.gng;
+            for (u64 l = 0; l < (u64)q.r; l++)
+                for (u64 ch = 0; ch < nch; ch++, gi++)
+                    lists[gi & 7].push_back({q.tile0 + l * q.tiles_per_l + ch * (u64)q.gng, (u64)q.gng});
+        }
+        std::vector<u64> pre, first;
+        for (int x = 0; x < 8; x++) {
+            xq[x] = pre.size();
+            u64 acc = 0;
+            for (auto &g : lists[x]) {
+                pre.push_back(acc);
+                first.push_back(g.first);
+                acc += g.second;
+            }
+            xq[9 + x] = acc;
+        }
+        xq[8] = pre.size();
+        xq.insert(xq.end(), pre.begin(), pre.end());
+        xq.insert(xq.end(), first.begin(), first.end());
+    }
     u32 *d_greq = nullptr;
     u64 *d_gcol0 = nullptr;
     if (hipMemcpyToSymbol(HIP_SYMBOL(c_primes), h_primes, sizeof(h_primes)) != hipSuccess ||
         hipMalloc((void **)&P->d_reqs, sizeof(Req) * P->reqs.size()) != hipSuccess ||
         hipMalloc((void **)&P->d_ws, sizeof(u64) * words) != hipSuccess ||
         hipMalloc((void **)&P->d_qreqs, sizeof(Req) * P->qreqs.size()) != hipSuccess ||
-        hipMalloc((void **)&P->d_counter, 2 * sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc((void **)&P->d_counter, kCounters * sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc((void **)&P->d_xq, sizeof(u64) * xq.size()) != hipSuccess ||
         (!greq.empty() && (hipMalloc((void **)&d_greq, sizeof(u32) * greq.size()) != hipSuccess ||
                            hipMalloc((void **)&d_gcol0, sizeof(u64) * gcol0.size()) != hipSuccess))) {
         (void)hipFree(d_greq);
@@ -1109,7 +1338,8 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
                   hipSuccess &&
               hipMemcpy(P->d_qreqs, P->qreqs.data(), sizeof(Req) * P->qreqs.size(), hipMemcpyHostToDevice) ==
                   hipSuccess &&
-              hipMemcpy(P->d_ws, ws_host.data(), sizeof(u64) * words, hipMemcpyHostToDevice) == hipSuccess;
+              hipMemcpy(P->d_ws, ws_host.data(), sizeof(u64) * words, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(P->d_xq, xq.data(), sizeof(u64) * xq.size(), hipMemcpyHostToDevice) == hipSuccess;
     if (ok && !greq.empty())
         ok = hipMemcpy(d_greq, greq.data(), sizeof(u32) * greq.size(), hipMemcpyHostToDevice) == hipSuccess &&
              hipMemcpy(d_gcol0, gcol0.data(), sizeof(u64) * gcol0.size(), hipMemcpyHostToDevice) == hipSuccess;
@@ -1156,6 +1386,7 @@ QMC_EXPORT void qmc_batch_free(void *h) {
     (void)hipFree(P->d_reqs);
     (void)hipFree(P->d_ws);
     (void)hipFree(P->d_counter);
+    (void)hipFree(P->d_xq);
     (void)hipFree(P->d_qreqs);
     delete P;
 }
@@ -1164,6 +1395,20 @@ QMC_EXPORT u64 qmc_batch_total_tiles(void *h) { return ((Plan *)h)->total_tiles;
 
 // Number of replication slabs (sum of r over the requests): the length of a digest array.
 QMC_EXPORT u64 qmc_batch_slabs(void *h) { return ((Plan *)h)->nslabs; }
+
+// CUs (persistent workgroups) of the gdn class; the lattice and dnb2 class
+// takes the others while both run, then the freed ones.  A tunable: 1 ..
+// the device's CU count.
+QMC_EXPORT int qmc_batch_set_gdn_cus(void *h, int n) {
+    Plan *P = (Plan *)h;
+    if (n < 1 || n > P->cus) return qmc_fail(QMC_ERR_ARG, "batch: gdn CUs must be in [1, %d]", P->cus);
+    if (n != P->grid1 && P->gexec) {
+        QMC_HIP_CHECK(hipGraphExecDestroy(P->gexec));
+        P->gexec = nullptr;
+    }
+    P->grid1 = n;
+    return QMC_OK;
+}
 
 // Make later runs of the plan add, for every replication slab, the wrap-around
 // sum of the stored float64 bit patterns into digest[slab] (device memory of
@@ -1235,14 +1480,14 @@ QMC_EXPORT int qmc_batch_run_per_call_dev(void *h, double *ring, qmc_stream_t s)
                 hipLaunchKernelGGL((batch_single_kernel<0, true>), dim3((unsigned)tiles), dim3(kBlock), lds, s,
                                    P->d_reqs + i, P->d_ws, ring, P->digest);
             else
-                hipLaunchKernelGGL((batch_single_kernel<1, true>), dim3((unsigned)tiles), dim3(kBlock), lds, s,
+                hipLaunchKernelGGL((batch_single_kernel<1, true>), dim3((unsigned)tiles), dim3(kBlock1), lds, s,
                                    P->d_reqs + i, P->d_ws, ring, P->digest);
         } else {
             if (c0)
                 hipLaunchKernelGGL((batch_single_kernel<0, false>), dim3((unsigned)tiles), dim3(kBlock), lds, s,
                                    P->d_reqs + i, P->d_ws, ring, P->digest);
             else
-                hipLaunchKernelGGL((batch_single_kernel<1, false>), dim3((unsigned)tiles), dim3(kBlock), lds, s,
+                hipLaunchKernelGGL((batch_single_kernel<1, false>), dim3((unsigned)tiles), dim3(kBlock1), lds, s,
                                    P->d_reqs + i, P->d_ws, ring, P->digest);
         }
     }

@@ -164,7 +164,7 @@ def test_gdn_requests_match_reference_chain(small):
     assert checked > 0
 
 
-@pytest.mark.parametrize("cap", [1, 3])
+@pytest.mark.parametrize("cap", [8, 12])
 def test_gdn_levels_past_the_tables_match_reference_chain(torch, cap):
     """With the tables' levels capped, most sums continue on the path that
     computes digits from the index and permutation values from the keys; the

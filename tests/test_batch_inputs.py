@@ -90,7 +90,7 @@ def test_gdn_base_constants():
         for t in range(52):
             assert rec[t] == want  # gdn_integer_to_float.c:34-38
             want /= b
-        assert levels in (8, 12, 16, 24, 32, 48, 52)
+        assert levels in (8, 10, 12, 16, 24, 48, 52)
         if b > 2:
             assert float(b) ** levels >= 2.0 ** 74
         assert all(thr[t] >= 1.0 for t in range(tmin)) and (tmin == 52 or thr[tmin] < 1.0)
