@@ -58,6 +58,9 @@ QMC_EXPORT void qmc_clear_error(void);
 /* read (and optionally clear) the current device's status bits that kernels
  * set on inputs the reference leaves undefined; synchronizes the device */
 QMC_EXPORT int qmc_device_status(unsigned *bits, int clear);
+/* clear the status bits on stream s, ordered with the caller's launches (the
+ * Python tensor path clears them before a call and reads them after it) */
+QMC_EXPORT int qmc_status_reset_dev(qmc_stream_t s);
 
 /* ------------------------------------------------- drop-ins: lattices
  * lat_gen_linear.c:3-13, lat_gen_natural.c:3-14, lat_gen_gray.c:3-14,

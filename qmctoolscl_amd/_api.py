@@ -92,6 +92,53 @@ SIGNATURES = {
 
 KERNEL_NAMES = tuple(SIGNATURES)
 
+# Elements of every array argument a call reads or writes, from the sizes in
+# the reference's prototype comments and index expressions (c_funcs/<name>.c:
+# e.g. lat_shift_mod_1.c:11-13, dnb2_digital_shift.c:32 for lshifts[l % r_x],
+# gdn_linear_matrix_scramble.c:33 for bases[(l % r_b) * d + j]).  The wrappers
+# reject a smaller array before any launch.
+SIZES = {
+    "lat_gen_linear": {"g": "r*d", "x": "r*n*d"},
+    "lat_gen_natural": {"g": "r*d", "x": "r*n*d"},
+    "lat_gen_gray": {"g": "r*d", "x": "r*n*d"},
+    "lat_shift_mod_1": {"x": "r_x*n*d", "shifts": "r*d", "xr": "r*n*d"},
+    "dnb2_gmat_lsb_to_msb": {"tmaxes": "r", "C_lsb": "r*d*mmax", "C_msb": "r*d*mmax"},
+    "dnb2_linear_matrix_scramble": {"S": "r*d*tmax_new", "C": "r_C*d*mmax", "C_lms": "r*d*mmax"},
+    "dnb2_gen_natural": {"C": "r*d*mmax", "xb": "r*n*d"},
+    "dnb2_gen_gray": {"C": "r*d*mmax", "xb": "r*n*d"},
+    "dnb2_digital_shift": {"lshifts": "r_x", "xb": "r_x*n*d", "shiftsb": "r*d", "xrb": "r*n*d"},
+    "dnb2_integer_to_float": {"tmaxes": "r", "xb": "r*n*d", "x": "r*n*d"},
+    "dnb2_interlace": {"C": "r*d*mmax", "C_alpha": "r*d_alpha*mmax"},
+    "dnb2_undo_interlace": {"C_alpha": "r*d_alpha*mmax", "C": "r*d*mmax"},
+    "gdn_gen_natural": {"bases": "r_b*d", "C": "r*d*mmax*tmax", "xdig": "r*n*d*tmax"},
+    "gdn_gen_natural_same_base": {"C": "r*d*mmax*tmax", "xdig": "r*n*d*tmax"},
+    "gdn_linear_matrix_scramble": {"bases": "r_b*d", "S": "r*d*tmax_new*tmax", "C": "r_C*d*mmax*tmax",
+                                   "C_lms": "r*d*mmax*tmax_new"},
+    "gdn_digital_permutation": {"perms": "r*d*tmax_new*bmax", "xdig": "r_x*n*d*tmax", "xdig_new": "r*n*d*tmax_new"},
+    "gdn_digital_shift": {"bases": "r_b*d", "shifts": "r*d*tmax_new", "xdig": "r_x*n*d*tmax",
+                          "xdig_new": "r*n*d*tmax_new"},
+    "gdn_integer_to_float": {"bases": "r_b*d", "xdig": "r*n*d*tmax", "x": "r*n*d"},
+    "gdn_interlace": {"C": "r*d*mmax*tmax", "C_alpha": "r*d_alpha*mmax*tmax_alpha"},
+    "gdn_undo_interlace": {"C_alpha": "r*d_alpha*mmax*tmax_alpha", "C": "r*d*mmax*tmax"},
+    "fwht_1d_radix2": {"x": "d1*d2*2*n_half"},
+    "fft_bro_1d_radix2": {"twiddler": "2*n_half", "twiddlei": "2*n_half", "xr": "d1*d2*2*n_half",
+                          "xi": "d1*d2*2*n_half"},
+    "ifft_bro_1d_radix2": {"twiddler": "2*n_half", "twiddlei": "2*n_half", "xr": "d1*d2*2*n_half",
+                           "xi": "d1*d2*2*n_half"},
+}
+
+
+def required_elements(name, scalars):
+    """{array argument: elements the call touches} for a call of `name` with
+    the scalar arguments `scalars` (a dict by argument name)."""
+    out = {}
+    for arg, expr in SIZES[name].items():
+        v = 1
+        for f in expr.split("*"):
+            v *= int(f) if f.isdigit() else int(scalars[f])
+        out[arg] = v
+    return out
+
 
 def arg_kind(spec):
     return spec[1]

@@ -49,6 +49,8 @@ def lib():
     L.qmc_init.restype = ctypes.c_int
     L.qmc_device_status.restype = ctypes.c_int
     L.qmc_device_status.argtypes = [ctypes.POINTER(ctypes.c_uint), ctypes.c_int]
+    L.qmc_status_reset_dev.restype = ctypes.c_int
+    L.qmc_status_reset_dev.argtypes = [ctypes.c_void_p]
     for name, sig in SIGNATURES.items():
         kinds = [_C[s[1]] for s in sig]
         host = getattr(L, name)
@@ -63,6 +65,17 @@ def lib():
     L.qmc_dnb2_gen_shift_float_dev.argtypes = ([ctypes.c_int] + [ctypes.c_uint64] * 6 + [ctypes.c_void_p] * 7)
     _lib = L
     return L
+
+
+def check_status(what):
+    """Read and clear the current device's status bits (after the caller's
+    stream has synchronized); raise QMCError(QMC_ERR_RANGE) if any is set."""
+    L = lib()
+    bits = ctypes.c_uint(0)
+    check(L.qmc_device_status(ctypes.byref(bits), 1))
+    if bits.value:
+        raise QMCError(2, "%s: input outside the reference's defined domain (status 0x%x): a base below 2 or "
+                          "an index that reads past an input array" % (what, bits.value))
 
 
 def check(rc=None):

@@ -59,6 +59,8 @@ u32 *qmc_status_ptr() {
     return g_status[dev];
 }
 
+__global__ void status_reset_kernel(u32 *p) { p[0] = 0; }
+
 static hipStream_t lib_stream() {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
@@ -87,6 +89,16 @@ QMC_EXPORT int qmc_init(void) {
     return QMC_OK;
 }
 
+// Clears the status word on stream s (a one-thread kernel, so the clear is
+// ordered with the caller's launches and can sit in a captured graph).
+QMC_EXPORT int qmc_status_reset_dev(qmc_stream_t s) {
+    u32 *p = qmc_status_ptr();
+    if (!p) return g_err_code;
+    hipLaunchKernelGGL(status_reset_kernel, dim3(1), dim3(1), 0, (hipStream_t)s, p);
+    QMC_HIP_CHECK(hipGetLastError());
+    return QMC_OK;
+}
+
 QMC_EXPORT int qmc_device_status(unsigned *bits, int clear) {
     u32 *p = qmc_status_ptr();
     if (!p) return g_err_code;
@@ -108,7 +120,13 @@ namespace {
 class Staging {
   public:
     Staging() : s_(lib_stream()) {
-        if (!s_) fail_ = qmc_fail(QMC_ERR_HIP, "no HIP stream (is a GPU visible?)");
+        if (!s_) {
+            fail_ = qmc_fail(QMC_ERR_HIP, "no HIP stream (is a GPU visible?)");
+            return;
+        }
+        // status bits left by earlier launches are not this call's
+        u32 *st = qmc_status_ptr();
+        if (st) hipLaunchKernelGGL(status_reset_kernel, dim3(1), dim3(1), 0, s_, st);
     }
     ~Staging() {
         for (auto &b : bufs_) (void)hipFreeAsync(b.dev, s_);

@@ -8,11 +8,20 @@ import ctypes
 from . import _lib
 
 
-def _ptr(t):
+def _ptr(t, need=0, f64=False, what="tensor"):
+    """Device pointer of a contiguous GPU tensor holding at least `need`
+    elements of float64 (f64) or 64-bit integers."""
     if t is None:
         return None
-    if not (t.is_cuda and t.is_contiguous()):
-        raise ctypes.ArgumentError("expected a contiguous GPU tensor")
+    import torch
+    if not (getattr(t, "is_cuda", False) and t.is_contiguous()):
+        raise ctypes.ArgumentError("%s: expected a contiguous GPU tensor" % what)
+    ok = (torch.float64,) if f64 else (torch.int64, getattr(torch, "uint64", torch.int64))
+    if t.dtype not in ok:
+        raise ctypes.ArgumentError("%s: expected %s data, got %s" % (what, "float64" if f64 else "64-bit integer",
+                                                                     t.dtype))
+    if t.numel() < need:
+        raise ctypes.ArgumentError("%s holds %d elements, the call touches %d" % (what, t.numel(), need))
     return t.data_ptr()
 
 
@@ -26,8 +35,10 @@ def lat_gen_shift(order, r, n, d, r_g, n_start, g, shifts, xr, stream=None):
     """xr = lat_shift_mod_1(r, n, d, r_g, lat_gen_<order>(r_g, n, d, [n_start,] g), shifts).
     order: 0 linear, 1 natural, 2 gray.  g: (r_g, d) uint64, shifts: (r, d)
     float64, xr: (r, n, d) float64."""
-    rc = _lib.lib().qmc_lat_gen_shift_dev(int(order), int(r), int(n), int(d), int(r_g), int(n_start),
-                                          _ptr(g), _ptr(shifts), _ptr(xr), _stream(stream, xr))
+    r, n, d, r_g = int(r), int(n), int(d), int(r_g)
+    rc = _lib.lib().qmc_lat_gen_shift_dev(int(order), r, n, d, r_g, int(n_start),
+                                          _ptr(g, r_g * d, what="g"), _ptr(shifts, r * d, True, "shifts"),
+                                          _ptr(xr, r * n * d, True, "xr"), _stream(stream, xr))
     _lib.check(rc)
 
 
@@ -38,9 +49,13 @@ def dnb2_gen_shift_float(natural, r, n, d, r_x, n_start, mmax, C, lshifts, shift
     (lshifts: (r_x,), shiftsb: (r, d)) and conversion to double (tmaxes:
     (r,)) in one pass; writes xrb (r, n, d) uint64 and/or x (r, n, d) float64."""
     like = xrb if xrb is not None else x
-    rc = _lib.lib().qmc_dnb2_gen_shift_float_dev(int(bool(natural)), int(r), int(n), int(d), int(r_x),
-                                                 int(n_start), int(mmax), _ptr(C), _ptr(lshifts), _ptr(shiftsb),
-                                                 _ptr(tmaxes), _ptr(xrb), _ptr(x), _stream(stream, like))
+    r, n, d, r_x, mmax = int(r), int(n), int(d), int(r_x), int(mmax)
+    rx = min(r_x, r)  # nets the call reads
+    rc = _lib.lib().qmc_dnb2_gen_shift_float_dev(int(bool(natural)), r, n, d, r_x, int(n_start), mmax,
+                                                 _ptr(C, rx * d * mmax, what="C"), _ptr(lshifts, rx, what="lshifts"),
+                                                 _ptr(shiftsb, r * d, what="shiftsb"), _ptr(tmaxes, r, what="tmaxes"),
+                                                 _ptr(xrb, r * n * d, what="xrb"), _ptr(x, r * n * d, True, "x"),
+                                                 _stream(stream, like))
     _lib.check(rc)
 
 
@@ -56,6 +71,9 @@ def gdn_gen_scramble_float(mode, r, n, d, r_b, mmax, tmax, tmax_new, n_start, bm
     fn = L.qmc_gdn_gen_scramble_float_dev
     fn.restype = ctypes.c_int
     fn.argtypes = [ctypes.c_int] + [ctypes.c_uint64] * 9 + [ctypes.c_void_p] * 5
-    rc = fn(int(mode), int(r), int(n), int(d), int(r_b), int(mmax), int(tmax), int(tmax_new), int(n_start),
-            int(bmax), _ptr(bases), _ptr(C), _ptr(tab) if tab is not None else None, _ptr(x), _stream(stream, x))
+    r, n, d, r_b, mmax, tmax, tmax_new, bmax = (int(v) for v in (r, n, d, r_b, mmax, tmax, tmax_new, bmax))
+    need_tab = {0: 0, 1: r * d * tmax_new * bmax, 2: r * d * tmax_new}.get(int(mode), 0)
+    rc = fn(int(mode), r, n, d, r_b, mmax, tmax, tmax_new, int(n_start), bmax, _ptr(bases, r_b * d, what="bases"),
+            _ptr(C, r * d * mmax * tmax, what="C"), _ptr(tab, need_tab, what="tab") if tab is not None else None,
+            _ptr(x, r * n * d, True, "x"), _stream(stream, x))
     _lib.check(rc)

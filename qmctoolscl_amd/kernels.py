@@ -16,7 +16,12 @@ Keyword arguments of the reference (backend, global_size, local_size, wait,
 context, queue, program, platform_id, device_id, profile) are accepted;
 backend must be one of "c", "cl", "hip", "gpu" (any case), others raise
 AssertionError as in qmctoolscl/util.py:52-54.  wait=False skips the final
-stream synchronization for tensor arguments.  There is no CPU path.
+stream synchronization for tensor arguments, and with it the check of the
+device status bits that kernels set on inputs the reference leaves undefined
+(read them with qmc_device_status); with wait=True such a call raises
+QMCError (QMC_ERR_RANGE), as the numpy path does.  An array smaller than
+the elements the call touches raises ctypes.ArgumentError before any launch.
+There is no CPU path.
 """
 import ctypes
 import time
@@ -24,7 +29,7 @@ import time
 import numpy as np
 
 from . import _lib
-from ._api import SIGNATURES
+from ._api import SIGNATURES, required_elements
 
 _BACKENDS = ("c", "cl", "hip", "gpu")
 
@@ -84,6 +89,12 @@ def _make(name):
             _PREPROCESS[name](*args)
         arrays = [(s, a) for s, a in zip(sig, args) if s[1] != "u64"]
         on_gpu = [_is_torch(a) for _, a in arrays]
+        need = required_elements(name, {s[0]: int(a) for s, a in zip(sig, args) if s[1] == "u64"})
+        for s, a in arrays:
+            have = a.numel() if _is_torch(a) else getattr(a, "size", None)
+            if have is not None and have < need[s[0]]:
+                raise ctypes.ArgumentError("%s: %s holds %d elements, the call touches %d"
+                                           % (name, s[0], have, need[s[0]]))
         L = _lib.lib()
         if all(on_gpu):
             import torch
@@ -92,11 +103,15 @@ def _make(name):
             dev = arrays[0][1].device
             with torch.cuda.device(dev):
                 stream = torch.cuda.current_stream(dev)
+                sp = ctypes.c_void_p(stream.cuda_stream)
                 cargs = [int(a) if s[1] == "u64" else a.data_ptr() for s, a in zip(sig, args)]
-                rc = getattr(L, name + "_dev")(*cargs, ctypes.c_void_p(stream.cuda_stream))
+                if wait:  # the status bits this call sets are read back below
+                    _lib.check(L.qmc_status_reset_dev(sp))
+                rc = getattr(L, name + "_dev")(*cargs, sp)
                 _lib.check(rc)
                 if wait:
                     stream.synchronize()
+                    _lib.check_status(name)
         elif not any(on_gpu):
             for s, a in arrays:
                 _np_ok(a, s[1])

@@ -130,3 +130,30 @@ def test_entry_points_reject_bad_arguments_before_any_gpu_call():
     assert fw(1, 1, 64, None, None, None) == ARG      # 2^7 is below the minimum
     assert fw(1, 1, 1 << 24, None, None, None) == ARG  # 2^25 is above the maximum
     assert fw(1, 1, 512, 8, None, None) == ARG         # misaligned pointer
+
+
+@pytest.mark.skipif(not os.path.exists(_lib.LIB_PATH), reason="library not built")
+def test_undersized_arrays_are_rejected_before_any_call():
+    """Every array argument must hold the elements the call touches (the
+    sizes of the reference's prototype comments); a smaller one raises
+    ctypes.ArgumentError before the library is asked to do anything."""
+    from qmctoolscl_amd._api import SIZES, required_elements
+    U = np.uint64
+    g = np.ones((2, 3), dtype=np.uint64)
+    with pytest.raises(ctypes.ArgumentError, match="x holds 17"):
+        q.lat_gen_linear(U(2), U(3), U(3), g, np.empty(17))        # needs 2*3*3 = 18
+    with pytest.raises(ctypes.ArgumentError, match="g holds 5"):
+        q.lat_gen_linear(U(2), U(3), U(3), g.ravel()[:5].copy(), np.empty(18))
+    with pytest.raises(ctypes.ArgumentError, match="perms"):
+        q.gdn_digital_permutation(U(1), U(4), U(2), U(1), U(1), U(3), U(3), U(5), np.zeros(29, np.uint64),
+                                  np.zeros(24, np.uint64), np.zeros(24, np.uint64))  # perms needs 1*2*3*5 = 30
+    with pytest.raises(ctypes.ArgumentError, match="lshifts"):
+        q.dnb2_digital_shift(U(4), U(2), U(2), U(3), np.zeros(2, np.uint64), np.zeros(12, np.uint64),
+                             np.zeros(8, np.uint64), np.zeros(16, np.uint64))  # lshifts[l % r_x]: r_x = 3
+    with pytest.raises(ctypes.ArgumentError, match="xi"):
+        q.fft_bro_1d_radix2(U(2), U(1), U(4), np.empty(8), np.empty(8), np.empty(16), np.empty(15))
+    # a spot check of the table against the reference's index arithmetic
+    assert required_elements("gdn_linear_matrix_scramble", dict(r=3, d=2, mmax=4, r_C=1, r_b=1, tmax=5,
+                                                                tmax_new=7)) == {
+        "bases": 2, "S": 3 * 2 * 7 * 5, "C": 1 * 2 * 4 * 5, "C_lms": 3 * 2 * 4 * 7}
+    assert set(SIZES) == set(SIGNATURES)

@@ -233,6 +233,48 @@ def test_errors_are_raised():
     q.lat_gen_gray(np.uint64(1), np.uint64(4), np.uint64(2), np.uint64(0), g, x)
 
 
+def test_tensor_path_reports_status_bits_to_its_own_call(gpu):
+    """A tensor call with a base of 1 raises (the bits it sets are read after
+    its stream synchronizes), and the next, valid numpy call does not see
+    them; with wait=False the bits stay for qmc_device_status."""
+    torch = gpu
+    U = np.uint64
+    bases = to_t(np.array([[1, 3]], dtype=np.uint64))
+    Cg = to_t(np.ones((1, 2, 2, 2), dtype=np.uint64))
+    xd = torch.empty((1, 3, 2, 2), dtype=torch.int64, device="cuda")
+    with pytest.raises(q.QMCError, match="QMC_ERR_RANGE"):
+        q.gdn_gen_natural(U(1), U(3), U(2), U(1), U(2), U(2), U(0), bases, Cg, xd)
+    g = np.ones((1, 2), dtype=np.uint64)
+    x = np.empty((1, 4, 2))
+    q.lat_gen_gray(U(1), U(4), U(2), U(0), g, x)
+    q.gdn_gen_natural(U(1), U(3), U(2), U(1), U(2), U(2), U(0), bases, Cg, xd, wait=False)
+    torch.cuda.synchronize()
+    import ctypes
+    bits = ctypes.c_uint(0)
+    q._lib.check(q._lib.lib().qmc_device_status(ctypes.byref(bits), 1))
+    assert bits.value != 0
+    # a valid tensor call after the cleared word
+    ok = to_t(np.array([[2, 3]], dtype=np.uint64))
+    q.gdn_gen_natural(U(1), U(3), U(2), U(1), U(2), U(2), U(0), ok, Cg, xd)
+
+
+def test_fused_entry_points_reject_undersized_tensors(gpu):
+    import ctypes
+    from qmctoolscl_amd import fused
+    torch = gpu
+    g = torch.ones((1, 8), dtype=torch.int64, device="cuda")
+    sh = torch.zeros((4, 8), dtype=torch.float64, device="cuda")
+    xr = torch.empty((4, 16, 8), dtype=torch.float64, device="cuda")
+    with pytest.raises(ctypes.ArgumentError, match="xr holds"):
+        fused.lat_gen_shift(1, 4, 32, 8, 1, 0, g, sh, xr)        # needs 4*32*8
+    with pytest.raises(ctypes.ArgumentError, match="shifts"):
+        fused.lat_gen_shift(1, 4, 16, 8, 1, 0, g, sh[:3], xr)
+    with pytest.raises(ctypes.ArgumentError, match="float64"):
+        fused.lat_gen_shift(1, 4, 16, 8, 1, 0, g, sh.float(), xr)
+    fused.lat_gen_shift(1, 4, 16, 8, 1, 0, g, sh, xr)
+    torch.cuda.synchronize()
+
+
 def test_empty_inputs_are_no_ops():
     g = np.ones((1, 2), dtype=np.uint64)
     x = np.empty((1, 0, 2))
