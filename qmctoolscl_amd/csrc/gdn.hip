@@ -13,6 +13,8 @@
 // including wrap-around: gdn_gen_natural.c:40-50 reduces after every digit,
 // x = (x + dig*C) % b.  Where no step can wrap, the walks use exact modular
 // sums; every other element takes the step-by-step path.
+#include <type_traits>
+
 #include "qmc_common.hpp"
 
 #include <algorithm>
@@ -520,7 +522,7 @@ struct FuseArgs {
     const u32 *Q, *okf;
     double *x;
     u32 *status;
-    u32 tmax, tnew, perm_lds, off_q, off_qs, off_rt, off_tab;
+    u32 tmax, tnew, perm_lds, off_q, off_qs, off_rt, off_tab, off_thr;
 };
 
 // Scrambled modes with a value table: V[z][t] = RN(recip_t * (double)y_t(z))
@@ -617,6 +619,13 @@ __global__ __launch_bounds__(TAB ? kFuseTabBlock : kBlock) void gdn_fused_kernel
     double *srtm = srt + tnew;
     u32 *sP = (u32 *)(smem + a.off_tab);
     u64 *sS = (u64 *)(smem + a.off_tab);
+    // table path: per-level largest term (as bits), thresholds, walked levels
+    unsigned long long *sM = (unsigned long long *)(smem + a.off_thr);
+    double *sThr = (double *)(smem + a.off_thr + 8 * 64);
+    u32 *sTW = (u32 *)(smem + a.off_thr + 8 * 64 + 8 * 65);
+    u32 bad0 = 0;
+    if (TAB)
+        for (u32 t = threadIdx.x; t < 64; t += blockDim.x) sM[t] = 0;
     const u64 pbase = lj * tnew * a.bmax;
     if (ok)
         for (u32 e = threadIdx.x; e < (u32)a.mmax * TP; e += blockDim.x) {
@@ -669,7 +678,34 @@ __global__ __launch_bounds__(TAB ? kFuseTabBlock : kBlock) void gdn_fused_kernel
                     y = (double)(t >= tmax ? sh : xs >= sh && xs < 2 * b ? (xs >= b ? xs - b : xs) : umod(xs, b));
                 }
             }
-            V[zz * (TP + 1) + t] = t < tnew ? srt[t] * y : 0.0;
+            const double term = t < tnew ? srt[t] * y : 0.0;
+            V[zz * (TP + 1) + t] = term;
+            if (t < tnew) atomicMax(&sM[t], (unsigned long long)__double_as_longlong(term));
+        }
+        // levels past the table: one tail term each
+        for (u32 t = TP + threadIdx.x; t < tnew; t += blockDim.x)
+            atomicMax(&sM[t], (unsigned long long)__double_as_longlong(
+                                  srt[t] * (double)fuse_tail<MODE>(a, t, pbase, sP, sS, false, bad0)));
+        __syncthreads();
+        // Exact early exit of the sums (as in the batch's gdn class): B_t =
+        // the largest term of any level s >= t (terms are >= 0, so their bit
+        // patterns order as the values); a partial sum v >= thr_t =
+        // 2^(ilogb(B_t) + 54) has ulp(v) / 2 > B_t, so every later add
+        // rounds back to v.  Levels from TW on are walked only while some
+        // lane of the wave is below the threshold; TW is the first level
+        // whose threshold is at most 2^-20.
+        if (threadIdx.x == 0) {
+            double B = 0.0;
+            u32 tw = TP;
+            for (int t = (int)tnew - 1; t >= 0; t--) {
+                const double mt = __longlong_as_double((long long)sM[t]);
+                B = mt > B ? mt : B;
+                const double th = B > 0.0 ? ldexp(1.0, ilogb(B) + 54) : 0.0;
+                sThr[t] = th;
+                if (th <= 0x1p-20 && (u32)t < tw) tw = (u32)t;
+            }
+            sThr[tnew] = 0.0;
+            sTW[0] = tw;
         }
         __syncthreads();
     }
@@ -741,26 +777,56 @@ __global__ __launch_bounds__(TAB ? kFuseTabBlock : kBlock) void gdn_fused_kernel
     u32 dlow = (k >> 32) ? (u32)(k % b) : (u32)k - div_q((u32)k, bdiv) * bb;
     if (TAB && fast) {
         constexpr u32 S = (TP + 1) * 8;
-        const u32 bS = bb * S;
+        const u32 tw = __builtin_amdgcn_readfirstlane(sTW[0]);  // levels a point usually needs (uniform)
 #pragma unroll
         for (int t = 0; t < TP; t++) z[t] *= S;
-        for (u32 h = 0;; h++) {
-            double v = 0.;
+        // walk and sum the first TL levels, TL the smallest of 16, 32, TP
+        // covering tw: a wave adds every one of them (terms past a lane's
+        // threshold leave its sum unchanged) and checks once, at TL
+        auto run = [&](auto tl) {
+            constexpr int TL = decltype(tl)::value;
+            const double thl = sThr[(u32)TL < tnew ? (u32)TL : tnew];
+            for (u32 h = 0;; h++) {
+                double v = 0.;
 #pragma unroll
-            for (int t = 0; t < TP; t++)  // + 0 past tmax_new
-                v += *(__attribute__((address_space(3))) const double *)(uintptr_t)(z[t] + 8u * t);
-            for (u32 t = TP; t < tnew; t++) v += srt[t] * (double)fuse_tail<MODE>(a, t, pbase, sP, sS, false, bad);
-            xo[(u64)h * a.d] = v;
-            if (h + 1 == H) break;
-            const u32 *qc = sQs + walk_carry(dlow, k, b, bb, bdiv, (u32)a.mmax) * TP;
+                for (int t = 0; t < TL; t++)  // + 0 past tmax_new
+                    v += *(__attribute__((address_space(3))) const double *)(uintptr_t)(z[t] + 8u * t);
+                if ((u32)TL < tnew && __builtin_amdgcn_ballot_w64(v < thl) != 0) {
+                    // the levels from TL on, for the lanes still below them:
+                    // the digit from the point's index, then the table (or
+                    // the tail value past tmax), the reference's product and add
+                    for (u32 t = TL; t < tnew; t++) {
+                        const bool act = v < sThr[t];
+                        if (__builtin_amdgcn_ballot_w64(act) == 0) break;
+                        if (!act) continue;
+                        u64 y;
+                        if (t < tmax)
+                            y = fuse_scramble<MODE>(a, t,
+                                                    gdn_digit(k, b, a.C, lj * a.mmax * tmax + t, tmax, a.Ctotal, bad),
+                                                    b, pbase, sP, sS, plds, bad);
+                        else
+                            y = fuse_tail<MODE>(a, t, pbase, sP, sS, plds, bad);
+                        v += srt[t] * (double)y;
+                    }
+                }
+                xo[(u64)h * a.d] = v;
+                if (h + 1 == H) break;
+                const u32 *qc = sQs + walk_carry(dlow, k, b, bb, bdiv, (u32)a.mmax) * TP;
 #pragma unroll
-            for (int t = 0; t < TP; t++) {
-                const u32 s2 = z[t] + qc[t];
-                z[t] = __builtin_elementwise_min(s2, s2 - bS);
+                for (int t = 0; t < TL; t++) {
+                    const u32 s2 = z[t] + qc[t];
+                    z[t] = __builtin_elementwise_min(s2, s2 - bb * S);
+                }
+                k++;
             }
-            k++;
-        }
-        if (bad) flag(a.status, bad);
+        };
+        if (TP > 16 && tw <= 16)
+            run(std::integral_constant<int, (TP > 16 ? 16 : TP)>());
+        else if (TP > 32 && tw <= 32)
+            run(std::integral_constant<int, (TP > 32 ? 32 : TP)>());
+        else
+            run(std::integral_constant<int, TP>());
+        if (bad | bad0) flag(a.status, bad | bad0);
         return;
     }
     const u32 bm = (u32)a.bmax;
@@ -958,7 +1024,9 @@ QMC_EXPORT int qmc_gdn_gen_scramble_float_dev(int mode, u64 r, u64 n, u64 d, u64
     a.off_qs = a.off_q + qbytes;
     a.off_rt = a.off_qs + (tabm ? qbytes : 0);
     a.off_tab = a.off_rt + (u32)(8 * (tmax_new + TP));
-    const size_t shm = a.off_tab + (mode == 1 ? (a.perm_lds ? 4 * rows * bmax : 0) : mode == 2 ? 8 * rows : 0);
+    a.off_thr = (u32)((a.off_tab + (mode == 1 ? (a.perm_lds ? 4 * rows * bmax : 0) : mode == 2 ? 8 * rows : 0) + 15) &
+                      ~(u64)15);
+    const size_t shm = a.off_thr + (tabm ? 8 * 64 + 8 * 65 + 16 : 0);
     const u64 nblocks = (a.ngroups + 7) / 8 * 8 * d;
     if (nblocks > 0x7fffffffull) return qmc_fail(QMC_ERR_ARG, "%s: r*n*d too large", who);
     u32 *ws = nullptr;
@@ -971,7 +1039,7 @@ QMC_EXPORT int qmc_gdn_gen_scramble_float_dev(int mode, u64 r, u64 n, u64 d, u64
     if (tabm) {
         // the value table and two Q tables exceed the default 64 KiB of LDS
         static PerDeviceOnce attrs;
-        const int most = (int)(kFuseTabBytes + 2 * sizeof(u32) * 64 * 64 + 8 * 128 + 8 * 64);
+        const int most = (int)(kFuseTabBytes + 2 * sizeof(u32) * 64 * 64 + 8 * 128 + 8 * 64 + 2048);
         const int rc = attrs.run([&]() -> int {
             QMC_HIP_CHECK(hipFuncSetAttribute((const void *)gdn_fused_kernel<16, 1, true>,
                                               hipFuncAttributeMaxDynamicSharedMemorySize, most));
