@@ -1,31 +1,39 @@
 """Benchmark of the QMC point-generation and transform hot path on one MI355X.
 
-Headline workload (one step): a randomized rank-1 lattice -- n = 2^20 points
-in natural order, d = 64, one generating vector of odd integers in [1, 2^20),
-r = 16 uniform random shifts mod 1 -- produced as the r x n x d float64 array
-(1.07e9 points, 8 GiB) by the fused generate+shift kernel, inputs resident in
-HBM before timing.  value = points/s (Gpts/s).  The same points through the
-two drop-in kernels (lat_gen_natural, then lat_shift_mod_1) are timed beside it.
+Headline workload (one step): a ragged batch of 512 generation requests --
+method uniform over {lattice + shifts, dnb2 LMS + digital shift, dnb2 with a
+hash-based nested uniform scramble, Halton-type gdn LMS + uniform digit
+permutations (52 digits, the reference's sequential float)}, n = 2^m with m
+uniform in [8, 22], d in [1, 256], r in [1, 64] -- 1.36e12 float64 points
+(10.9 TB) streamed through a 32 GiB ring by the persistent batch kernels,
+replayed from a hipGraph; random inputs counter-based, built on the device
+before timing.  value = points/s (Gpts/s); tile sizes and the CU split
+between the two kernel classes come from the autotuner.  Beside it: the same
+requests with one launch per request, and, on a bounded ragged sub-batch, the
+reference's own call sequence per request through the drop-in kernels.
 
 Further sections (same JSON line, under "sections"):
-  dnb2     base-2 net, n=2^16, d=8, r=4: LMS (drop-in kernel) then the fused
-           natural-order generation + 64-bit digital shift + float, writing the
-           uint64 and float64 outputs; vs the four drop-in kernels chained.
-  batch    512 ragged requests (method in {lattice, dnb2 LMS+shift, dnb2 NUS,
-           Halton LMS+perm}, n=2^m with m in [8,22], d in [1,256], r in [1,64])
-           in one persistent hipGraph-replayed launch vs one launch per request,
-           streamed through a 32 GiB ring; tile size and grid from the autotuner.
-  transforms  FWHT (float64 radix-2, bit-exact; and bf16-MFMA with 3-way split)
-           and FFT (float64) at 4096 x 2^12, 64 x 2^20, 4 x 2^24, with the MFMA
-           result's relative L2 / max error against the float64 transform.
+  lattice     rank-1 lattice, n=2^20, d=64, r=16 shifts mod 1, fused kernel vs
+              the two drop-ins (the reference's chain), with its own roofline
+  dnb2        base-2 net, n=2^16, d=8, r=4: LMS + natural order + 64-bit shift
+              + float, fused vs the four drop-ins; nested uniform scrambling
+  gdn         Halton-type net, first 32 primes, n=3^12, d=32, r=8, LMS to 52
+              digits, permutations, float: the drop-in chain and the fused
+              kernel; NUS on the first 3^7 points; dnb2 interlace
+  transforms  FWHT / FFT / IFFT, float64 drop-ins (bit-exact / 1e-12) and the
+              matrix-core forms, at 4096 x 2^12, 64 x 2^20, 4 x 2^24
+  kernels     kernel-matrix columns + transforms (lattice and net kernels)
+  halton      generalized Halton, n=2^20, d=32
 
-Roofline: the headline kernel's algorithmic bytes (8 B written per point)
-over its duration from HIP events on the launching stream, against 8 TB/s.
-CPU baseline: the reference C backend built from its own sources (oracle/_ref)
--- or this project's C restatement of it where absent -- single-threaded as
-the reference C backend is, on a bounded sample of the headline workload.
+Roofline: the headline's algorithmic bytes (8 B written per point) over the
+duration of one graph replay (the two persistent batch kernels, running
+side by side) from HIP events on the launching stream, against 8 TB/s.
+CPU baseline: the reference C backend built from its own sources
+(oracle/_ref; this project's C restatement of it where absent), one core as
+the reference runs it (qmctoolscl/util.py:107), on bounded samples of each
+configuration, with the scale stated.
 
-    python bench.py [--gpus 1] [--steps 50] [--warmup 20] [--sections main,dnb2,batch,transforms]
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--sections ...] [--quick-batch]
 """
 import argparse
 import ctypes
@@ -41,45 +49,13 @@ sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
-N_LOG2, D, R = 20, 64, 16
-PROFILE_TRAFFIC = os.path.join(HERE, "profiles", "lat_gen_shift_traffic.json")
+FP32_MFMA_PEAK_TFLOPS = 157.3
+BATCH_TRAFFIC = os.path.join(HERE, "profiles", "batch_512_ragged_hbm_traffic.json")
+LATTICE_TRAFFIC = os.path.join(HERE, "profiles", "lat_gen_shift_traffic.json")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
-
-
-def cpu_baseline(g, shifts, rows):
-    """Time lat_gen_natural(r=1) + lat_shift_mod_1(r=16) on `rows` rows."""
-    ref = os.path.join(HERE, "oracle", "_ref", "libqmcref.so")
-    port = os.path.join(HERE, "oracle", "liboracle.so")
-    u = ctypes.c_uint64
-    vp = ctypes.c_void_p
-    x = np.empty((1, rows, D))
-    xr = np.empty((R, rows, D))
-    if os.path.exists(ref):
-        L = ctypes.CDLL(ref)
-        kind = "reference"
-        gen = lambda: L.lat_gen_natural(u(1), u(rows), u(D), u(1), u(rows), u(D), u(0), vp(g.ctypes.data),
-                                        vp(x.ctypes.data))
-        sh = lambda: L.lat_shift_mod_1(u(R), u(rows), u(D), u(R), u(rows), u(D), u(1), vp(x.ctypes.data),
-                                       vp(shifts.ctypes.data), vp(xr.ctypes.data))
-    elif os.path.exists(port):
-        L = ctypes.CDLL(port)
-        kind = "port"
-        gen = lambda: L.oracle_lat_gen_natural(u(1), u(rows), u(D), u(0), vp(g.ctypes.data), vp(x.ctypes.data))
-        sh = lambda: L.oracle_lat_shift_mod_1(u(R), u(rows), u(D), u(1), vp(x.ctypes.data),
-                                              vp(shifts.ctypes.data), vp(xr.ctypes.data))
-    else:
-        return None
-    t0 = time.perf_counter()
-    gen()
-    sh()
-    dt = time.perf_counter() - t0
-    pts = R * rows * D
-    return {"value": round(pts / dt / 1e9, 4), "unit": "Gpts/s", "cores": 1, "kind": kind,
-            "sample": "lat_gen_natural(r=1) + lat_shift_mod_1(r=16) over the first %d of 2^%d rows "
-                      "(n_start=0), d=%d: %.3g points in %.2f s" % (rows, N_LOG2, D, pts, dt)}
 
 
 class Timer:
@@ -106,6 +82,402 @@ class Timer:
 
 def t64(torch, a, dev):
     return torch.from_numpy(a.view(np.int64) if a.dtype == np.uint64 else a).to(dev)
+
+
+def committed_traffic(path):
+    """HBM bytes per launch from a committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary (not measured in this run), or None."""
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+# ---------------------------------------------------------------- headline
+
+def batch_requests(quick):
+    from qmctoolscl_amd import batch as B
+    meth, m, d, r, seeds = B.sample_requests(512, 2024)
+    if quick:
+        m = np.minimum(m, 16).astype(np.int32)
+    return meth, m, d, r, seeds
+
+
+def section_batch(torch, dev, timed, args):
+    """The config-3 batch: autotuned persistent launch under a hipGraph, vs one
+    launch per request; per-slab digests of both at full size."""
+    from qmctoolscl_amd import batch as B
+    meth, m, d, r, seeds = batch_requests(args.quick_batch)
+    ring_elems = 4 << 30  # 32 GiB of doubles
+    ring = torch.empty(ring_elems, dtype=torch.float64, device=dev)
+    # tune on the same requests at a quarter of the points per replication
+    m_tune = np.maximum(m - 2, 8).astype(np.int32)
+    t0 = time.perf_counter()
+    if args.tune:
+        p, g, gp, gc = (int(v) for v in args.tune.split(","))
+        tune = {"points_per_tile": p, "grid": g, "gdn_points_per_tile": gp, "gdn_cus": gc, "ms": None,
+                "table": "given on the command line"}
+    else:
+        tune = B.autotune(meth, m_tune, d, r, seeds, ring, reps=1, cache=False)
+    tune_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    plan = B.Batch(meth, m, d, r, seeds, points_per_tile=tune["points_per_tile"], grid=tune["grid"],
+                   ring_elems=ring_elems, gdn_points_per_tile=tune["gdn_points_per_tile"], gdn_cus=tune["gdn_cus"])
+    plan_s = time.perf_counter() - t0
+    pts = plan.points
+    log("batch: %d requests, %.3g points (%.2f TB written); tuned %s" % (len(m), pts, 8 * pts / 1e12,
+                                                                         {k: tune[k] for k in tune if k != "table"}))
+    ms_b, wall = timed(lambda: plan.run(ring, graph=True), args.steps, args.warmup)
+    if args.batch_only:
+        del ring
+        torch.cuda.empty_cache()
+        gbs = 8 * pts / ms_b / 1e6
+        return {"points": pts, "ms": ms_b, "wall_s": wall, "GBs": gbs, "per_call_ms": float("nan"),
+                "detail": {"autotune": {k: tune[k] for k in tune if k != "table"}, "batched_ms": round(ms_b, 2)}}
+    ms_p, _ = timed(lambda: plan.run_per_call(ring), 1, 0)
+    dg_b = plan.digests(ring)
+    dg_p = plan.digests(ring, per_call=True)
+    per_method = {}
+    names = ("lattice", "dnb2_lms_shift", "dnb2_nus_hash", "gdn_lms_perm")
+    for k, name in enumerate(names):
+        sel = meth == k
+        per_method[name] = int(np.sum((np.int64(1) << m[sel].astype(np.int64)) * d[sel] * r[sel]))
+    # each method's requests alone (m capped at 16 to bound the time), tuned shapes
+    by_method = {}
+    mq = np.minimum(m, 16).astype(np.int32)
+    for k, name in enumerate(names):
+        sel = meth == k
+        sub = B.Batch(meth[sel], mq[sel], d[sel], r[sel], seeds[sel], points_per_tile=tune["points_per_tile"],
+                      grid=tune["grid"], ring_elems=ring_elems, gdn_points_per_tile=tune["gdn_points_per_tile"])
+        ms_k, _ = timed(lambda: sub.run(ring, graph=True), 2, 1)
+        by_method[name] = {"points": sub.points, "ms": round(ms_k, 3), "Gpts_s": round(sub.points / ms_k / 1e6, 1),
+                           "frac_hbm": round(8 * sub.points / ms_k / 1e6 / HBM_PEAK_GBS, 4)}
+        del sub
+    del ring
+    torch.cuda.empty_cache()
+    gbs = 8 * pts / ms_b / 1e6
+    return {"points": pts, "ms": ms_b, "wall_s": wall, "GBs": gbs, "per_call_ms": ms_p,
+            "detail": {"workload": "512 ragged requests, m in [%d,%d], d in [1,256], r in [1,64], counter-based "
+                                   "inputs" % (int(m.min()), int(m.max())),
+                       "points_by_method": per_method, "tiles": plan.total_tiles,
+                       "bitexact_vs_per_call": bool(np.array_equal(dg_b, dg_p)), "slabs_checked": int(dg_b.size),
+                       "autotune": {k: tune[k] for k in tune if k != "table"}, "autotune_table": tune["table"],
+                       "autotune_s": round(tune_s, 1), "plan_setup_s": round(plan_s, 2),
+                       "batched_ms": round(ms_b, 2), "batched_Gpts_s": round(pts / ms_b / 1e6, 2),
+                       "batched_frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
+                       "per_call_ms": round(ms_p, 2), "per_call_Gpts_s": round(pts / ms_p / 1e6, 2),
+                       "speedup_vs_per_call": round(ms_p / ms_b, 3),
+                       "by_method_m_le_16": by_method}}
+
+
+def section_lattice(torch, q, dev, timed, steps, warmup):
+    """Config 1: randomized rank-1 lattice, n=2^20, d=64, r=16, natural order:
+    the fused generate+shift kernel vs lat_gen_natural + lat_shift_mod_1."""
+    from qmctoolscl_amd import fused
+    n_log2, D, R = 20, 64, 16
+    n = 1 << n_log2
+    rng = np.random.Generator(np.random.SFC64(0))
+    g = (2 * rng.integers(0, 2**19, size=(1, D), dtype=np.uint64) + 1)
+    shifts = q.lat_get_shifts(rng, R, D)
+    g_t, s_t = t64(torch, g, dev), t64(torch, shifts, dev)
+    xr = torch.empty((R, n, D), dtype=torch.float64, device=dev)
+    x1 = torch.empty((1, n, D), dtype=torch.float64, device=dev)
+    pts = R * n * D
+    U = np.uint64
+    ms_f, _ = timed(lambda: fused.lat_gen_shift(1, R, n, D, 1, 0, g_t, s_t, xr), steps, warmup)
+    ms_c, _ = timed(lambda: (q.lat_gen_natural(U(1), U(n), U(D), U(0), g_t, x1, wait=False),
+                             q.lat_shift_mod_1(U(R), U(n), U(D), U(1), x1, s_t, xr, wait=False)),
+                    max(3, steps // 4), 1)
+    del xr, x1
+    torch.cuda.empty_cache()
+    ach = 8.0 * pts / (ms_f * 1e-3) / 1e9
+    return {"workload": "randomized rank-1 lattice: natural order, n=2^20, d=64, r=16 shifts mod 1 (fused "
+                        "generate+shift, float64 r x n x d)", "points": pts,
+            "fused_ms": round(ms_f, 4), "fused_Gpts_s": round(pts / ms_f / 1e6, 2),
+            "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": committed_traffic(LATTICE_TRAFFIC),
+                         "traffic_source": "committed rocprofv3 summary (profiles/), not this run"},
+            "dropin_chain_ms": round(ms_c, 4), "dropin_chain_Gpts_s": round(pts / ms_c / 1e6, 2),
+            "dropin_chain": "lat_gen_natural + lat_shift_mod_1"}
+
+
+# ------------------------------------------------------------ CPU baselines
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+class RefC:
+    """The reference C backend (oracle/_ref, built from the reference's own
+    sources) called as its Python wrapper calls it, batch sizes equal to the
+    sizes (qmctoolscl/util.py:107); or this project's C restatement of it
+    (oracle/liboracle.so) where the reference build is absent."""
+
+    def __init__(self):
+        from qmctoolscl_amd._api import SIGNATURES
+        self.sig = SIGNATURES
+        ref = os.path.join(HERE, "oracle", "_ref", "libqmcref.so")
+        port = os.path.join(HERE, "oracle", "liboracle.so")
+        if os.path.exists(ref):
+            self.L, self.kind, self.prefix = ctypes.CDLL(ref), "reference", ""
+        elif os.path.exists(port):
+            self.L, self.kind, self.prefix = ctypes.CDLL(port), "port", "oracle_"
+        else:
+            self.L = None
+
+    def __call__(self, name, *args):
+        spec = self.sig[name]
+        argv = []
+        for s, a in zip(spec, args):
+            argv.append(ctypes.c_uint64(int(a)) if s[1] == "u64" else ctypes.c_void_p(a.ctypes.data))
+        if self.prefix == "":
+            argv = argv[:3] + [ctypes.c_uint64(int(a)) for a in args[:3]] + argv[3:]
+        fn = getattr(self.L, self.prefix + name)
+        fn.restype = None
+        fn(*argv)
+
+
+def _timeit(fn):
+    t0 = time.perf_counter()
+    fn()
+    return time.perf_counter() - t0
+
+
+def cpu_batch(C, quick):
+    """Config 3 on the CPU: the first requests of each method with a reference
+    chain (lattice, dnb2 LMS + shift, gdn LMS + permutation), the reference's
+    calls per request on the request's exported inputs, m capped at 14 and r
+    at 2 (gdn: 11 and 1); the rate of each method weighted by the full batch's
+    points (the hash-NUS method, which has no reference counterpart, at the
+    dnb2 rate)."""
+    from qmctoolscl_amd import batch as B
+    meth, m, d, r, seeds = batch_requests(quick)
+    rates, spent, sample_pts = {}, 0.0, 0
+    for k in (0, 1, 3):
+        pts, sec = 0, 0.0
+        for q in np.flatnonzero(meth == k)[:6]:
+            mm, dd, rr, sd = min(int(m[q]), 14 if k != 3 else 11), int(d[q]), min(int(r[q]), 2 if k != 3 else 1), \
+                int(seeds[q])
+            n = 1 << mm
+            if k == 0:
+                g, sh = B.lattice_inputs(sd, mm, dd, rr)
+                x = np.empty((1, n, dd))
+                xr = np.empty((rr, n, dd))
+                sec += _timeit(lambda: (C("lat_gen_natural", 1, n, dd, 0, g, x),
+                                        C("lat_shift_mod_1", rr, n, dd, 1, x, sh, xr)))
+            elif k == 1:
+                Cm, S, sh = B.dnb2_inputs(sd, 1, dd, rr)
+                Cl = np.empty((rr, dd, 32), dtype=np.uint64)
+                xb = np.empty((rr, n, dd), dtype=np.uint64)
+                xrb = np.empty_like(xb)
+                x = np.empty((rr, n, dd))
+                ls, tm = np.full(rr, 32, np.uint64), np.full(rr, 64, np.uint64)
+                sec += _timeit(lambda: (C("dnb2_linear_matrix_scramble", rr, dd, 32, 1, 32, S, Cm, Cl),
+                                        C("dnb2_gen_natural", rr, n, dd, 0, 32, Cl, xb),
+                                        C("dnb2_digital_shift", rr, n, dd, rr, ls, xb, sh, xrb),
+                                        C("dnb2_integer_to_float", rr, n, dd, tm, xrb, x)))
+            else:
+                bases, S, perms = B.gdn_inputs(sd, mm, rr, j0=0, dj=dd)
+                bmax = int(bases.max())
+                Cm = np.ascontiguousarray(np.tile(np.eye(mm, dtype=np.uint64)[None, None], (1, dd, 1, 1)))
+                Cl = np.empty((rr, dd, mm, 52), dtype=np.uint64)
+                xd = np.empty((rr, n, dd, 52), dtype=np.uint64)
+                xp = np.empty_like(xd)
+                x = np.empty((rr, n, dd))
+                sec += _timeit(lambda: (C("gdn_linear_matrix_scramble", rr, dd, mm, 1, 1, mm, 52, bases, S, Cm, Cl),
+                                        C("gdn_gen_natural", rr, n, dd, 1, mm, 52, 0, bases, Cl, xd),
+                                        C("gdn_digital_permutation", rr, n, dd, rr, 1, 52, 52, bmax, perms, xd, xp),
+                                        C("gdn_integer_to_float", rr, n, dd, 1, 52, bases, xp, x)))
+            pts += rr * n * dd
+        rates[k] = pts / sec
+        spent += sec
+        sample_pts += pts
+    full = {k: float(np.sum((np.int64(1) << m[meth == k].astype(np.int64)) * d[meth == k] * r[meth == k]))
+            for k in range(4)}
+    rates[2] = rates[1]
+    t_full = sum(full[k] / rates[k] for k in range(4))
+    return {"value": round(sum(full.values()) / t_full / 1e9, 5), "unit": "Gpts/s",
+            "sample": "first 6 requests of the lattice, dnb2 LMS+shift and gdn LMS+permutation methods, m capped at "
+                      "14, r at 2 (gdn 11, 1): the reference's call chain per request on the exported inputs (%.3g "
+                      "points in "
+                      "%.1f s); per-method rates weighted by the batch's points (hash NUS at the dnb2 rate: no "
+                      "reference counterpart)" % (sample_pts, spent),
+            "per_method_Mpts_s": {n_: round(rates[k] / 1e6, 3) for k, n_ in
+                                  ((0, "lattice"), (1, "dnb2_lms_shift"), (3, "gdn_lms_perm"))},
+            "projected_full_batch_s": round(t_full, 1)}
+
+
+def cpu_legs(C):
+    """Configs 0, 1, 2, 4 on one core, bounded samples, scale stated."""
+    from qmctoolscl_amd import rand_funcs as rf
+    legs = {}
+    U = np.uint64
+    # config 0: dnb2, n=2^16, d=8, r=4, full size
+    rng = np.random.Generator(np.random.SFC64(0))
+    n, d, r, t = 1 << 16, 8, 4, 32
+    Cm = np.empty((1, d, t), dtype=np.uint64)
+    for j in range(d):
+        for k in range(t):
+            above = rng.integers(0, 2, size=k, dtype=np.uint64)
+            Cm[0, j, k] = (1 << (t - 1 - k)) + sum(int(b) << (t - 1 - c) for c, b in enumerate(above))
+    S = rf.dnb2_get_linear_scramble_matrix(rng, r, d, t, t, False)
+    sh = rf.dnb2_get_digital_shifts(rng, r, d, 64)
+    Cl = np.empty((r, d, t), dtype=np.uint64)
+    xb = np.empty((r, n, d), dtype=np.uint64)
+    xrb = np.empty_like(xb)
+    x = np.empty((r, n, d))
+    sec = _timeit(lambda: (C("dnb2_linear_matrix_scramble", r, d, t, 1, t, S, Cm, Cl),
+                           C("dnb2_gen_natural", r, n, d, 0, t, Cl, xb),
+                           C("dnb2_digital_shift", r, n, d, r, np.full(r, 32, U), xb, sh, xrb),
+                           C("dnb2_integer_to_float", r, n, d, np.full(r, 64, U), xrb, x)))
+    legs["config0_dnb2"] = {"value": round(r * n * d / sec / 1e9, 5), "unit": "Gpts/s",
+                            "sample": "full size (n=2^16, d=8, r=4): LMS, natural order, 64-bit shift, float in %.2f s"
+                                      % sec}
+    # config 1: lattice, first 2^15 of 2^20 rows, d=64, r=16
+    rows, D, R = 1 << 15, 64, 16
+    g = (2 * rng.integers(0, 2**19, size=(1, D), dtype=np.uint64) + 1)
+    shifts = rf.lat_get_shifts(rng, R, D)
+    x1 = np.empty((1, rows, D))
+    xr = np.empty((R, rows, D))
+    sec = _timeit(lambda: (C("lat_gen_natural", 1, rows, D, 0, g, x1), C("lat_shift_mod_1", R, rows, D, 1, x1, shifts,
+                                                                         xr)))
+    legs["config1_lattice"] = {"value": round(R * rows * D / sec / 1e9, 5), "unit": "Gpts/s",
+                               "sample": "lat_gen_natural(r=1) + lat_shift_mod_1(r=16), first 2^15 of the 2^20 rows "
+                                         "(1/32 of the points), d=64, in %.2f s" % sec}
+    # config 2: gdn chain on the first 3^7 of 3^12 points, r=1 of 8
+    primes = [2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61, 67, 71, 73, 79, 83, 89, 97, 101,
+              103, 107, 109, 113, 127, 131]
+    n2, d2, mmax, tnew = 3 ** 7, 32, 20, 52
+    bases = np.array([primes], dtype=np.uint64)
+    Cg = rf.gdn_get_halton_generating_matrix(1, d2, mmax)
+    S2 = rf.gdn_get_linear_scramble_matrix(rng, 1, d2, mmax, tnew, 1, bases)
+    perms = rf.gdn_get_digital_permutations(rng, 1, d2, tnew, 1, bases)
+    Cl2 = np.empty((1, d2, mmax, tnew), dtype=np.uint64)
+    xd = np.empty((1, n2, d2, tnew), dtype=np.uint64)
+    xp = np.empty_like(xd)
+    x2 = np.empty((1, n2, d2))
+    sec = _timeit(lambda: (C("gdn_linear_matrix_scramble", 1, d2, mmax, 1, 1, mmax, tnew, bases, S2, Cg, Cl2),
+                           C("gdn_gen_natural", 1, n2, d2, 1, mmax, tnew, 0, bases, Cl2, xd),
+                           C("gdn_digital_permutation", 1, n2, d2, 1, 1, tnew, tnew, 131, perms, xd, xp),
+                           C("gdn_integer_to_float", 1, n2, d2, 1, tnew, bases, xp, x2)))
+    legs["config2_gdn"] = {"value": round(n2 * d2 / sec / 1e9, 6), "unit": "Gpts/s",
+                           "sample": "LMS -> gen_natural -> digital_permutation -> integer_to_float, first 3^7 of the "
+                                     "3^12 points, r=1 of 8 (1/648 of the points), d=32, 52 digits, in %.2f s" % sec}
+    # config 4: transforms, one vector each at 2^20, 256 vectors at 2^12
+    for m, V in ((20, 1), (12, 256)):
+        nn = 1 << m
+        xw = rng.standard_normal((V, nn))
+        sec = _timeit(lambda: C("fwht_1d_radix2", 1, V, nn // 2, xw))
+        yr, yi = rng.standard_normal((V, nn)), rng.standard_normal((V, nn))
+        twr, twi = np.empty(nn), np.empty(nn)
+        sec_f = _timeit(lambda: C("fft_bro_1d_radix2", 1, V, nn // 2, twr, twi, yr, yi))
+        legs["config4_2^%d" % m] = {"fwht_GBs": round(16 * V * nn / sec / 1e9, 3),
+                                    "fft_GBs": round(32 * V * nn / sec_f / 1e9, 3),
+                                    "sample": "%d vector(s) of 2^%d (config 4 batches 64 at 2^20, 4096 at 2^12): fwht "
+                                              "%.3f s, fft %.3f s" % (V, m, sec, sec_f)}
+    return legs
+
+
+def cpu_baseline(quick):
+    C = RefC()
+    if C.L is None:
+        return None
+    out = cpu_batch(C, quick)
+    out.update({"cores": 1, "kind": C.kind, "cpu_model": _cpu_model(), "nproc": os.cpu_count(),
+                "opencl_cpu": "not measured: no pyopencl / CPU OpenCL runtime in this image"})
+    out["legs"] = cpu_legs(C)
+    return out
+
+
+# ------------------------------------------------------------ sections
+
+def section_batch_vs_chain(torch, q, dev, timed):
+    """A bounded ragged sample drawn like the batch (lattice, dnb2 LMS+shift,
+    gdn LMS+permutation requests) three ways: the persistent batch, one fused
+    launch per request, and the reference's call sequence per request through
+    the drop-in kernels on the requests' own inputs (lat_gen_natural ->
+    lat_shift_mod_1; dnb2_linear_matrix_scramble -> dnb2_gen_natural ->
+    dnb2_digital_shift -> dnb2_integer_to_float; gdn_linear_matrix_scramble ->
+    gdn_gen_natural -> gdn_digital_permutation -> gdn_integer_to_float on
+    52-digit arrays), all on device tensors."""
+    from qmctoolscl_amd import batch as B
+    U = np.uint64
+    meth, m, d, r, seeds = B.sample_requests(48, 7, m_range=(8, 14), d_range=(1, 64), r_range=(1, 8),
+                                             methods=(0, 1, 3))
+    plan = B.Batch(meth, m, d, r, seeds, points_per_tile=1 << 15)
+    ring = torch.empty(plan.ring_elems, dtype=torch.float64, device=dev)
+    pts = plan.points
+    ms_b, _ = timed(lambda: plan.run(ring, graph=True), 5, 2)
+    ms_f, _ = timed(lambda: plan.run_per_call(ring), 5, 2)
+    del ring
+    # inputs per request, device-resident before timing; scratch shared across requests
+    n_max = int((1 << m.astype(np.int64)).max())
+    big = int(max((1 << int(mm)) * int(dd) * int(rr) for mm, dd, rr in zip(m, d, r)))
+    xdig = torch.empty(big * 52, dtype=torch.int64, device=dev)
+    xdig2 = torch.empty(big * 52, dtype=torch.int64, device=dev)
+    xb = torch.empty(big, dtype=torch.int64, device=dev)
+    xrb = torch.empty(big, dtype=torch.int64, device=dev)
+    out = torch.empty(big, dtype=torch.float64, device=dev)
+    x1 = torch.empty(n_max * 64, dtype=torch.float64, device=dev)
+    calls = []
+    for i in range(len(m)):
+        n, dd, rr, mm = 1 << int(m[i]), int(d[i]), int(r[i]), int(m[i])
+        if meth[i] == 0:
+            g, sh = B.lattice_inputs(seeds[i], mm, dd, rr)
+            g_t, sh_t = t64(torch, g.reshape(1, dd), dev), t64(torch, sh, dev)
+            calls.append(lambda n=n, dd=dd, rr=rr, g_t=g_t, sh_t=sh_t: (
+                q.lat_gen_natural(U(1), U(n), U(dd), U(0), g_t, x1, wait=False),
+                q.lat_shift_mod_1(U(rr), U(n), U(dd), U(1), x1, sh_t, out, wait=False)))
+        elif meth[i] == 1:
+            C, S, sh = B.dnb2_inputs(seeds[i], 1, dd, rr)
+            C_t, S_t, sh_t = t64(torch, C, dev), t64(torch, S, dev), t64(torch, sh, dev)
+            Cl = torch.empty((rr, dd, 32), dtype=torch.int64, device=dev)
+            ls = t64(torch, np.full(rr, 32, np.uint64), dev)
+            tm = t64(torch, np.full(rr, 64, np.uint64), dev)
+            calls.append(lambda n=n, dd=dd, rr=rr, C_t=C_t, S_t=S_t, sh_t=sh_t, Cl=Cl, ls=ls, tm=tm: (
+                q.dnb2_linear_matrix_scramble(U(rr), U(dd), U(32), U(1), U(32), S_t, C_t, Cl, wait=False),
+                q.dnb2_gen_natural(U(rr), U(n), U(dd), U(0), U(32), Cl, xb, wait=False),
+                q.dnb2_digital_shift(U(rr), U(n), U(dd), U(rr), ls, xb, sh_t, xrb, wait=False),
+                q.dnb2_integer_to_float(U(rr), U(n), U(dd), tm, xrb, out, wait=False)))
+        else:
+            # the request's own inputs: identity C (m x m), LMS to 52 digits, permutations
+            bases, S, perms = B.gdn_inputs(seeds[i], mm, rr, j0=0, dj=dd)
+            bmax = int(bases.max())
+            Cg = np.ascontiguousarray(np.tile(np.eye(mm, dtype=np.uint64)[None, None], (1, dd, 1, 1)))
+            b_t, S_t, C_t, p_t = (t64(torch, a, dev) for a in (bases, S, Cg, perms))
+            Cl = torch.empty((rr, dd, mm, 52), dtype=torch.int64, device=dev)
+            calls.append(lambda n=n, dd=dd, rr=rr, mm=mm, bmax=bmax, b_t=b_t, S_t=S_t, C_t=C_t, p_t=p_t, Cl=Cl: (
+                q.gdn_linear_matrix_scramble(U(rr), U(dd), U(mm), U(1), U(1), U(mm), U(52), b_t, S_t, C_t, Cl,
+                                             wait=False),
+                q.gdn_gen_natural(U(rr), U(n), U(dd), U(1), U(mm), U(52), U(0), b_t, Cl, xdig, wait=False),
+                q.gdn_digital_permutation(U(rr), U(n), U(dd), U(rr), U(1), U(52), U(52), U(bmax), p_t, xdig,
+                                          xdig2, wait=False),
+                q.gdn_integer_to_float(U(rr), U(n), U(dd), U(1), U(52), b_t, xdig2, out, wait=False)))
+
+    def chain():
+        for f in calls:
+            f()
+
+    ms_c, _ = timed(chain, 3, 1)
+    by = {}
+    for k, name in enumerate(("lattice", "dnb2_lms_shift", "gdn_lms_perm")):
+        sel = meth == (0, 1, 3)[k]
+        by[name] = int(np.sum((np.int64(1) << m[sel].astype(np.int64)) * d[sel] * r[sel]))
+    return {"workload": "48 ragged requests (lattice, dnb2 LMS+shift, gdn LMS+permutation), m in [8,14], "
+                        "d in [1,64], r in [1,8]", "points": pts, "points_by_method": by,
+            "batched_ms": round(ms_b, 4), "batched_Gpts_s": round(pts / ms_b / 1e6, 2),
+            "fused_per_request_ms": round(ms_f, 4), "fused_per_request_Gpts_s": round(pts / ms_f / 1e6, 2),
+            "dropin_chain_per_request_ms": round(ms_c, 4),
+            "dropin_chain_per_request_Gpts_s": round(pts / ms_c / 1e6, 2),
+            "speedup_vs_fused_per_request": round(ms_f / ms_b, 3),
+            "speedup_vs_dropin_chain": round(ms_c / ms_b, 3)}
 
 
 def section_dnb2(torch, q, fused, timed, dev):
@@ -264,146 +636,6 @@ def section_gdn(torch, q, dev, timed, r=8):
     return out
 
 
-def section_batch(torch, dev, timed, quick):
-    from qmctoolscl_amd import batch as B
-    meth, m, d, r, seeds = B.sample_requests(512, 2024)
-    if quick:
-        m = np.minimum(m, 16).astype(np.int32)
-    ring_elems = 4 << 30  # 32 GiB of doubles
-    ring = torch.empty(ring_elems, dtype=torch.float64, device=dev)
-    # tune on the same requests at 1/16 of the points per replication
-    m_small = np.maximum(m - 4, 4).astype(np.int32)
-    tune = B.autotune(meth, m_small, d, r, seeds, ring, tiles=(1 << 14, 1 << 15, 1 << 16, 1 << 17),
-                      grids_per_cu=(4, 8), reps=1, cache=False)
-    plan = B.Batch(meth, m, d, r, seeds, points_per_tile=tune["points_per_tile"], grid=tune["grid"],
-                   ring_elems=ring_elems)
-    pts = plan.points
-    log("batch: %d requests, %.3g points (%.2f TB written)" % (len(m), pts, 8 * pts / 1e12))
-    ms_b, _ = timed(lambda: plan.run(ring, graph=True), 2, 1)
-    ms_p, _ = timed(lambda: plan.run_per_call(ring), 2, 1)
-    per_method = {}
-    for mm in range(4):
-        sel = meth == mm
-        per_method[["lattice", "dnb2_lms_shift", "dnb2_nus_hash", "gdn_lms_perm"][mm]] = int(
-            np.sum((np.int64(1) << m[sel].astype(np.int64)) * d[sel] * r[sel]))
-    gbs = 8 * pts / ms_b / 1e6
-    # exactness at full size: per-slab digests of the batched and per-call runs
-    dg_b = plan.digests(ring)
-    dg_p = plan.digests(ring, per_call=True)
-    # each method's requests alone (capped at m = 16 to bound the time), same tile size
-    by_method = {}
-    mq = np.minimum(m, 16).astype(np.int32)
-    for k, name in enumerate(("lattice", "dnb2_lms_shift", "dnb2_nus_hash", "gdn_lms_perm")):
-        sel = meth == k
-        sub = B.Batch(meth[sel], mq[sel], d[sel], r[sel], seeds[sel], points_per_tile=tune["points_per_tile"],
-                      grid=tune["grid"], ring_elems=ring_elems)
-        ms_k, _ = timed(lambda: sub.run(ring, graph=True), 3, 1)
-        by_method[name] = {"points": sub.points, "ms": round(ms_k, 3), "Gpts_s": round(sub.points / ms_k / 1e6, 1),
-                           "GBs_written": round(8 * sub.points / ms_k / 1e6, 1),
-                           "frac_hbm": round(8 * sub.points / ms_k / 1e6 / HBM_PEAK_GBS, 4)}
-        del sub
-    return {"workload": "512 ragged requests, m in [%d,%d], d in [1,256], r in [1,64], counter-based inputs"
-                        % (int(m.min()), int(m.max())),
-            "bitexact_vs_per_call": bool(np.array_equal(dg_b, dg_p)), "slabs_checked": int(dg_b.size),
-            "by_method_m_le_16": by_method,
-            "points": pts, "points_by_method": per_method, "tiles": plan.total_tiles,
-            "autotune": {k: tune[k] for k in ("points_per_tile", "grid")}, "autotune_table": tune["table"],
-            "batched_ms": round(ms_b, 2), "batched_Gpts_s": round(pts / ms_b / 1e6, 2),
-            "batched_GBs_written": round(gbs, 1), "batched_frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
-            "per_call_ms": round(ms_p, 2), "per_call_Gpts_s": round(pts / ms_p / 1e6, 2),
-            "speedup_vs_per_call": round(ms_p / ms_b, 3)}
-
-
-def section_batch_vs_chain(torch, q, dev, timed):
-    """A bounded ragged sample (lattice, dnb2 LMS+shift, gdn LMS+permutation
-    requests) three ways: the persistent batch, one fused launch per request,
-    and the reference's call sequence per request through the drop-in kernels
-    (lat_gen_natural -> lat_shift_mod_1; dnb2_linear_matrix_scramble ->
-    dnb2_gen_natural -> dnb2_digital_shift -> dnb2_integer_to_float;
-    gdn_gen_natural -> gdn_digital_permutation -> gdn_integer_to_float on
-    tmax = 52 digit arrays), all on device tensors."""
-    from qmctoolscl_amd import batch as B
-    U = np.uint64
-    meth, m, d, r, seeds = B.sample_requests(48, 7, m_range=(8, 14), d_range=(1, 64), r_range=(1, 8),
-                                             methods=(0, 1, 3))
-    plan = B.Batch(meth, m, d, r, seeds, points_per_tile=1 << 15)
-    ring = torch.empty(plan.ring_elems, dtype=torch.float64, device=dev)
-    pts = plan.points
-    ms_b, _ = timed(lambda: plan.run(ring, graph=True), 5, 2)
-    ms_f, _ = timed(lambda: plan.run_per_call(ring), 5, 2)
-    del ring
-    primes = []
-    c = 2
-    while len(primes) < 64:
-        if all(c % p for p in primes):
-            primes.append(c)
-        c += 1
-    # inputs per request, device-resident before timing; scratch shared across requests
-    n_max = int((1 << m.astype(np.int64)).max())
-    big = int(max((1 << int(mm)) * int(dd) * int(rr) for mm, dd, rr in zip(m, d, r)))
-    xdig = torch.empty(big * 52, dtype=torch.int64, device=dev)
-    xdig2 = torch.empty(big * 52, dtype=torch.int64, device=dev)
-    xb = torch.empty(big, dtype=torch.int64, device=dev)
-    xrb = torch.empty(big, dtype=torch.int64, device=dev)
-    out = torch.empty(big, dtype=torch.float64, device=dev)
-    x1 = torch.empty(n_max * 64, dtype=torch.float64, device=dev)
-    rng = np.random.default_rng(3)
-    calls = []
-    for i in range(len(m)):
-        n, dd, rr, mm = 1 << int(m[i]), int(d[i]), int(r[i]), int(m[i])
-        if meth[i] == 0:
-            g, sh = B.lattice_inputs(seeds[i], mm, dd, rr)
-            g_t, sh_t = t64(torch, g.reshape(1, dd), dev), t64(torch, sh, dev)
-            calls.append(lambda n=n, dd=dd, rr=rr, g_t=g_t, sh_t=sh_t: (
-                q.lat_gen_natural(U(1), U(n), U(dd), U(0), g_t, x1, wait=False),
-                q.lat_shift_mod_1(U(rr), U(n), U(dd), U(1), x1, sh_t, out, wait=False)))
-        elif meth[i] == 1:
-            C, S, sh = B.dnb2_inputs(seeds[i], 1, dd, rr)
-            C_t, S_t, sh_t = t64(torch, C, dev), t64(torch, S, dev), t64(torch, sh, dev)
-            Cl = torch.empty((rr, dd, 32), dtype=torch.int64, device=dev)
-            ls = t64(torch, np.full(rr, 32, np.uint64), dev)
-            tm = t64(torch, np.full(rr, 64, np.uint64), dev)
-            calls.append(lambda n=n, dd=dd, rr=rr, C_t=C_t, S_t=S_t, sh_t=sh_t, Cl=Cl, ls=ls, tm=tm: (
-                q.dnb2_linear_matrix_scramble(U(rr), U(dd), U(32), U(1), U(32), S_t, C_t, Cl, wait=False),
-                q.dnb2_gen_natural(U(rr), U(n), U(dd), U(0), U(32), Cl, xb, wait=False),
-                q.dnb2_digital_shift(U(rr), U(n), U(dd), U(rr), ls, xb, sh_t, xrb, wait=False),
-                q.dnb2_integer_to_float(U(rr), U(n), U(dd), tm, xrb, out, wait=False)))
-        else:
-            bases = np.array([primes[:dd]], dtype=np.uint64)
-            bmax = int(bases.max())
-            mmax = mm  # base 2 needs m digits of the index
-            Cg = np.zeros((rr, dd, mmax, 52), dtype=np.uint64)
-            perms = np.zeros((rr, dd, 52, bmax), dtype=np.uint64)
-            for j in range(dd):
-                b = int(bases[0, j])
-                Cg[:, j] = rng.integers(0, b, size=(rr, mmax, 52), dtype=np.uint64)
-                perms[:, j, :, :b] = rng.integers(0, b, size=(rr, 52, b), dtype=np.uint64)
-            b_t, C_t, p_t = t64(torch, bases, dev), t64(torch, Cg, dev), t64(torch, perms, dev)
-            calls.append(lambda n=n, dd=dd, rr=rr, mmax=mmax, bmax=bmax, b_t=b_t, C_t=C_t, p_t=p_t: (
-                q.gdn_gen_natural(U(rr), U(n), U(dd), U(1), U(mmax), U(52), U(0), b_t, C_t, xdig, wait=False),
-                q.gdn_digital_permutation(U(rr), U(n), U(dd), U(rr), U(1), U(52), U(52), U(bmax), p_t, xdig,
-                                          xdig2, wait=False),
-                q.gdn_integer_to_float(U(rr), U(n), U(dd), U(1), U(52), b_t, xdig2, out, wait=False)))
-
-    def chain():
-        for f in calls:
-            f()
-
-    ms_c, _ = timed(chain, 3, 1)
-    by = {}
-    for k, name in enumerate(("lattice", "dnb2_lms_shift", "gdn_lms_perm")):
-        sel = meth == (0, 1, 3)[k]
-        by[name] = int(np.sum((np.int64(1) << m[sel].astype(np.int64)) * d[sel] * r[sel]))
-    return {"workload": "48 ragged requests (lattice, dnb2 LMS+shift, gdn LMS+permutation), m in [8,14], "
-                        "d in [1,64], r in [1,8]", "points": pts, "points_by_method": by,
-            "batched_ms": round(ms_b, 4), "batched_Gpts_s": round(pts / ms_b / 1e6, 2),
-            "fused_per_request_ms": round(ms_f, 4), "fused_per_request_Gpts_s": round(pts / ms_f / 1e6, 2),
-            "dropin_chain_per_request_ms": round(ms_c, 4),
-            "dropin_chain_per_request_Gpts_s": round(pts / ms_c / 1e6, 2),
-            "speedup_vs_fused_per_request": round(ms_f / ms_b, 3),
-            "speedup_vs_dropin_chain": round(ms_c / ms_b, 3)}
-
-
 def section_transforms(torch, q, dev, timed0):
     from qmctoolscl_amd.mfma import fft_bro_mfma, fwht_mfma, ifft_bro_mfma, relative_errors
 
@@ -455,15 +687,17 @@ def section_transforms(torch, q, dev, timed0):
                                    np.stack([yr.cpu().numpy(), yi.cpu().numpy()]))
         del yr, yi, zr, zi, fwork
         pts = V * n
-        # MFMA work: per 16-point axis, two 16x16x32 products per 256 outputs
+        # matrix-core work of the FWHT: per radix-16 axis, four 16x16x4 fp32
+        # products per 16x16 block = 32 flops per value, on the fp32 MFMA peak
         axes = m // 4
-        mfma_flops = pts * axes * 2 * 2 * 32 * 16 / 16
+        fwht_mfma_flops = pts * axes * 4 * (2 * 16 * 16 * 4) / 256
         out.append({"n": "2^%d" % m, "vectors": V,
                     "fwht_fp64_ms": round(ms64, 4), "fwht_fp64_GBs": round(16 * pts / ms64 / 1e6, 1),
                     "fwht_mfma_ms": round(msm, 4), "fwht_mfma_GBs": round(16 * pts / msm / 1e6, 1),
                     "mfma_speedup": round(ms64 / msm, 3),
                     "mfma_rel_l2": float("%.3g" % l2), "mfma_rel_max": float("%.3g" % mx),
-                    "mfma_util": float("%.3g" % (mfma_flops / (msm * 1e-3) / (BF16_PEAK_TFLOPS * 1e12))),
+                    "mfma_util": float("%.3g" % (fwht_mfma_flops / (msm * 1e-3) / (FP32_MFMA_PEAK_TFLOPS * 1e12))),
+                    "mfma_util_peak": "fp32 MFMA, %.1f TFLOP/s" % FP32_MFMA_PEAK_TFLOPS,
                     "fft_fp64_ms": round(msf, 4), "fft_fp64_GBs": round(32 * pts / msf / 1e6, 1),
                     "fft_mfma_ms": round(msfm, 4), "fft_mfma_GBs": round(32 * pts / msfm / 1e6, 1),
                     "fft_mfma_speedup": round(msf / msfm, 3),
@@ -471,9 +705,10 @@ def section_transforms(torch, q, dev, timed0):
                     "ifft_fp64_ms": round(msi, 4), "ifft_mfma_ms": round(msim, 4),
                     "ifft_mfma_speedup": round(msi / msim, 3),
                     "ifft_mfma_rel_l2": float("%.3g" % il2), "ifft_mfma_rel_max": float("%.3g" % imx),
-                    # 12 products of 16x16x32 per 256 complex values per radix-16 level
+                    # 12 products of 16x16x32 bf16 per 256 complex values per radix-16 level
                     "fft_mfma_util": float("%.3g" % (pts * (m // 4) * 768 / (msfm * 1e-3) /
-                                                     (BF16_PEAK_TFLOPS * 1e12)))})
+                                                     (BF16_PEAK_TFLOPS * 1e12))),
+                    "fft_mfma_util_peak": "bf16 MFMA, %.0f TFLOP/s" % BF16_PEAK_TFLOPS})
         del x0, x, ref, xm, work, xr, xi
         torch.cuda.empty_cache()
     return out
@@ -554,15 +789,18 @@ def section_halton(torch, dev, timed):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--cpu-rows", type=int, default=1 << 19)
+    ap.add_argument("--steps", type=int, default=3, help="timed batch steps (one step: the whole 512-request batch)")
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--sections", default="main,dnb2,transforms,gdn,halton,kernels,batch")
+    ap.add_argument("--sections", default="lattice,dnb2,transforms,gdn,halton,kernels,chain",
+                    help="further sections after the headline batch (also: none)")
     ap.add_argument("--quick-batch", action="store_true", help="cap request sizes at n=2^16")
+    ap.add_argument("--tune", default="", help="points_per_tile,grid,gdn_points_per_tile,gdn_cus (skip the autotuner)")
     ap.add_argument("--calibrate-copy", action="store_true",
-                    help="first run one plain 2 GiB device-to-device copy (a known-size kernel that scales "
+                    help="first run a plain 2 GiB fill and a 2 GiB device-to-device copy (known-size kernels that scale "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="time the batched launches only (no per-call run, digests or per-method runs): for profiles")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (generation shards over "
@@ -572,31 +810,11 @@ def main():
     sections = set(args.sections.split(","))
 
     import torch
-    from qmctoolscl_amd import fused
     import qmctoolscl_amd as q
 
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream(dev)
     timed = Timer(torch, stream)
-    n = 1 << N_LOG2
-    rng = np.random.Generator(np.random.SFC64(0))
-    g = (2 * rng.integers(0, 2**19, size=(1, D), dtype=np.uint64) + 1)
-    shifts = q.lat_get_shifts(rng, R, D)
-    g_t = t64(torch, g, dev)
-    s_t = t64(torch, shifts, dev)
-    xr = torch.empty((R, n, D), dtype=torch.float64, device=dev)
-    x1 = torch.empty((1, n, D), dtype=torch.float64, device=dev)
-    pts = R * n * D
-    U = np.uint64
-
-    def step_fused():
-        fused.lat_gen_shift(1, R, n, D, 1, 0, g_t, s_t, xr)
-
-    def step_per_call():
-        q.lat_gen_natural(U(1), U(n), U(D), U(0), g_t, x1, wait=False)
-        q.lat_shift_mod_1(U(R), U(n), U(D), U(1), x1, s_t, xr, wait=False)
-
-    ceiling = None
     if args.calibrate_copy:
         a = torch.ones(1 << 28, dtype=torch.float64, device=dev)
         b = torch.empty_like(a)
@@ -604,56 +822,51 @@ def main():
         torch.cuda.synchronize()
         del a, b
         torch.cuda.empty_cache()
-        # write-rate references over the same 8 GiB the workload writes: torch's fill and the
-        # library's 16-byte full-grid fill
-        L = q._lib.lib()
-        L.qmc_fill_dev.restype = ctypes.c_int
-        L.qmc_fill_dev.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_double, ctypes.c_void_p]
-        buf = torch.empty(pts, dtype=torch.float64, device=dev)
-        ms_tf, _ = timed(lambda: buf.fill_(0.5), 5, 1)
-        ms_qf, _ = timed(lambda: q._lib.check(L.qmc_fill_dev(buf.data_ptr(), pts, 0.5, ctypes.c_void_p(
-            torch.cuda.current_stream(dev).cuda_stream))), 5, 1)
-        ceiling = {"bytes": 8 * pts, "torch_fill_GBs": round(8 * pts / ms_tf / 1e6, 1),
-                   "fill16_GBs": round(8 * pts / ms_qf / 1e6, 1)}
-        del buf
-        torch.cuda.empty_cache()
-    log("main: lattice n=2^20 d=64 r=16")
-    ms_fused, wall = timed(step_fused, args.steps, args.warmup)
-    ms_pc, _ = timed(step_per_call, max(3, args.steps // 4), 1)
-    achieved = 8.0 * pts / (ms_fused * 1e-3) / 1e9  # GB/s written, algorithmic
-    traffic = None
-    if os.path.exists(PROFILE_TRAFFIC):
-        with open(PROFILE_TRAFFIC) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+    log("headline: 512-request ragged batch")
+    b = section_batch(torch, dev, timed, args)
+    pts = b["points"]
     out = {
         "metric": "batched vs per-call Gpts/s (% HBM peak); MFMA vs FP64 FWHT/FFT GB/s @2e-6",
-        "value": round(pts / (ms_fused * 1e-3) / 1e9, 3),
+        "value": round(pts / (b["ms"] * 1e-3) / 1e9, 3),
         "unit": "Gpts/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_fused, 4),
+        "ms_per_step": round(b["ms"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "randomized rank-1 lattice: natural order, n=2^20, d=64, r=16 shifts mod 1 "
-                               "(fused generate+shift, float64 r x n x d output)",
-                   "points_per_step": pts, "bytes_written_per_step": 8 * pts},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
-        "per_call": {"ms_per_step": round(ms_pc, 4), "value": round(pts / (ms_pc * 1e-3) / 1e9, 3),
-                     "unit": "Gpts/s", "kernels": "lat_gen_natural + lat_shift_mod_1"},
-        "wall_s_timed": round(wall, 3),
-        "sections": {},
+        "config": {"workload": "ragged batch of 512 generation requests (method uniform over lattice+shift, "
+                               "dnb2 LMS+shift, dnb2 hash-NUS, gdn LMS+permutation; n=2^m, m in [8,22], d in "
+                               "[1,256], r in [1,64]) in persistent hipGraph launches" +
+                               (" (m capped at 16)" if args.quick_batch else ""),
+                   "requests": 512, "points_per_step": pts, "bytes_written_per_step": 8 * pts},
+        "roofline": {"bound": "hbm", "achieved": round(b["GBs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(b["GBs"] / HBM_PEAK_GBS, 4),
+                     "traffic": None if args.quick_batch else committed_traffic(BATCH_TRAFFIC),
+                     "traffic_source": "committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (profiles/), not this run",
+                     "kernel": "batch_persistent_kernel (store-bound and gdn classes side by side), one graph replay"},
+        "per_call": {"ms_per_step": round(b["per_call_ms"], 3), "value": round(pts / b["per_call_ms"] / 1e6, 3),
+                     "unit": "Gpts/s", "speedup_of_batched": round(b["per_call_ms"] / b["ms"], 3),
+                     "form": "the same requests, one launch of the same kernels per request"},
+        "wall_s_timed": round(b["wall_s"], 3),
+        "sections": {"batch": b["detail"]},
     }
-    if ceiling is not None:
-        out["write_ceiling"] = ceiling
-    del x1
-    torch.cuda.empty_cache()
+    if "chain" in sections:
+        log("section: batch vs per-request drop-in chains")
+        out["sections"]["batch_vs_dropin_chain"] = section_batch_vs_chain(torch, q, dev, timed)
+        c = out["sections"]["batch_vs_dropin_chain"]
+        out["per_call"]["reference_chain_sub_batch"] = {k: c[k] for k in ("workload", "batched_Gpts_s",
+                                                                          "dropin_chain_per_request_Gpts_s",
+                                                                          "speedup_vs_dropin_chain")}
+    if "lattice" in sections:
+        log("section: lattice (config 1)")
+        out["sections"]["lattice"] = section_lattice(torch, q, dev, timed, 50, 20)
     if "dnb2" in sections:
         log("section: dnb2")
+        from qmctoolscl_amd import fused
         out["sections"]["dnb2"] = section_dnb2(torch, q, fused, timed, dev)
     if "transforms" in sections:
         log("section: transforms")
@@ -667,17 +880,9 @@ def main():
     if "halton" in sections:
         log("section: halton")
         out["sections"]["halton"] = section_halton(torch, dev, timed)
-    if "batch" in sections:
-        del xr
-        torch.cuda.empty_cache()
-        log("section: batch")
-        out["sections"]["batch"] = section_batch(torch, dev, timed, args.quick_batch)
-        torch.cuda.empty_cache()
-        log("section: batch vs per-request drop-in chains")
-        out["sections"]["batch_vs_chain"] = section_batch_vs_chain(torch, q, dev, timed)
     if not args.no_cpu:
         log("cpu baseline")
-        out["cpu_baseline"] = cpu_baseline(g, shifts, args.cpu_rows)
+        out["cpu_baseline"] = cpu_baseline(args.quick_batch)
     print(json.dumps(out))
 
 

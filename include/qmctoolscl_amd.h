@@ -315,13 +315,14 @@ QMC_EXPORT int qmc_mfma_probe_dev(const float *a, const float *b, float *d, qmc_
  * 3 Halton-type base-b LMS to 52 digits + uniform digit permutations; r x 2^m
  * x d float64 points, natural order, counter-based random inputs from
  * seeds[q]).  Outputs go to a ring of ring_elems doubles, each replication
- * slab contiguous at qmc_batch_slab_offset(plan, q, l).  gdn_level_cap > 0
+ * slab contiguous at qmc_batch_slab_offset(plan, q, l).  points_per_tile
+ * and gdn_points_per_tile (0 = the same) size the tiles.  gdn_level_cap > 0
  * limits the permutation levels gdn tiles read from their tables (the output
  * does not change; a test hook for the other path).  Calls on one plan must
  * not overlap. */
 QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t *m, const int32_t *d,
                                 const int32_t *r, const qmc_u64 *seeds, qmc_u64 points_per_tile, int grid,
-                                qmc_u64 ring_elems, int gdn_level_cap);
+                                qmc_u64 ring_elems, qmc_u64 gdn_points_per_tile, int gdn_level_cap);
 QMC_EXPORT void qmc_batch_free(void *plan);
 QMC_EXPORT qmc_u64 qmc_batch_total_tiles(void *plan);
 QMC_EXPORT qmc_u64 qmc_batch_points(void *plan);

@@ -39,7 +39,7 @@ def _declare(L):
         return
     L.qmc_batch_plan.restype = ctypes.c_void_p
     L.qmc_batch_plan.argtypes = [ctypes.c_int] + [_vp] * 5 + [ctypes.c_uint64, ctypes.c_int, ctypes.c_uint64,
-                                                               ctypes.c_int]
+                                                               ctypes.c_uint64, ctypes.c_int]
     L.qmc_batch_free.argtypes = [_vp]
     L.qmc_batch_total_tiles.restype = ctypes.c_uint64
     L.qmc_batch_total_tiles.argtypes = [_vp]
@@ -87,7 +87,7 @@ def sample_requests(count, seed, m_range=(8, 22), d_range=(1, 256), r_range=(1, 
 
 class Batch:
     def __init__(self, methods, m, d, r, seeds, points_per_tile=1 << 16, grid=0, ring_elems=None,
-                 gdn_level_cap=0):
+                 gdn_points_per_tile=0, gdn_cus=0, gdn_level_cap=0):
         self.methods = np.ascontiguousarray(methods, dtype=np.int32)
         self.m = np.ascontiguousarray(m, dtype=np.int32)
         self.d = np.ascontiguousarray(d, dtype=np.int32)
@@ -101,11 +101,15 @@ class Batch:
         L = _L()
         h = L.qmc_batch_plan(len(self.m), self.methods.ctypes.data, self.m.ctypes.data, self.d.ctypes.data,
                              self.r.ctypes.data, self.seeds.ctypes.data, self.points_per_tile, self.grid,
-                             self.ring_elems, int(gdn_level_cap))
+                             self.ring_elems, int(gdn_points_per_tile), int(gdn_level_cap))
         if not h:
             _lib.check()
         self._h = h
         self._free = L.qmc_batch_free
+        self.gdn_points_per_tile = int(gdn_points_per_tile) or self.points_per_tile
+        self.gdn_cus = 0
+        if gdn_cus:
+            self.set_gdn_cus(gdn_cus)
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -228,8 +232,10 @@ def owen_hash64(x, seed):
 
 # ------------------------------------------------------------------ autotuner
 
-DEFAULT_TILES = (1 << 13, 1 << 14, 1 << 15, 1 << 16, 1 << 17)
-DEFAULT_GRIDS_PER_CU = (2, 4, 8)
+DEFAULT_TILES = (1 << 14, 1 << 15, 1 << 16, 1 << 17)
+DEFAULT_GRIDS_PER_CU = (4, 8)
+DEFAULT_GDN_TILES = (1 << 15, 1 << 16, 1 << 17)
+DEFAULT_GDN_SHARES = (0.5, 0.625, 0.75, 0.875, 1.0)
 
 
 def _cache_path():
@@ -237,18 +243,35 @@ def _cache_path():
     return os.path.join(base, "batch_autotune.json")
 
 
-def autotune(methods, m, d, r, seeds, ring, tiles=DEFAULT_TILES, grids_per_cu=DEFAULT_GRIDS_PER_CU, reps=2,
-             cache=True):
-    """Time the persistent batch for every (points_per_tile, grid) candidate on
-    this batch and return the fastest as {"points_per_tile", "grid", "ms",
-    "table"}.  Results are cached per (device, batch signature)."""
+def _time_plan(plan, ring, reps):
     import torch
+    plan.run(ring)  # warm: captures the graph
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plan.run(ring)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / reps
+
+
+def autotune(methods, m, d, r, seeds, ring, tiles=DEFAULT_TILES, grids_per_cu=DEFAULT_GRIDS_PER_CU,
+             gdn_tiles=DEFAULT_GDN_TILES, gdn_shares=DEFAULT_GDN_SHARES, reps=2, cache=True):
+    """Pick the batch's tunables by timing it: the tile size and grid of the
+    lattice/dnb2 class on those requests alone, the gdn tile size on the gdn
+    requests alone, then the split of the CUs between the two classes on the
+    whole batch (the gdn kernel's share; the store-bound kernel runs on the
+    rest beside it).  Returns {"points_per_tile", "grid",
+    "gdn_points_per_tile", "gdn_cus", "ms", "table"}, cached per (device,
+    batch signature).  The searched shapes replace the reference's global-size
+    sweeps (gs_run.py:42-88) for this path."""
+    import torch
+    methods = np.asarray(methods, np.int32)
+    m, d, r, seeds = (np.asarray(a) for a in (m, d, r, seeds))
     dev = ring.device
     props = torch.cuda.get_device_properties(dev)
     cus = props.multi_processor_count
-    sig = "%s|%d|%d|%s" % (props.name, len(m), int(np.sum((np.int64(1) << np.asarray(m, np.int64)) *
-                                                          np.asarray(d) * np.asarray(r))),
-                           hash(np.asarray(seeds).tobytes()))
+    sig = "%s|%d|%d|%s" % (props.name, len(m), int(np.sum((np.int64(1) << m.astype(np.int64)) * d * r)),
+                           hash(seeds.tobytes() + methods.tobytes()))
     path = _cache_path()
     if cache and os.path.exists(path):
         try:
@@ -259,21 +282,43 @@ def autotune(methods, m, d, r, seeds, ring, tiles=DEFAULT_TILES, grids_per_cu=DE
         except (OSError, ValueError):
             pass
     table = []
-    best = None
-    for p in tiles:
-        for g in grids_per_cu:
-            plan = Batch(methods, m, d, r, seeds, points_per_tile=p, grid=g * cus, ring_elems=ring.numel())
-            plan.run(ring)  # warm: captures the graph
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(reps):
-                plan.run(ring)
-            torch.cuda.synchronize()
-            ms = (time.perf_counter() - t0) * 1e3 / reps
-            table.append({"points_per_tile": p, "grid": g * cus, "ms": round(ms, 3)})
-            if best is None or ms < best["ms"]:
-                best = {"points_per_tile": p, "grid": g * cus, "ms": round(ms, 3)}
+    n_ring = ring.numel()
+    c0 = methods != 3
+    best0 = (tiles[0], grids_per_cu[0] * cus)
+    if c0.any():
+        ms0 = None
+        for p in tiles:
+            for g in grids_per_cu:
+                plan = Batch(methods[c0], m[c0], d[c0], r[c0], seeds[c0], points_per_tile=p, grid=g * cus,
+                             ring_elems=n_ring)
+                ms = _time_plan(plan, ring, reps)
+                table.append({"class": "lattice+dnb2", "points_per_tile": p, "grid": g * cus, "ms": round(ms, 3)})
+                if ms0 is None or ms < ms0:
+                    ms0, best0 = ms, (p, g * cus)
+                del plan
+    bestg = gdn_tiles[0]
+    if (~c0).any():
+        msg = None
+        for p in gdn_tiles:
+            plan = Batch(methods[~c0], m[~c0], d[~c0], r[~c0], seeds[~c0], points_per_tile=p, ring_elems=n_ring)
+            ms = _time_plan(plan, ring, reps)
+            table.append({"class": "gdn", "gdn_points_per_tile": p, "ms": round(ms, 3)})
+            if msg is None or ms < msg:
+                msg, bestg = ms, p
             del plan
+    plan = Batch(methods, m, d, r, seeds, points_per_tile=best0[0], grid=best0[1], ring_elems=n_ring,
+                 gdn_points_per_tile=bestg)
+    best = None
+    shares = gdn_shares if (c0.any() and (~c0).any()) else (1.0,)
+    for sh in shares:
+        n_cus = max(1, min(cus, int(round(sh * cus))))
+        plan.set_gdn_cus(n_cus)
+        ms = _time_plan(plan, ring, reps)
+        table.append({"class": "both", "gdn_cus": n_cus, "ms": round(ms, 3)})
+        if best is None or ms < best["ms"]:
+            best = {"points_per_tile": best0[0], "grid": best0[1], "gdn_points_per_tile": bestg, "gdn_cus": n_cus,
+                    "ms": round(ms, 3)}
+    del plan
     best["table"] = table
     if cache:
         try:

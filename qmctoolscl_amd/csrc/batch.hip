@@ -1132,12 +1132,13 @@ QMC_EXPORT void qmc_batch_free(void *h);
 // (1..31 for lattices), d[q] in [1, 256], r[q] >= 1, seeds[q]; `points_per_tile`
 // (power of two) and `grid` (persistent workgroups; 0 = 4 per CU) are the
 // tunables; `ring_elems` is the output ring size in doubles (every replication
-// slab must fit); `gdn_level_cap` (0 = none) limits the permutation levels a
-// gdn tile reads from its tables, the rest taking the index-and-key path
-// (a test hook for that path; the output does not change).
+// slab must fit); `gdn_points_per_tile` (0 = points_per_tile) sizes the gdn
+// tiles; `gdn_level_cap` (0 = none) limits the permutation levels a gdn tile
+// reads from its tables, the rest taking the index-and-key path (a test hook
+// for that path; the output does not change).
 QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t *m, const int32_t *d,
                                 const int32_t *r, const u64 *seeds, u64 points_per_tile, int grid, u64 ring_elems,
-                                int gdn_level_cap) {
+                                u64 gdn_points_per_tile, int gdn_level_cap) {
     fill_primes();
     Plan *P = new Plan();
     P->ring_elems = ring_elems;
@@ -1159,7 +1160,7 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
         const u64 n = 1ull << q.m;
         u64 rows;
         if (q.method == M_GDN_LMS) {
-            rows = points_per_tile / kGdnCols;  // a gdn tile is kGdnCols columns
+            rows = (gdn_points_per_tile ? gdn_points_per_tile : points_per_tile) / kGdnCols;  // kGdnCols columns
             if (rows < (u64)kGdnLanes) rows = kGdnLanes;
         } else {
             rows = points_per_tile / (u64)q.d;

@@ -208,11 +208,25 @@ def test_autotuner_picks_a_measured_candidate(torch, tmp_path, monkeypatch):
     meth, m, d, r, seeds = B.sample_requests(16, 3, m_range=(6, 10), d_range=(1, 32), r_range=(1, 4))
     plan = B.Batch(meth, m, d, r, seeds)
     ring = torch.empty(plan.ring_elems, dtype=torch.float64, device="cuda")
-    best = B.autotune(meth, m, d, r, seeds, ring, tiles=(1 << 10, 1 << 12), grids_per_cu=(2, 4), reps=1)
-    assert len(best["table"]) == 4
-    assert best["ms"] == min(t["ms"] for t in best["table"])
-    again = B.autotune(meth, m, d, r, seeds, ring, tiles=(1 << 10, 1 << 12), grids_per_cu=(2, 4), reps=1)
-    assert again["points_per_tile"] == best["points_per_tile"]  # served from the cache
+    assert (meth == 3).any() and (meth != 3).any()
+    kw = dict(tiles=(1 << 10, 1 << 12), grids_per_cu=(2, 4), gdn_tiles=(1 << 10, 1 << 12), gdn_shares=(0.5, 1.0),
+              reps=1)
+    best = B.autotune(meth, m, d, r, seeds, ring, **kw)
+    tab = best["table"]
+    assert len(tab) == 4 + 2 + 2
+    # each stage keeps its fastest candidate, and the result is the fastest split
+    c0 = min((t for t in tab if t["class"] == "lattice+dnb2"), key=lambda t: t["ms"])
+    g = min((t for t in tab if t["class"] == "gdn"), key=lambda t: t["ms"])
+    both = [t for t in tab if t["class"] == "both"]
+    assert (best["points_per_tile"], best["grid"]) == (c0["points_per_tile"], c0["grid"])
+    assert best["gdn_points_per_tile"] == g["gdn_points_per_tile"]
+    assert best["ms"] == min(t["ms"] for t in both)
+    again = B.autotune(meth, m, d, r, seeds, ring, **kw)
+    assert again == best  # served from the cache
+    # the tuned plan computes the same points as any other
+    tuned = B.Batch(meth, m, d, r, seeds, points_per_tile=best["points_per_tile"], grid=best["grid"],
+                    gdn_points_per_tile=best["gdn_points_per_tile"], gdn_cus=best["gdn_cus"])
+    assert np.array_equal(tuned.digests(ring), plan.digests(ring))
 
 
 # Requests around the kernels' shape edges: one column, columns around a wave

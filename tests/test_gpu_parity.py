@@ -419,6 +419,8 @@ def _oracle_gdn_chain(mode, r, n, d, r_b, mmax, tmax, tnew, n_start, bmax, bases
     (1, 2, 1200, 2, 1, 6, 6, 8, 0, "b593"),           # value table too large (b = 593): u32 rows in LDS
     (2, 1, 900, 3, 1, 8, 12, 40, 0, "random"),        # shift table of 16 rows, raw shifts 16..39 past it
     (2, 1, 800, 2, 1, 6, 60, 60, 0, "b131"),          # b = 131 above the 126 rows a 64-digit shift table holds
+    (1, 2, 2000, 6, 1, 12, 52, 52, 0, "zerolead"),    # sums still 0 after 20 levels: every point past the
+    (1, 1, 3000, 3, 1, 12, 40, 52, 0, "zerolead"),    # walked levels (digits from the index), tail past tmax
 ])
 def test_fused_gdn_equals_chain(mode, r, n, d, r_b, mmax, tmax, tnew, n_start, kind):
     """qmc_gdn_gen_scramble_float_dev against the oracle's three-call chain,
@@ -460,6 +462,10 @@ def test_fused_gdn_equals_chain(mode, r, n, d, r_b, mmax, tmax, tnew, n_start, k
                     tab[l, j, t] = rng.permutation(bmax).astype(np.uint64)
         if kind == "wide":
             tab[:, :, ::2, 1] += np.uint64(2**40)
+        if kind == "zerolead":
+            # every digit of the first 20 levels maps to 0 (a table the reference
+            # accepts): the sums stay 0 there, so no point can stop before level 20
+            tab[:, :, :20, :] = 0
     elif mode == 2 and kind == "smallshift":
         tab = rng.integers(0, 2**64 - 1, size=(r, d, tnew), dtype=np.uint64)  # raw past tmax
         for l in range(r):
