@@ -50,7 +50,7 @@ enum {
 };
 
 /* ---------------------------------------------------------------- runtime */
-QMC_EXPORT int qmc_init(void);                 /* allocate per-device state on the current device */
+QMC_EXPORT int qmc_init(void);                 /* set up per-device state on the current device */
 QMC_EXPORT const char *qmc_version(void);
 QMC_EXPORT const char *qmc_last_error(void);
 QMC_EXPORT int qmc_last_error_code(void);
@@ -258,6 +258,19 @@ QMC_EXPORT int qmc_gdn_gen_scramble_float_dev(int mode, qmc_u64 r, qmc_u64 n, qm
                                               qmc_u64 mmax, qmc_u64 tmax, qmc_u64 tmax_new, qmc_u64 n_start,
                                               qmc_u64 bmax, const qmc_u64 *bases, const qmc_u64 *C,
                                               const qmc_u64 *tab, double *x, qmc_stream_t s);
+/* No counterpart in the reference.  The same call as
+ * qmc_gdn_gen_scramble_float_dev with identical results, for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops): it calls no hipMalloc/hipFree and no allocating library routine.
+ * Its workspace is a 4 MiB array per device that the library holds until the
+ * process ends; a call needing more than 2^20 words of it
+ * (r*d*mmax*tmax + r*d) fails with QMC_ERR_ARG before any launch.  Calls that
+ * overlap on one device share that array and race. */
+QMC_EXPORT int qmc_gdn_gen_scramble_float_noalloc_dev(int mode, qmc_u64 r, qmc_u64 n, qmc_u64 d, qmc_u64 r_b,
+                                                      qmc_u64 mmax, qmc_u64 tmax, qmc_u64 tmax_new,
+                                                      qmc_u64 n_start, qmc_u64 bmax, const qmc_u64 *bases,
+                                                      const qmc_u64 *C, const qmc_u64 *tab, double *x,
+                                                      qmc_stream_t s);
 
 /* ------------------------------------------- nested uniform scrambling
  * Replaces the pure-Python dnb2_nested_uniform_scramble (rand_funcs.py:162-266)

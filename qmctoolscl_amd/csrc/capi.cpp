@@ -26,6 +26,8 @@ hipStream_t g_stream[kMaxDevices] = {};
 
 }  // namespace
 
+int gdn_scratch_setup();
+
 int qmc_fail(int code, const char *fmt, ...) {
     va_list ap;
     va_start(ap, fmt);
@@ -41,6 +43,11 @@ int qmc_fail_hip(hipError_t e, const char *what) {
 
 // Device word that kernels OR status bits into when they meet input the
 // reference leaves undefined (a base below 2, a read past the end of an input).
+// It is a device symbol, zero when the code object loads, so no entry point
+// allocates memory for it; each device holds its own copy, found once per
+// device with hipGetSymbolAddress.
+static __device__ __align__(256) u32 status_word[64];
+
 u32 *qmc_status_ptr() {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
@@ -50,8 +57,9 @@ u32 *qmc_status_ptr() {
     std::lock_guard<std::mutex> lk(g_mu);
     if (!g_status[dev]) {
         u32 *p = nullptr;
-        if (hipMalloc((void **)&p, 256) != hipSuccess || hipMemset(p, 0, 256) != hipSuccess) {
-            qmc_fail(QMC_ERR_NOMEM, "cannot allocate the device status word");
+        const hipError_t e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(status_word));
+        if (e != hipSuccess || !p) {
+            qmc_fail(QMC_ERR_HIP, "cannot find the device status word: %s", hipGetErrorString(e));
             return nullptr;
         }
         g_status[dev] = p;
@@ -86,7 +94,7 @@ QMC_EXPORT const char *qmc_version(void) { return "qmctoolscl_amd 0.1.0 (gfx950)
 QMC_EXPORT int qmc_init(void) {
     if (!qmc_status_ptr()) return g_err_code ? g_err_code : QMC_ERR_HIP;
     if (!lib_stream()) return qmc_fail(QMC_ERR_HIP, "cannot create the library stream");
-    return QMC_OK;
+    return gdn_scratch_setup();
 }
 
 // Clears the status word on stream s (a one-thread kernel, so the clear is

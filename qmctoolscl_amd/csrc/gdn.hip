@@ -13,6 +13,7 @@
 // including wrap-around: gdn_gen_natural.c:40-50 reduces after every digit,
 // x = (x + dig*C) % b.  Where no step can wrap, the walks use exact modular
 // sums; every other element takes the step-by-step path.
+#include <mutex>
 #include <type_traits>
 
 #include "qmc_common.hpp"
@@ -883,6 +884,41 @@ Rows make_rows(u64 n, u64 d, u64 tdigits) {
 
 u32 *qmc_status_ptr();
 
+// Scratch of qmc_gdn_gen_scramble_float_noalloc_dev: the prefix table Q
+// (r*d*mmax*tmax words) followed by one walk flag per column (r*d words).
+// gdn_chain_setup_kernel writes every word the fused kernel reads in the same
+// call, so nothing in it needs clearing.  Each device holds its own copy until
+// the process ends; calls that overlap on one device race on it.
+constexpr u64 kScratchWords = 1ull << 20;  // 4 MiB
+static __device__ __align__(256) u32 scratch[kScratchWords];
+
+namespace {
+std::mutex g_scratch_mu;
+u32 *g_scratch[64] = {};
+
+// The current device's address of `scratch`, looked up once per device (in
+// host code the array's name is only a stand-in for it).
+int scratch_for_device(u32 **out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return qmc_fail(QMC_ERR_HIP, "no usable HIP device");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        u32 *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return qmc_fail(QMC_ERR_HIP, "cannot find the gdn scratch: %s", hipGetErrorString(e));
+        g_scratch[dev] = p;
+    }
+    *out = g_scratch[dev];
+    return QMC_OK;
+}
+}  // namespace
+
+// Called by qmc_init on the current device.
+int gdn_scratch_setup() {
+    u32 *p = nullptr;
+    return scratch_for_device(&p);
+}
+
 extern "C" {
 
 static int gen_common(const char *who, u64 r, u64 n, u64 d, u64 r_b, u64 mmax, u64 tmax, u64 n_start,
@@ -976,10 +1012,11 @@ static int digitwise(bool permute, const char *who, u64 r, u64 n, u64 d, u64 r_x
 // gdn_digital_shift, each with r_x = r -> gdn_integer_to_float, writing only
 // x (r, n, d).  Mode 0 converts the generated digits directly (tmax_new must
 // equal tmax).  tab: perms (r, d, tmax_new, bmax) or shifts (r, d, tmax_new).
-QMC_EXPORT int qmc_gdn_gen_scramble_float_dev(int mode, u64 r, u64 n, u64 d, u64 r_b, u64 mmax, u64 tmax,
-                                              u64 tmax_new, u64 n_start, u64 bmax, const u64 *bases, const u64 *C,
-                                              const u64 *tab, double *x, hipStream_t s) {
-    const char *who = "gdn_gen_scramble_float";
+// noalloc: the prefix table lives in `scratch` instead of a stream-ordered
+// allocation, and a call needing more than kScratchWords words is rejected.
+static int fused_gdn(const char *who, bool noalloc, int mode, u64 r, u64 n, u64 d, u64 r_b, u64 mmax, u64 tmax,
+                     u64 tmax_new, u64 n_start, u64 bmax, const u64 *bases, const u64 *C, const u64 *tab, double *x,
+                     hipStream_t s) {
     if (!(r && n && d)) return QMC_OK;
     if (mode < 0 || mode > 2) return qmc_fail(QMC_ERR_ARG, "%s: mode must be 0, 1 or 2", who);
     if (r_b == 0) return qmc_fail(QMC_ERR_ARG, "%s: r_b must be positive", who);
@@ -1030,7 +1067,16 @@ QMC_EXPORT int qmc_gdn_gen_scramble_float_dev(int mode, u64 r, u64 n, u64 d, u64
     const u64 nblocks = (a.ngroups + 7) / 8 * 8 * d;
     if (nblocks > 0x7fffffffull) return qmc_fail(QMC_ERR_ARG, "%s: r*n*d too large", who);
     u32 *ws = nullptr;
-    if (walk) QMC_HIP_CHECK(hipMallocAsync((void **)&ws, sizeof(u32) * (qwords + r * d), s));
+    if (walk && noalloc) {
+        if (qwords + r * d > kScratchWords)
+            return qmc_fail(QMC_ERR_ARG,
+                            "%s: needs %llu words of scratch (r*d*mmax*tmax + r*d), above the %llu it holds", who,
+                            (unsigned long long)(qwords + r * d), (unsigned long long)kScratchWords);
+        const int rc = scratch_for_device(&ws);
+        if (rc != QMC_OK) return rc;
+    } else if (walk) {
+        QMC_HIP_CHECK(hipMallocAsync((void **)&ws, sizeof(u32) * (qwords + r * d), s));
+    }
     a.Q = ws;
     a.okf = walk ? ws + qwords : nullptr;
     if (walk)
@@ -1064,7 +1110,7 @@ QMC_EXPORT int qmc_gdn_gen_scramble_float_dev(int mode, u64 r, u64 n, u64 d, u64
             return QMC_OK;
         });
         if (rc != QMC_OK) {
-            if (ws) (void)hipFreeAsync(ws, s);
+            if (ws && !noalloc) (void)hipFreeAsync(ws, s);
             return rc;
         }
     }
@@ -1098,9 +1144,25 @@ QMC_EXPORT int qmc_gdn_gen_scramble_float_dev(int mode, u64 r, u64 n, u64 d, u64
 #undef QMC_FUSE
         e = hipGetLastError();
     }
-    if (ws) (void)hipFreeAsync(ws, s);
+    if (ws && !noalloc) (void)hipFreeAsync(ws, s);
     if (e != hipSuccess) return qmc_fail_hip(e, who);
     return QMC_OK;
+}
+
+QMC_EXPORT int qmc_gdn_gen_scramble_float_dev(int mode, u64 r, u64 n, u64 d, u64 r_b, u64 mmax, u64 tmax,
+                                              u64 tmax_new, u64 n_start, u64 bmax, const u64 *bases, const u64 *C,
+                                              const u64 *tab, double *x, hipStream_t s) {
+    return fused_gdn("gdn_gen_scramble_float", false, mode, r, n, d, r_b, mmax, tmax, tmax_new, n_start, bmax, bases,
+                     C, tab, x, s);
+}
+
+// The same computation with no device allocation (for callers that cannot
+// allocate, e.g. under graph capture); the workspace is `scratch`.
+QMC_EXPORT int qmc_gdn_gen_scramble_float_noalloc_dev(int mode, u64 r, u64 n, u64 d, u64 r_b, u64 mmax, u64 tmax,
+                                                      u64 tmax_new, u64 n_start, u64 bmax, const u64 *bases,
+                                                      const u64 *C, const u64 *tab, double *x, hipStream_t s) {
+    return fused_gdn("gdn_gen_scramble_float_noalloc", true, mode, r, n, d, r_b, mmax, tmax, tmax_new, n_start, bmax,
+                     bases, C, tab, x, s);
 }
 
 int gdn_digital_permutation_dev(u64 r, u64 n, u64 d, u64 r_x, u64 r_b, u64 tmax, u64 tmax_new, u64 bmax,

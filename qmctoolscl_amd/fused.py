@@ -59,16 +59,19 @@ def dnb2_gen_shift_float(natural, r, n, d, r_x, n_start, mmax, C, lshifts, shift
     _lib.check(rc)
 
 
-def gdn_gen_scramble_float(mode, r, n, d, r_b, mmax, tmax, tmax_new, n_start, bmax, bases, C, tab, x, stream=None):
+def gdn_gen_scramble_float(mode, r, n, d, r_b, mmax, tmax, tmax_new, n_start, bmax, bases, C, tab, x, stream=None,
+                           no_alloc=False):
     """x = gdn_integer_to_float(r, n, d, r_b, tmax_new, bases, S(gdn_gen_natural(r, n, d, r_b, mmax, tmax,
     n_start, bases, C))) with S = the identity (mode 0, tmax_new == tmax),
     gdn_digital_permutation(r, n, d, r, r_b, tmax, tmax_new, bmax, tab, .) (mode 1)
     or gdn_digital_shift(r, n, d, r, r_b, tmax, tmax_new, bases, tab, .) (mode 2),
     in one pass that materializes no digits.  bases: (r_b, d), C: (r, d, mmax,
     tmax), tab: perms (r, d, tmax_new, bmax) or shifts (r, d, tmax_new),
-    x: (r, n, d) float64."""
+    x: (r, n, d) float64.  no_alloc=True takes the entry point that allocates
+    no device memory (its fixed workspace holds r*d*mmax*tmax + r*d <= 2^20
+    words; calls overlapping on one device race on it)."""
     L = _lib.lib()
-    fn = L.qmc_gdn_gen_scramble_float_dev
+    fn = L.qmc_gdn_gen_scramble_float_noalloc_dev if no_alloc else L.qmc_gdn_gen_scramble_float_dev
     fn.restype = ctypes.c_int
     fn.argtypes = [ctypes.c_int] + [ctypes.c_uint64] * 9 + [ctypes.c_void_p] * 5
     r, n, d, r_b, mmax, tmax, tmax_new, bmax = (int(v) for v in (r, n, d, r_b, mmax, tmax, tmax_new, bmax))
