@@ -61,7 +61,6 @@ struct Req {
     u32 clog;            // dnb2: log2 of the row stride between a chain's points
     u32 gcap;            // gdn: most permutation levels a tile reads from its tables
     u32 gng;             // gdn: column groups (tiles per row range)
-    u32 gslot;           // gdn: LDS bytes per column of a tile (the plan's largest)
     u64 gblk;            // gdn: word offset of the replication blocks in the request's tables
     u64 lw;              // gdn: words of one replication's column tables
     DivU32 gdiv;         // gdn: division by gng
@@ -270,23 +269,9 @@ __device__ __forceinline__ u32 fast_div(u32 x, u64 M) { return (u32)__umul64hi(M
 //             b = 2: u64 cmask (bit 51 - t = pi_t(0)), Q_0 .. Q_{m-1} (bits
 //                    51 - t, mod 2)
 //           info_j = colo_j | TB << 32 | mm << 40 | tmin << 48;
-//           the info words are followed by the column groups of the tiles
-//           (j0 | ncols << 16: up to kGdnCols adjacent columns with one TB),
-//           then the replication blocks
-
-// column groups of a gdn request: runs of up to 4 adjacent columns with the
-// same levels in LDS (j0, ncols)
-inline std::vector<std::pair<u32, u32>> gdn_groups(int d) {
-    std::vector<std::pair<u32, u32>> g;
-    for (int j = 0; j < d; j++) {
-        if (!g.empty() && g.back().second < 4 && j > 1 &&
-            gdn_levels(h_primes[j]) == gdn_levels(h_primes[g.back().first]))
-            g.back().second++;
-        else
-            g.push_back({(u32)j, 1u});
-    }
-    return g;
-}
+//           then desc_j (column j's LDS tables in its tile, gdn_plan), the
+//           column groups of the tiles (j0 | ncols << 16 | logC << 24), and
+//           the replication blocks
 
 // u16 entries per row of a column's tables in the workspace
 __host__ __device__ inline u32 gdn_row_even(u32 TB) { return (TB + 1) & ~1u; }
@@ -297,13 +282,99 @@ inline u64 gdn_col_words(u32 b, int m) {
     return ((b + mm) * TE * 2 + 7) / 8;
 }
 
-// LDS bytes of a gdn tile of column j: the permutation rows, the walk table
-// twice (scaled to row offsets, and plain), the reciprocals
-inline u32 gdn_col_lds(u32 b, int m) {
-    if (b == 2) return 8 * (1 + (u32)m);
-    const u32 TB = (u32)gdn_levels(b), mm = (u32)gdn_mm(b, m);
-    const u32 TP = (TB + 1) / 2;
-    return ((b * gdn_row_u16(TB) * 2 + 15) & ~15u) + 4 * mm * TP + 8 * mm * TP + 8 * (TB + 2);
+constexpr u32 kBlock1 = 1024;                                 // threads of a gdn workgroup
+constexpr u32 kGdnStage = 2 * kBlock1 * (u32)sizeof(double);  // two stages of step values, at LDS offset 0
+
+// Kinds of a column's LDS tables: base 2 (the walk words), permutation rows
+// (u16) or value rows (doubles); see WalkOdd
+enum { GK_B2 = 0, GK_PERM = 1, GK_VAL = 2 };
+
+// levels a column walks: the stored TB, or a smaller bucket under the level cap
+inline u32 gdn_walked(u32 TB, u32 gcap) {
+    return gcap >= TB ? TB : gcap >= 24 ? 24 : gcap >= 16 ? 16 : gcap >= 12 ? 12 : gcap >= 10 ? 10 : 8;
+}
+
+// LDS bytes of one column's tables in a tile (16-byte multiples): the rows,
+// the walk table twice (packed row offsets and plain), and for permutation
+// rows the reciprocals.  Row offsets are packed in u16 halves whose sums must
+// stay below 2^16: b * rowb < 2^15 (0 when that fails).
+inline u32 gdn_slot_bytes(u32 b, int m, u32 TW, int kind) {
+    if (kind == GK_B2) return (8 * (1 + (u32)m) + 15) & ~15u;
+    const u32 mm = (u32)gdn_mm(b, m);
+    const u32 rowb = kind == GK_VAL ? 8 * (TW + 1) : 2 * gdn_row_u16(TW);
+    if ((u64)b * rowb >= (1u << 15)) return 0;
+    const u32 rows = (b * rowb + 15) & ~15u, walk = (12 * mm * ((TW + 1) / 2) + 15) & ~15u;
+    return rows + walk + (kind == GK_VAL ? 0u : (8 * TW + 15) & ~15u);
+}
+
+// The column groups of a gdn request and every column's LDS tables: the 16
+// columns of each 128-byte line of a row in one group if their permutation
+// rows fit in the LDS left beside the stages, else halves of it, recursively;
+// then, in each group, value rows instead of permutation rows for the columns
+// with the most levels first, while the group still fits.  desc_j = LDS byte
+// offset | kind << 32 | walked levels << 40.
+constexpr u32 kGdnTablesMax = 160 * 1024 - 16 - kGdnStage;  // LDS beside the stages and the queue slots
+struct GdnPlan {
+    std::vector<u64> groups, desc;
+    u32 lds = 0;     // LDS bytes of the largest tile, stages included
+    int bad = -1;    // a column whose tables fit nowhere
+};
+
+inline void gdn_plan_cols(GdnPlan &p, int s, int e, int m, u32 gcap) {
+    const int n = e - s;
+    u32 C = 1, logC = 0;
+    while ((int)C < n) C *= 2, logC++;
+    std::vector<int> kind(n);
+    std::vector<u32> TW(n), bytes(n);
+    u64 total = 0;
+    for (int c = 0; c < n; c++) {
+        const u32 b = h_primes[s + c];
+        const u32 TB = (u32)gdn_levels(b);
+        TW[c] = b == 2 ? TB : gdn_walked(TB, gcap);
+        // 24 or 48 levels only for b <= 23: value rows of at most 4.6 KB
+        kind[c] = b == 2 ? GK_B2 : TW[c] >= 24 ? GK_VAL : GK_PERM;
+        bytes[c] = gdn_slot_bytes(b, m, TW[c], kind[c]);
+        if (!bytes[c]) {
+            p.bad = s + c;
+            return;
+        }
+        total += bytes[c];
+    }
+    if (total > kGdnTablesMax) {
+        if (n == 1) {
+            p.bad = s;
+            return;
+        }
+        gdn_plan_cols(p, s, s + (int)C / 2, m, gcap);
+        gdn_plan_cols(p, s + (int)C / 2, e, m, gcap);
+        return;
+    }
+    std::vector<int> order(n);
+    for (int c = 0; c < n; c++) order[c] = c;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return TW[a] > TW[b]; });
+    for (int c : order) {
+        if (kind[c] != GK_PERM) continue;
+        const u32 vb = gdn_slot_bytes(h_primes[s + c], m, TW[c], GK_VAL);
+        if (vb && total - bytes[c] + vb <= kGdnTablesMax) {
+            total = total - bytes[c] + vb;
+            bytes[c] = vb;
+            kind[c] = GK_VAL;
+        }
+    }
+    u32 off = kGdnStage;
+    for (int c = 0; c < n; c++) {
+        p.desc[s + c] = off | ((u64)kind[c] << 32) | ((u64)TW[c] << 40);
+        off += bytes[c];
+    }
+    p.groups.push_back((u64)s | ((u64)n << 16) | ((u64)logC << 24));
+    p.lds = std::max(p.lds, off);
+}
+
+inline GdnPlan gdn_plan(int d, int m, u32 gcap) {
+    GdnPlan p;
+    p.desc.assign(d, 0);
+    for (int a = 0; a < d && p.bad < 0; a += 16) gdn_plan_cols(p, a, std::min(d, a + 16), m, gcap);
+    return p;
 }
 
 u64 table_words(Req &q) {
@@ -316,8 +387,8 @@ u64 table_words(Req &q) {
             u64 w = 0;
             for (u64 j = 0; j < d; j++) w += gdn_col_words(h_primes[j], q.m);
             q.lw = w;
-            q.gng = (u32)gdn_groups((int)d).size();
-            q.gblk = d + q.gng;
+            q.gng = (u32)gdn_plan((int)d, q.m, q.gcap).groups.size();
+            q.gblk = 2 * d + q.gng;
             return q.gblk + r * w;
         }
     }
@@ -580,19 +651,18 @@ __device__ __forceinline__ void dnb2_tile(const Req &q, u64 l, u64 row0, const u
     }
 }
 
-// A gdn tile is kGdnCols adjacent columns (with the same number of levels in
-// LDS) times RT rows; each column has kGdnLanes lanes of the workgroup, lane i
-// walking rows i H .. i H + H - 1.  After every step the columns' values meet
-// in LDS and are stored row by row, so one store instruction writes 16 rows x
-// 32 bytes instead of 64 rows x 8 bytes (the L2 takes one write request per
-// line an instruction touches; with one column per instruction those requests
-// were the gdn class's bound).
-constexpr int kGdnCols = 4;
-#ifndef QMC_GDN_LANES
-#define QMC_GDN_LANES 256
-#endif
-constexpr int kGdnLanes = QMC_GDN_LANES;        // lanes (row chains) per column
-constexpr int kBlock1 = kGdnCols * kGdnLanes;   // threads of a gdn workgroup
+// A gdn tile is a group of C = 2^logC adjacent columns (C <= 16) times RT rows
+// of one replication.  Column c of the group has L = 1024 / C lanes of the
+// workgroup (threads [c L, (c + 1) L): whole waves), lane i walking the H = RT / L
+// consecutive points row0 + i H .. row0 + i H + H - 1.  Groups follow the
+// 128-byte lines of the rows: the 16 columns of a line share a group whenever
+// their tables fit in LDS together, else its halves or quarters do.  After
+// every step the group's values meet in LDS and are stored row by row, so one
+// store instruction writes whole lines (C = 16) or 64- or 32-byte pieces of
+// them.  A piece that other workgroups complete reaches HBM as a partial line
+// whenever the L2 evicts the line first, and HBM writes 32-byte pieces as
+// read-modify-writes (685 GB/s against 4.97 TB/s for whole lines, measured on
+// a plain kernel): with four columns per group those writes bounded the class.
 
 // Base 2 (j = 0).  Q = cmask, Q_0 .. Q_{m-1}.  The permuted digits of point k
 // are cmask ^ (XOR of S_i = Q_i ^ Q_{i-1} over the set bits i of k), and a
@@ -624,15 +694,18 @@ struct Walk2 {
 struct GdnCol {
     const double *rec, *thr;  // the base's constants (kTNmax each)
     const u32 *Qs, *Qu;       // walk table in LDS: packed pairs of row offsets, and plain
-    const double *srec;       // the reciprocals of the walked levels in LDS (read as broadcasts)
+    const double *srec;       // permutation rows: the reciprocals of the walked levels in LDS
     u64 seed, l, j, M;
-    u32 b, mm, rowb, base;    // base: LDS byte address of the permutation rows
+    u32 b, mm, rowb, base;    // rowb: bytes per table row; base: LDS byte address of the rows
 };
 
-// u16 at byte address a of the dynamic LDS, which starts at address 0 in the
-// kernels that use this (they declare no static LDS)
+// u16 and double at byte address a of the dynamic LDS, which starts at address
+// 0 in the kernels that use these (they declare no static LDS)
 __device__ __forceinline__ u32 lds_u16(u32 a) {
     return *(__attribute__((address_space(3))) const unsigned short *)(uintptr_t)a;
+}
+__device__ __forceinline__ double lds_f64(u32 a) {
+    return *(__attribute__((address_space(3))) const double *)(uintptr_t)a;
 }
 
 __device__ __forceinline__ double uniform_f64(const double *p) {
@@ -666,13 +739,18 @@ __device__ __forceinline__ u32 as_word(u16x2 p) { return __builtin_bit_cast(u32,
 
 // Consecutive points of a column of odd base b, the first TW levels from LDS.
 // A lane keeps the digits z_t of its point as byte offsets z_t * rowb into the
-// permutation rows, two levels per register (u16 halves: b * rowb < 2^15,
-// host-checked), so pi_t(z_t) is one u16 load with t in the immediate offset,
-// and a step adds a table row of packed offsets modulo b * rowb.  Every point
-// adds all TW levels: a lane whose sum has passed a level's threshold adds
-// terms that leave it unchanged (gdn_levels), so only the check at level TW,
-// once per point, decides whether the rare tail runs.
-template <int TW>
+// column's table rows, two levels per register (u16 halves: b * rowb < 2^15,
+// host-checked), so a level's entry is one load with t in the immediate
+// offset, and a step adds a table row of packed offsets modulo b * rowb.  The
+// rows hold either the permutations pi_t(z) (u16; the term is then
+// rec_t * pi_t(z), rec_t read from LDS as a broadcast) or, for VT, the terms
+// themselves, V[z][t] = RN(rec_t * pi_t(z)) as doubles: the reference's
+// product, rounded once as there (gdn_integer_to_float.c:36-37), so a level
+// costs one 8-byte load and the add.  Every point adds all TW levels: a lane
+// whose sum has passed a level's threshold adds terms that leave it unchanged
+// (gdn_levels), so only the check at level TW, once per point, decides whether
+// the rare tail runs.
+template <int TW, bool VT>
 struct WalkOdd {
     static constexpr int TP = (TW + 1) / 2;  // level pairs (the last half is padding when TW is odd)
     u32 z[TP];
@@ -704,19 +782,22 @@ struct WalkOdd {
                    ((acc[2 * p + 1] - fast_div(acc[2 * p + 1], M) * b) * c.rowb) << 16;
         dlow = k0 - fast_div(k0, M) * b;
     }
+    // the LDS address of level t's entry (t a constant after unrolling)
+    __device__ __forceinline__ u32 addr(const GdnCol &c, int t) const {
+        const u32 o = (t & 1) ? (z[t / 2] >> 16) : (z[t / 2] & 0xffffu);
+        return o + c.base + (VT ? 8u : 2u) * (u32)t;
+    }
+    // gdn_integer_to_float.c:36-37; v starts as the first term (0.0 + p = p)
     __device__ __forceinline__ double value(const GdnCol &c, double thw) const {
-        // gdn_integer_to_float.c:36-37; v starts as the first term (0.0 + p =
-        // p).  The reciprocals are read from LDS on every point (as
-        // broadcasts) rather than held in registers: the pointer passes
-        // through opaque().
-        const double *sr = c.srec + opaque(0);
-        const u32 base = c.base;
-        double v = sr[0] * (double)lds_u16((z[0] & 0xffffu) + base);
-        v += sr[1] * (double)lds_u16((z[0] >> 16) + base + 2);
+        // the reciprocals of the permutation rows are read from LDS on every
+        // point (as broadcasts) rather than held in registers: the pointer
+        // passes through opaque()
+        const double *sr = c.srec + (VT ? 0 : opaque(0));
+        double v = 0.0;
 #pragma unroll
-        for (int p = 1; p < TP; p++) {
-            v += sr[2 * p] * (double)lds_u16((z[p] & 0xffffu) + base + 4 * p);
-            if (2 * p + 1 < TW) v += sr[2 * p + 1] * (double)lds_u16((z[p] >> 16) + base + 4 * p + 2);
+        for (int t = 0; t < TW; t++) {
+            const double p = VT ? lds_f64(addr(c, t)) : sr[t] * (double)lds_u16(addr(c, t));
+            v = t == 0 ? p : v + p;
         }
         if (TW < kTNmax && __builtin_amdgcn_ballot_w64(v < thw) != 0) v = gdn_tail(c, k, TW, v);
         return v;
@@ -752,45 +833,75 @@ struct WalkOdd {
     }
 };
 
+// Byte offset of the walk tables in an odd-base column's LDS slot (after the
+// rows), and of the reciprocals (permutation rows only, after the walk tables)
+__host__ __device__ inline u32 gdn_qoff(u32 b, u32 TW, bool vt) {
+    return vt ? ((b * (TW + 1) * 8 + 15) & ~15u) : ((b * gdn_row_u16(TW) * 2 + 15) & ~15u);
+}
+__host__ __device__ inline u32 gdn_walk_bytes(u32 mm, u32 TW) { return (12 * mm * ((TW + 1) / 2) + 15) & ~15u; }
+
 // LDS fill of an odd-base column for TW walked levels out of TB stored, by the
-// column's kGdnLanes threads (t): the permutation rows (level pairs as u32
-// words; TW, TB and the row length are even), the walk table as packed pairs
-// of row offsets and plain, and the reciprocals.  Eight loads in flight per
-// thread before their LDS stores.
-template <int TW>
-__device__ __forceinline__ void gdn_fill_odd(u32 t, const u64 *__restrict__ blk, u32 b, u32 mm, u32 TB, u32 rowu,
-                                             unsigned char *Pl8, u32 *Qs, u32 *Qu, const double *__restrict__ rec,
-                                             double *srec) {
-    const u32 *Pg = (const u32 *)blk;
-    u32 *Pl = (u32 *)Pl8;
+// column's L threads (t): the rows -- permutations (level pairs as u32 words;
+// TW, TB and the row length are even) or the terms V[z][t] (rows of TW + 1
+// doubles, an odd count, so that random rows spread over the banks) -- then the
+// walk table as packed pairs of row offsets and plain, and for permutation rows
+// the reciprocals.  Eight loads in flight per thread before their LDS stores.
+template <int TW, bool VT>
+__device__ __forceinline__ void gdn_fill_odd(u32 t, u32 L, const u64 *__restrict__ blk, u32 b, u32 mm, u32 TB,
+                                             unsigned char *slot, u32 *Qs, u32 *Qu, const double *__restrict__ rec,
+                                             double *srec, u32 rowb) {
     constexpr u32 TP = (TW + 1) / 2;
-    const u32 TE = gdn_row_even(TB), TBp = TE / 2;
     constexpr u32 kF = 8;
-    const u32 nw = b * TP;
-    for (u32 e0 = 0; e0 < nw; e0 += kF * kGdnLanes) {
-        u32 w[kF];
+    const u32 TE = gdn_row_even(TB);
+    if (VT) {
+        const unsigned short *Pg = (const unsigned short *)blk;
+        double *V = (double *)slot;
+        const u32 nv = b * TW;
+        for (u32 e0 = 0; e0 < nv; e0 += kF * L) {
+            u32 w[kF];
 #pragma unroll
-        for (u32 f = 0; f < kF; f++) {
-            const u32 e = e0 + f * kGdnLanes + t;
-            const u32 z = e / TP, tp = e - z * TP;
-            w[f] = e < nw ? Pg[z * TBp + tp] : 0u;
-        }
+            for (u32 f = 0; f < kF; f++) {
+                const u32 e = e0 + f * L + t;
+                const u32 z = e / TW, tt = e - z * TW;
+                w[f] = e < nv ? Pg[z * TE + tt] : 0u;
+            }
 #pragma unroll
-        for (u32 f = 0; f < kF; f++) {
-            const u32 e = e0 + f * kGdnLanes + t;
-            const u32 z = e / TP, tp = e - z * TP;
-            if (e < nw) Pl[z * (rowu / 2) + tp] = w[f];
+            for (u32 f = 0; f < kF; f++) {
+                const u32 e = e0 + f * L + t;
+                const u32 z = e / TW, tt = e - z * TW;
+                if (e < nv) V[z * (TW + 1) + tt] = rec[tt] * (double)w[f];
+            }
         }
+    } else {
+        const u32 *Pg = (const u32 *)blk;
+        u32 *Pl = (u32 *)slot;
+        const u32 TBp = TE / 2, rw = rowb / 4;  // u32 words per global and per LDS row
+        const u32 nw = b * TP;
+        for (u32 e0 = 0; e0 < nw; e0 += kF * L) {
+            u32 w[kF];
+#pragma unroll
+            for (u32 f = 0; f < kF; f++) {
+                const u32 e = e0 + f * L + t;
+                const u32 z = e / TP, tp = e - z * TP;
+                w[f] = e < nw ? Pg[z * TBp + tp] : 0u;
+            }
+#pragma unroll
+            for (u32 f = 0; f < kF; f++) {
+                const u32 e = e0 + f * L + t;
+                const u32 z = e / TP, tp = e - z * TP;
+                if (e < nw) Pl[z * rw + tp] = w[f];
+            }
+        }
+        if (t < TW) srec[t] = rec[t];
     }
     const unsigned short *Qg = (const unsigned short *)blk + (u64)b * TE;
-    for (u32 e = t; e < mm * TP; e += kGdnLanes) {
+    for (u32 e = t; e < mm * TP; e += L) {
         const u32 cc = e / TP, tp = e - cc * TP;
         const u32 lo = Qg[cc * TE + 2 * tp], hi = 2 * tp + 1 < TW ? Qg[cc * TE + 2 * tp + 1] : 0u;
-        Qs[e] = (lo * 2 * rowu) | ((hi * 2 * rowu) << 16);
+        Qs[e] = (lo * rowb) | ((hi * rowb) << 16);
         Qu[2 * e] = lo;
         Qu[2 * e + 1] = hi;
     }
-    if (t < TW) srec[t] = rec[t];
 }
 
 // Tiles are drained by two persistent kernels: class 0 = lattice and dnb2
@@ -799,21 +910,28 @@ __device__ __forceinline__ void gdn_fill_odd(u32 t, const u64 *__restrict__ blk,
 __host__ __device__ constexpr int method_class(int method) { return method == M_GDN_LMS ? 1 : 0; }
 
 // The step loop of a gdn tile: every lane of an active column computes its
-// point, the values meet in LDS, and the workgroup stores them row by row
-// (thread e: row e / 4 of the step, column e % 4).  H barriers for every
-// workgroup, whatever its columns.
+// point, the values meet in LDS and the workgroup stores them row by row
+// (thread e: row e / C, column e % C of the step).  The stage slot of (row i,
+// column c) is i C + (c ^ ((i >> (4 - logC)) mod C)): 16 consecutive lanes of
+// a column, and 16 consecutive threads of the store, touch 16 different
+// 8-byte bank pairs.  H barriers for every wave of the workgroup, whatever its
+// column's kind (the waves of one group run different instantiations of this
+// loop, with equal barrier counts).
 template <class W, class SinkT, class Value, class Step>
-__device__ __forceinline__ void gdn_steps(W &w, bool act, u32 H, u32 i, u32 cg, u32 ncols, u32 rows, u64 row0,
-                                          u32 d, u32 j0, double *__restrict__ slab, double *stage, SinkT &out,
-                                          Value value, Step step) {
-    const u32 rr = threadIdx.x >> 2, cc = threadIdx.x & 3;
+__device__ __forceinline__ void gdn_steps(W &w, bool act, u32 H, u32 i, u32 cg, u32 logC, u32 ncols, u32 rows,
+                                          u64 row0, u32 d, u32 j0, double *__restrict__ slab, double *stage,
+                                          SinkT &out, Value value, Step step) {
+    const u32 C = 1u << logC, sh = 4 - logC;
+    const u32 rr = threadIdx.x >> logC, cc = threadIdx.x & (C - 1);
     const bool st = cc < ncols && rr < rows;
+    const u32 wslot = (i << logC) + (cg ^ ((i >> sh) & (C - 1)));
+    const u32 rslot = (rr << logC) + (cc ^ ((rr >> sh) & (C - 1)));
     double *dst = slab + (row0 + (u64)rr * H) * d + j0 + cc;
     for (u32 h = 0; h < H; h++) {
         double *sb = stage + (h & 1) * kBlock1;
-        if (act) sb[cg * kGdnLanes + i] = value(w);
+        if (act) sb[wslot] = value(w);
         __syncthreads();
-        if (st) out.put(dst, sb[cc * kGdnLanes + rr]);
+        if (st) out.put(dst, sb[rslot]);
         dst += d;
         if (act && h + 1 < H) step(w);
     }
@@ -821,9 +939,8 @@ __device__ __forceinline__ void gdn_steps(W &w, bool act, u32 H, u32 i, u32 cg, 
 
 // A gdn tile: rows [chunk RT, (chunk + 1) RT) of the column group g of
 // replication l, rem = chunk * G + g (the column groups of one row range are
-// consecutive tiles).  The group's columns j0 .. j0 + ncols - 1 share their
-// level count; workgroup threads [c kGdnLanes, (c + 1) kGdnLanes) take column
-// j0 + c.
+// consecutive tiles).  Group word: j0 | ncols << 16 | logC << 24; column word
+// (desc): LDS byte offset of its tables | kind << 32 | walked levels << 40.
 template <class SinkT>
 __device__ __forceinline__ void gdn_tile(const Req &q, u64 l, u32 rem, const u64 *__restrict__ ws,
                                          const u64 *__restrict__ tab, double *__restrict__ slab,
@@ -831,47 +948,50 @@ __device__ __forceinline__ void gdn_tile(const Req &q, u64 l, u32 rem, const u64
     const u32 d = (u32)q.d;
     const u32 chunk = div_q(rem, q.gdiv);
     const u32 g = rem - chunk * q.gng;
-    const u64 gw = tab[d + g];
+    const u64 gw = tab[2 * (u64)d + g];
     const u32 j0 = __builtin_amdgcn_readfirstlane((u32)(gw & 0xffff));
     const u32 ncols = __builtin_amdgcn_readfirstlane((u32)((gw >> 16) & 0xff));
-    const u32 cg = __builtin_amdgcn_readfirstlane(threadIdx.x / kGdnLanes);
-    const u32 i = threadIdx.x - cg * kGdnLanes;
+    const u32 logC = __builtin_amdgcn_readfirstlane((u32)((gw >> 24) & 0x7));
+    const u32 logL = 10 - logC, L = 1u << logL;  // lanes per column
+    const u32 cg = __builtin_amdgcn_readfirstlane(threadIdx.x >> logL);
+    const u32 i = threadIdx.x & (L - 1);
     const bool colact = cg < ncols;
     const u32 j = j0 + (colact ? cg : 0u);
     const u32 RT = q.rows_per_tile;
-    const u64 info = tab[j];
+    const u64 info = tab[j], desc = tab[d + j];
     const u32 colo = __builtin_amdgcn_readfirstlane((u32)info);
     const u32 TB = __builtin_amdgcn_readfirstlane((u32)((info >> 32) & 0xff));
     const u32 mm = __builtin_amdgcn_readfirstlane((u32)((info >> 40) & 0xff));
+    const u32 soff = __builtin_amdgcn_readfirstlane((u32)desc);
+    const u32 kind = __builtin_amdgcn_readfirstlane((u32)((desc >> 32) & 0xff));
+    const u32 TW = __builtin_amdgcn_readfirstlane((u32)((desc >> 40) & 0xff));
     const u64 *blk = tab + q.gblk + l * q.lw + colo;
     const u32 b = c_primes[j];
-    const u32 H = RT >= (u32)kGdnLanes ? RT / (u32)kGdnLanes : 1u;
-    const u32 rows = RT >= (u32)kGdnLanes ? (u32)kGdnLanes : RT;  // lanes with points
+    const u32 H = RT >= L ? RT >> logL : 1u;
+    const u32 rows = RT >= L ? L : RT;  // lanes with points
     const bool act = colact && i < rows;
     const u64 row0 = (u64)chunk * RT;
     const u32 k = (u32)row0 + i * H;
-    unsigned char *slot = lds + cg * q.gslot;
-    double *stage = (double *)(lds + kGdnCols * q.gslot);
-    if (b == 2) {  // column 0, alone in its group
+    unsigned char *slot = lds + soff;
+    double *stage = (double *)lds;
+    if (kind == GK_B2) {  // column 0
         u64 *Q = (u64 *)slot;
         if (colact)
-            for (u32 e = i; e <= (u32)q.m; e += kGdnLanes) Q[e] = blk[e];
+            for (u32 e = i; e <= (u32)q.m; e += L) Q[e] = blk[e];
         __syncthreads();
         Walk2 w;
         if (act) w.init(Q, k);
-        gdn_steps(w, act, H, i, cg, ncols, rows, row0, d, j0, slab, stage, out,
+        gdn_steps(w, act, H, i, cg, logC, ncols, rows, row0, d, j0, slab, stage, out,
                   [](Walk2 &x) { return x.value(); }, [](Walk2 &x) { x.step(); });
         return;
     }
-    // levels walked: the stored TB, or a smaller bucket under the level cap
-    const u32 TW = q.gcap >= TB ? TB : q.gcap >= 24 ? 24 : q.gcap >= 16 ? 16 : q.gcap >= 12 ? 12 : q.gcap >= 10 ? 10 : 8;
-    const u32 rowu = gdn_row_u16(TW);
+    const bool vt = kind == GK_VAL;
     const u32 TP = (TW + 1) / 2;
     // byte offsets in the slot (pointer arithmetic keeps the LDS address space)
-    const u32 qoff = (b * rowu * 2 + 15) & ~15u;
+    const u32 qoff = gdn_qoff(b, TW, vt);
     u32 *Qs = (u32 *)(slot + qoff);
     u32 *Qu = Qs + mm * TP;
-    double *srec = (double *)(slot + ((qoff + 12 * mm * TP + 7) & ~7u));
+    double *srec = (double *)(slot + qoff + gdn_walk_bytes(mm, TW));
     GdnCol c;
     c.rec = (const double *)ws + (u64)j * kTNmax;
     c.thr = (const double *)ws + (u64)(256 + j) * kTNmax;
@@ -884,21 +1004,23 @@ __device__ __forceinline__ void gdn_tile(const Req &q, u64 l, u32 rem, const u64
     c.M = recip_magic(b);
     c.b = b;
     c.mm = mm;
-    c.rowb = 2 * rowu;
+    c.rowb = vt ? 8 * (TW + 1) : 2 * gdn_row_u16(TW);
     c.base = (u32)(uintptr_t)(__attribute__((address_space(3))) unsigned char *)slot;
     const double thw = uniform_f64(c.thr + (TW < (u32)kTNmax ? TW : 0u));
-#define QMC_GDN_TW(T_)                                                                                      \
-    case T_: {                                                                                              \
-        if (colact) gdn_fill_odd<T_>(i, blk, b, mm, TB, rowu, slot, Qs, Qu, c.rec, srec);                  \
-        __syncthreads();                                                                                    \
-        WalkOdd<T_> w;                                                                                      \
-        if (act) w.init(c, k);                                                                              \
-        gdn_steps(w, act, H, i, cg, ncols, rows, row0, d, j0, slab, stage, out,                            \
-                  [&](WalkOdd<T_> &x) { return x.value(c, thw); }, [&](WalkOdd<T_> &x) { x.step(c); });   \
+#define QMC_GDN_TW(T_, VT_)                                                                                    \
+    case (T_) | ((VT_) ? 0x100 : 0): {                                                                         \
+        if (colact) gdn_fill_odd<T_, VT_>(i, L, blk, b, mm, TB, slot, Qs, Qu, c.rec, srec, c.rowb);            \
+        __syncthreads();                                                                                       \
+        WalkOdd<T_, VT_> w;                                                                                    \
+        if (act) w.init(c, k);                                                                                 \
+        gdn_steps(w, act, H, i, cg, logC, ncols, rows, row0, d, j0, slab, stage, out,                         \
+                  [&](WalkOdd<T_, VT_> &x) { return x.value(c, thw); }, [&](WalkOdd<T_, VT_> &x) { x.step(c); }); \
     } break;
-    switch (TW) {
-        QMC_GDN_TW(8) QMC_GDN_TW(10) QMC_GDN_TW(12) QMC_GDN_TW(16) QMC_GDN_TW(24) QMC_GDN_TW(48)
-        default: break;  // never: gdn_levels gives one of these
+    switch (TW | (vt ? 0x100u : 0u)) {
+        QMC_GDN_TW(8, false) QMC_GDN_TW(10, false) QMC_GDN_TW(12, false) QMC_GDN_TW(16, false)
+        QMC_GDN_TW(8, true) QMC_GDN_TW(10, true) QMC_GDN_TW(12, true) QMC_GDN_TW(16, true)
+        QMC_GDN_TW(24, true) QMC_GDN_TW(48, true)
+        default: break;  // never: the planner gives one of these
     }
 #undef QMC_GDN_TW
 }
@@ -1160,8 +1282,8 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
         const u64 n = 1ull << q.m;
         u64 rows;
         if (q.method == M_GDN_LMS) {
-            rows = (gdn_points_per_tile ? gdn_points_per_tile : points_per_tile) / kGdnCols;  // kGdnCols columns
-            if (rows < (u64)kGdnLanes) rows = kGdnLanes;
+            rows = (gdn_points_per_tile ? gdn_points_per_tile : points_per_tile) / 4;  // a tile of 4 columns
+            if (rows < 256) rows = 256;
         } else {
             rows = points_per_tile / (u64)q.d;
             if ((q.method == M_DNB2_LMS || q.method == M_DNB2_NUS) && rows < 1024) rows = 1024;  // 64-step chains at 2^s = 16
@@ -1170,6 +1292,7 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
         while (p2 * 2 <= rows) p2 *= 2;
         if (p2 > n) p2 = n;
         q.rows_per_tile = (u32)p2;
+        q.gcap = gdn_level_cap > 0 ? (u32)gdn_level_cap : 64u;
         q.tab = words;
         words += table_words(q);
         q.tiles_per_l = (u32)(n / p2) * (q.method == M_GDN_LMS ? q.gng : 1u);
@@ -1177,18 +1300,16 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
         tile += (u64)q.r * q.tiles_per_l;
         q.ddiv = make_div((u32)q.d);
         q.gdiv = make_div(q.method == M_GDN_LMS ? q.gng : 1u);
-        q.gcap = gdn_level_cap > 0 ? (u32)gdn_level_cap : 64u;
         if (q.method == M_DNB2_LMS || q.method == M_DNB2_NUS) q.clog = dnb2_chain_log(q.rows_per_tile, (u32)q.d);
-        if (q.method == M_GDN_LMS)
-            for (int j = 0; j < q.d; j++) {
-                P->lds_bytes[1] = std::max(P->lds_bytes[1], (gdn_col_lds(h_primes[j], q.m) + 15) & ~15u);
-                // the walk's packed row offsets: two of them add below 2^16
-                if (h_primes[j] > 2 && h_primes[j] * 2 * gdn_row_u16((u32)gdn_levels(h_primes[j])) >= (1u << 15)) {
-                    qmc_fail(QMC_ERR_ARG, "batch: gdn column %d has too large a permutation table", j);
-                    qmc_batch_free(P);
-                    return nullptr;
-                }
+        if (q.method == M_GDN_LMS) {
+            const GdnPlan gp = gdn_plan(q.d, q.m, q.gcap);
+            if (gp.bad >= 0) {
+                qmc_fail(QMC_ERR_ARG, "batch: gdn column %d has too large a permutation table", gp.bad);
+                qmc_batch_free(P);
+                return nullptr;
             }
+            P->lds_bytes[1] = std::max(P->lds_bytes[1], gp.lds);
+        }
         q.slabs = words;
         words += q.r;
         q.slab0 = (u32)P->nslabs;
@@ -1204,11 +1325,6 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
     }
     P->total_tiles = tile;
     P->ws_words = words;
-    // a gdn workgroup's LDS: kGdnCols column slots of the plan's largest
-    // column tables, then the two stages of a step's values
-    const u32 gslot = P->lds_bytes[1];
-    for (Req &q : P->reqs) q.gslot = gslot;
-    if (gslot) P->lds_bytes[1] = kGdnCols * gslot + 2 * kBlock1 * (u32)sizeof(double);
     for (int c = 0; c < 2; c++)
         for (const Req &q0 : P->reqs) {
             if (method_class(q0.method) != c) continue;
@@ -1271,8 +1387,9 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
             ws_host[q.tab + j] = colo | ((u64)gdn_levels(b) << 32) | ((u64)gdn_mm(b, q.m) << 40) | (tmin << 48);
             colo += gdn_col_words(b, q.m);
         }
-        const auto grp = gdn_groups(q.d);
-        for (size_t g = 0; g < grp.size(); g++) ws_host[q.tab + q.d + g] = grp[g].first | ((u64)grp[g].second << 16);
+        const GdnPlan gp = gdn_plan(q.d, q.m, q.gcap);
+        for (int j = 0; j < q.d; j++) ws_host[q.tab + q.d + j] = gp.desc[j];
+        for (size_t g = 0; g < gp.groups.size(); g++) ws_host[q.tab + 2 * (u64)q.d + g] = gp.groups[g];
         greq.push_back((u32)qi);
         gcol0.push_back(gcols);
         gcols += (u64)q.r * (u64)q.d;
