@@ -510,12 +510,19 @@ def section_dnb2(torch, q, fused, timed, dev):
         q.dnb2_digital_shift(U(r), U(n), U(d), U(r), lsh, xb, shd, xrb, wait=False)
         q.dnb2_integer_to_float(U(r), U(n), U(d), tm, xrb, x, wait=False)
 
+    def one_launch_step():
+        fused.dnb2_lms_gen_shift_float(1, r, n, d, 1, 0, t, t, Sd, Cd, lsh, shd, tm, xrb, x)
+
+    ms_1, _ = timed(one_launch_step, 200, 10)
     ms_f, _ = timed(fused_step, 200, 10)
     ms_c, _ = timed(chain_step, 200, 10)
     pts = r * n * d
-    out = {"workload": "dnb2 LMS + natural order + 64-bit digital shift + float, n=2^16, d=8, r=4, tmax=32",
-           "points": pts, "fused_ms": round(ms_f, 4), "fused_Gpts_s": round(pts / ms_f / 1e6, 2),
-           "fused_GBs_written": round(16 * pts / ms_f / 1e6, 1),
+    out = {"workload": "dnb2 LMS + natural order + 64-bit digital shift + float, n=2^16, d=8, r=4, tmax=32 "
+                       "(writes the shifted integers and their doubles, 32 MiB)",
+           "points": pts, "one_launch_ms": round(ms_1, 4), "one_launch_Gpts_s": round(pts / ms_1 / 1e6, 2),
+           "one_launch_GBs_written": round(16 * pts / ms_1 / 1e6, 1),
+           "fused_ms": round(ms_f, 4), "fused_Gpts_s": round(pts / ms_f / 1e6, 2),
+           "fused_GBs_written": round(16 * pts / ms_f / 1e6, 1), "fused": "LMS launch + fused generation launch",
            "per_call_ms": round(ms_c, 4), "per_call_Gpts_s": round(pts / ms_c / 1e6, 2)}
     # nested uniform scrambling of the same unscrambled net (host trees with numpy draws in the
     # reference's order, GPU walk), 32 -> 64 bits; a repeat call on the grown trees draws nothing

@@ -248,6 +248,18 @@ QMC_EXPORT int qmc_dnb2_gen_shift_float_dev(int natural, qmc_u64 r, qmc_u64 n, q
                                             qmc_u64 n_start, qmc_u64 mmax, const qmc_u64 *C,
                                             const qmc_u64 *lshifts, const qmc_u64 *shiftsb,
                                             const qmc_u64 *tmaxes, qmc_u64 *xrb, double *x, qmc_stream_t s);
+/* dnb2_linear_matrix_scramble(r, d, mmax, r_C, tmax_new, S, C) ->
+ * dnb2_gen_<natural|gray>(r, ...) -> dnb2_digital_shift(r, ..., r_x = r, ...) ->
+ * dnb2_integer_to_float(r, ...) in one launch (the scrambled columns never
+ * leave LDS); S (r, d, tmax_new), C (r_C, d, mmax); needs mmax, tmax_new <= 64,
+ * d <= 256 and d * (2 mmax + tmax_new) <= 6144; xrb and/or x may be NULL.  Replaces the chain of
+ * dnb2_linear_matrix_scramble.c:3-16, dnb2_gen_natural.c:3-15 (or
+ * dnb2_gen_gray.c:3-15), dnb2_digital_shift.c:3-16, dnb2_integer_to_float.c:3-14. */
+QMC_EXPORT int qmc_dnb2_lms_gen_shift_float_dev(int natural, qmc_u64 r, qmc_u64 n, qmc_u64 d, qmc_u64 r_C,
+                                                qmc_u64 n_start, qmc_u64 mmax, qmc_u64 tmax_new,
+                                                const qmc_u64 *S, const qmc_u64 *C, const qmc_u64 *lshifts,
+                                                const qmc_u64 *shiftsb, const qmc_u64 *tmaxes, qmc_u64 *xrb,
+                                                double *x, qmc_stream_t s);
 
 /* gdn_gen_natural(r, ...) -> mode 1: gdn_digital_permutation(r, ..., r_x = r, ...)
  * or mode 2: gdn_digital_shift(r, ..., r_x = r, ...) -> gdn_integer_to_float,

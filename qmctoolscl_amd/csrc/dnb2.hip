@@ -16,6 +16,9 @@
 
 namespace {
 
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
 // XOR of the columns selected by the bits of q
 __device__ __forceinline__ u64 xor_cols(const u64 *__restrict__ col, u64 q) {
     u64 v = 0;
@@ -92,6 +95,112 @@ __global__ __launch_bounds__(kBlock) void dnb2_gen_kernel(RunMap m, u64 mmax,
             for (int b = 0; b <= c; b++) v ^= col[b];
         } else {
             v ^= col[c];
+        }
+    }
+}
+
+// dnb2_linear_matrix_scramble -> dnb2_gen_<order> -> dnb2_digital_shift (r_x
+// = r) -> dnb2_integer_to_float in one launch.  Block (chunk, l) first builds
+// the d * mmax scrambled columns of replication l in LDS (each a GF(2)
+// product of S's rows, staged in LDS, and one column of C,
+// dnb2_linear_matrix_scramble.c:33-48) and their prefix XORs, then computes
+// rows [chunk R, (chunk + 1) R), R = 8 floor(256 / d): thread t walks the 8
+// rows of run t / d in column t % d (the first value XORs the columns
+// selected by the bits of its index -- Gray order: of the index's Gray code --
+// and every later one a prefix XOR (natural order) or one column (Gray
+// order), as the index grows by one), the shifted integers meet in LDS, and
+// the block writes its R * d consecutive elements of both outputs with
+// 16-byte stores.  Natural order over a range that Gray codes do not map onto
+// itself writes only the rows the reference's walk reaches (the rest stay
+// untouched, as there).
+constexpr u32 kLmsRun = 8;  // rows per thread
+
+template <bool NATURAL>
+__global__ __launch_bounds__(kBlock) void dnb2_lms_gen_kernel(u64 n, u64 d, u64 n_start, u64 mmax, u64 nC,
+                                                              u64 tmax_new, u32 R, int check,
+                                                              const u64 *__restrict__ S, const u64 *__restrict__ C,
+                                                              const u64 *__restrict__ lshifts,
+                                                              const u64 *__restrict__ shiftsb,
+                                                              const u64 *__restrict__ tmaxes, u64 *__restrict__ xrb,
+                                                              double *__restrict__ x) {
+    extern __shared__ u64 lds_u64[];
+    u64 *cl = lds_u64;               // [d][mmax] scrambled columns
+    u64 *pre = cl + d * mmax;        // [d][mmax] prefix XORs
+    u64 *srows = pre + d * mmax;     // [d][tmax_new] rows of S
+    u64 *stage = srows + d * tmax_new;  // [R * d] shifted integers of the block's rows
+    const u64 l = blockIdx.y;
+    for (u64 e = threadIdx.x; e < d * tmax_new; e += blockDim.x) srows[e] = S[l * d * tmax_new + e];
+    __syncthreads();
+    for (u64 e = threadIdx.x; e < d * mmax; e += blockDim.x) {
+        const u64 j = e / mmax;
+        const u64 col = C[(l * d * mmax + e) % nC];
+        const u64 *srow = srows + j * tmax_new;
+        u64 out = 0;
+        for (u64 t = 0; t < tmax_new; t++)
+            if (__popcll(srow[t] & col) & 1) out += 1ull << ((tmax_new - t - 1) & 63);
+        cl[e] = out;
+    }
+    __syncthreads();
+    for (u64 e = threadIdx.x; e < d * mmax; e += blockDim.x) {
+        const u64 j = e / mmax;
+        u64 acc = 0;
+        for (u64 f = j * mmax; f <= e; f++) acc ^= cl[f];
+        pre[e] = acc;
+    }
+    __syncthreads();
+    const u64 i0 = (u64)blockIdx.x * R;
+    const u32 rows = (u32)(n - i0 < (u64)R ? n - i0 : (u64)R);
+    const u64 lsh = lshifts[l] & 63;  // x86 masks variable shift counts to 6 bits
+    const u32 run = threadIdx.x / (u32)d, j = threadIdx.x - run * (u32)d;
+    if (run * kLmsRun < rows) {
+        const u64 *colj = cl + j * mmax, *prej = pre + j * mmax;
+        const u64 sh = shiftsb[l * d + j];
+        const u32 r0 = run * kLmsRun, r1 = r0 + kLmsRun < rows ? r0 + kLmsRun : rows;
+        u64 k = i0 + r0 + n_start;
+        u64 v = xor_cols(colj, NATURAL ? k : gray_of(k));
+        for (u32 i = r0;;) {
+            stage[(u64)i * d + j] = (v << lsh) ^ sh;
+            if (++i >= r1) break;
+            k++;
+            const int c = __ffsll((long long)k) - 1;  // ctz(k)
+            v ^= NATURAL ? prej[c] : colj[c];
+        }
+    }
+    __syncthreads();
+    const int tm = (int)tmaxes[l];
+    const u64 base = l * n * d + i0 * d, cnt = (u64)rows * d;
+    for (u64 e = 2 * (u64)threadIdx.x; e < cnt; e += 2 * (u64)blockDim.x) {
+        const bool two = e + 1 < cnt;
+        bool w0 = true, w1 = two;
+        if (NATURAL && check) {
+            const u64 k0 = i0 + e / d + n_start, k1 = i0 + (e + 1) / d + n_start;
+            const u64 g0 = gray_inverse(k0), g1 = gray_inverse(k1);
+            w0 = g0 >= n_start && g0 - n_start < n;
+            w1 = two && g1 >= n_start && g1 - n_start < n;
+        }
+        const u64 a = stage[e], b = two ? stage[e + 1] : 0;
+        if (w0 && w1 && ((base + e) & 1) == 0) {
+            if (xrb) {
+                u64x2 o;
+                o.x = a;
+                o.y = b;
+                store_stream((u64x2 *)(xrb + base + e), o);
+            }
+            if (x) {
+                f64x2 o;
+                o.x = ldexp((double)a, -tm);
+                o.y = ldexp((double)b, -tm);
+                store_stream((f64x2 *)(x + base + e), o);
+            }
+        } else {
+            if (w0) {
+                if (xrb) store_stream(xrb + base + e, a);
+                if (x) store_stream(x + base + e, ldexp((double)a, -tm));
+            }
+            if (w1) {
+                if (xrb) store_stream(xrb + base + e + 1, b);
+                if (x) store_stream(x + base + e + 1, ldexp((double)b, -tm));
+            }
         }
     }
 }
@@ -327,6 +436,38 @@ int qmc_dnb2_gen_shift_float_dev(int natural, u64 r, u64 n, u64 d, u64 r_x, u64 
     const u64 rx = r_x < r ? r_x : r;
     FuseArgs f{r, lshifts, shiftsb, tmaxes, xrb, x};
     return launch_gen<1>(natural != 0, rx, n, d, n_start, mmax, C, nullptr, f, s);
+}
+
+// Fused dnb2_linear_matrix_scramble(r, d, mmax, r_C, tmax_new, S, C) ->
+// dnb2_gen_<order>(r) -> dnb2_digital_shift(r, r_x = r) ->
+// dnb2_integer_to_float(r) in one launch, writing xrb and/or x.
+int qmc_dnb2_lms_gen_shift_float_dev(int natural, u64 r, u64 n, u64 d, u64 r_C, u64 n_start, u64 mmax,
+                                     u64 tmax_new, const u64 *S, const u64 *C, const u64 *lshifts,
+                                     const u64 *shiftsb, const u64 *tmaxes, u64 *xrb, double *x, hipStream_t s) {
+    const char *who = "dnb2_lms_gen_shift_float";
+    if (!(r && n && d)) return QMC_OK;
+    if (r_C == 0) return qmc_fail(QMC_ERR_ARG, "%s: r_C must be positive", who);
+    if (mmax == 0 || mmax > 64 || tmax_new == 0 || tmax_new > 64)
+        return qmc_fail(QMC_ERR_ARG, "%s: needs 1 <= mmax <= 64 and 1 <= tmax_new <= 64", who);
+    if (d > (u64)kBlock || d * (2 * mmax + tmax_new) > 6144)
+        return qmc_fail(QMC_ERR_ARG, "%s: needs d <= %d and d * (2 mmax + tmax_new) <= 6144 (LDS)", who, kBlock);
+    if (r > 65535) return qmc_fail(QMC_ERR_ARG, "%s: r above 65535 is not supported", who);
+    int rc = check_bits(who, n_start, n, mmax);
+    if (rc) return rc;
+    const int check = natural && !(pow2_or_zero(n_start) && pow2_or_zero(n_start + n));
+    const u32 R = kLmsRun * (u32)(kBlock / d);  // rows per block: one run of kLmsRun rows per thread
+    const u64 chunks = (n + R - 1) / R;
+    if (chunks > 0x7fffffffull) return qmc_fail(QMC_ERR_ARG, "%s: n too large", who);
+    const size_t shm = sizeof(u64) * (d * (2 * mmax + tmax_new) + (u64)R * d);
+    const dim3 grid((unsigned)chunks, (unsigned)r);
+    if (natural)
+        hipLaunchKernelGGL(dnb2_lms_gen_kernel<true>, grid, dim3(kBlock), shm, s, n, d, n_start, mmax, r_C * d * mmax,
+                           tmax_new, R, check, S, C, lshifts, shiftsb, tmaxes, xrb, x);
+    else
+        hipLaunchKernelGGL(dnb2_lms_gen_kernel<false>, grid, dim3(kBlock), shm, s, n, d, n_start, mmax,
+                           r_C * d * mmax, tmax_new, R, check, S, C, lshifts, shiftsb, tmaxes, xrb, x);
+    QMC_HIP_CHECK(hipGetLastError());
+    return QMC_OK;
 }
 
 }  // extern "C"

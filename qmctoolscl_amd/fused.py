@@ -59,6 +59,25 @@ def dnb2_gen_shift_float(natural, r, n, d, r_x, n_start, mmax, C, lshifts, shift
     _lib.check(rc)
 
 
+def dnb2_lms_gen_shift_float(natural, r, n, d, r_C, n_start, mmax, tmax_new, S, C, lshifts, shiftsb, tmaxes,
+                             xrb=None, x=None, stream=None):
+    """dnb2_linear_matrix_scramble(r, d, mmax, r_C, tmax_new, S, C) -> dnb2_gen_<order>(r) ->
+    dnb2_digital_shift(r, r_x = r) -> dnb2_integer_to_float(r) in one launch.  S: (r, d, tmax_new),
+    C: (r_C, d, mmax), lshifts: (r,), shiftsb: (r, d), tmaxes: (r,); writes xrb (r, n, d) uint64
+    and/or x (r, n, d) float64."""
+    L = _lib.lib()
+    fn = L.qmc_dnb2_lms_gen_shift_float_dev
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_int] + [ctypes.c_uint64] * 7 + [ctypes.c_void_p] * 8
+    like = xrb if xrb is not None else x
+    r, n, d, r_C, mmax, tmax_new = (int(v) for v in (r, n, d, r_C, mmax, tmax_new))
+    rc = fn(int(bool(natural)), r, n, d, r_C, int(n_start), mmax, tmax_new, _ptr(S, r * d * tmax_new, what="S"),
+            _ptr(C, r_C * d * mmax, what="C"), _ptr(lshifts, r, what="lshifts"), _ptr(shiftsb, r * d, what="shiftsb"),
+            _ptr(tmaxes, r, what="tmaxes"), _ptr(xrb, r * n * d, what="xrb"), _ptr(x, r * n * d, True, "x"),
+            _stream(stream, like))
+    _lib.check(rc)
+
+
 def gdn_gen_scramble_float(mode, r, n, d, r_b, mmax, tmax, tmax_new, n_start, bmax, bases, C, tab, x, stream=None,
                            no_alloc=False):
     """x = gdn_integer_to_float(r, n, d, r_b, tmax_new, bases, S(gdn_gen_natural(r, n, d, r_b, mmax, tmax,

@@ -43,8 +43,9 @@ def to_t(torch, a):
 def test_config0_full_size(torch):
     """dnb2_linear_matrix_scramble -> dnb2_gen_natural -> dnb2_digital_shift
     -> dnb2_integer_to_float at n = 2^16, d = 8, r = 4, tmax = 32 (upper
-    triangular unit-diagonal generating matrices), drop-ins and the fused
-    kernel, bit for bit against the oracle."""
+    triangular unit-diagonal generating matrices): the drop-ins, the fused
+    generation kernel and the one-launch chain, bit for bit against the
+    oracle."""
     from qmctoolscl_amd import fused
     rng = np.random.Generator(np.random.SFC64(0))
     n, d, r, t = 1 << 16, 8, 4, 32
@@ -80,6 +81,13 @@ def test_config0_full_size(torch):
     fx = torch.empty_like(tx)
     fxrb = torch.empty_like(txrb)
     fused.dnb2_gen_shift_float(1, r, n, d, r, 0, t, tCl, tl, tsh, ttm, fxrb, fx)
+    torch.cuda.synchronize()
+    assert np.array_equal(fxrb.cpu().numpy().view(np.uint64), xrb)
+    assert np.array_equal(fx.cpu().numpy().view(np.uint64), x.view(np.uint64))
+    # the whole chain in one launch
+    fx.zero_()
+    fxrb.zero_()
+    fused.dnb2_lms_gen_shift_float(1, r, n, d, 1, 0, t, t, tS, tC, tl, tsh, ttm, fxrb, fx)
     torch.cuda.synchronize()
     assert np.array_equal(fxrb.cpu().numpy().view(np.uint64), xrb)
     assert np.array_equal(fx.cpu().numpy().view(np.uint64), x.view(np.uint64))

@@ -15,6 +15,7 @@ import _clib
 import cases
 import replay
 import qmctoolscl_amd as q
+from qmctoolscl_amd import rand_funcs as rf
 from qmctoolscl_amd._api import KERNEL_NAMES, SIGNATURES
 
 pytestmark = pytest.mark.gpu
@@ -184,6 +185,47 @@ def test_fused_dnb2_equals_chain():
                                    o_xrb, o_x)
         assert np.array_equal(from_t(o_xrb, True), xrb)
         assert np.array_equal(o_x.cpu().numpy().view(np.uint64), x.view(np.uint64))
+
+
+@pytest.mark.parametrize("natural,n,n_start,r_C,tmax_new,lsh,d", [
+    (1, 1 << 10, 0, 3, 32, 32, 5),     # config 0's chain, one matrix set per replication
+    (0, 1 << 10, 0, 1, 32, 32, 5),     # Gray order, the matrices shared by every replication
+    (1, 1 << 9, 1 << 9, 3, 64, 0, 3),  # a later closed block, 64-bit LMS, no left shift
+    (0, 1000, 37, 2, 48, 16, 9),       # ragged Gray range, r_C between 1 and r
+    (1, 1 << 10, 1 << 10, 3, 32, 32, 2),
+    (1, 1 << 8, 0, 1, 64, 0, 32),      # d * (2 mmax + tmax_new) near the LDS limit (60-bit columns)
+])
+def test_fused_dnb2_lms_equals_oracle_chain(natural, n, n_start, r_C, tmax_new, lsh, d):
+    """qmc_dnb2_lms_gen_shift_float_dev against the oracle's four-call chain
+    (LMS, generation, digital shift with r_x = r, conversion), bit for bit.
+    (Natural order over a range that Gray codes do not map onto itself makes
+    the reference write outside its output, so the oracle is not run there.)"""
+    import torch
+    from qmctoolscl_amd import fused
+    rng = np.random.Generator(np.random.SFC64(n + d))
+    r, mmax = 3, 32 if tmax_new < 64 or lsh else 60
+    C = rng.integers(0, 2**mmax if mmax < 64 else 2**63, size=(r_C, d, mmax), dtype=np.uint64)
+    S = rf.dnb2_get_linear_scramble_matrix(rng, r, d, mmax, tmax_new, False)
+    sh = rf.dnb2_get_digital_shifts(rng, r, d, 64)
+    lshifts = np.full(r, lsh, dtype=np.uint64)
+    tm = np.full(r, 64, dtype=np.uint64)
+    Cl = np.empty((r, d, mmax), dtype=np.uint64)
+    _clib.call_oracle("dnb2_linear_matrix_scramble", dict(r=r, d=d, mmax=mmax, r_C=r_C, tmax_new=tmax_new, S=S,
+                                                          C=C, C_lms=Cl))
+    xb = np.zeros((r, n, d), dtype=np.uint64)
+    _clib.call_oracle("dnb2_gen_natural" if natural else "dnb2_gen_gray",
+                      dict(r=r, n=n, d=d, n_start=n_start, mmax=mmax, C=Cl, xb=xb))
+    xrb = np.empty_like(xb)
+    _clib.call_oracle("dnb2_digital_shift", dict(r=r, n=n, d=d, r_x=r, lshifts=lshifts, xb=xb, shiftsb=sh, xrb=xrb))
+    x = np.empty((r, n, d))
+    _clib.call_oracle("dnb2_integer_to_float", dict(r=r, n=n, d=d, tmaxes=tm, xb=xrb, x=x))
+    o_xrb = torch.zeros((r, n, d), dtype=torch.int64, device="cuda")
+    o_x = torch.zeros((r, n, d), dtype=torch.float64, device="cuda")
+    fused.dnb2_lms_gen_shift_float(natural, r, n, d, r_C, n_start, mmax, tmax_new, to_t(S), to_t(C),
+                                   to_t(lshifts), to_t(sh), to_t(tm), o_xrb, o_x)
+    torch.cuda.synchronize()
+    assert np.array_equal(from_t(o_xrb, True), xrb)
+    assert np.array_equal(o_x.cpu().numpy().view(np.uint64), x.view(np.uint64))
 
 
 def test_full_size_lattice_rows_match_oracle():
