@@ -56,6 +56,26 @@ __device__ __forceinline__ void split3f(float x, __bf16 &a, __bf16 &b, __bf16 &c
     c = (__bf16)(r1 - (float)b);
 }
 
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+// two floats rounded to bf16 (one v_cvt_pk_bf16_f32), as one 32-bit word
+__device__ __forceinline__ u32 pack_bf16(float x0, float x1) {
+    f32x2 v;
+    v.x = x0;
+    v.y = x1;
+    return __builtin_bit_cast(u32, __builtin_convertvector(v, bf16x2));
+}
+
+// split3f of a pair of floats, the pieces packed as the MFMA operands hold
+// them: the bf16 of the low half is word << 16, of the high half word & ~0xffff
+__device__ __forceinline__ void split3_pair(float x0, float x1, u32 &a, u32 &b, u32 &c) {
+    a = pack_bf16(x0, x1);
+    const float r0 = x0 - __uint_as_float(a << 16), r1 = x1 - __uint_as_float(a & 0xffff0000u);
+    b = pack_bf16(r0, r1);
+    c = pack_bf16(r0 - __uint_as_float(b << 16), r1 - __uint_as_float(b & 0xffff0000u));
+}
+
 __device__ __forceinline__ void split3d(double x, __bf16 &a, __bf16 &b, __bf16 &c) {
     a = (__bf16)(float)x;
     const double r1 = x - (double)(float)a;
@@ -161,57 +181,76 @@ __device__ __forceinline__ f32x2 cmul(f32x2 a, f32x2 b) {
     return r;
 }
 
-// radix-16 level on tile bits [ta, ta+4), level `lev` of the pass; with tw,
-// output k of the column with in-tile bits c above the level (b of them) is
-// multiplied by w_{2^(b+4)}^(brev_b(c) k) * f[lev][k]
-template <bool INV>
-__device__ void dft16_level(float *re, float *im, int T, int ta, bool tw, int wave, int lane, const TwLds &tl,
-                            int lev, u32 salt) {
+// Twiddle of level `lev` (tile bits [ta, ta+4), b = T - ta - 4 tile bits above
+// it) for tile position t: row k = the level's digit of t, column bits c = t's
+// bits above the level; w = w_{2^(b+4)}^(brev_b(c) k) * f[lev][k], the factor f
+// skipped when the tile fixes no bits above the level (it is exactly 1 then).
+__device__ __forceinline__ f32x2 level_twiddle(const TwLds &tl, int T, int ta, int lev, bool fone, int t) {
+    const int k = (t >> ta) & 15;
+    const int b = T - ta - 4;
+    // b = 0: J = 0 and t1[0] t2[0] = 1 exactly; without branches the lookups of
+    // several outputs can all be in flight together
+    const int rb = b > 0 ? brev_bits(t >> (ta + 4), b) : 0;
+    const int J = ((rb * k) & ((16 << b) - 1)) << (9 - b);  // w_{2^(b+4)}^j = w_{2^13}^(j 2^(9-b))
+    const f32x2 w = cmul(tl.t1[J >> 7], tl.t2[J & 127]);
+    return fone ? w : cmul(w, tl.f[lev][k]);
+}
+
+// radix-16 level on tile bits [ta, ta+4), level `lev` of the pass.  Forward,
+// with tw: output k of the column with in-tile bits c above the level (b of
+// them) is multiplied by the level's twiddle (level_twiddle).  Inverse (the
+// transposed network): the conjugate twiddles of a level act on its inputs;
+// with tw_in they are applied here on the way in, and with tw_next this
+// level's outputs take the conjugate twiddles of level lev - 1 (the next one
+// run), which then reads its inputs as they are -- so each element is
+// twiddled once per level either way, by the lane that holds both its parts.
+template <bool INV, bool WHOLE>
+__device__ void dft16_level(float *re, float *im, int T, int ta, bool tw, bool tw_in, bool tw_next, bool fone_,
+                            bool fone_next_, int wave, int lane, const TwLds &tl, int lev, u32 salt) {
+    // a tile that is a whole vector fixes no index bits: every factor f is 1
+    const bool fone = WHOLE || fone_, fone_next = WHOLE || fone_next_;
     const Wtile W = tl.w[lane];
     const int nblk = 1 << (T - 8);
     const int n = lane & 15, g = lane >> 4;
     const int c0 = (g & 1) * 8;
     float *src = g < 2 ? re : im;
-    const int b = T - ta - 4;  // tile bits above this level (<= 9)
+    // LDS word of row c of a column: pad(base + (c << ta)) = pad(base) + c * st for ta = 0 or
+    // ta >= 4 (no carry out of the low four bits), which every radix-16 level has
+    const int st = (1 << ta) + ((1 << ta) >> 4);
     for (int blk = wave; blk < nblk; blk += (int)(blockDim.x >> 6)) {
         const int col = blk * 16 + n;
         const int lo = col & ((1 << ta) - 1);
         const int base = ((col >> ta) << (ta + 4)) | lo;
+        const int pin = pad(base + (c0 << ta));
         const u32 sg = dither_sign((u32)col, salt);
-        const int rb = b > 0 ? brev_bits(col >> ta, b) : 0;
-        // twiddle of input/output row k of this column: w_{2^(b+4)}^(brev_b(c_in) k) f[lev][k]
-        auto twid = [&](int k) -> f32x2 {
-            f32x2 w = tl.f[lev][k];
-            if (b > 0) {
-                const int J = ((rb * k) & ((16 << b) - 1)) << (9 - b);  // w_{2^(b+4)}^j = w_{2^13}^(j 2^(9-b))
-                w = cmul(cmul(tl.t1[J >> 7], tl.t2[J & 127]), w);
-            }
-            return w;
-        };
-        bf16x8 x[3];
+        float v[8];
 #pragma unroll
         for (int i = 0; i < 8; i++) {
-            __bf16 s0, s1, s2;
-            float v;
-            if (INV) {
-                // the inverse applies the conjugate twiddle to the inputs (the transposed network)
-                const int k = c0 + i;
-                float vr = re[pad(base + (k << ta))], vi = im[pad(base + (k << ta))];
-                if (tw) {
-                    const f32x2 w = twid(k);
-                    const float tr = vr * w.x + vi * w.y;
-                    vi = vi * w.x - vr * w.y;
-                    vr = tr;
-                }
-                v = g < 2 ? vr : vi;
+            if (INV && tw_in) {
+                const int t = base + ((c0 + i) << ta);
+                float vr = re[pin + i * st], vi = im[pin + i * st];
+                const f32x2 w = level_twiddle(tl, T, ta, lev, fone, t);
+                const float tr = vr * w.x + vi * w.y;
+                vi = vi * w.x - vr * w.y;
+                vr = tr;
+                v[i] = flip(g < 2 ? vr : vi, sg);
             } else {
-                v = src[pad(base + ((c0 + i) << ta))];
+                v[i] = flip(src[pin + i * st], sg);
             }
-            split3f(flip(v, sg), s0, s1, s2);
-            x[0][i] = s0;
-            x[1][i] = s1;
-            x[2][i] = s2;
         }
+        u32x4 pa, pb, pc;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            u32 wa, wb, wc;
+            split3_pair(v[2 * i], v[2 * i + 1], wa, wb, wc);
+            pa[i] = wa;
+            pb[i] = wb;
+            pc[i] = wc;
+        }
+        bf16x8 x[3];
+        x[0] = __builtin_bit_cast(bf16x8, pa);
+        x[1] = __builtin_bit_cast(bf16x8, pb);
+        x[2] = __builtin_bit_cast(bf16x8, pc);
         f32x4 yr = {0.f, 0.f, 0.f, 0.f}, yi = {0.f, 0.f, 0.f, 0.f};
         // sum over pieces p + q <= 2 of W_p * X_q, smallest terms first
         yr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.Ar[2], x[0], yr, 0, 0, 0);
@@ -226,20 +265,30 @@ __device__ void dft16_level(float *re, float *im, int T, int ta, bool tw, int wa
         yi = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.Ai[1], x[0], yi, 0, 0, 0);
         yi = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.Ai[0], x[1], yi, 0, 0, 0);
         yi = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.Ai[0], x[0], yi, 0, 0, 0);
-        // lane holds outputs k = 4g + i of column n
+        // lane holds outputs k = 4g + i of column n; the twiddles of all four first (their
+        // lookups in flight together), then the products
+        const int pout = pad(base + ((4 * g) << ta));
+        const bool twout = INV ? tw_next : tw;
+        f32x2 w[4];
+        if (twout) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int t = base + ((4 * g + i) << ta);
+                w[i] = INV ? level_twiddle(tl, T, ta - 4, lev - 1, fone_next, t) : level_twiddle(tl, T, ta, lev, fone, t);
+            }
+        }
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const int k = 4 * g + i;
-            const int t = base + (k << ta);
             float vr = flip(yr[i], sg), vi = flip(yi[i], sg);
-            if (!INV && tw) {
-                const f32x2 w = twid(k);
-                const float tr = vr * w.x - vi * w.y;
-                vi = vr * w.y + vi * w.x;
+            if (twout) {
+                // forward: times w; inverse: times conj(w)
+                const float wy = INV ? -w[i].y : w[i].y;
+                const float tr = vr * w[i].x - vi * wy;
+                vi = vr * wy + vi * w[i].x;
                 vr = tr;
             }
-            re[pad(t)] = vr;
-            im[pad(t)] = vi;
+            re[pout + i * st] = vr;
+            im[pout + i * st] = vi;
         }
     }
 }
@@ -277,7 +326,7 @@ __device__ void dft_small_level(float *re, float *im, int T, int ta, int rb) {
 // float64.  Workgroups are persistent: each walks tiles blockIdx.x,
 // +gridDim.x, ... with all NP value pairs (of re and of im) of a thread
 // (2^T = 2 NP blockDim.x) loaded at once.
-template <int IN64, int OUT64, int NP, bool INV>
+template <int IN64, int OUT64, int NP, bool INV, bool WHOLE = false>
 __global__ __launch_bounds__(kThreads) void fft_mfma_pass(PassGeom G, int glo, int ghi, const void *__restrict__ sr,
                                                         const void *__restrict__ si, void *__restrict__ dr,
                                                         void *__restrict__ di, double scale, Twiddles tw, u64 ntiles) {
@@ -335,8 +384,8 @@ __global__ __launch_bounds__(kThreads) void fft_mfma_pass(PassGeom G, int glo, i
                 im[pad(t + 1)] = (float)qc[p].y;
             }
         }
-        // the tile's fixed twiddle factors: level lev, output k
-        for (int idx = tid; idx < 16 * nlev; idx += blockDim.x) {
+        // the tile's fixed twiddle factors: level lev, output k (none for a whole vector)
+        for (int idx = tid; idx < (WHOLE ? 0 : 16 * nlev); idx += blockDim.x) {
             const int lev = idx >> 4, k = idx & 15;
             const int ga = glo + 4 * lev;
             const int hi_bits = G.m - ga - 4;
@@ -355,17 +404,28 @@ __global__ __launch_bounds__(kThreads) void fft_mfma_pass(PassGeom G, int glo, i
             tl.f[lev][k] = w;
         }
         __syncthreads();
-        // forward: levels upward; inverse (the transposed network): downward, twiddles on the inputs
+        // forward: levels upward; inverse (the transposed network): downward, each level's
+        // conjugate twiddles applied to its inputs -- by the level run before it when that is a
+        // radix-16 level of this pass, else on the way in.  The levels of a pass sit on
+        // consecutive tile bits (ta(lev - 1) = ta(lev) - 4).
+        bool pre = false;  // this step's input twiddles were applied by the previous step
         for (int step = 0; step < nlev; step++) {
             const int lev = INV ? nlev - 1 - step : step;
             const int ga = glo + 4 * lev;
             const int ltid = opaque(threadIdx.x);
             const int ta = ga < G.lc ? ga : G.lc + (ga - G.lo);  // tile bit of global bit ga
-            if (ghi - ga >= 4)
-                dft16_level<INV>(re, im, T, ta, G.m - ga - 4 > 0, ltid >> 6, ltid & 63, tl, lev,
-                                 (u32)tix * 0x632BE5ABu + (u32)ga * 0x27D4EB2Fu);
-            else
+            const bool tw = G.m - ga - 4 > 0;                    // index bits above the level
+            const bool fone = (G.m - ga - 4) - (T - ta - 4) <= 0;  // the tile fixes none of them
+            if (ghi - ga >= 4) {
+                const bool next = INV && lev > 0;  // level lev - 1 runs next (it always has bits above)
+                const bool fone_next = next && (G.m - ga) - (T - ta) <= 0;
+                dft16_level<INV, WHOLE>(re, im, T, ta, tw, INV && tw && !pre, next, fone, fone_next, ltid >> 6, ltid & 63,
+                                 tl, lev, (u32)tix * 0x632BE5ABu + (u32)ga * 0x27D4EB2Fu);
+                pre = next;
+            } else {
                 dft_small_level<INV>(re, im, T, ta, ghi - ga);  // only the top level can be short
+                pre = false;
+            }
             __syncthreads();
         }
         const int stid = opaque(threadIdx.x);
@@ -448,6 +508,8 @@ int run_mfma_fft(const char *who, u64 d1, u64 d2, u64 n_half, double *xr, double
         for (int k = 0; k < 4; k++)
             QMC_HIP_CHECK(hipFuncSetAttribute(pass_fn<INV>(k & 1, k >> 1), hipFuncAttributeMaxDynamicSharedMemorySize,
                                               bytes));
+        QMC_HIP_CHECK(hipFuncSetAttribute((const void *)fft_mfma_pass<1, 1, 8, INV, true>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
         return QMC_OK;
     });
     if (arc) return arc;
@@ -474,18 +536,18 @@ int run_mfma_fft(const char *who, u64 d1, u64 d2, u64 n_half, double *xr, double
         const int np = m >= 10 ? 8 : 1 << (m - 7);
         const int threads = (1 << m) / (2 * np);
         const size_t shm = shm_bytes(m);
-        const void *fn = np == 8 ? (const void *)fft_mfma_pass<1, 1, 8, INV>
-                         : np == 4 ? (const void *)fft_mfma_pass<1, 1, 4, INV>
-                                   : (const void *)fft_mfma_pass<1, 1, 2, INV>;
+        const void *fn = np == 8 ? (const void *)fft_mfma_pass<1, 1, 8, INV, true>
+                         : np == 4 ? (const void *)fft_mfma_pass<1, 1, 4, INV, true>
+                                   : (const void *)fft_mfma_pass<1, 1, 2, INV, true>;
         const unsigned grid = grid_of(fn, threads, shm, V);
         if (np == 8)
-            hipLaunchKernelGGL((fft_mfma_pass<1, 1, 8, INV>), dim3(grid), dim3(threads), shm, s, G, 0, m,
+            hipLaunchKernelGGL((fft_mfma_pass<1, 1, 8, INV, true>), dim3(grid), dim3(threads), shm, s, G, 0, m,
                                (const void *)xr, (const void *)xi, (void *)xr, (void *)xi, scale, tw, V);
         else if (np == 4)
-            hipLaunchKernelGGL((fft_mfma_pass<1, 1, 4, INV>), dim3(grid), dim3(threads), shm, s, G, 0, m,
+            hipLaunchKernelGGL((fft_mfma_pass<1, 1, 4, INV, true>), dim3(grid), dim3(threads), shm, s, G, 0, m,
                                (const void *)xr, (const void *)xi, (void *)xr, (void *)xi, scale, tw, V);
         else
-            hipLaunchKernelGGL((fft_mfma_pass<1, 1, 2, INV>), dim3(grid), dim3(threads), shm, s, G, 0, m,
+            hipLaunchKernelGGL((fft_mfma_pass<1, 1, 2, INV, true>), dim3(grid), dim3(threads), shm, s, G, 0, m,
                                (const void *)xr, (const void *)xi, (void *)xr, (void *)xi, scale, tw, V);
         if (hipGetLastError() != hipSuccess) rc = qmc_fail(QMC_ERR_HIP, "%s: launch failed", who);
     } else {
