@@ -295,8 +295,8 @@ inline u32 gdn_walked(u32 TB, u32 gcap) {
 }
 
 // LDS bytes of one column's tables in a tile (16-byte multiples): the rows,
-// the walk table twice (packed row offsets and plain), and for permutation
-// rows the reciprocals.  Row offsets are packed in u16 halves whose sums must
+// the walk table twice (packed row offsets and plain), the early-exit
+// thresholds of the walked levels, and for permutation rows the reciprocals.  Row offsets are packed in u16 halves whose sums must
 // stay below 2^16: b * rowb < 2^15 (0 when that fails).
 inline u32 gdn_slot_bytes(u32 b, int m, u32 TW, int kind) {
     if (kind == GK_B2) return (8 * (1 + (u32)m) + 15) & ~15u;
@@ -304,7 +304,8 @@ inline u32 gdn_slot_bytes(u32 b, int m, u32 TW, int kind) {
     const u32 rowb = kind == GK_VAL ? 8 * (TW + 1) : 2 * gdn_row_u16(TW);
     if ((u64)b * rowb >= (1u << 15)) return 0;
     const u32 rows = (b * rowb + 15) & ~15u, walk = (12 * mm * ((TW + 1) / 2) + 15) & ~15u;
-    return rows + walk + (kind == GK_VAL ? 0u : (8 * TW + 15) & ~15u);
+    const u32 lv = (8 * TW + 15) & ~15u;
+    return rows + walk + lv + (kind == GK_VAL ? 0u : lv);
 }
 
 // The column groups of a gdn request and every column's LDS tables: the 16
@@ -564,6 +565,16 @@ __device__ __forceinline__ double dnb2_emit(u32 v, u64 xs, double lo) {
 // row index changes in bits clog..clog+c (c = ctz(g+1)), so the value changes by
 // pre[clog+c] ^ pre[clog-1] (pre = prefix XOR of the columns).  Steps come in
 // groups of 8 whose first 7 transitions use c = 0,1,0,2,0,1,0.
+//
+// The chains read their column constants from LDS, staged once per tile:
+// one counter (vmcnt) tracks a wave's loads and stores in issue order, so a
+// global load issued after a chain's stores would wait for all of them to
+// complete, and every chain would start on an empty store queue.  Per column
+// (kChainWords(s) words): cols[0 .. s), the transitions
+// xk[k] = pre[s + k] ^ pre[s - 1] for k < 3 + kMaxGroupLog, the 64-bit shift
+// or seed (low word first).
+__host__ __device__ constexpr u32 kChainWords(u32 s) { return s + 3 + kMaxGroupLog + 2; }
+
 template <bool LMS, bool DIG, class SinkT>
 __device__ __forceinline__ void dnb2_tile(const Req &q, u64 l, u64 row0, const u64 *__restrict__ tab,
                                           double *__restrict__ slab, u32 *v0, SinkT &out) {
@@ -575,6 +586,23 @@ __device__ __forceinline__ void dnb2_tile(const Req &q, u64 l, u64 row0, const u
     const u32 *T32 = (const u32 *)tab;
     const u64 *extra = tab + 32 * nset + l * d;  // shifts or NUS seeds
     const u64 stride = (u64)R * d;
+    const u32 W = kChainWords(s);
+    u32 *cst = v0 + 256;  // column constants, W words per column
+    for (u32 e = threadIdx.x; e < d * W; e += blockDim.x) {
+        const u32 j = e / W, w = e - j * W;
+        const u32 *pre = T32 + (set + j) * 64;
+        u32 val;
+        if (w < s) {
+            val = pre[32 + w];
+        } else if (w < s + 3 + kMaxGroupLog) {
+            const u32 k = w - s;
+            val = s + k < (u32)kTB ? pre[s + k] ^ (s ? pre[s - 1] : 0u) : 0u;
+        } else {
+            const u64 xs = extra[j];
+            val = w == W - 2 ? (u32)xs : (u32)(xs >> 32);
+        }
+        cst[e] = val;
+    }
     // value of the tile's first row per column, shared by the column's chains
     // (all column loads issued at once: one memory round trip)
     for (u32 j = threadIdx.x; j < d; j += blockDim.x) {
@@ -593,11 +621,11 @@ __device__ __forceinline__ void dnb2_tile(const Req &q, u64 l, u64 row0, const u
     for (u32 sig = threadIdx.x; sig < R * d; sig += blockDim.x) {
         const u32 rr0 = div_q(sig, q.ddiv);
         const u32 j = sig - rr0 * d;
-        const u32 *pre = T32 + (set + j) * 64;
-        const u32 *cols = pre + 32;
-        const u64 xs = extra[j];
+        const u32 *cj = cst + j * W;
+        const u32 *cols = cj;
+        const u32 *xkj = cj + s;
+        const u64 xs = (u64)cj[W - 2] | ((u64)cj[W - 1] << 32);
         const double lo = (double)(u32)xs * 0x1p-64;
-        const u32 base = s ? pre[s - 1] : 0u;
         // row0 + rr0 with rr0 < 2^s <= the tile: the low bits add independently
         u32 v = v0[j];
 #pragma unroll
@@ -611,7 +639,7 @@ __device__ __forceinline__ void dnb2_tile(const Req &q, u64 l, u64 row0, const u
             // covers them all
             u32 xk[3 + kMaxGroupLog];
 #pragma unroll
-            for (int k = 0; k < 3 + kMaxGroupLog; k++) xk[k] = (s + k < (u32)kTB) ? pre[s + k] ^ base : 0u;
+            for (int k = 0; k < 3 + kMaxGroupLog; k++) xk[k] = xkj[k];
             const u32 x0 = xk[0], x1 = xk[1], x2 = xk[2];
             for (u32 h = 0; h < H; h += 8) {
                 // the transition after this group: c = 3 + ctz((h + 8) / 8), uniform over the block
@@ -640,7 +668,7 @@ __device__ __forceinline__ void dnb2_tile(const Req &q, u64 l, u64 row0, const u
             }
         } else {
             // H <= 4: transitions c = ctz(h+1) <= 1
-            const u32 x0 = pre[s] ^ base, x1 = H > 1 ? pre[s + 1] ^ base : 0u;
+            const u32 x0 = xkj[0], x1 = H > 1 ? xkj[1] : 0u;
             for (u32 h = 0;; h++) {
                 out.put(dst, dnb2_emit<LMS>(v, xs, lo));
                 if (h + 1 == H) break;
@@ -695,6 +723,7 @@ struct GdnCol {
     const double *rec, *thr;  // the base's constants (kTNmax each)
     const u32 *Qs, *Qu;       // walk table in LDS: packed pairs of row offsets, and plain
     const double *srec;       // permutation rows: the reciprocals of the walked levels in LDS
+    const double *sthr;       // the early-exit thresholds of the walked levels in LDS
     u64 seed, l, j, M;
     u32 b, mm, rowb, base;    // rowb: bytes per table row; base: LDS byte address of the rows
 };
@@ -746,10 +775,27 @@ __device__ __forceinline__ u32 as_word(u16x2 p) { return __builtin_bit_cast(u32,
 // rec_t * pi_t(z), rec_t read from LDS as a broadcast) or, for VT, the terms
 // themselves, V[z][t] = RN(rec_t * pi_t(z)) as doubles: the reference's
 // product, rounded once as there (gdn_integer_to_float.c:36-37), so a level
-// costs one 8-byte load and the add.  Every point adds all TW levels: a lane
-// whose sum has passed a level's threshold adds terms that leave it unchanged
-// (gdn_levels), so only the check at level TW, once per point, decides whether
-// the rare tail runs.
+// costs one 8-byte load and the add.  From level tchk on, a wave stops as soon
+// as every one of its sums has reached the level's threshold (a lane whose sum
+// has passed it would only add terms that leave it unchanged, gdn_levels);
+// after TW levels one check decides whether the rare tail runs.
+// Levels at which a wave of a TW-level column checks whether all its sums are
+// final: for the bases that walk TW levels, the levels at which the 64 sums of
+// a wave (spread uniformly over [0, 1)) have all passed the threshold with a
+// good chance.  A check costs a compare and exposes the latency of the next
+// LDS reads (they cannot be issued ahead of the branch), so there are few of
+// them; at a level below the column's own first useful one (gdn_tmin) a check
+// never ends the sum.  Fixed per TW: a check level read at run time made the
+// compiler spill hundreds of registers.
+__host__ __device__ constexpr bool gdn_check_level(int TW, int t) {
+    return TW == 8    ? t == 7
+           : TW == 10 ? t == 8
+           : TW == 12 ? t == 10
+           : TW == 16 ? (t == 12 || t == 14)
+           : TW == 24 ? (t == 15 || t == 18)
+                      : (t == 26 || t == 38 || t == 42);
+}
+
 template <int TW, bool VT>
 struct WalkOdd {
     static constexpr int TP = (TW + 1) / 2;  // level pairs (the last half is padding when TW is odd)
@@ -793,9 +839,13 @@ struct WalkOdd {
         // point (as broadcasts) rather than held in registers: the pointer
         // passes through opaque()
         const double *sr = c.srec + (VT ? 0 : opaque(0));
+        const double *sh = c.sthr + opaque(0);
         double v = 0.0;
 #pragma unroll
         for (int t = 0; t < TW; t++) {
+            // at the check levels, a wave whose sums have all reached thr_t is
+            // done: every later term leaves them unchanged (gdn_levels)
+            if (gdn_check_level(TW, t) && __builtin_amdgcn_ballot_w64(v < sh[t]) == 0) return v;
             const double p = VT ? lds_f64(addr(c, t)) : sr[t] * (double)lds_u16(addr(c, t));
             v = t == 0 ? p : v + p;
         }
@@ -849,7 +899,7 @@ __host__ __device__ inline u32 gdn_walk_bytes(u32 mm, u32 TW) { return (12 * mm 
 template <int TW, bool VT>
 __device__ __forceinline__ void gdn_fill_odd(u32 t, u32 L, const u64 *__restrict__ blk, u32 b, u32 mm, u32 TB,
                                              unsigned char *slot, u32 *Qs, u32 *Qu, const double *__restrict__ rec,
-                                             double *srec, u32 rowb) {
+                                             double *srec, const double *__restrict__ thr, double *sthr, u32 rowb) {
     constexpr u32 TP = (TW + 1) / 2;
     constexpr u32 kF = 8;
     const u32 TE = gdn_row_even(TB);
@@ -894,6 +944,7 @@ __device__ __forceinline__ void gdn_fill_odd(u32 t, u32 L, const u64 *__restrict
         }
         if (t < TW) srec[t] = rec[t];
     }
+    if (t < TW) sthr[t] = thr[t];
     const unsigned short *Qg = (const unsigned short *)blk + (u64)b * TE;
     for (u32 e = t; e < mm * TP; e += L) {
         const u32 cc = e / TP, tp = e - cc * TP;
@@ -991,13 +1042,15 @@ __device__ __forceinline__ void gdn_tile(const Req &q, u64 l, u32 rem, const u64
     const u32 qoff = gdn_qoff(b, TW, vt);
     u32 *Qs = (u32 *)(slot + qoff);
     u32 *Qu = Qs + mm * TP;
-    double *srec = (double *)(slot + qoff + gdn_walk_bytes(mm, TW));
+    double *sthr = (double *)(slot + qoff + gdn_walk_bytes(mm, TW));
+    double *srec = sthr + (((8 * TW + 15) & ~15u) / 8);
     GdnCol c;
     c.rec = (const double *)ws + (u64)j * kTNmax;
     c.thr = (const double *)ws + (u64)(256 + j) * kTNmax;
     c.Qs = Qs;
     c.Qu = Qu;
     c.srec = srec;
+    c.sthr = sthr;
     c.seed = q.seed;
     c.l = l;
     c.j = j;
@@ -1009,7 +1062,7 @@ __device__ __forceinline__ void gdn_tile(const Req &q, u64 l, u32 rem, const u64
     const double thw = uniform_f64(c.thr + (TW < (u32)kTNmax ? TW : 0u));
 #define QMC_GDN_TW(T_, VT_)                                                                                    \
     case (T_) | ((VT_) ? 0x100 : 0): {                                                                         \
-        if (colact) gdn_fill_odd<T_, VT_>(i, L, blk, b, mm, TB, slot, Qs, Qu, c.rec, srec, c.rowb);            \
+        if (colact) gdn_fill_odd<T_, VT_>(i, L, blk, b, mm, TB, slot, Qs, Qu, c.rec, srec, c.thr, sthr, c.rowb); \
         __syncthreads();                                                                                       \
         WalkOdd<T_, VT_> w;                                                                                    \
         if (act) w.init(c, k);                                                                                 \
@@ -1037,16 +1090,22 @@ __device__ void run_tile(const Req &q, u64 t, const u64 *__restrict__ ws, double
     if (CLASS == 0) {
         const u64 row0 = rem * (u64)q.rows_per_tile;
         if (q.method == M_LATTICE) {
+            // g (as exact doubles) and the replication's shifts staged in LDS:
+            // a global load after the stores would wait for them (dnb2_tile)
             const u32 pts = q.rows_per_tile * (u32)d;
-            const u64 *g = tab;
-            const double *sh = (const double *)(tab + d) + l * d;
+            double *sg = (double *)lds, *ssh = sg + d;
+            for (u32 j = threadIdx.x; j < d; j += blockDim.x) {
+                sg[j] = (double)tab[j];
+                ssh[j] = ((const double *)(tab + d))[l * d + j];
+            }
+            __syncthreads();
             for (u32 e = threadIdx.x; e < pts; e += blockDim.x) {
                 const u32 rr = div_q(e, q.ddiv);
                 const u32 j = e - rr * q.d;
                 const u64 row = row0 + rr;
                 const double phi = (double)__brevll(row) * 0x1p-64;  // phi2(row), exact
-                const double x = frac_mod1((double)g[j] * phi);      // lat_gen_natural.c:45
-                out.put(slab + row * d + j, frac_mod1(x + sh[j]));  // lat_shift_mod_1.c:31
+                const double x = frac_mod1(sg[j] * phi);             // lat_gen_natural.c:45
+                out.put(slab + row * d + j, frac_mod1(x + ssh[j]));  // lat_shift_mod_1.c:31
             }
         } else if (q.method == M_DNB2_LMS) {
             dnb2_tile<true, DIG>(q, l, row0, tab, slab, (u32 *)lds, out);
@@ -1174,7 +1233,13 @@ u32 dnb2_chain_log(u32 RT, u32 d) {
     return best_al != ~0u ? best_al : best;
 }
 
-constexpr u32 kLdsClass0 = 256 * sizeof(u32);  // dnb2 first-row values
+// LDS of a class-0 tile: dnb2 first-row values (256 words) and chain
+// constants (dnb2_tile), or the lattice's g and shifts
+constexpr u32 kLdsClass0 = 256 * sizeof(u32);
+inline u32 class0_lds(const Req &q) {
+    if (q.method == M_LATTICE) return 16u * (u32)q.d;
+    return 4u * (256u + (u32)q.d * kChainWords(q.clog));
+}
 
 struct Plan {
     std::vector<Req> reqs;
@@ -1301,6 +1366,7 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
         q.ddiv = make_div((u32)q.d);
         q.gdiv = make_div(q.method == M_GDN_LMS ? q.gng : 1u);
         if (q.method == M_DNB2_LMS || q.method == M_DNB2_NUS) q.clog = dnb2_chain_log(q.rows_per_tile, (u32)q.d);
+        if (method_class(q.method) == 0) P->lds_bytes[0] = std::max(P->lds_bytes[0], class0_lds(q));
         if (q.method == M_GDN_LMS) {
             const GdnPlan gp = gdn_plan(q.d, q.m, q.gcap);
             if (gp.bad >= 0) {
