@@ -294,7 +294,15 @@ QMC_EXPORT int qmc_gdn_gen_scramble_float_noalloc_dev(int mode, qmc_u64 r, qmc_u
  * draw values (the root's first) and *_apply_dev scrambles on the GPU.
  * Base-b rows are 16-bit (b <= 2^16).  Trees persist across calls until
  * freed; each *_apply_dev keeps the packed device copy of the last set of
- * trees it was given and uploads again only when a tree grew. */
+ * trees it was given and uploads again only when a tree grew.
+ *
+ * Base b with numpy's SFC64 generators: qmc_nusb_grow inserts a call's points
+ * into all r*d trees on host threads and replays each tree's generator
+ * (Generator.permutation's draws, rand_funcs.py:22-32) from its state, so no
+ * row is drawn in Python; a tree materializes only its internal nodes' rows
+ * and keeps one generator state per leaf, whose rows the device regenerates.
+ * With other bit generators the caller draws the rows (*_insert counts,
+ * *_add_rows) and all of them are materialized. */
 QMC_EXPORT void *qmc_nus2_tree_new(int tmax_new);
 QMC_EXPORT void qmc_nus2_tree_free(void *tree);
 QMC_EXPORT qmc_u64 qmc_nus2_tree_nodes(void *tree);
@@ -311,6 +319,18 @@ QMC_EXPORT qmc_u64 qmc_nusb_tree_rows_needed(void *tree);
 QMC_EXPORT int64_t qmc_nusb_insert(void *tree, const qmc_u64 *v, qmc_u64 n, qmc_u64 stride_pt,
                                    qmc_u64 stride_dig, uint32_t *counts);
 QMC_EXPORT int qmc_nusb_add_rows(void *tree, const uint16_t *rows, qmc_u64 nrows);
+QMC_EXPORT qmc_u64 qmc_nusb_tree_states(void *tree);
+QMC_EXPORT qmc_u64 qmc_nusb_tree_nodes(void *tree);
+QMC_EXPORT qmc_u64 qmc_nusb_tree_bytes(void *tree);
+QMC_EXPORT int qmc_nusb_root_rows(void *tree, uint16_t *out);
+/* trees[l*d + j] takes xdig[(l % r_x), i, j, 0..tmax) for i < n; where
+ * replay[k] != 0 the new rows come from the SFC64 state states[5k..5k+5)
+ * (numpy's s[0..3], has_uint32 << 32 | uinteger), advanced in place. */
+QMC_EXPORT int qmc_nusb_grow(void **trees, qmc_u64 nt, const qmc_u64 *xdig, qmc_u64 r, qmc_u64 n, qmc_u64 d,
+                             qmc_u64 r_x, qmc_u64 *states, const uint8_t *replay, int threads);
+/* count Generator.permutation(b) draws from SFC64 state w (advanced in place)
+ * into out (count x b): the replay qmc_nusb_grow uses, exported for checks */
+QMC_EXPORT void qmc_nus_sfc64_permutations(qmc_u64 *w, qmc_u64 b, qmc_u64 count, uint16_t *out);
 QMC_EXPORT int qmc_nusb_apply_dev(void **trees, qmc_u64 r, qmc_u64 n, qmc_u64 d, qmc_u64 r_x, qmc_u64 r_b,
                                   int tmax, int tmax_new, const qmc_u64 *bases, const qmc_u64 *xdig,
                                   qmc_u64 *xrdig, qmc_stream_t s);

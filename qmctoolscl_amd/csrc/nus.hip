@@ -25,8 +25,10 @@
 #include <cstring>
 #include <vector>
 
+#include <algorithm>
 #include <atomic>
 #include <mutex>
+#include <thread>
 
 #include "qmc_common.hpp"
 
@@ -102,100 +104,334 @@ void insert2(Tree2 *T, u64 v, std::vector<uint32_t> &widths) {
 }
 
 // ------------------------------------------------------------------ base b
+//
+// A reference leaf holds tmax_new - depth permutation rows of b entries
+// (random_uint64_permutations, rand_funcs.py:22-32), drawn when the leaf
+// appears: at n = 3^12 points, 52 digits and bases up to 131 that is
+// terabytes per point set.  Here a tree materializes only the row of each
+// internal node (its own level) and keeps, per leaf, the state of the tree's
+// generator where the leaf's rows begin (5 words); a point that ends at the
+// leaf regenerates those rows on the device.  For numpy's SFC64 the builder
+// replays the generator itself (Sfc64); for any other bit generator the caller
+// draws every block's rows with numpy and all of them are materialized.
+
+// numpy's SFC64 bit generator (C. Doty-Humphrey's Small Fast Chaotic PRNG:
+// 256-bit state, one 64-bit output per step) with numpy's 32-bit buffering (a
+// 64-bit output serves two 32-bit draws, low half first), and the draws of
+// Generator.permutation(b): a Fisher-Yates shuffle of 0..b-1, for i = b-1 down
+// to 1 swapping entries i and j = random_interval(i) (a 32-bit draw masked to
+// the bit width of i, rejected while above i).  Checked on the host against
+// numpy's own Generator (qmc_nus_sfc64_permutations).
+struct Sfc64 {
+    u64 s[4];
+    u32 has, buf;
+    __host__ __device__ __forceinline__ u64 next64() {
+        const u64 t = s[0] + s[1] + s[3]++;
+        s[0] = s[1] ^ (s[1] >> 11);
+        s[1] = s[2] + (s[2] << 3);
+        s[2] = ((s[2] << 24) | (s[2] >> 40)) + t;
+        return t;
+    }
+    __host__ __device__ __forceinline__ u32 next32() {
+        if (has) {
+            has = 0;
+            return buf;
+        }
+        const u64 v = next64();
+        has = 1;
+        buf = (u32)(v >> 32);
+        return (u32)v;
+    }
+    // random_interval(mx) for 1 <= mx < 2^32
+    __host__ __device__ __forceinline__ u32 interval(u32 mx) {
+        u32 mask = mx;
+        mask |= mask >> 1;
+        mask |= mask >> 2;
+        mask |= mask >> 4;
+        mask |= mask >> 8;
+        mask |= mask >> 16;
+        u32 v;
+        do {
+            v = next32() & mask;
+        } while (v > mx);
+        return v;
+    }
+    // packed as 5 words: s[0..3], has << 32 | buf
+    __host__ __device__ void load(const u64 *w) {
+        for (int k = 0; k < 4; k++) s[k] = w[k];
+        has = (u32)(w[4] >> 32);
+        buf = (u32)w[4];
+    }
+    __host__ __device__ void store(u64 *w) const {
+        for (int k = 0; k < 4; k++) w[k] = s[k];
+        w[4] = ((u64)has << 32) | buf;
+    }
+};
+
+// one Generator.permutation(b) into out[0..b)
+void sfc_perm(Sfc64 &g, u32 b, uint16_t *out) {
+    for (u32 k = 0; k < b; k++) out[k] = (uint16_t)k;
+    for (u32 i = b - 1; i >= 1; i--) {
+        const u32 j = g.interval(i);
+        const uint16_t a = out[i];
+        out[i] = out[j];
+        out[j] = a;
+    }
+}
+// The draws of `rows` permutations of b, not kept (most rows: the leaves'
+// rows are regenerated on the device).  Written as a scan over the stream of
+// 32-bit values (low half of each 64-bit output first) without data-dependent
+// branches: a value is accepted when (v & mask) <= i, which moves i down by
+// one (and halves mask when i falls to half of it); a row ends when i reaches
+// 0.  The same values are consumed as by Sfc64::interval, about four times
+// faster (a rejected draw no longer costs a mispredicted branch).
+void sfc_skip(Sfc64 &g, u32 b, u64 rows) {
+    if (b < 2 || rows == 0) return;
+    u64 s0 = g.s[0], s1 = g.s[1], s2 = g.s[2], s3 = g.s[3];
+    u32 full = b - 1;
+    full |= full >> 1;
+    full |= full >> 2;
+    full |= full >> 4;
+    full |= full >> 8;
+    full |= full >> 16;
+    u32 i = b - 1, mask = full;
+    u64 left = rows;
+    auto take = [&](u32 v) -> bool {  // true when the last row ends
+        i -= (v & mask) <= i ? 1u : 0u;
+        mask = i <= (mask >> 1) ? mask >> 1 : mask;
+        if (i == 0) {
+            if (--left == 0) return true;
+            i = b - 1;
+            mask = full;
+        }
+        return false;
+    };
+    if (g.has) {
+        g.has = 0;
+        if (take(g.buf)) return;
+    }
+    for (;;) {
+        const u64 t = s0 + s1 + s3++;
+        s0 = s1 ^ (s1 >> 11);
+        s1 = s2 + (s2 << 3);
+        s2 = ((s2 << 24) | (s2 >> 40)) + t;
+        if (take((u32)t)) {
+            g.has = 1;
+            g.buf = (u32)(t >> 32);
+            break;
+        }
+        if (take((u32)(t >> 32))) {
+            g.has = 0;  // numpy leaves the used high half in its buffer word
+            g.buf = (u32)(t >> 32);
+            break;
+        }
+    }
+    g.s[0] = s0;
+    g.s[1] = s1;
+    g.s[2] = s2;
+    g.s[3] = s3;
+}
 
 struct NodeB {
-    int32_t kids;    // offset of b child slots in Tree::kids, -1 for a leaf
-    int32_t owner;   // permutation block
-    int64_t rest;    // leaf: offset of its (tmax_new - depth) remaining digits in Tree::digits
+    int32_t kids;  // offset of b child slots in TreeB::kids, -1 for a leaf
+    int32_t owner; // block (leaf creation) whose rows the node uses: row (depth - depth0) at its level
+    int64_t rest;  // its digits at levels depth .. tmax-1 in TreeB::digits (later levels are 0)
+    int32_t row;   // materialized row of its own level, or -1 (leaves of blocks that are not materialized)
+    int32_t st;    // leaf of a replayed block: generator state where its rows begin, or -1
 };
 
 struct Block {
-    int64_t row0;    // first row of the block in the row pool (filled when the draws arrive)
-    int32_t depth0;  // depth of the block's first row
-    int32_t rows;
+    int32_t depth0, rows;  // first level and number of its rows (rand_funcs.py:327, 345: tmax_new - t - 1)
+    int32_t first;         // the leaf created with it
+    int32_t mat0;          // first of its rows in TreeB::rows when all of them are materialized, or -1
 };
 
 struct TreeB {
     int tmax, tmax_new;
-    u64 b;                    // <= 65536, so a permuted digit fits 16 bits
-    u64 id, version;          // identity and mutation count: the device copy is reused while both match
+    u64 b;                        // <= 65536, so a digit fits 16 bits
+    u64 id, version;              // identity and mutation count: the device copy is reused while both match
     std::vector<NodeB> nodes;
     std::vector<int32_t> kids;
-    std::vector<u64> digits;  // leaf digit pool
-    std::vector<Block> blocks;
-    std::vector<uint16_t> rows;  // permutation rows, b values each, in draw order
+    std::vector<uint16_t> digits;  // leaf digit pool
+    std::vector<Block> blocks;    // in draw order
+    size_t ready = 0;             // blocks whose rows are drawn (replayed or supplied)
+    u64 supplied = 0;             // rows received through qmc_nusb_add_rows (numpy mode) not yet assigned
+    std::vector<uint16_t> pending;  // those rows
+    std::vector<uint16_t> rows;   // materialized rows, b entries each
+    std::vector<u64> states;      // leaf generator states, 5 words each
 };
 
 std::atomic<u64> g_treeb_ids{1};
 
-int32_t new_leafB(TreeB *T, const u64 *rest, int len, int32_t owner) {
+int32_t push_rowB(TreeB *T, Sfc64 &g) {
+    const size_t o = T->rows.size();
+    T->rows.resize(o + T->b);
+    sfc_perm(g, (u32)T->b, &T->rows[o]);
+    return (int32_t)(o / T->b);
+}
+
+int32_t push_stateB(TreeB *T, const Sfc64 &g) {
+    const size_t o = T->states.size();
+    T->states.resize(o + 5);
+    g.store(&T->states[o]);
+    return (int32_t)(o / 5);
+}
+
+inline u32 digit_atB(const TreeB *T, const NodeB &N, int depth) {
+    return depth < T->tmax ? T->digits[N.rest] : 0u;
+}
+
+// a new leaf at `depth` for the point pd (its digits at levels >= depth), with
+// a new block of tmax_new - depth rows
+int32_t new_leafB(TreeB *T, int depth, const uint16_t *pd, std::vector<uint32_t> *counts) {
+    Block bl;
+    bl.depth0 = depth;
+    bl.rows = T->tmax_new - depth;
+    bl.first = (int32_t)T->nodes.size();
+    bl.mat0 = -1;
+    T->blocks.push_back(bl);
+    if (counts) counts->push_back((uint32_t)bl.rows);
     NodeB nd;
     nd.kids = -1;
-    nd.owner = owner;
+    nd.owner = (int32_t)T->blocks.size() - 1;
     nd.rest = (int64_t)T->digits.size();
-    T->digits.insert(T->digits.end(), rest, rest + len);
+    nd.row = -1;
+    nd.st = -1;
+    if (depth < T->tmax) T->digits.insert(T->digits.end(), pd + depth, pd + T->tmax);
     T->nodes.push_back(nd);
-    return (int32_t)T->nodes.size() - 1;
+    return bl.first;
 }
 
-int32_t new_blockB(TreeB *T, int depth0, int rows, std::vector<uint32_t> &counts) {
-    Block bl;
-    bl.row0 = -1;
-    bl.depth0 = depth0;
-    bl.rows = rows;
-    T->blocks.push_back(bl);
-    counts.push_back((uint32_t)rows);
-    return (int32_t)T->blocks.size() - 1;
+// The leaf `node` at depth t becomes internal: its contents move one level
+// down (rand_funcs.py:336-340), keeping the owner block.  Its row for level t
+// comes from the materialized block, or is drawn now from the leaf's stored
+// state (the moved leaf keeps the state after it); for a block whose rows are
+// not drawn yet, the replay resolves the chain.
+void split_leafB(TreeB *T, int32_t node, int t, u32 ndg) {
+    NodeB moved;
+    moved.kids = -1;
+    moved.owner = T->nodes[node].owner;
+    moved.rest = T->nodes[node].rest + 1;
+    moved.row = -1;
+    moved.st = -1;
+    const Block &B = T->blocks[moved.owner];
+    if (B.mat0 >= 0) {
+        T->nodes[node].row = B.mat0 + (t - B.depth0);
+        moved.row = T->nodes[node].row + 1;
+    } else if (T->nodes[node].st >= 0) {
+        Sfc64 g;
+        g.load(&T->states[5 * (size_t)T->nodes[node].st]);
+        T->nodes[node].row = push_rowB(T, g);
+        T->nodes[node].st = -1;
+        moved.st = push_stateB(T, g);
+    }
+    const int32_t mid = (int32_t)T->nodes.size();
+    T->nodes.push_back(moved);
+    const int32_t koff = (int32_t)T->kids.size();
+    T->kids.insert(T->kids.end(), (size_t)T->b, -1);
+    T->nodes[node].kids = koff;
+    T->kids[koff + ndg] = mid;
 }
 
-// rand_funcs.py:318-353 for one point's digits (tmax of them)
-void insertB(TreeB *T, const u64 *dig, u64 stride, std::vector<uint32_t> &counts) {
+// rand_funcs.py:318-353 for one point: pd holds its tmax digits (each < b).
+// Where the reference's loop would read past the last digit (tmax ==
+// tmax_new, rand_funcs.py:323-325, an IndexError there) the point's path is
+// already complete and the insertion stops.
+void insertB(TreeB *T, const uint16_t *pd, std::vector<uint32_t> *counts) {
     const int tn = T->tmax_new, tm = T->tmax;
-    std::vector<u64> rest(tn);
     int32_t node = 0;
-    for (int t = 0; t <= tm; t++) {
-        const int len = tn - t;
-        for (int k = 0; k < len; k++) rest[k] = (k < tm - t) ? dig[(u64)(t + k) * stride] : 0;
-        const u64 dg = rest[0];
-        if (dg >= T->b) return;  // a digit outside [0, b): the reference would index past a permutation
-        NodeB &N = T->nodes[node];
-        if (N.kids >= 0) {
-            const int32_t kid = T->kids[N.kids + dg];
+    for (int t = 0; t <= tm && t < tn; t++) {
+        const u32 dg = t < tm ? pd[t] : 0u;
+        if (T->nodes[node].kids >= 0) {
+            const int32_t kid = T->kids[T->nodes[node].kids + dg];
             if (kid < 0) {
-                const int32_t blk = new_blockB(T, t + 1, tn - t - 1, counts);
-                const int32_t k = new_leafB(T, rest.data() + 1, len - 1, blk);
+                const int32_t k = new_leafB(T, t + 1, pd, counts);
                 T->kids[T->nodes[node].kids + dg] = k;
                 return;
             }
             node = kid;
-        } else if (std::memcmp(&T->digits[N.rest], rest.data(), sizeof(u64) * len) == 0) {
+            continue;
+        }
+        const int nd = tm - t;  // digits the leaf holds (levels t .. tmax-1)
+        if (nd <= 0 || std::memcmp(&T->digits[T->nodes[node].rest], pd + t, 2 * (size_t)nd) == 0)
             return;  // seen before
-        } else {
-            const u64 ndg = T->digits[N.rest];
-            const int32_t owner = N.owner;
-            const int64_t old_rest = N.rest;
-            // the leaf's contents move one level down, keeping its owner block
-            NodeB moved;
-            moved.kids = -1;
-            moved.owner = owner;
-            moved.rest = old_rest + 1;  // its digits shifted by one level (shares the pool)
-            T->nodes.push_back(moved);
-            const int32_t mid = (int32_t)T->nodes.size() - 1;
-            const int32_t koff = (int32_t)T->kids.size();
-            T->kids.insert(T->kids.end(), (size_t)T->b, -1);
-            NodeB &M = T->nodes[node];
-            M.kids = koff;
-            T->kids[koff + ndg] = mid;
-            if (ndg == dg) {
-                node = mid;
-            } else {
-                const int32_t blk = new_blockB(T, t + 1, tn - t - 1, counts);
-                const int32_t k = new_leafB(T, rest.data() + 1, len - 1, blk);
-                T->kids[koff + dg] = k;
-                return;
+        const u32 ndg = T->digits[T->nodes[node].rest];
+        split_leafB(T, node, t, ndg);
+        if (ndg == dg) {
+            node = T->kids[T->nodes[node].kids + dg];
+            continue;
+        }
+        const int32_t k = new_leafB(T, t + 1, pd, counts);
+        T->kids[T->nodes[node].kids + dg] = k;
+        return;
+    }
+}
+
+// All rows of block B are materialized from row m0 on: give every node of its
+// chain (its first leaf, pushed down by later splits) its level's row.
+void assign_chainB(TreeB *T, Block &B, int32_t m0) {
+    B.mat0 = m0;
+    int32_t node = B.first;
+    for (int D = B.depth0;; D++) {
+        NodeB &N = T->nodes[node];
+        N.row = m0 + (D - B.depth0);
+        if (N.kids < 0) break;
+        node = T->kids[N.kids + digit_atB(T, N, D)];
+    }
+}
+
+// Draw the rows of the blocks not drawn yet, in creation order, from g: the
+// root's block is materialized whole (the reference draws it when the tree
+// starts and keeps it as root_node.perm); in every other block the rows of
+// the chain's internal nodes are materialized and the leaf where the chain
+// ends keeps the state at its first row, the remaining rows only advancing g.
+void replayB(TreeB *T, Sfc64 &g) {
+    const u32 b = (u32)T->b;
+    for (; T->ready < T->blocks.size(); T->ready++) {
+        Block &B = T->blocks[T->ready];
+        if (T->ready == 0) {
+            int32_t m0 = -1;
+            for (int k = 0; k < B.rows; k++) {
+                const int32_t rw = push_rowB(T, g);
+                if (k == 0) m0 = rw;
             }
+            if (m0 < 0) m0 = (int32_t)(T->rows.size() / b);
+            assign_chainB(T, B, m0);
+            continue;
+        }
+        int32_t node = B.first;
+        int D = B.depth0;
+        for (int k = 0; k < B.rows; k++, D++) {
+            NodeB &N = T->nodes[node];
+            if (N.kids < 0) {
+                N.st = push_stateB(T, g);
+                sfc_skip(g, b, (u64)(B.rows - k));
+                break;
+            }
+            N.row = push_rowB(T, g);
+            node = T->kids[N.kids + digit_atB(T, N, D)];
         }
     }
+    T->version++;
+}
+
+// numpy mode: the rows of the blocks not drawn yet arrive in draw order;
+// every block they cover is materialized.
+void take_rowsB(TreeB *T) {
+    const u64 b = T->b;
+    size_t used = 0;
+    while (T->ready < T->blocks.size()) {
+        Block &B = T->blocks[T->ready];
+        const size_t need = (size_t)B.rows * b;
+        if (T->pending.size() - used < need) break;
+        const int32_t m0 = (int32_t)(T->rows.size() / b);
+        T->rows.insert(T->rows.end(), T->pending.begin() + used, T->pending.begin() + used + need);
+        used += need;
+        assign_chainB(T, B, m0);
+        T->ready++;
+    }
+    T->pending.erase(T->pending.begin(), T->pending.begin() + used);
+    T->version++;
 }
 
 // ------------------------------------------------------------ device walks
@@ -239,50 +475,73 @@ __global__ __launch_bounds__(kBlock) void nus2_apply_kernel(u64 r, u64 n, u64 d,
 }
 
 struct PackedB {
-    const int32_t *kids_off;   // per node, -1 for a leaf
-    const int32_t *owner;      // per node: block id (tree-local)
-    const int32_t *kids;       // child slots (tree-local node ids)
-    const int64_t *blk_row0;   // per block: first row (tree-local)
-    const int32_t *blk_depth0; // per block
-    const uint16_t *rows;      // b values per row, each tree's rows packed
-    const int64_t *node0, *kid0, *blk0, *row0;  // per tree offsets (row0 in values)
-    const uint32_t *tb;        // per tree base
+    const int32_t *kids_off;  // per node, -1 for a leaf
+    const int32_t *row;       // per node: materialized row of its level (tree-local), or -1
+    const int32_t *st;        // per node: leaf state (tree-local), or -1
+    const int32_t *kids;      // child slots (tree-local node ids)
+    const uint16_t *rows;     // b values per row, each tree's rows packed
+    const u64 *states;        // 5 words per state
+    const int64_t *node0, *kid0, *row0, *st0;  // per tree offsets (row0 in values, st0 in states)
+    const uint32_t *tb;       // per tree base
 };
 
-__global__ __launch_bounds__(kBlock) void nusb_apply_kernel(u64 r, u64 n, u64 d, u64 r_x, u64 r_b, int tmax,
-                                                            int tmax_new, u64 bmax, PackedB P,
-                                                            const u64 *__restrict__ xdig,
-                                                            u64 *__restrict__ xrdig) {
-    const u64 total = r * n * d;
-    const u64 stride = (u64)gridDim.x * blockDim.x;
-    for (u64 e = (u64)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
-        const u64 j = e % d;
-        const u64 li = e / d;
-        const u64 i = li % n;
-        const u64 l = li / n;
-        const u64 tree = l * d + j;
-        const u64 *src = xdig + (((l % r_x) * n + i) * d + j) * tmax;
-        u64 *out = xrdig + e * tmax_new;
-        const int64_t nb = P.node0[tree], kb = P.kid0[tree], bb = P.blk0[tree];
+constexpr int kNusbLanes = 64;
+
+// One wave per workgroup; point e = tree * n + i, so the lanes of a wave take
+// consecutive points of one tree.  A point walks its tree from the root: an
+// internal node gives the permuted digit of its level from its row
+// (rand_funcs.py:320-327); at its leaf the remaining levels come from the
+// leaf's rows (rand_funcs.py:330-331), read from a materialized block or
+// regenerated from the leaf's state, one Generator.permutation(b) per level.
+// Only entry x of a regenerated permutation is needed: the shuffle's entry x
+// is x sent through its swaps (i <-> j_i) in reverse order of drawing, so the
+// draws j_{b-1} .. j_1 go to a per-lane buffer (jb, one u16 per lane and
+// level, lanes interleaved) and are read back from j_1 up.
+template <bool LBUF>
+__global__ __launch_bounds__(kNusbLanes) void nusb_apply_kernel(u64 r, u64 n, u64 d, u64 r_x, int tmax, int tmax_new,
+                                                              u64 bmax, PackedB P, const u64 *__restrict__ xdig,
+                                                              u64 *__restrict__ xrdig, uint16_t *scratch) {
+    extern __shared__ uint16_t lbuf[];
+    uint16_t *jb = (LBUF ? lbuf : scratch + (u64)blockIdx.x * kNusbLanes * bmax) + threadIdx.x;
+    const u64 total = r * d * n;
+    for (u64 e = (u64)blockIdx.x * kNusbLanes + threadIdx.x; e < total; e += (u64)gridDim.x * kNusbLanes) {
+        const u64 tree = e / n, i = e - tree * n;
+        const u64 l = tree / d, j = tree - l * d;
+        const u64 *src = xdig + (((l % r_x) * n + i) * d + j) * (u64)tmax;
+        u64 *out = xrdig + ((l * n + i) * d + j) * (u64)tmax_new;
+        const int64_t nb = P.node0[tree], kb = P.kid0[tree];
         const uint16_t *rows = P.rows + P.row0[tree];
-        const int64_t b = P.tb[tree];
-        int64_t node = 0;  // tree-local
-        int t = 0;
-        for (; t < tmax_new; t++) {
-            const u64 dg = t < tmax ? src[t] : 0;
-            const int32_t blk = P.owner[nb + node];
-            const int64_t row = P.blk_row0[bb + blk] + (t - P.blk_depth0[bb + blk]);
-            out[t] = rows[row * b + (int64_t)dg];
+        const u32 b = P.tb[tree];
+        int64_t node = 0;
+        for (int t = 0; t < tmax_new; t++) {
+            const u32 dg = t < tmax ? (u32)src[t] : 0u;
             const int32_t ko = P.kids_off[nb + node];
-            if (ko < 0) {  // leaf: its block's rows cover every deeper level
-                for (int tt = t + 1; tt < tmax_new; tt++) {
-                    const u64 g = tt < tmax ? src[tt] : 0;
-                    const int64_t rw = P.blk_row0[bb + blk] + (tt - P.blk_depth0[bb + blk]);
-                    out[tt] = rows[rw * b + (int64_t)g];
-                }
-                break;
+            if (ko >= 0) {
+                out[t] = rows[(u64)P.row[nb + node] * b + dg];
+                node = P.kids[kb + ko + dg];
+                continue;
             }
-            node = P.kids[kb + ko + (int64_t)dg];
+            const int32_t st = P.st[nb + node];
+            if (st < 0) {  // materialized rows, consecutive from the leaf's own level
+                const u64 rw = (u64)P.row[nb + node];
+                for (int tt = t; tt < tmax_new; tt++) {
+                    const u32 x = tt < tmax ? (u32)src[tt] : 0u;
+                    out[tt] = rows[(rw + (u64)(tt - t)) * b + x];
+                }
+            } else {
+                Sfc64 g;
+                g.load(P.states + (u64)(P.st0[tree] + st) * 5);
+                for (int tt = t; tt < tmax_new; tt++) {
+                    for (u32 q = b - 1; q >= 1; q--) jb[(u64)q * kNusbLanes] = (uint16_t)g.interval(q);
+                    u32 p = tt < tmax ? (u32)src[tt] : 0u;
+                    for (u32 q = 1; q < b; q++) {
+                        const u32 jq = jb[(u64)q * kNusbLanes];
+                        p = p == q ? jq : (p == jq ? q : p);
+                    }
+                    out[tt] = p;
+                }
+            }
+            break;
         }
     }
 }
@@ -456,10 +715,10 @@ QMC_EXPORT void *qmc_nusb_tree_new(int tmax, int tmax_new, u64 b) {
     T->b = b;
     T->id = g_treeb_ids++;
     T->version = 0;
-    std::vector<uint32_t> dummy;
-    new_blockB(T, 0, tmax_new, dummy);  // block 0: the root's tmax_new rows, drawn by the caller
-    std::vector<u64> zeros(tmax_new, 0);
-    new_leafB(T, zeros.data(), tmax_new, 0);  // the root starts as a leaf for the all-zero digits
+    // the root starts as a leaf for the all-zero digits, with block 0: its
+    // tmax_new rows (rand_funcs.py:312-317)
+    std::vector<uint16_t> zeros(tmax > 0 ? tmax : 1, 0);
+    new_leafB(T, 0, zeros.data(), nullptr);
     return T;
 }
 
@@ -468,109 +727,197 @@ QMC_EXPORT u64 qmc_nusb_tree_rows(void *h) { return ((TreeB *)h)->rows.size() / 
 QMC_EXPORT u64 qmc_nusb_tree_rows_needed(void *h) {
     TreeB *T = (TreeB *)h;
     u64 k = 0;
-    for (auto &bl : T->blocks) k += bl.rows;
-    return k;
+    for (size_t q = T->ready; q < T->blocks.size(); q++) k += (u64)T->blocks[q].rows;
+    return k - T->pending.size() / T->b;
+}
+QMC_EXPORT u64 qmc_nusb_tree_states(void *h) { return ((TreeB *)h)->states.size() / 5; }
+QMC_EXPORT u64 qmc_nusb_tree_nodes(void *h) { return ((TreeB *)h)->nodes.size(); }
+
+// Host bytes a tree holds (nodes, child slots, leaf digits, materialized rows,
+// leaf states).
+QMC_EXPORT u64 qmc_nusb_tree_bytes(void *h) {
+    TreeB *T = (TreeB *)h;
+    return T->nodes.size() * sizeof(NodeB) + T->kids.size() * 4 + T->digits.size() * 2 +
+           T->blocks.size() * sizeof(Block) + T->rows.size() * 2 + T->states.size() * 8;
 }
 
 // Insert n points (digits at v[k*stride_pt + t*stride_dig], t < tmax) in
-// order; writes the row counts of the new permutation blocks to counts
-// (capacity n) and returns how many blocks were added (-1 on a digit >= b).
+// order; writes the row counts of the new blocks to counts (capacity n) and
+// returns how many blocks were added (-1 on a digit >= b).
 QMC_EXPORT int64_t qmc_nusb_insert(void *h, const u64 *v, u64 n, u64 stride_pt, u64 stride_dig, uint32_t *counts) {
     TreeB *T = (TreeB *)h;
     std::vector<uint32_t> c;
+    std::vector<uint16_t> pd(T->tmax > 0 ? T->tmax : 1);
     for (u64 k = 0; k < n; k++) {
         const u64 *p = v + k * stride_pt;
-        for (int t = 0; t < T->tmax; t++)
-            if (p[(u64)t * stride_dig] >= T->b) {
-                qmc_fail(QMC_ERR_RANGE, "nus: digit %llu is not below the base %llu",
-                         (unsigned long long)p[(u64)t * stride_dig], (unsigned long long)T->b);
+        for (int t = 0; t < T->tmax; t++) {
+            const u64 x = p[(u64)t * stride_dig];
+            if (x >= T->b) {
+                qmc_fail(QMC_ERR_RANGE, "nus: digit %llu is not below the base %llu", (unsigned long long)x,
+                         (unsigned long long)T->b);
                 return -1;
             }
-        const size_t nodes = T->nodes.size();
-        insertB(T, p, stride_dig, c);
-        if (T->nodes.size() != nodes) T->version++;
+            pd[t] = (uint16_t)x;
+        }
+        const size_t nodes0 = T->nodes.size();
+        insertB(T, pd.data(), &c);
+        if (T->nodes.size() != nodes0) T->version++;
     }
     if (!c.empty()) std::memcpy(counts, c.data(), c.size() * sizeof(uint32_t));
     return (int64_t)c.size();
 }
 
-// Append permutation rows (b values each, below b) in draw order.
+// numpy mode: append permutation rows (b values each, below b) in draw order;
+// every block they complete is materialized.
 QMC_EXPORT int qmc_nusb_add_rows(void *h, const uint16_t *rows, u64 nrows) {
     TreeB *T = (TreeB *)h;
-    const u64 have = T->rows.size() / T->b;
-    if (have + nrows > qmc_nusb_tree_rows_needed(h)) return qmc_fail(QMC_ERR_ARG, "nus: more rows than asked for");
-    T->rows.insert(T->rows.end(), rows, rows + nrows * T->b);
-    T->version++;
-    // assign row offsets to the blocks now fully covered
-    u64 acc = 0;
-    for (auto &bl : T->blocks) {
-        if (bl.row0 < 0 && acc + bl.rows <= have + nrows) bl.row0 = (int64_t)acc;
-        acc += bl.rows;
-    }
+    if (nrows > qmc_nusb_tree_rows_needed(h)) return qmc_fail(QMC_ERR_ARG, "nus: more rows than asked for");
+    T->pending.insert(T->pending.end(), rows, rows + nrows * T->b);
+    take_rowsB(T);
     return QMC_OK;
 }
 
+// The root's tmax_new rows (block 0) once drawn, b values each: the
+// reference's root_node.perm (rand_funcs.py:316).
+QMC_EXPORT int qmc_nusb_root_rows(void *h, uint16_t *out) {
+    TreeB *T = (TreeB *)h;
+    if (T->ready == 0 || T->blocks[0].mat0 < 0) return qmc_fail(QMC_ERR_ARG, "nus: the root's rows are not drawn");
+    std::memcpy(out, &T->rows[(size_t)T->blocks[0].mat0 * T->b], (size_t)T->blocks[0].rows * T->b * 2);
+    return QMC_OK;
+}
+
+// One call's host work for nt = r*d trees: tree l*d + j takes the points
+// xdig[(l % r_x), i, j, 0 .. tmax) for i < n, in order; then, where replay[k]
+// is set, the rows of its new blocks are drawn from the SFC64 state
+// states[5k .. 5k+5) (numpy's s[0..3], has_uint32 << 32 | uinteger), which is
+// advanced in place exactly as the reference's draws would advance it.
+// Trees are independent: up to `threads` host threads take them in turn.
+QMC_EXPORT int qmc_nusb_grow(void **trees, u64 nt, const u64 *xdig, u64 r, u64 n, u64 d, u64 r_x, u64 *states,
+                             const uint8_t *replay, int threads) {
+    if (nt != r * d) return qmc_fail(QMC_ERR_ARG, "nus: %llu trees for r*d = %llu", (unsigned long long)nt,
+                                     (unsigned long long)(r * d));
+    // largest bases first: a tree's replay costs about (b - 1) draws per row
+    std::vector<u64> order(nt);
+    for (u64 k = 0; k < nt; k++) order[k] = k;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](u64 a, u64 c) { return ((TreeB *)trees[a])->b > ((TreeB *)trees[c])->b; });
+    std::atomic<u64> next{0};
+    std::atomic<int> bad{0};
+    std::atomic<u64> bad_digit{0}, bad_tree{0};
+    auto work = [&]() {
+        for (;;) {
+            const u64 q = next++;
+            if (q >= nt || bad.load()) return;
+            const u64 k = order[q];
+            TreeB *T = (TreeB *)trees[k];
+            const u64 l = k / d, j = k % d;
+            const u64 tm = (u64)T->tmax;
+            const size_t nodes0 = T->nodes.size(), blocks0 = T->blocks.size();
+            std::vector<uint16_t> pd(tm > 0 ? tm : 1);
+            for (u64 i = 0; i < n; i++) {
+                const u64 *p = xdig + (((l % r_x) * n + i) * d + j) * tm;
+                for (u64 t = 0; t < tm; t++) {
+                    if (p[t] >= T->b) {
+                        bad_digit = p[t];
+                        bad_tree = k;
+                        bad = 1;
+                        return;
+                    }
+                    pd[t] = (uint16_t)p[t];
+                }
+                insertB(T, pd.data(), nullptr);
+            }
+            // the device copy is kept while no tree changes (a repeat call on the same points)
+            if (T->nodes.size() != nodes0 || T->blocks.size() != blocks0) T->version++;
+            if (replay && replay[k] && T->ready < T->blocks.size()) {
+                Sfc64 g;
+                g.load(states + 5 * k);
+                replayB(T, g);
+                g.store(states + 5 * k);
+            }
+        }
+    };
+    const int nth = (int)std::max<u64>(1, std::min<u64>((u64)std::max(threads, 1), nt));
+    std::vector<std::thread> pool;
+    for (int q = 1; q < nth; q++) pool.emplace_back(work);
+    work();
+    for (auto &th : pool) th.join();
+    if (bad.load())
+        return qmc_fail(QMC_ERR_RANGE, "nus: digit %llu of tree %llu is not below its base",
+                        (unsigned long long)bad_digit.load(), (unsigned long long)bad_tree.load());
+    return QMC_OK;
+}
+
+// count Generator.permutation(b) draws from the SFC64 state w (5 words,
+// advanced in place) into out (count x b): the replay the builder uses
+QMC_EXPORT void qmc_nus_sfc64_permutations(u64 *w, u64 b, u64 count, uint16_t *out) {
+    Sfc64 g;
+    g.load(w);
+    for (u64 k = 0; k < count; k++) sfc_perm(g, (u32)b, out + k * b);
+    g.store(w);
+}
 
 QMC_EXPORT int qmc_nusb_apply_dev(void **trees, u64 r, u64 n, u64 d, u64 r_x, u64 r_b, int tmax, int tmax_new,
                                   const u64 *bases, const u64 *xdig, u64 *xrdig, qmc_stream_t s) {
     (void)bases;  // each tree carries its base (checked against the bases when it was grown)
+    (void)r_b;
     if (!(r && n && d)) return QMC_OK;
     const u64 nt = r * d;
     std::vector<u64> sig;
     sig.reserve(2 * nt + 2);
     sig.push_back(r);
     sig.push_back(d);
+    u64 bmax = 1;
     for (u64 k = 0; k < nt; k++) {
         TreeB *T = (TreeB *)trees[k];
         if (T->tmax != tmax || T->tmax_new != tmax_new)
             return qmc_fail(QMC_ERR_ARG, "nus: tree %llu was built with other tmax/tmax_new", (unsigned long long)k);
-        if (T->rows.size() / T->b != qmc_nusb_tree_rows_needed(T))
+        if (T->ready != T->blocks.size())
             return qmc_fail(QMC_ERR_ARG, "nus: tree %llu is missing permutation rows", (unsigned long long)k);
         sig.push_back(T->id);
         sig.push_back(T->version);
+        bmax = std::max<u64>(bmax, T->b);
     }
     int dev = 0;
     QMC_HIP_CHECK(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lock(g_nusb_mu);
     if (g_nusb.buf == nullptr || g_nusb.device != dev || g_nusb.sig != sig) {
-        std::vector<int64_t> node0(nt), kid0(nt), blk0(nt), row0(nt);
+        std::vector<int64_t> node0(nt), kid0(nt), row0(nt), st0(nt);
         std::vector<uint32_t> tb(nt);
-        u64 nn = 0, nk = 0, nb = 0, nr = 0;
+        u64 nn = 0, nk = 0, nr = 0, ns = 0;
         for (u64 k = 0; k < nt; k++) {
             TreeB *T = (TreeB *)trees[k];
             node0[k] = (int64_t)nn;
             kid0[k] = (int64_t)nk;
-            blk0[k] = (int64_t)nb;
             row0[k] = (int64_t)nr;
+            st0[k] = (int64_t)ns;
             tb[k] = (uint32_t)T->b;
             nn += T->nodes.size();
             nk += T->kids.size();
-            nb += T->blocks.size();
             nr += T->rows.size();
+            ns += T->states.size() / 5;
         }
-        std::vector<int32_t> kids_off(nn), owner(nn), kids(nk > 0 ? nk : 1), bdepth(nb);
-        std::vector<int64_t> brow(nb);
+        std::vector<int32_t> kids_off(nn), row(nn), st(nn), kids(nk > 0 ? nk : 1);
         std::vector<uint16_t> rows(nr > 0 ? nr : 1);
+        std::vector<u64> states(ns > 0 ? 5 * ns : 5);
         for (u64 k = 0; k < nt; k++) {
             TreeB *T = (TreeB *)trees[k];
             for (size_t q = 0; q < T->nodes.size(); q++) {
                 kids_off[node0[k] + q] = T->nodes[q].kids;
-                owner[node0[k] + q] = T->nodes[q].owner;
+                row[node0[k] + q] = T->nodes[q].row;
+                st[node0[k] + q] = T->nodes[q].st;
             }
             for (size_t q = 0; q < T->kids.size(); q++) kids[kid0[k] + q] = T->kids[q] < 0 ? 0 : T->kids[q];
-            for (size_t q = 0; q < T->blocks.size(); q++) {
-                brow[blk0[k] + q] = T->blocks[q].row0;
-                bdepth[blk0[k] + q] = T->blocks[q].depth0;
-            }
             if (!T->rows.empty()) std::memcpy(&rows[row0[k]], T->rows.data(), T->rows.size() * sizeof(uint16_t));
+            if (!T->states.empty()) std::memcpy(&states[5 * st0[k]], T->states.data(), T->states.size() * 8);
         }
         if (g_nusb.buf) {
             (void)hipFree(g_nusb.buf);
             g_nusb.buf = nullptr;
         }
         g_nusb.sig.clear();
-        const size_t total = kids_off.size() * 4 + owner.size() * 4 + kids.size() * 4 + brow.size() * 8 +
-                             bdepth.size() * 4 + rows.size() * 2 + 4 * nt * 8 + nt * 4 + 128;
+        const size_t total = (kids_off.size() + row.size() + st.size() + kids.size()) * 4 + rows.size() * 2 +
+                             states.size() * 8 + 4 * nt * 8 + nt * 4 + 128;
         char *buf = nullptr;
         if (hipMalloc((void **)&buf, total) != hipSuccess)
             return qmc_fail(QMC_ERR_NOMEM, "nus: device allocation of %zu bytes for the trees failed", total);
@@ -585,15 +932,15 @@ QMC_EXPORT int qmc_nusb_apply_dev(void **trees, u64 r, u64 n, u64 d, u64 r_x, u6
         };
         PackedB P;
         P.kids_off = (const int32_t *)put(kids_off.data(), kids_off.size() * 4);
-        P.owner = (const int32_t *)put(owner.data(), owner.size() * 4);
+        P.row = (const int32_t *)put(row.data(), row.size() * 4);
+        P.st = (const int32_t *)put(st.data(), st.size() * 4);
         P.kids = (const int32_t *)put(kids.data(), kids.size() * 4);
-        P.blk_row0 = (const int64_t *)put(brow.data(), brow.size() * 8);
-        P.blk_depth0 = (const int32_t *)put(bdepth.data(), bdepth.size() * 4);
         P.rows = (const uint16_t *)put(rows.data(), rows.size() * 2);
+        P.states = (const u64 *)put(states.data(), states.size() * 8);
         P.node0 = (const int64_t *)put(node0.data(), nt * 8);
         P.kid0 = (const int64_t *)put(kid0.data(), nt * 8);
-        P.blk0 = (const int64_t *)put(blk0.data(), nt * 8);
         P.row0 = (const int64_t *)put(row0.data(), nt * 8);
+        P.st0 = (const int64_t *)put(st0.data(), nt * 8);
         P.tb = (const uint32_t *)put(tb.data(), nt * 4);
         if (ce != hipSuccess) {
             (void)hipFree(buf);
@@ -604,9 +951,29 @@ QMC_EXPORT int qmc_nusb_apply_dev(void **trees, u64 r, u64 n, u64 d, u64 r_x, u6
         g_nusb.device = dev;
         g_nusb.sig = sig;
     }
-    hipLaunchKernelGGL(nusb_apply_kernel, dim3(grid_for(r * n * d, kBlock, 2048 * 8)), dim3(kBlock), 0, s, r, n, d,
-                       r_x, r_b, tmax, tmax_new, (u64)0, g_nusb.P, xdig, xrdig);
-    hipError_t e = hipGetLastError();
+    // the per-lane draw buffers: 64 lanes x bmax u16 per workgroup, in LDS up
+    // to 64 KB, else in a device scratch of at most 512 MB
+    const u64 total = r * n * d;
+    const size_t per_wg = (size_t)kNusbLanes * bmax * 2;
+    hipError_t e = hipSuccess;
+    if (per_wg <= 64 * 1024) {
+        const unsigned grid = grid_for(total, kNusbLanes, 4096);
+        hipLaunchKernelGGL((nusb_apply_kernel<true>), dim3(grid), dim3(kNusbLanes), per_wg, s, r, n, d, r_x, tmax,
+                           tmax_new, bmax, g_nusb.P, xdig, xrdig, (uint16_t *)nullptr);
+        e = hipGetLastError();
+    } else {
+        const unsigned grid =
+            (unsigned)std::max<u64>(1, std::min<u64>(grid_for(total, kNusbLanes, 4096), (512ull << 20) / per_wg));
+        uint16_t *scratch = nullptr;
+        e = hipMallocAsync((void **)&scratch, (size_t)grid * per_wg, s);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL((nusb_apply_kernel<false>), dim3(grid), dim3(kNusbLanes), 0, s, r, n, d, r_x, tmax,
+                               tmax_new, bmax, g_nusb.P, xdig, xrdig, scratch);
+            e = hipGetLastError();
+            const hipError_t f = hipFreeAsync(scratch, s);
+            if (e == hipSuccess) e = f;
+        }
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return qmc_fail_hip(e, "nusb_apply_kernel");
     return QMC_OK;

@@ -10,12 +10,19 @@ array of NUSNode_dnb2() / NUSNode_gdn() that keep each tree between calls.
 
 Each tree is grown on the host by the library's C++ builder in point order,
 which records how many random bits (base 2) or permutation rows (base b)
-every new leaf needs; those draws are then made with the tree's Generator in
-that same order -- one vectorized numpy call per tree, which consumes the
-stream exactly as the reference's one call per leaf does -- and the GPU walks
-the finished trees to scramble every point in parallel.  Outputs equal the
-reference's for the same Generators, bit for bit.
+every new leaf needs.  Base 2: the draws are made with the tree's Generator in
+that order -- one vectorized numpy call per tree, which consumes the stream
+exactly as the reference's one call per leaf does.  Base b with numpy's SFC64
+(the generators the reference's documentation uses): the builder replays the
+generator itself on host threads, one tree per thread, and keeps one
+generator state per leaf instead of its rows (a reference leaf holds
+tmax_new - depth rows of b entries); the Generator objects are left in the
+state the reference's draws would leave them.  Other bit generators: the rows
+are drawn with numpy and all of them kept.  The GPU then walks the finished
+trees to scramble every point in parallel, regenerating leaf rows from their
+states.  Outputs equal the reference's for the same Generators, bit for bit.
 """
+import os
 import ctypes
 import time
 
@@ -47,6 +54,18 @@ def _declare(L):
     L.qmc_nusb_insert.argtypes = [ctypes.c_void_p, _u64p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, _u64p]
     L.qmc_nusb_add_rows.restype = ctypes.c_int
     L.qmc_nusb_add_rows.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]
+    L.qmc_nusb_tree_rows_needed.restype = ctypes.c_uint64
+    L.qmc_nusb_tree_rows_needed.argtypes = [ctypes.c_void_p]
+    for f in ("qmc_nusb_tree_rows", "qmc_nusb_tree_states", "qmc_nusb_tree_nodes", "qmc_nusb_tree_bytes"):
+        getattr(L, f).restype = ctypes.c_uint64
+        getattr(L, f).argtypes = [ctypes.c_void_p]
+    L.qmc_nusb_root_rows.restype = ctypes.c_int
+    L.qmc_nusb_root_rows.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    L.qmc_nusb_grow.restype = ctypes.c_int
+    L.qmc_nusb_grow.argtypes = ([ctypes.c_void_p, ctypes.c_uint64, _u64p] + [ctypes.c_uint64] * 4 +
+                                [_u64p, ctypes.c_void_p, ctypes.c_int])
+    L.qmc_nus_sfc64_permutations.restype = None
+    L.qmc_nus_sfc64_permutations.argtypes = [_u64p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_void_p]
     L.qmc_nusb_apply_dev.restype = ctypes.c_int
     L.qmc_nusb_apply_dev.argtypes = ([ctypes.c_void_p] + [ctypes.c_uint64] * 5 + [ctypes.c_int] * 2 +
                                      [_u64p, _u64p, _u64p, ctypes.c_void_p])
@@ -181,6 +200,40 @@ def dnb2_nested_uniform_scramble(r, n, d, r_x, tmax, tmax_new, rngs, root_nodes,
     return time.perf_counter() - t0_perf, time.process_time() - t0_proc
 
 
+def _sfc64_state(rng):
+    """numpy's SFC64 state as the library's 5 words, or None for other bit generators."""
+    bg = rng.bit_generator
+    if not isinstance(bg, np.random.SFC64):
+        return None
+    st = bg.state
+    w = np.empty(5, dtype=np.uint64)
+    w[:4] = np.asarray(st["state"]["state"], dtype=np.uint64)
+    w[4] = (int(st["has_uint32"]) << 32) | (int(st["uinteger"]) & 0xFFFFFFFF)
+    return w
+
+
+def _set_sfc64_state(rng, w):
+    rng.bit_generator.state = {"bit_generator": "SFC64", "state": {"state": np.array(w[:4], dtype=np.uint64)},
+                               "has_uint32": int(w[4] >> np.uint64(32)), "uinteger": int(w[4] & np.uint64(0xFFFFFFFF))}
+
+
+def sfc64_permutations(rng, count, b):
+    """`count` rows of Generator.permutation(b) drawn by the library's SFC64
+    replay from rng's state (rng advanced as numpy's draws would advance it);
+    the routine the base-b builder uses, exposed for checks."""
+    w = _sfc64_state(rng)
+    if w is None:
+        raise TypeError("sfc64_permutations needs a Generator over numpy's SFC64")
+    out = np.empty((int(count), int(b)), dtype=np.uint16)
+    _lib_nus().qmc_nus_sfc64_permutations(w.ctypes.data, int(b), int(count), out.ctypes.data)
+    _set_sfc64_state(rng, w)
+    return out
+
+
+def _host_threads():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
 def gdn_nested_uniform_scramble(r, n, d, r_x, r_b, tmax, tmax_new, rngs, root_nodes, bases, xdig, xrdig):
     """Nested uniform scramble of a base-b digital net (rand_funcs.py:274-356).
     xdig: (r_x, n, d, tmax) uint64, xrdig: (r, n, d, tmax_new) uint64 out,
@@ -190,46 +243,71 @@ def gdn_nested_uniform_scramble(r, n, d, r_x, r_b, tmax, tmax_new, rngs, root_no
     if not (0 <= tmax <= tmax_new):
         raise ValueError("gdn_nested_uniform_scramble: need tmax <= tmax_new")
     L = _lib_nus()
-    host = _host_u64(xdig).reshape(-1, n, d, tmax)
+    host = np.ascontiguousarray(_host_u64(xdig).reshape(-1, n, d, tmax))
+    if host.shape[0] < r_x:
+        raise ctypes.ArgumentError("xdig holds %d replications, r_x = %d" % (host.shape[0], r_x))
     b2 = np.atleast_2d(_host_u64(bases) if not isinstance(bases, np.ndarray) else bases)
-    handles = (ctypes.c_void_p * (r * d))()
-    counts = np.empty(max(n, 1), dtype=np.uint32)
+    nt = r * d
+    handles = (ctypes.c_void_p * max(nt, 1))()
+    states = np.zeros((max(nt, 1), 5), dtype=np.uint64)
+    replay = np.zeros(max(nt, 1), dtype=np.uint8)
+    drawn = []  # trees whose rows numpy draws (bit generators other than SFC64)
+    fresh = []
     for l in range(r):
-        pts = host[l % r_x]
         for j in range(d):
+            k = l * d + j
             node = root_nodes[l, j]
             if not isinstance(node, NUSNode_gdn):
                 raise TypeError("root_nodes must hold NUSNode_gdn objects")
             rng = rngs[l, j]
-            b = b2[l % r_b, j]
+            b = int(b2[l % r_b, j])
+            w = _sfc64_state(rng)
             if node._tree is None:
-                h = L.qmc_nusb_tree_new(tmax, tmax_new, int(b))
+                h = L.qmc_nusb_tree_new(tmax, tmax_new, b)
                 if not h:
                     _lib.check()
                 node._tree = _NativeTree(h, L.qmc_nusb_tree_free)
-                node._key = (tmax, tmax_new, int(b))
-                rows = random_uint64_permutations(rng, tmax_new, b)  # rand_funcs.py:316
-                rows16 = np.ascontiguousarray(rows, dtype=np.uint16)  # b <= 2^16: every entry fits
-                _lib.check(L.qmc_nusb_add_rows(h, rows16.ctypes.data, tmax_new))
-                node.perm, node.xdig = rows, np.zeros(tmax_new, dtype=np.uint64)
-            elif node._key != (tmax, tmax_new, int(b)):
+                node._key = (tmax, tmax_new, b)
+                node.xdig = np.zeros(tmax_new, dtype=np.uint64)
+                fresh.append(k)
+                if w is None:  # rand_funcs.py:316, drawn now as the reference does
+                    rows = random_uint64_permutations(rng, tmax_new, b)
+                    rows16 = np.ascontiguousarray(rows, dtype=np.uint16)  # kept alive through the call
+                    _lib.check(L.qmc_nusb_add_rows(h, rows16.ctypes.data, tmax_new))
+                    node.perm = rows
+            elif node._key != (tmax, tmax_new, b):
                 raise ValueError("root node was grown with (tmax, tmax_new, b) = %s" % (node._key,))
-            h = node._tree.handle
-            col = np.ascontiguousarray(pts[:, j, :])
-            k = L.qmc_nusb_insert(h, col.ctypes.data, n, tmax, 1, counts.ctypes.data)
-            if k < 0:
-                _lib.check()
-            total = int(counts[:k].sum()) if k else 0
-            if total:
-                bi = int(b)
-                # Generator.permuted shuffles each row with the same draws whatever
-                # the dtype, so 16-bit rows equal the reference's 64-bit ones.
-                rows = rng.permuted(np.tile(np.arange(bi, dtype=np.uint16), (total, 1)), axis=1)
-                rows = np.ascontiguousarray(rows)
-                _lib.check(L.qmc_nusb_add_rows(h, rows.ctypes.data, total))
-                assert rows.shape == (total, bi)
-            handles[l * d + j] = h
-    io = _Device(xdig, xrdig)
+            if w is None:
+                drawn.append(k)
+            else:
+                states[k] = w
+                replay[k] = 1
+            handles[k] = node._tree.handle
+    if nt:
+        _lib.check(L.qmc_nusb_grow(handles, nt, host.ctypes.data, r, n, d, r_x, states.ctypes.data,
+                                   replay.ctypes.data, _host_threads()))
+    for k in range(nt):
+        l, j = divmod(k, d)
+        node = root_nodes[l, j]
+        if replay[k]:
+            _set_sfc64_state(rngs[l, j], states[k])
+            if k in fresh:
+                b = node._key[2]
+                rows = np.empty((tmax_new, b), dtype=np.uint16)
+                _lib.check(L.qmc_nusb_root_rows(node._tree.handle, rows.ctypes.data))
+                node.perm = rows.astype(np.uint64)
+    for k in drawn:
+        l, j = divmod(k, d)
+        h = root_nodes[l, j]._tree.handle
+        total = int(L.qmc_nusb_tree_rows_needed(h))
+        if total:
+            bi = root_nodes[l, j]._key[2]
+            # Generator.permuted shuffles each row with the same draws whatever
+            # the dtype, so 16-bit rows equal the reference's 64-bit ones.
+            rows = rngs[l, j].permuted(np.tile(np.arange(bi, dtype=np.uint16), (total, 1)), axis=1)
+            rows = np.ascontiguousarray(rows)
+            _lib.check(L.qmc_nusb_add_rows(h, rows.ctypes.data, total))
+    io = _Device(host if not _is_torch(xdig) else xdig, xrdig)
     import torch
     bases_t = torch.from_numpy(np.ascontiguousarray(b2, dtype=np.uint64).view(np.int64)).to(io.dev)
     stream = torch.cuda.current_stream(io.dev)

@@ -105,6 +105,79 @@ def test_gdn_draw_schedule_matches_reference_order():
         assert got == want, (b, tmax, tmax_new)
 
 
+@needs_lib
+def test_sfc64_permutation_replay_matches_numpy():
+    """The builder's SFC64 replay of Generator.permutation(b) equals numpy's own
+    draws, row for row, and leaves the generator in numpy's state, from states
+    with and without a buffered 32-bit half."""
+    for seed in range(4):
+        for b in (2, 3, 5, 7, 131, 1619, 65536):
+            a = np.random.Generator(np.random.SFC64(seed))
+            c = np.random.Generator(np.random.SFC64(seed))
+            if seed % 2:  # leave half of a 64-bit output buffered
+                a.integers(0, 2**31, dtype=np.uint32)
+                c.integers(0, 2**31, dtype=np.uint32)
+            count = 3 if b > 2000 else 12
+            want = np.stack([a.permutation(b) for _ in range(count)])
+            got = nus.sfc64_permutations(c, count, b)
+            assert np.array_equal(got.astype(np.int64), want), (seed, b)
+            sa, sc = a.bit_generator.state, c.bit_generator.state
+            assert list(sa["state"]["state"]) == list(sc["state"]["state"]), (seed, b)
+            assert (sa["has_uint32"], sa["uinteger"]) == (sc["has_uint32"], sc["uinteger"]), (seed, b)
+
+
+@needs_lib
+@pytest.mark.parametrize("bases,tmax,tmax_new", [([2, 3, 5], 3, 8), ([3, 7], 4, 6), ([131], 2, 5), ([3], 12, 52)])
+def test_gdn_replay_consumes_the_stream_as_the_reference(bases, tmax, tmax_new):
+    """After two calls that grow the same trees, each tree's generator is where
+    the reference's draws leave it (rand_funcs.py:312-352, through the oracle),
+    and the root rows equal the reference's root_node.perm.  Host side only:
+    the builder's insertion and replay run without a GPU."""
+    L = nus._lib_nus()
+    rng = np.random.default_rng(len(bases) * 7 + tmax)
+    d, r = len(bases), 2
+    n = 200 if tmax > 4 else 60
+    b = np.array([bases], dtype=np.uint64)
+    xdig = np.ascontiguousarray(np.stack([rng.integers(0, bb, size=(1, n, tmax), dtype=np.uint64)
+                                          for bb in bases], axis=2))
+    xdig[:, 7] = xdig[:, 3]
+    roots_o = np.array([nus_oracle.NodeB() for _ in range(r * d)]).reshape(r, d)
+    rngs_o, rngs_p = _rngs(r, d, 11), _rngs(r, d, 11)
+    trees = [L.qmc_nusb_tree_new(tmax, tmax_new, int(bb)) for _ in range(r) for bb in bases]
+    handles = (ctypes.c_void_p * (r * d))(*trees)
+    try:
+        for part in (xdig[:, :n // 2], xdig[:, n // 2:]):
+            part = np.ascontiguousarray(part)
+            m = part.shape[1]
+            want = np.zeros((r, m, d, tmax_new), dtype=np.uint64)
+            nus_oracle.gdn_nus(r, m, d, 1, 1, tmax, tmax_new, rngs_o, roots_o, b, part, want)
+            states = np.stack([nus._sfc64_state(g) for g in rngs_p.ravel()])
+            replay = np.ones(r * d, dtype=np.uint8)
+            _lib.check(L.qmc_nusb_grow(handles, r * d, part.ctypes.data, r, m, d, 1, states.ctypes.data,
+                                       replay.ctypes.data, 4))
+            for k, g in enumerate(rngs_p.ravel()):
+                nus._set_sfc64_state(g, states[k])
+            for go, gp in zip(rngs_o.ravel(), rngs_p.ravel()):
+                so, sp = go.bit_generator.state, gp.bit_generator.state
+                assert list(so["state"]["state"]) == list(sp["state"]["state"])
+                assert (so["has_uint32"], so["uinteger"]) == (sp["has_uint32"], sp["uinteger"])
+        for k in range(r * d):
+            bb = bases[k % d]
+            rows = np.empty((tmax_new, bb), dtype=np.uint16)
+            _lib.check(L.qmc_nusb_root_rows(trees[k], rows.ctypes.data))
+            root = roots_o.ravel()[k]
+            first = root.perm if root.perm.ndim == 2 else None
+            if first is not None:  # the root is still a leaf in the oracle: all its rows
+                assert np.array_equal(rows.astype(np.uint64), first)
+            else:  # split: its own row is the first one drawn
+                assert np.array_equal(rows[0].astype(np.uint64), root.perm)
+            # leaves keep a state, not rows: materialized rows stay far below the reference's
+            assert L.qmc_nusb_tree_rows(trees[k]) <= tmax_new + L.qmc_nusb_tree_nodes(trees[k])
+    finally:
+        for h in trees:
+            L.qmc_nusb_tree_free(h)
+
+
 def _rngs(r, d, seed):
     seeds = np.random.SeedSequence(seed).spawn(r * d)
     return np.array([np.random.Generator(np.random.SFC64(s)) for s in seeds]).reshape(r, d)
@@ -149,6 +222,62 @@ def test_gdn_nus_matches_oracle_gpu(bases, tmax, tmax_new):
         got = np.zeros_like(want)
         nus_oracle.gdn_nus(r, part.shape[1], d, r_x, 1, tmax, tmax_new, rngs_o, roots_o, b, part, want)
         q.gdn_nested_uniform_scramble(r, part.shape[1], d, r_x, 1, tmax, tmax_new, rngs_p, roots_p, b, part, got)
+        assert np.array_equal(got, want)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("gen", ["sfc64", "pcg64"])
+def test_gdn_nus_large_bases_and_other_generators_gpu(gen):
+    """Leaf rows regenerated on the device for a base whose per-lane draw
+    buffers exceed the LDS budget (1619: device scratch), beside small bases;
+    with PCG64 generators every row is drawn by numpy and kept instead."""
+    rng = np.random.default_rng(40)
+    bases = [3, 1619, 131]
+    d, r, n, tmax, tmax_new = len(bases), 2, 90, 3, 9
+    b = np.array([bases], dtype=np.uint64)
+    xdig = np.ascontiguousarray(np.stack([rng.integers(0, bb, size=(1, n, tmax), dtype=np.uint64)
+                                          for bb in bases], axis=2))
+    xdig[:, 11] = xdig[:, 5]
+
+    def gens(seed):
+        ss = np.random.SeedSequence(seed).spawn(r * d)
+        bg = np.random.SFC64 if gen == "sfc64" else np.random.PCG64
+        return np.array([np.random.Generator(bg(s)) for s in ss]).reshape(r, d)
+
+    roots_o = np.array([nus_oracle.NodeB() for _ in range(r * d)]).reshape(r, d)
+    roots_p = np.array([q.NUSNode_gdn() for _ in range(r * d)]).reshape(r, d)
+    rngs_o, rngs_p = gens(8), gens(8)
+    for part in (xdig[:, :50], xdig[:, 50:]):
+        part = np.ascontiguousarray(part)
+        want = np.zeros((r, part.shape[1], d, tmax_new), dtype=np.uint64)
+        got = np.zeros_like(want)
+        nus_oracle.gdn_nus(r, part.shape[1], d, 1, 1, tmax, tmax_new, rngs_o, roots_o, b, part, want)
+        q.gdn_nested_uniform_scramble(r, part.shape[1], d, 1, 1, tmax, tmax_new, rngs_p, roots_p, b, part, got)
+        assert np.array_equal(got, want)
+    assert all(a.integers(0, 2**40) == c.integers(0, 2**40) for a, c in zip(rngs_o.ravel(), rngs_p.ravel()))
+
+
+@pytest.mark.gpu
+def test_gdn_nus_first_32_primes_many_points_gpu():
+    """The config-2 shape in miniature: the first 32 primes, 20 -> 52 digits,
+    3^6 points in two calls, against the oracle (which materializes every
+    row); most leaves' rows come from stored generator states here."""
+    from qmctoolscl_amd.batch import first_primes
+    rng = np.random.default_rng(41)
+    bases = first_primes(32)
+    d, r, n, tmax, tmax_new = 32, 1, 3 ** 6, 20, 52
+    b = np.array([bases], dtype=np.uint64)
+    xdig = np.ascontiguousarray(np.stack([rng.integers(0, bb, size=(1, n, tmax), dtype=np.uint64)
+                                          for bb in bases], axis=2))
+    roots_o = np.array([nus_oracle.NodeB() for _ in range(r * d)]).reshape(r, d)
+    roots_p = np.array([q.NUSNode_gdn() for _ in range(r * d)]).reshape(r, d)
+    rngs_o, rngs_p = _rngs(r, d, 12), _rngs(r, d, 12)
+    for part in (xdig[:, :300], xdig[:, 300:]):
+        part = np.ascontiguousarray(part)
+        want = np.zeros((r, part.shape[1], d, tmax_new), dtype=np.uint64)
+        got = np.zeros_like(want)
+        nus_oracle.gdn_nus(r, part.shape[1], d, 1, 1, tmax, tmax_new, rngs_o, roots_o, b, part, want)
+        q.gdn_nested_uniform_scramble(r, part.shape[1], d, 1, 1, tmax, tmax_new, rngs_p, roots_p, b, part, got)
         assert np.array_equal(got, want)
 
 
