@@ -515,7 +515,7 @@ __global__ __launch_bounds__(kBlock) void gdn_interlace_kernel(int undo, u64 r, 
 // every digit from its index (gdn_digit).  The d blocks of one point range
 // are consecutive on one XCD (block b runs on XCD b % 8), so the 8-byte
 // stores of a point's row meet in that XCD's L2.
-constexpr u32 kFuseH = 64;  // consecutive points per lane
+constexpr u32 kFuseH = 128;  // consecutive points per lane (64: 1.53 ms against 1.40 on config 2)
 
 struct FuseArgs {
     u64 n, d, r_b, mmax, n_start, bmax, Ctotal, tab_total, ngroups, nchunks;
@@ -625,8 +625,6 @@ __global__ __launch_bounds__(TAB ? kFuseTabBlock : kBlock) void gdn_fused_kernel
     double *sThr = (double *)(smem + a.off_thr + 8 * 64);
     u32 *sTW = (u32 *)(smem + a.off_thr + 8 * 64 + 8 * 65);
     u32 bad0 = 0;
-    if (TAB)
-        for (u32 t = threadIdx.x; t < 64; t += blockDim.x) sM[t] = 0;
     const u64 pbase = lj * tnew * a.bmax;
     if (ok)
         for (u32 e = threadIdx.x; e < (u32)a.mmax * TP; e += blockDim.x) {
@@ -681,12 +679,25 @@ __global__ __launch_bounds__(TAB ? kFuseTabBlock : kBlock) void gdn_fused_kernel
             }
             const double term = t < tnew ? srt[t] * y : 0.0;
             V[zz * (TP + 1) + t] = term;
-            if (t < tnew) atomicMax(&sM[t], (unsigned long long)__double_as_longlong(term));
         }
-        // levels past the table: one tail term each
-        for (u32 t = TP + threadIdx.x; t < tnew; t += blockDim.x)
-            atomicMax(&sM[t], (unsigned long long)__double_as_longlong(
-                                  srt[t] * (double)fuse_tail<MODE>(a, t, pbase, sP, sS, false, bad0)));
+        __syncthreads();
+        // the largest term of each level: a scan of the table's column by one
+        // thread per level (terms are >= 0, so their bit patterns order as the
+        // values; LDS atomics on one word per level serialized the fill);
+        // levels past the table have one tail term each
+        for (u32 t = threadIdx.x; t < tnew; t += blockDim.x) {
+            unsigned long long mx = 0;
+            if (t < (u32)TP) {
+                for (u32 zz = 0; zz < vrows; zz++) {
+                    const unsigned long long w = (unsigned long long)__double_as_longlong(V[zz * (TP + 1) + t]);
+                    mx = w > mx ? w : mx;
+                }
+            } else {
+                mx = (unsigned long long)__double_as_longlong(srt[t] *
+                                                               (double)fuse_tail<MODE>(a, t, pbase, sP, sS, false, bad0));
+            }
+            sM[t] = mx;
+        }
         __syncthreads();
         // Exact early exit of the sums (as in the batch's gdn class): B_t =
         // the largest term of any level s >= t (terms are >= 0, so their bit
@@ -781,9 +792,9 @@ __global__ __launch_bounds__(TAB ? kFuseTabBlock : kBlock) void gdn_fused_kernel
         const u32 tw = __builtin_amdgcn_readfirstlane(sTW[0]);  // levels a point usually needs (uniform)
 #pragma unroll
         for (int t = 0; t < TP; t++) z[t] *= S;
-        // walk and sum the first TL levels, TL the smallest of 16, 32, TP
-        // covering tw: a wave adds every one of them (terms past a lane's
-        // threshold leave its sum unchanged) and checks once, at TL
+        // walk and sum the first TL levels (TL covering tw, below): a wave adds
+        // every one of them (terms past a lane's threshold leave its sum
+        // unchanged) and checks once, at TL
         auto run = [&](auto tl) {
             constexpr int TL = decltype(tl)::value;
             const double thl = sThr[(u32)TL < tnew ? (u32)TL : tnew];
@@ -821,12 +832,16 @@ __global__ __launch_bounds__(TAB ? kFuseTabBlock : kBlock) void gdn_fused_kernel
                 k++;
             }
         };
-        if (TP > 16 && tw <= 16)
-            run(std::integral_constant<int, (TP > 16 ? 16 : TP)>());
-        else if (TP > 32 && tw <= 32)
-            run(std::integral_constant<int, (TP > 32 ? 32 : TP)>());
-        else
-            run(std::integral_constant<int, TP>());
+        // TL: the smallest of 12, 16, 20, 24, 28, 32, 40, 48 (below TP) or TP
+        // at or above tw (each walked level costs its walk step, load and add
+        // on every point: the buckets follow the bases closely)
+#define QMC_FUSE_TL(L_)                                                  \
+    else if (TP > (L_) && tw <= (L_)) run(std::integral_constant<int, (TP > (L_) ? (L_) : TP)>());
+        if (false) {
+        }
+        QMC_FUSE_TL(12) QMC_FUSE_TL(16) QMC_FUSE_TL(20) QMC_FUSE_TL(24) QMC_FUSE_TL(28) QMC_FUSE_TL(32)
+        QMC_FUSE_TL(40) QMC_FUSE_TL(48) else run(std::integral_constant<int, TP>());
+#undef QMC_FUSE_TL
         if (bad | bad0) flag(a.status, bad | bad0);
         return;
     }
