@@ -368,6 +368,10 @@ QMC_EXPORT int qmc_mfma_probe_dev(const float *a, const float *b, float *d, qmc_
 QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t *m, const int32_t *d,
                                 const int32_t *r, const qmc_u64 *seeds, qmc_u64 points_per_tile, int grid,
                                 qmc_u64 ring_elems, qmc_u64 gdn_points_per_tile, int gdn_level_cap);
+/* the same with a tile size per method: tile_points[method], method 0..3 */
+QMC_EXPORT void *qmc_batch_plan_tiles(int nreq, const int32_t *methods, const int32_t *m, const int32_t *d,
+                                      const int32_t *r, const qmc_u64 *seeds, const qmc_u64 *tile_points, int grid,
+                                      qmc_u64 ring_elems, int gdn_level_cap);
 QMC_EXPORT void qmc_batch_free(void *plan);
 QMC_EXPORT qmc_u64 qmc_batch_total_tiles(void *plan);
 QMC_EXPORT qmc_u64 qmc_batch_points(void *plan);

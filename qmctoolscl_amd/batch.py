@@ -40,6 +40,8 @@ def _declare(L):
     L.qmc_batch_plan.restype = ctypes.c_void_p
     L.qmc_batch_plan.argtypes = [ctypes.c_int] + [_vp] * 5 + [ctypes.c_uint64, ctypes.c_int, ctypes.c_uint64,
                                                                ctypes.c_uint64, ctypes.c_int]
+    L.qmc_batch_plan_tiles.restype = ctypes.c_void_p
+    L.qmc_batch_plan_tiles.argtypes = [ctypes.c_int] + [_vp] * 6 + [ctypes.c_int, ctypes.c_uint64, ctypes.c_int]
     L.qmc_batch_free.argtypes = [_vp]
     L.qmc_batch_total_tiles.restype = ctypes.c_uint64
     L.qmc_batch_total_tiles.argtypes = [_vp]
@@ -86,6 +88,10 @@ def sample_requests(count, seed, m_range=(8, 22), d_range=(1, 256), r_range=(1, 
 
 
 class Batch:
+    """A plan.  points_per_tile: one tile size (points) for the lattice and
+    dnb2 methods, or a sequence of three (lattice, dnb2 LMS, dnb2 hash NUS);
+    gdn_points_per_tile (0 = the lattice's) sizes the gdn tiles."""
+
     def __init__(self, methods, m, d, r, seeds, points_per_tile=1 << 16, grid=0, ring_elems=None,
                  gdn_points_per_tile=0, gdn_cus=0, gdn_level_cap=0):
         self.methods = np.ascontiguousarray(methods, dtype=np.int32)
@@ -96,17 +102,21 @@ class Batch:
         slabs = (np.int64(1) << self.m.astype(np.int64)) * self.d
         self.points = int(np.sum(slabs * self.r))
         self.ring_elems = int(ring_elems) if ring_elems is not None else self.points
-        self.points_per_tile = int(points_per_tile)
+        tp = [int(v) for v in points_per_tile] if np.ndim(points_per_tile) else [int(points_per_tile)] * 3
+        if len(tp) != 3:
+            raise ValueError("points_per_tile: one size or three (lattice, dnb2 LMS, dnb2 hash NUS)")
+        self.points_per_tile = tp[0] if len(set(tp)) == 1 else tp
+        self.gdn_points_per_tile = int(gdn_points_per_tile) or tp[0]
+        self.tile_points = np.array(tp + [self.gdn_points_per_tile], dtype=np.uint64)
         self.grid = int(grid)
         L = _L()
-        h = L.qmc_batch_plan(len(self.m), self.methods.ctypes.data, self.m.ctypes.data, self.d.ctypes.data,
-                             self.r.ctypes.data, self.seeds.ctypes.data, self.points_per_tile, self.grid,
-                             self.ring_elems, int(gdn_points_per_tile), int(gdn_level_cap))
+        h = L.qmc_batch_plan_tiles(len(self.m), self.methods.ctypes.data, self.m.ctypes.data, self.d.ctypes.data,
+                                   self.r.ctypes.data, self.seeds.ctypes.data, self.tile_points.ctypes.data,
+                                   self.grid, self.ring_elems, int(gdn_level_cap))
         if not h:
             _lib.check()
         self._h = h
         self._free = L.qmc_batch_free
-        self.gdn_points_per_tile = int(gdn_points_per_tile) or self.points_per_tile
         self.gdn_cus = 0
         if gdn_cus:
             self.set_gdn_cus(gdn_cus)
@@ -232,7 +242,7 @@ def owen_hash64(x, seed):
 
 # ------------------------------------------------------------------ autotuner
 
-DEFAULT_TILES = (1 << 14, 1 << 15, 1 << 16, 1 << 17)
+DEFAULT_TILES = (1 << 13, 1 << 14, 1 << 15, 1 << 16, 1 << 17)
 DEFAULT_GRIDS_PER_CU = (4, 8)
 DEFAULT_GDN_TILES = (1 << 15, 1 << 16, 1 << 17)
 DEFAULT_GDN_SHARES = (0.5, 0.625, 0.75, 0.875, 1.0)
@@ -256,11 +266,17 @@ def _time_plan(plan, ring, reps):
 
 def autotune(methods, m, d, r, seeds, ring, tiles=DEFAULT_TILES, grids_per_cu=DEFAULT_GRIDS_PER_CU,
              gdn_tiles=DEFAULT_GDN_TILES, gdn_shares=DEFAULT_GDN_SHARES, reps=2, cache=True):
-    """Pick the batch's tunables by timing it: the tile size and grid of the
-    lattice/dnb2 class on those requests alone, the gdn tile size on the gdn
-    requests alone, then the split of the CUs between the two classes on the
-    whole batch (the gdn kernel's share; the store-bound kernel runs on the
-    rest beside it).  Returns {"points_per_tile", "grid",
+    """Pick the batch's tunables by timing it, one dimension at a time:
+    1. each store-bound method's tile size (rows of all d columns per tile: the
+       split of its n x d slabs; lattice, dnb2 LMS, dnb2 hash NUS) on that
+       method's requests alone;
+    2. the persistent grid of the lattice/dnb2 class on those requests with
+       their chosen tiles;
+    3. the gdn tile size (rows of a column group: the n split of gdn slabs,
+       whose d split follows the 128-byte lines) on the gdn requests alone;
+    4. the split of the CUs between the two classes on the whole batch (the
+       gdn kernel's share; the store-bound kernel runs on the rest beside it).
+    Returns {"points_per_tile": [lattice, dnb2, nus], "grid",
     "gdn_points_per_tile", "gdn_cus", "ms", "table"}, cached per (device,
     batch signature).  The searched shapes replace the reference's global-size
     sweeps (gs_run.py:42-88) for this path."""
@@ -270,8 +286,8 @@ def autotune(methods, m, d, r, seeds, ring, tiles=DEFAULT_TILES, grids_per_cu=DE
     dev = ring.device
     props = torch.cuda.get_device_properties(dev)
     cus = props.multi_processor_count
-    sig = "%s|%d|%d|%s" % (props.name, len(m), int(np.sum((np.int64(1) << m.astype(np.int64)) * d * r)),
-                           hash(seeds.tobytes() + methods.tobytes()))
+    sig = "%s|%d|%d|%s|v2" % (props.name, len(m), int(np.sum((np.int64(1) << m.astype(np.int64)) * d * r)),
+                              hash(seeds.tobytes() + methods.tobytes()))
     path = _cache_path()
     if cache and os.path.exists(path):
         try:
@@ -283,19 +299,34 @@ def autotune(methods, m, d, r, seeds, ring, tiles=DEFAULT_TILES, grids_per_cu=DE
             pass
     table = []
     n_ring = ring.numel()
+    names = ("lattice", "dnb2_lms", "dnb2_nus")
     c0 = methods != 3
-    best0 = (tiles[0], grids_per_cu[0] * cus)
+    tp = [tiles[0]] * 3
+    grid0 = grids_per_cu[0] * cus
+    for k in range(3):
+        sel = methods == k
+        if not sel.any():
+            continue
+        best = None
+        for p in tiles:
+            plan = Batch(methods[sel], m[sel], d[sel], r[sel], seeds[sel], points_per_tile=p, grid=grid0,
+                         ring_elems=n_ring)
+            ms = _time_plan(plan, ring, reps)
+            table.append({"class": names[k], "points_per_tile": p, "grid": grid0, "ms": round(ms, 3)})
+            if best is None or ms < best:
+                best, tp[k] = ms, p
+            del plan
+    best0 = grid0
     if c0.any():
         ms0 = None
-        for p in tiles:
-            for g in grids_per_cu:
-                plan = Batch(methods[c0], m[c0], d[c0], r[c0], seeds[c0], points_per_tile=p, grid=g * cus,
-                             ring_elems=n_ring)
-                ms = _time_plan(plan, ring, reps)
-                table.append({"class": "lattice+dnb2", "points_per_tile": p, "grid": g * cus, "ms": round(ms, 3)})
-                if ms0 is None or ms < ms0:
-                    ms0, best0 = ms, (p, g * cus)
-                del plan
+        for g in grids_per_cu:
+            plan = Batch(methods[c0], m[c0], d[c0], r[c0], seeds[c0], points_per_tile=tp, grid=g * cus,
+                         ring_elems=n_ring)
+            ms = _time_plan(plan, ring, reps)
+            table.append({"class": "lattice+dnb2", "points_per_tile": list(tp), "grid": g * cus, "ms": round(ms, 3)})
+            if ms0 is None or ms < ms0:
+                ms0, best0 = ms, g * cus
+            del plan
     bestg = gdn_tiles[0]
     if (~c0).any():
         msg = None
@@ -306,7 +337,7 @@ def autotune(methods, m, d, r, seeds, ring, tiles=DEFAULT_TILES, grids_per_cu=DE
             if msg is None or ms < msg:
                 msg, bestg = ms, p
             del plan
-    plan = Batch(methods, m, d, r, seeds, points_per_tile=best0[0], grid=best0[1], ring_elems=n_ring,
+    plan = Batch(methods, m, d, r, seeds, points_per_tile=tp, grid=best0, ring_elems=n_ring,
                  gdn_points_per_tile=bestg)
     best = None
     shares = gdn_shares if (c0.any() and (~c0).any()) else (1.0,)
@@ -316,7 +347,7 @@ def autotune(methods, m, d, r, seeds, ring, tiles=DEFAULT_TILES, grids_per_cu=DE
         ms = _time_plan(plan, ring, reps)
         table.append({"class": "both", "gdn_cus": n_cus, "ms": round(ms, 3)})
         if best is None or ms < best["ms"]:
-            best = {"points_per_tile": best0[0], "grid": best0[1], "gdn_points_per_tile": bestg, "gdn_cus": n_cus,
+            best = {"points_per_tile": list(tp), "grid": best0, "gdn_points_per_tile": bestg, "gdn_cus": n_cus,
                     "ms": round(ms, 3)}
     del plan
     best["table"] = table

@@ -1316,16 +1316,17 @@ extern "C" {
 QMC_EXPORT void qmc_batch_free(void *h);
 
 // Build a plan for nreq requests: methods[q] in 0..3, m[q] = log2 n in [1, 32]
-// (1..31 for lattices), d[q] in [1, 256], r[q] >= 1, seeds[q]; `points_per_tile`
-// (power of two) and `grid` (persistent workgroups; 0 = 4 per CU) are the
-// tunables; `ring_elems` is the output ring size in doubles (every replication
-// slab must fit); `gdn_points_per_tile` (0 = points_per_tile) sizes the gdn
-// tiles; `gdn_level_cap` (0 = none) limits the permutation levels a gdn tile
-// reads from its tables, the rest taking the index-and-key path (a test hook
-// for that path; the output does not change).
-QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t *m, const int32_t *d,
-                                const int32_t *r, const u64 *seeds, u64 points_per_tile, int grid, u64 ring_elems,
-                                u64 gdn_points_per_tile, int gdn_level_cap) {
+// (1..31 for lattices), d[q] in [1, 256], r[q] >= 1, seeds[q]; the tunables are
+// tile_points[method] (the points a tile of that method holds: rows of all d
+// columns for methods 0..2, rows of a group of columns for gdn, rounded down
+// to a power of two of rows) and `grid` (persistent workgroups of the
+// lattice/dnb2 class; 0 = 4 per CU); `ring_elems` is the output ring size in
+// doubles (every replication slab must fit); `gdn_level_cap` (0 = none) limits
+// the permutation levels a gdn tile reads from its tables, the rest taking the
+// index-and-key path (a test hook for that path; the output does not change).
+QMC_EXPORT void *qmc_batch_plan_tiles(int nreq, const int32_t *methods, const int32_t *m, const int32_t *d,
+                                      const int32_t *r, const u64 *seeds, const u64 *tile_points, int grid,
+                                      u64 ring_elems, int gdn_level_cap) {
     fill_primes();
     Plan *P = new Plan();
     P->ring_elems = ring_elems;
@@ -1347,10 +1348,10 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
         const u64 n = 1ull << q.m;
         u64 rows;
         if (q.method == M_GDN_LMS) {
-            rows = (gdn_points_per_tile ? gdn_points_per_tile : points_per_tile) / 4;  // a tile of 4 columns
+            rows = tile_points[3] / 4;  // a tile of 4 columns
             if (rows < 256) rows = 256;
         } else {
-            rows = points_per_tile / (u64)q.d;
+            rows = tile_points[q.method] / (u64)q.d;
             if ((q.method == M_DNB2_LMS || q.method == M_DNB2_NUS) && rows < 1024) rows = 1024;  // 64-step chains at 2^s = 16
         }
         u64 p2 = 1;
@@ -1557,6 +1558,16 @@ QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t 
         return nullptr;
     }
     return P;
+}
+
+// The same plan with one tile size for the lattice and dnb2 methods and
+// `gdn_points_per_tile` (0 = points_per_tile) for gdn.
+QMC_EXPORT void *qmc_batch_plan(int nreq, const int32_t *methods, const int32_t *m, const int32_t *d,
+                                const int32_t *r, const u64 *seeds, u64 points_per_tile, int grid, u64 ring_elems,
+                                u64 gdn_points_per_tile, int gdn_level_cap) {
+    const u64 tp[4] = {points_per_tile, points_per_tile, points_per_tile,
+                       gdn_points_per_tile ? gdn_points_per_tile : points_per_tile};
+    return qmc_batch_plan_tiles(nreq, methods, m, d, r, seeds, tp, grid, ring_elems, gdn_level_cap);
 }
 
 QMC_EXPORT void qmc_batch_free(void *h) {
