@@ -114,7 +114,9 @@ def section_batch(torch, dev, timed, args):
     m_tune = np.maximum(m - 2, 8).astype(np.int32)
     t0 = time.perf_counter()
     if args.tune:
-        p, g, gp, gc = (int(v) for v in args.tune.split(","))
+        p, g, gp, gc = args.tune.split(",")
+        p = [int(v) for v in p.split("/")] if "/" in p else int(p)
+        g, gp, gc = int(g), int(gp), int(gc)
         tune = {"points_per_tile": p, "grid": g, "gdn_points_per_tile": gp, "gdn_cus": gc, "ms": None,
                 "table": "given on the command line"}
     else:
@@ -816,7 +818,8 @@ def main():
     ap.add_argument("--sections", default="lattice,dnb2,transforms,gdn,halton,kernels,chain",
                     help="further sections after the headline batch (also: none)")
     ap.add_argument("--quick-batch", action="store_true", help="cap request sizes at n=2^16")
-    ap.add_argument("--tune", default="", help="points_per_tile,grid,gdn_points_per_tile,gdn_cus (skip the autotuner)")
+    ap.add_argument("--tune", default="", help="points_per_tile,grid,gdn_points_per_tile,gdn_cus (skip the autotuner; "
+                                               "points_per_tile may be lattice/dnb2/nus)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a plain 2 GiB fill and a 2 GiB device-to-device copy (known-size kernels that scale "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
