@@ -155,6 +155,23 @@ def section_batch(torch, dev, timed, args):
         by_method[name] = {"points": sub.points, "ms": round(ms_k, 3), "Gpts_s": round(sub.points / ms_k / 1e6, 1),
                            "frac_hbm": round(8 * sub.points / ms_k / 1e6 / HBM_PEAK_GBS, 4)}
         del sub
+    # the batch's dnb2 LMS+shift requests at their full sizes, as a batch of their
+    # own: batched LMS+shift digital-net generation against one launch per request
+    sel = meth == 1
+    lp = B.Batch(meth[sel], m[sel], d[sel], r[sel], seeds[sel], points_per_tile=tune["points_per_tile"],
+                 grid=tune["grid"], ring_elems=ring_elems)
+    ms_l, _ = timed(lambda: lp.run(ring, graph=True), max(args.steps, 2), 1)
+    ms_lp, _ = timed(lambda: lp.run_per_call(ring), 1, 0)
+    dl_b, dl_p = lp.digests(ring), lp.digests(ring, per_call=True)
+    lms_shift = {"requests": int(sel.sum()), "points": lp.points,
+                 "m": [int(m[sel].min()), int(m[sel].max())], "batched_ms": round(ms_l, 3),
+                 "batched_Gpts_s": round(lp.points / ms_l / 1e6, 1),
+                 "batched_frac_hbm": round(8 * lp.points / ms_l / 1e6 / HBM_PEAK_GBS, 4),
+                 "per_call_ms": round(ms_lp, 3), "speedup_vs_per_call": round(ms_lp / ms_l, 3),
+                 "bitexact_vs_per_call": bool(np.array_equal(dl_b, dl_p)),
+                 "form": "the dnb2 LMS+shift requests of the 512-request batch (full m), batched in one "
+                         "persistent graph launch; per call = one launch of the same fused kernel per request"}
+    del lp
     del ring
     torch.cuda.empty_cache()
     gbs = 8 * pts / ms_b / 1e6
@@ -169,7 +186,7 @@ def section_batch(torch, dev, timed, args):
                        "batched_frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
                        "per_call_ms": round(ms_p, 2), "per_call_Gpts_s": round(pts / ms_p / 1e6, 2),
                        "speedup_vs_per_call": round(ms_p / ms_b, 3),
-                       "by_method_m_le_16": by_method}}
+                       "by_method_m_le_16": by_method, "lms_shift": lms_shift}}
 
 
 def section_lattice(torch, q, dev, timed, steps, warmup):
@@ -468,6 +485,19 @@ def section_batch_vs_chain(torch, q, dev, timed):
             f()
 
     ms_c, _ = timed(chain, 3, 1)
+    # the dnb2 LMS+shift requests alone, both ways
+    sel = meth == 1
+    lplan = B.Batch(meth[sel], m[sel], d[sel], r[sel], seeds[sel], points_per_tile=1 << 15)
+    lring = torch.empty(lplan.ring_elems, dtype=torch.float64, device=dev)
+    ms_lb, _ = timed(lambda: lplan.run(lring, graph=True), 5, 2)
+    del lring
+    lcalls = [f for f, k in zip(calls, meth) if k == 1]
+
+    def lchain():
+        for f in lcalls:
+            f()
+
+    ms_lc, _ = timed(lchain, 3, 1)
     by = {}
     for k, name in enumerate(("lattice", "dnb2_lms_shift", "gdn_lms_perm")):
         sel = meth == (0, 1, 3)[k]
@@ -479,7 +509,10 @@ def section_batch_vs_chain(torch, q, dev, timed):
             "dropin_chain_per_request_ms": round(ms_c, 4),
             "dropin_chain_per_request_Gpts_s": round(pts / ms_c / 1e6, 2),
             "speedup_vs_fused_per_request": round(ms_f / ms_b, 3),
-            "speedup_vs_dropin_chain": round(ms_c / ms_b, 3)}
+            "speedup_vs_dropin_chain": round(ms_c / ms_b, 3),
+            "lms_shift_only": {"requests": int(sel.sum()), "points": lplan.points,
+                               "batched_ms": round(ms_lb, 4), "dropin_chain_ms": round(ms_lc, 4),
+                               "speedup_vs_dropin_chain": round(ms_lc / ms_lb, 3)}}
 
 
 def section_dnb2(torch, q, fused, timed, dev):
@@ -879,6 +912,12 @@ def main():
         "wall_s_timed": round(b["wall_s"], 3),
         "sections": {"batch": b["detail"]},
     }
+    if "lms_shift" in b["detail"]:
+        ls = b["detail"]["lms_shift"]
+        out["lms_shift_batched"] = {"value": ls["batched_Gpts_s"], "unit": "Gpts/s",
+                                    "frac_hbm": ls["batched_frac_hbm"],
+                                    "speedup_vs_per_call": ls["speedup_vs_per_call"],
+                                    "workload": "the batch's %d dnb2 LMS+shift requests at full size" % ls["requests"]}
     if "chain" in sections:
         log("section: batch vs per-request drop-in chains")
         out["sections"]["batch_vs_dropin_chain"] = section_batch_vs_chain(torch, q, dev, timed)
@@ -892,6 +931,8 @@ def main():
                     "launch of the same requests",
             "workload": c["workload"], "value": c["dropin_chain_per_request_Gpts_s"], "unit": "Gpts/s",
             "batched_Gpts_s": c["batched_Gpts_s"], "speedup_of_batched": c["speedup_vs_dropin_chain"]})
+        if "lms_shift_batched" in out:
+            out["lms_shift_batched"]["speedup_vs_dropin_chain"] = c["lms_shift_only"]["speedup_vs_dropin_chain"]
     if "lattice" in sections:
         log("section: lattice (config 1)")
         out["sections"]["lattice"] = section_lattice(torch, q, dev, timed, 50, 20)
