@@ -326,10 +326,12 @@ def test_empty_inputs_are_no_ops():
                     np.ones((0, 2, 8), dtype=np.uint64), xb)
 
 
-@pytest.mark.parametrize("m,V", [(12, 3), (13, 2), (14, 2), (16, 2), (18, 1), (20, 1)])
+@pytest.mark.parametrize("m,V", [(12, 3), (13, 2), (14, 2), (16, 2), (18, 1), (20, 1), (25, 1)])
 def test_large_transforms_match_oracle(m, V):
     """Lengths past one LDS tile exercise the register passes (and the
-    stage-major twiddle table): FWHT still bit-exact, FFT/IFFT within 1e-12."""
+    stage-major twiddle table; 2^25: register groups of 4, 4 and 4 stages and
+    one of a single stage above the 12-bit tile): FWHT still bit-exact,
+    FFT/IFFT within 1e-12."""
     rng = np.random.default_rng(m)
     n = 1 << m
     x = rng.standard_normal((1, V, n))
