@@ -465,6 +465,8 @@ def _oracle_gdn_chain(mode, r, n, d, r_b, mmax, tmax, tnew, n_start, bmax, bases
     (2, 1, 800, 2, 1, 6, 60, 60, 0, "b131"),          # b = 131 above the 126 rows a 64-digit shift table holds
     (1, 2, 2000, 6, 1, 12, 52, 52, 0, "zerolead"),    # sums still 0 after 20 levels: every point past the
     (1, 1, 3000, 3, 1, 12, 40, 52, 0, "zerolead"),    # walked levels (digits from the index), tail past tmax
+    (1, 2, 3000, 12, 1, 8, 52, 52, 0, "halton"),      # bases 29..37 in the 16-register instance beside the 52
+    (1, 1, 2000, 12, 1, 10, 30, 52, 0, "halton"),     # the same with digits 16..29 past its registers, tail rows
 ])
 def test_fused_gdn_equals_chain(mode, r, n, d, r_b, mmax, tmax, tnew, n_start, kind):
     """qmc_gdn_gen_scramble_float_dev against the oracle's three-call chain,
