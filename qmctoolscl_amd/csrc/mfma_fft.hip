@@ -203,6 +203,48 @@ __device__ __forceinline__ f32x2 level_twiddle(const TwLds &tl, int T, int ta, i
     return fone ? w : cmul(w, tl.f[lev][k]);
 }
 
+// w_{2^(b+4)}^x (x < 2^(b+4)) from the tables, times f[lev][k] unless fone
+__device__ __forceinline__ f32x2 twiddle_of(const TwLds &tl, int b, int x, int lev, int k, bool fone) {
+    f32x2 w;
+    if (b <= 4) {
+        w = tl.t3[x << (4 - b)];
+    } else {
+        const int J = x << (9 - b);
+        w = cmul(tl.t1[J >> 7], tl.t2[J & 127]);
+    }
+    return fone ? w : cmul(w, tl.f[lev][k]);
+}
+
+// level_twiddle for the four outputs k = 4 g + i (i < 4) of the column whose
+// tile position with k = 0 is `base` (no bits in [ta, ta+4)), with the index
+// products strength-reduced (checked against level_twiddle on the CPU for
+// every tile size and level): forward, the level's own twiddles (one column
+// c, so brev_b(c) k = rb 4g + rb i); inverse, level lev - 1's, on tile bits
+// [ta-4, ta) (one k' per column; the outputs' bits [ta, ta+4) are the low
+// bits of c', so brev(c') = R0 + brev4(4g+i) 2^(b'-4) with brev4(4g+i) =
+// 4 brev2(i) + brev2(g))
+template <bool INV>
+__device__ __forceinline__ void out_twiddles(const TwLds &tl, int T, int ta, int lev, bool fone, int base, int g,
+                                             f32x2 w[4]) {
+    if (!INV) {
+        const int b = T - ta - 4;
+        const int rb = b > 0 ? brev_bits(base >> (ta + 4), b) : 0;
+        const int M = (16 << b) - 1, x0 = rb * 4 * g;
+#pragma unroll
+        for (int i = 0; i < 4; i++) w[i] = twiddle_of(tl, b, (x0 + i * rb) & M, lev, 4 * g + i, fone);
+    } else {
+        const int tap = ta - 4, b = T - ta;  // level lev - 1: tile bits [ta - 4, ta), b >= 4 bits above
+        const int kp = (base >> tap) & 15;
+        const int x0 = brev_bits(base >> ta, b) * kp + (((g >> 1 | (g & 1) << 1) * kp) << (b - 4));
+        const int U = kp << (b - 2), M = (16 << b) - 1;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int bi = (i >> 1) | ((i & 1) << 1);  // brev2(i)
+            w[i] = twiddle_of(tl, b, (x0 + bi * U) & M, lev - 1, kp, fone);
+        }
+    }
+}
+
 // radix-16 level on tile bits [ta, ta+4), level `lev` of the pass.  Forward,
 // with tw: output k of the column with in-tile bits c above the level (b of
 // them) is multiplied by the level's twiddle (level_twiddle).  Inverse (the
@@ -234,13 +276,16 @@ __device__ void dft16_level(float *re, float *im, int T, int ta, bool tw, bool t
         // (relative L2 and max error unchanged at 2^10, 2^12 and 2^13 for
         // normal, smooth and constant inputs)
         const u32 sg = WHOLE ? 0u : dither_sign((u32)col, salt);
+        // input twiddles (inverse, tw_in): rows k = c0 + i of one column, so
+        // brev_b(c) k = rb c0 + rb i
+        const int bin = T - ta - 4;
+        const int rbin = INV && tw_in && bin > 0 ? brev_bits(base >> (ta + 4), bin) : 0;
         float v[8];
 #pragma unroll
         for (int i = 0; i < 8; i++) {
             if (INV && tw_in) {
-                const int t = base + ((c0 + i) << ta);
                 float vr = re[pin + i * st], vi = im[pin + i * st];
-                const f32x2 w = level_twiddle(tl, T, ta, lev, fone, t);
+                const f32x2 w = twiddle_of(tl, bin, (rbin * (c0 + i)) & ((16 << bin) - 1), lev, c0 + i, fone);
                 const float tr = vr * w.x + vi * w.y;
                 vi = vi * w.x - vr * w.y;
                 vr = tr;
@@ -281,13 +326,7 @@ __device__ void dft16_level(float *re, float *im, int T, int ta, bool tw, bool t
         const int pout = pad(base + ((4 * g) << ta));
         const bool twout = INV ? tw_next : tw;
         f32x2 w[4];
-        if (twout) {
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int t = base + ((4 * g + i) << ta);
-                w[i] = INV ? level_twiddle(tl, T, ta - 4, lev - 1, fone_next, t) : level_twiddle(tl, T, ta, lev, fone, t);
-            }
-        }
+        if (twout) out_twiddles<INV>(tl, T, ta, lev, INV ? fone_next : fone, base, g, w);
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             float vr = flip(yr[i], sg), vi = flip(yi[i], sg);
