@@ -176,10 +176,12 @@ struct TwLds {
 
 static inline size_t shm_bytes(int T) { return padded_bytes(T) + sizeof(TwLds); }
 
+// complex product, one rounding fewer per part than mul-mul-add (the file is
+// built without contraction, so the fused forms are written out)
 __device__ __forceinline__ f32x2 cmul(f32x2 a, f32x2 b) {
     f32x2 r;
-    r.x = a.x * b.x - a.y * b.y;
-    r.y = a.x * b.y + a.y * b.x;
+    r.x = __builtin_fmaf(a.x, b.x, -(a.y * b.y));
+    r.y = __builtin_fmaf(a.x, b.y, a.y * b.x);
     return r;
 }
 
@@ -254,8 +256,8 @@ __device__ __forceinline__ void out_twiddles(const TwLds &tl, int T, int ta, int
 // run), which then reads its inputs as they are -- so each element is
 // twiddled once per level either way, by the lane that holds both its parts.
 template <bool INV, bool WHOLE>
-__device__ void dft16_level(float *re, float *im, int T, int ta, bool tw, bool tw_in, bool tw_next, bool fone_,
-                            bool fone_next_, int wave, int lane, const Wtile &W, const TwLds &tl, int lev, u32 salt) {
+__device__ __forceinline__ void dft16_level(float *re, float *im, int T, int ta, bool tw, bool tw_in, bool tw_next, bool fone_,
+                            bool fone_next_, int wave, int nwave, int lane, const Wtile &W, const TwLds &tl, int lev, u32 salt) {
     // a tile that is a whole vector fixes no index bits: every factor f is 1
     const bool fone = WHOLE || fone_, fone_next = WHOLE || fone_next_;
     const int nblk = 1 << (T - 8);
@@ -265,7 +267,7 @@ __device__ void dft16_level(float *re, float *im, int T, int ta, bool tw, bool t
     // LDS word of row c of a column: pad(base + (c << ta)) = pad(base) + c * st for ta = 0 or
     // ta >= 4 (no carry out of the low four bits), which every radix-16 level has
     const int st = (1 << ta) + ((1 << ta) >> 4);
-    for (int blk = wave; blk < nblk; blk += (int)(blockDim.x >> 6)) {
+    for (int blk = wave; blk < nblk; blk += nwave) {
         const int col = blk * 16 + n;
         const int lo = col & ((1 << ta) - 1);
         const int base = ((col >> ta) << (ta + 4)) | lo;
@@ -286,8 +288,8 @@ __device__ void dft16_level(float *re, float *im, int T, int ta, bool tw, bool t
             if (INV && tw_in) {
                 float vr = re[pin + i * st], vi = im[pin + i * st];
                 const f32x2 w = twiddle_of(tl, bin, (rbin * (c0 + i)) & ((16 << bin) - 1), lev, c0 + i, fone);
-                const float tr = vr * w.x + vi * w.y;
-                vi = vi * w.x - vr * w.y;
+                const float tr = __builtin_fmaf(vr, w.x, vi * w.y);
+                vi = __builtin_fmaf(vi, w.x, -(vr * w.y));
                 vr = tr;
                 v[i] = flip(g < 2 ? vr : vi, sg);
             } else {
@@ -333,8 +335,8 @@ __device__ void dft16_level(float *re, float *im, int T, int ta, bool tw, bool t
             if (twout) {
                 // forward: times w; inverse: times conj(w)
                 const float wy = INV ? -w[i].y : w[i].y;
-                const float tr = vr * w[i].x - vi * wy;
-                vi = vr * wy + vi * w[i].x;
+                const float tr = __builtin_fmaf(vr, w[i].x, -(vi * wy));
+                vi = __builtin_fmaf(vr, wy, vi * w[i].x);
                 vr = tr;
             }
             re[pout + i * st] = vr;
@@ -376,13 +378,16 @@ __device__ void dft_small_level(float *re, float *im, int T, int ta, int rb) {
 // float64.  Workgroups are persistent: each walks tiles blockIdx.x,
 // +gridDim.x, ... with all NP value pairs (of re and of im) of a thread
 // (2^T = 2 NP blockDim.x) loaded at once.
-template <int IN64, int OUT64, int NP, bool INV, bool WHOLE = false>
+// TT > 0 (whole vectors of 2^TT only): the tile size at compile time, so the
+// levels unroll with their strides, twiddle cases and loop counts fixed.
+template <int IN64, int OUT64, int NP, bool INV, bool WHOLE = false, int TT = 0>
 __global__ __launch_bounds__(kThreads) void fft_mfma_pass(PassGeom G, int glo, int ghi, const void *__restrict__ sr,
                                                         const void *__restrict__ si, void *__restrict__ dr,
                                                         void *__restrict__ di, double scale, Twiddles tw, u64 ntiles) {
     typedef typename std::conditional<IN64 != 0, f64x2, f32x2>::type In2;
     extern __shared__ float lds[];
-    const int T = G.lr + G.lc;
+    static_assert(TT == 0 || WHOLE, "a compile-time tile size is for whole vectors");
+    const int T = TT ? TT : G.lr + G.lc;
     const int P = (1 << T) + (1 << T) / 16;
     float *re = lds;
     float *im = lds + P;
@@ -417,7 +422,7 @@ __global__ __launch_bounds__(kThreads) void fft_mfma_pass(PassGeom G, int glo, i
         const u32 q = 2u * (u32)p * blockDim.x;
         return ((u64)(q >> G.lc) << G.lo) + (q & ((1u << G.lc) - 1));
     };
-    const int nlev = (ghi - glo + 3) / 4;
+    const int nlev = TT ? (TT + 3) / 4 : (ghi - glo + 3) / 4;
     for (u64 tix = blockIdx.x; tix < ntiles; tix += gridDim.x) {
         const u64 cb = tix & cmask_tile;
         const int tid = opaque(threadIdx.x);
@@ -462,25 +467,35 @@ __global__ __launch_bounds__(kThreads) void fft_mfma_pass(PassGeom G, int glo, i
         // conjugate twiddles applied to its inputs -- by the level run before it when that is a
         // radix-16 level of this pass, else on the way in.  The levels of a pass sit on
         // consecutive tile bits (ta(lev - 1) = ta(lev) - 4).
+        // a whole vector of 2^TT: global bits [0, TT) are tile bits [0, TT)
+        const int gm = TT ? TT : G.m, gl = TT ? 0 : glo, gh = TT ? TT : ghi;
+        const int glc = TT ? TT : G.lc, glo_ = TT ? TT : G.lo;
+        const int nwave = TT ? (1 << TT) / (128 * NP) : (int)(blockDim.x >> 6);
         bool pre = false;  // this step's input twiddles were applied by the previous step
-        for (int step = 0; step < nlev; step++) {
+        auto level_step = [&](int step) {
             const int lev = INV ? nlev - 1 - step : step;
-            const int ga = glo + 4 * lev;
+            const int ga = gl + 4 * lev;
             const int ltid = opaque(threadIdx.x);
-            const int ta = ga < G.lc ? ga : G.lc + (ga - G.lo);  // tile bit of global bit ga
-            const bool tw = G.m - ga - 4 > 0;                    // index bits above the level
-            const bool fone = (G.m - ga - 4) - (T - ta - 4) <= 0;  // the tile fixes none of them
-            if (ghi - ga >= 4) {
+            const int ta = ga < glc ? ga : glc + (ga - glo_);  // tile bit of global bit ga
+            const bool tw = gm - ga - 4 > 0;                   // index bits above the level
+            const bool fone = (gm - ga - 4) - (T - ta - 4) <= 0;  // the tile fixes none of them
+            if (gh - ga >= 4) {
                 const bool next = INV && lev > 0;  // level lev - 1 runs next (it always has bits above)
-                const bool fone_next = next && (G.m - ga) - (T - ta) <= 0;
-                dft16_level<INV, WHOLE>(re, im, T, ta, tw, INV && tw && !pre, next, fone, fone_next, ltid >> 6, ltid & 63,
-                                 W, tl, lev, (u32)tix * 0x632BE5ABu + (u32)ga * 0x27D4EB2Fu);
+                const bool fone_next = next && (gm - ga) - (T - ta) <= 0;
+                dft16_level<INV, WHOLE>(re, im, T, ta, tw, INV && tw && !pre, next, fone, fone_next, ltid >> 6, nwave,
+                                        ltid & 63, W, tl, lev, (u32)tix * 0x632BE5ABu + (u32)ga * 0x27D4EB2Fu);
                 pre = next;
             } else {
-                dft_small_level<INV>(re, im, T, ta, ghi - ga);  // only the top level can be short
+                dft_small_level<INV>(re, im, T, ta, gh - ga);  // only the top level can be short
                 pre = false;
             }
             __syncthreads();
+        };
+        if constexpr (TT != 0) {
+#pragma unroll
+            for (int step = 0; step < (TT + 3) / 4; step++) level_step(step);
+        } else {
+            for (int step = 0; step < nlev; step++) level_step(step);
         }
         const int stid = opaque(threadIdx.x);
         const u64 a0 = g0(tix, stid);
@@ -539,6 +554,20 @@ const void *pass_fn(bool in, bool out) {
                      : (const void *)fft_mfma_pass<0, 0, 8, INV>;
 }
 
+// the whole-vector kernel for length 2^m (m <= kT): 8 value pairs per thread
+// and the length at compile time from 2^10 up, fewer pairs below
+template <bool INV>
+const void *whole_fn(int m) {
+    switch (m) {
+    case 8: return (const void *)fft_mfma_pass<1, 1, 2, INV, true>;
+    case 9: return (const void *)fft_mfma_pass<1, 1, 4, INV, true>;
+    case 10: return (const void *)fft_mfma_pass<1, 1, 8, INV, true, 10>;
+    case 11: return (const void *)fft_mfma_pass<1, 1, 8, INV, true, 11>;
+    case 12: return (const void *)fft_mfma_pass<1, 1, 8, INV, true, 12>;
+    default: return (const void *)fft_mfma_pass<1, 1, 8, INV, true, 13>;
+    }
+}
+
 // The forward transform (INV = false, fft_bro_1d_radix2's) runs its passes
 // from the lowest index bits up; the inverse (ifft_bro_1d_radix2's) runs the
 // conjugate transpose of the same network: the same passes in reverse order,
@@ -562,8 +591,7 @@ int run_mfma_fft(const char *who, u64 d1, u64 d2, u64 n_half, double *xr, double
         for (int k = 0; k < 4; k++)
             QMC_HIP_CHECK(hipFuncSetAttribute(pass_fn<INV>(k & 1, k >> 1), hipFuncAttributeMaxDynamicSharedMemorySize,
                                               bytes));
-        QMC_HIP_CHECK(hipFuncSetAttribute((const void *)fft_mfma_pass<1, 1, 8, INV, true>,
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+        QMC_HIP_CHECK(hipFuncSetAttribute(whole_fn<INV>(kT), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
         return QMC_OK;
     });
     if (arc) return arc;
@@ -590,20 +618,16 @@ int run_mfma_fft(const char *who, u64 d1, u64 d2, u64 n_half, double *xr, double
         const int np = m >= 10 ? 8 : 1 << (m - 7);
         const int threads = (1 << m) / (2 * np);
         const size_t shm = shm_bytes(m);
-        const void *fn = np == 8 ? (const void *)fft_mfma_pass<1, 1, 8, INV, true>
-                         : np == 4 ? (const void *)fft_mfma_pass<1, 1, 4, INV, true>
-                                   : (const void *)fft_mfma_pass<1, 1, 2, INV, true>;
+        const void *fn = whole_fn<INV>(m);
         const unsigned grid = grid_of(fn, threads, shm, V);
-        if (np == 8)
-            hipLaunchKernelGGL((fft_mfma_pass<1, 1, 8, INV, true>), dim3(grid), dim3(threads), shm, s, G, 0, m,
-                               (const void *)xr, (const void *)xi, (void *)xr, (void *)xi, scale, tw, V);
-        else if (np == 4)
-            hipLaunchKernelGGL((fft_mfma_pass<1, 1, 4, INV, true>), dim3(grid), dim3(threads), shm, s, G, 0, m,
-                               (const void *)xr, (const void *)xi, (void *)xr, (void *)xi, scale, tw, V);
-        else
-            hipLaunchKernelGGL((fft_mfma_pass<1, 1, 2, INV, true>), dim3(grid), dim3(threads), shm, s, G, 0, m,
-                               (const void *)xr, (const void *)xi, (void *)xr, (void *)xi, scale, tw, V);
-        if (hipGetLastError() != hipSuccess) rc = qmc_fail(QMC_ERR_HIP, "%s: launch failed", who);
+        int glo = 0, ghi = m;
+        const void *sr = xr, *si = xi;
+        void *dr = xr, *di = xi;
+        double sc = scale;
+        u64 nt = V;
+        void *args[] = {&G, &glo, &ghi, &sr, &si, &dr, &di, &sc, &tw, &nt};
+        if (hipLaunchKernel(fn, dim3(grid), dim3(threads), args, shm, s) != hipSuccess)
+            rc = qmc_fail(QMC_ERR_HIP, "%s: launch failed", who);
     } else {
         buf = work;
         if (!buf) QMC_HIP_CHECK(hipMallocAsync((void **)&buf, 2 * sizeof(float) * V * n, s));
