@@ -85,9 +85,9 @@ __device__ void h2_axes(float *tile, int T, int a, int k) {
 // index = v*2^m + row*row_stride + cb*2^lc + col), applying the Hadamard
 // product along tile bits [tb0, tb0 + nb).  IN/OUT: 0 fp32, 1 float64 (the
 // output is then scaled by `scale`).  A workgroup walks tiles blockIdx.x,
-// +gridDim.x, ... (the launches below use one workgroup per tile, which
-// measured faster than fewer resident ones), with all NP value pairs of a
-// thread (2^T = 2 NP blockDim.x) loaded at once.
+// +gridDim.x, ..., with all NP value pairs of a thread (2^T = 2 NP
+// blockDim.x) loaded at once, the next tile's loads issued as soon as the
+// current tile is staged in LDS.
 template <int IN64, int OUT64, int NP>
 __global__ __launch_bounds__(kThreads) void fwht_mfma_pass(int m, int lr, int lc, u64 row_stride, int tb0, int nb,
                                                          const void *__restrict__ src, void *__restrict__ dst,
@@ -108,21 +108,24 @@ __global__ __launch_bounds__(kThreads) void fwht_mfma_pass(int m, int lr, int lc
         const u32 q = 2u * (u32)p * blockDim.x;
         return (u64)(q >> lc) * row_stride + (q & (u32)cmask);
     };
+    In2 r[NP];
+    auto load = [&](u64 ti) {
+        const In2 *s0 = (const In2 *)src + gidx(ti, opaque(threadIdx.x)) / 2;
+#pragma unroll
+        for (int p = 0; p < NP; p++) {
+            // the caller's float64 input streams through; the fp32 intermediate is read with ordinary
+            // loads, so the L2 keeps whole lines for the neighbouring tile when a tile row is narrower
+            // than a line (2^24: 15% faster)
+            if (IN64)
+                r[p] = __builtin_nontemporal_load(s0 + poff(p) / 2);
+            else
+                r[p] = s0[poff(p) / 2];
+        }
+    };
+    if (blockIdx.x < ntiles) load(blockIdx.x);
     for (u64 tix = blockIdx.x; tix < ntiles; tix += gridDim.x) {
         {
             const int tid = opaque(threadIdx.x);
-            const In2 *s0 = (const In2 *)src + gidx(tix, tid) / 2;
-            In2 r[NP];
-#pragma unroll
-            for (int p = 0; p < NP; p++) {
-                // the caller's float64 input streams through; the fp32 intermediate is read with ordinary
-                // loads, so the L2 keeps whole lines for the neighbouring tile when a tile row is narrower
-                // than a line (2^24: 15% faster)
-                if (IN64)
-                    r[p] = __builtin_nontemporal_load(s0 + poff(p) / 2);
-                else
-                    r[p] = s0[poff(p) / 2];
-            }
 #pragma unroll
             for (int p = 0; p < NP; p++) {
                 const int t = 2 * (tid + p * blockDim.x);
@@ -131,6 +134,8 @@ __global__ __launch_bounds__(kThreads) void fwht_mfma_pass(int m, int lr, int lc
             }
         }
         __syncthreads();
+        // the next tile's loads are in flight while this one is transformed
+        if (tix + gridDim.x < ntiles) load(tix + gridDim.x);
         int a = tb0;
         for (; a + 4 <= tb0 + nb; a += 4) {
             const int ltid = opaque(threadIdx.x);
@@ -208,8 +213,14 @@ QMC_EXPORT int qmc_fwht_mfma_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, double 
         const int np = m >= 10 ? 8 : 1 << (m - 7);
         const unsigned threads = (1u << m) / (2u * np);
         const size_t shm = padded_bytes(m);
-        // one workgroup per vector (10% faster at 2^12 than resident workgroups looping over vectors)
-        const unsigned grid = (unsigned)std::min<u64>(V, 1u << 30);
+        // resident workgroups looping over vectors, each loading its next vector while it transforms
+        // the current one (2^12: 1-2% faster than one workgroup per vector; half as many resident
+        // workgroups: 10% slower)
+        const void *fn = np == 8   ? (const void *)fwht_mfma_pass<1, 1, 8>
+                         : np == 4 ? (const void *)fwht_mfma_pass<1, 1, 4>
+                                   : (const void *)fwht_mfma_pass<1, 1, 2>;
+        const unsigned grid =
+            (unsigned)std::min<u64>(V, (u64)resident_per_cu(fn, (int)threads, shm) * (u64)device_cus());
         switch (np) {
             case 8:
                 hipLaunchKernelGGL((fwht_mfma_pass<1, 1, 8>), dim3(grid), dim3(threads), shm, s, m, 0, m,
