@@ -699,6 +699,25 @@ def section_transforms(torch, q, dev, timed0):
         # the best of three rounds for every path alike (the FP64 and the matrix-core
         # transforms run back to back here, and single rounds vary by up to 15%)
         return min(timed0(fn, steps, warmup if k == 0 else 1) for k in range(3))
+    # cold-cache timing: a 1 GiB buffer (4x the 256 MB Infinity Cache) is rewritten before
+    # every call, so the call reads its vectors from HBM.  Back to back, a working set that
+    # fits the cache (2^12 x 4096: 134 MB for the FWHT, 268 MB for the FFT's two parts) is
+    # partly served from it, which helps the paths with ordinary loads (the FP64 ones)
+    flush = torch.empty(1 << 28, dtype=torch.float32, device=dev)
+
+    def cold(fn, steps=20):
+        ts = []
+        for _ in range(steps):
+            flush.add_(1.0)
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record()
+            fn()
+            b.record()
+            torch.cuda.synchronize()
+            ts.append(a.elapsed_time(b))
+        return float(np.median(ts))
+
     out = []
     U = np.uint64
     for m, V in ((12, 4096), (20, 64), (24, 4)):
@@ -741,7 +760,14 @@ def section_transforms(torch, q, dev, timed0):
         torch.cuda.synchronize()
         il2, imx = relative_errors(np.stack([zr.cpu().numpy(), zi.cpu().numpy()]),
                                    np.stack([yr.cpu().numpy(), yi.cpu().numpy()]))
-        del yr, yi, zr, zi, fwork
+        del yr, yi, zr, zi
+        c64 = cold(lambda: q.fwht_1d_radix2(U(1), U(V), U(n // 2), x, wait=False))
+        cm = cold(lambda: fwht_mfma(1, V, n // 2, x, work))
+        cf = cold(lambda: q.fft_bro_1d_radix2(U(1), U(V), U(n // 2), twr, twi, xr, xi, wait=False))
+        cfm = cold(lambda: fft_bro_mfma(1, V, n // 2, xr, xi, fwork))
+        ci = cold(lambda: q.ifft_bro_1d_radix2(U(1), U(V), U(n // 2), twr, twi, xr, xi, wait=False))
+        cim = cold(lambda: ifft_bro_mfma(1, V, n // 2, xr, xi, fwork))
+        del fwork
         pts = V * n
         # matrix-core work of the FWHT: per radix-16 axis, four 16x16x4 fp32
         # products per 16x16 block = 32 flops per value, on the fp32 MFMA peak
@@ -761,12 +787,19 @@ def section_transforms(torch, q, dev, timed0):
                     "ifft_fp64_ms": round(msi, 4), "ifft_mfma_ms": round(msim, 4),
                     "ifft_mfma_speedup": round(msi / msim, 3),
                     "ifft_mfma_rel_l2": float("%.3g" % il2), "ifft_mfma_rel_max": float("%.3g" % imx),
+                    "cold_cache_ms": {"fwht_fp64": round(c64, 4), "fwht_mfma": round(cm, 4),
+                                      "fft_fp64": round(cf, 4), "fft_mfma": round(cfm, 4),
+                                      "ifft_fp64": round(ci, 4), "ifft_mfma": round(cim, 4)},
+                    "cold_cache_speedup": {"fwht": round(c64 / cm, 3), "fft": round(cf / cfm, 3),
+                                           "ifft": round(ci / cim, 3)},
                     # 12 products of 16x16x32 bf16 per 256 complex values per radix-16 level
                     "fft_mfma_util": float("%.3g" % (pts * (m // 4) * 768 / (msfm * 1e-3) /
                                                      (BF16_PEAK_TFLOPS * 1e12))),
                     "fft_mfma_util_peak": "bf16 MFMA, %.0f TFLOP/s" % BF16_PEAK_TFLOPS})
         del x0, x, ref, xm, work, xr, xi
         torch.cuda.empty_cache()
+    del flush
+    torch.cuda.empty_cache()
     return out
 
 
