@@ -129,23 +129,42 @@ __global__ __launch_bounds__(kBlock) void dnb2_lms_gen_kernel(u64 n, u64 d, u64 
     u64 *srows = pre + d * mmax;     // [d][tmax_new] rows of S
     u64 *stage = srows + d * tmax_new;  // [R * d] shifted integers of the block's rows
     const u64 l = blockIdx.y;
+    // at most one column per thread (d mmax <= blockDim): its load goes out with those of S,
+    // and when mmax divides 64 a dimension's columns sit in consecutive lanes of one wave,
+    // so their prefix XORs are a lane scan instead of a walk through LDS
+    const bool one = d * mmax <= (u64)blockDim.x;
+    const bool scan = one && (64 % mmax) == 0;
+    u64 col0 = 0;
+    if (one && threadIdx.x < d * mmax) col0 = C[(l * d * mmax + threadIdx.x) % nC];
     for (u64 e = threadIdx.x; e < d * tmax_new; e += blockDim.x) srows[e] = S[l * d * tmax_new + e];
     __syncthreads();
+    u64 mine = 0;
     for (u64 e = threadIdx.x; e < d * mmax; e += blockDim.x) {
         const u64 j = e / mmax;
-        const u64 col = C[(l * d * mmax + e) % nC];
+        const u64 col = one ? col0 : C[(l * d * mmax + e) % nC];
         const u64 *srow = srows + j * tmax_new;
         u64 out = 0;
         for (u64 t = 0; t < tmax_new; t++)
             if (__popcll(srow[t] & col) & 1) out += 1ull << ((tmax_new - t - 1) & 63);
         cl[e] = out;
+        mine = out;
     }
-    __syncthreads();
-    for (u64 e = threadIdx.x; e < d * mmax; e += blockDim.x) {
-        const u64 j = e / mmax;
-        u64 acc = 0;
-        for (u64 f = j * mmax; f <= e; f++) acc ^= cl[f];
-        pre[e] = acc;
+    if (scan) {
+        const u32 c = threadIdx.x % (u32)mmax;
+        u64 acc = mine;
+        for (u32 off = 1; off < (u32)mmax; off <<= 1) {
+            const u64 o = __shfl_up(acc, off, 64);
+            if (c >= off) acc ^= o;
+        }
+        if (threadIdx.x < d * mmax) pre[threadIdx.x] = acc;
+    } else {
+        __syncthreads();
+        for (u64 e = threadIdx.x; e < d * mmax; e += blockDim.x) {
+            const u64 j = e / mmax;
+            u64 acc = 0;
+            for (u64 f = j * mmax; f <= e; f++) acc ^= cl[f];
+            pre[e] = acc;
+        }
     }
     __syncthreads();
     const u64 i0 = (u64)blockIdx.x * R;

@@ -187,15 +187,19 @@ def test_fused_dnb2_equals_chain():
         assert np.array_equal(o_x.cpu().numpy().view(np.uint64), x.view(np.uint64))
 
 
-@pytest.mark.parametrize("natural,n,n_start,r_C,tmax_new,lsh,d", [
-    (1, 1 << 10, 0, 3, 32, 32, 5),     # config 0's chain, one matrix set per replication
-    (0, 1 << 10, 0, 1, 32, 32, 5),     # Gray order, the matrices shared by every replication
-    (1, 1 << 9, 1 << 9, 3, 64, 0, 3),  # a later closed block, 64-bit LMS, no left shift
-    (0, 1000, 37, 2, 48, 16, 9),       # ragged Gray range, r_C between 1 and r
-    (1, 1 << 10, 1 << 10, 3, 32, 32, 2),
-    (1, 1 << 8, 0, 1, 64, 0, 32),      # d * (2 mmax + tmax_new) near the LDS limit (60-bit columns)
+@pytest.mark.parametrize("natural,n,n_start,r_C,tmax_new,lsh,d,mm", [
+    (1, 1 << 10, 0, 3, 32, 32, 5, None),     # config 0's chain, one matrix set per replication
+    (0, 1 << 10, 0, 1, 32, 32, 5, None),     # Gray order, the matrices shared by every replication
+    (1, 1 << 9, 1 << 9, 3, 64, 0, 3, None),  # a later closed block, 64-bit LMS, no left shift
+    (0, 1000, 37, 2, 48, 16, 9, None),       # ragged Gray range, r_C between 1 and r
+    (1, 1 << 10, 1 << 10, 3, 32, 32, 2, None),
+    (1, 1 << 8, 0, 1, 64, 0, 32, None),      # d * (2 mmax + tmax_new) near the LDS limit (60-bit columns)
+    # prefix XORs as a lane scan (mmax dividing 64, d mmax <= 256): 16 and 64 columns
+    (1, 1 << 12, 0, 3, 16, 48, 16, 16),
+    (0, 1 << 11, 5, 2, 64, 0, 4, 64),
+    (1, 1 << 10, 0, 1, 64, 0, 4, 64),
 ])
-def test_fused_dnb2_lms_equals_oracle_chain(natural, n, n_start, r_C, tmax_new, lsh, d):
+def test_fused_dnb2_lms_equals_oracle_chain(natural, n, n_start, r_C, tmax_new, lsh, d, mm):
     """qmc_dnb2_lms_gen_shift_float_dev against the oracle's four-call chain
     (LMS, generation, digital shift with r_x = r, conversion), bit for bit.
     (Natural order over a range that Gray codes do not map onto itself makes
@@ -203,7 +207,7 @@ def test_fused_dnb2_lms_equals_oracle_chain(natural, n, n_start, r_C, tmax_new, 
     import torch
     from qmctoolscl_amd import fused
     rng = np.random.Generator(np.random.SFC64(n + d))
-    r, mmax = 3, 32 if tmax_new < 64 or lsh else 60
+    r, mmax = 3, mm if mm else 32 if tmax_new < 64 or lsh else 60
     C = rng.integers(0, 2**mmax if mmax < 64 else 2**63, size=(r_C, d, mmax), dtype=np.uint64)
     S = rf.dnb2_get_linear_scramble_matrix(rng, r, d, mmax, tmax_new, False)
     sh = rf.dnb2_get_digital_shifts(rng, r, d, 64)
