@@ -129,35 +129,56 @@ __global__ __launch_bounds__(kBlock) void dnb2_lms_gen_kernel(u64 n, u64 d, u64 
     u64 *srows = pre + d * mmax;     // [d][tmax_new] rows of S
     u64 *stage = srows + d * tmax_new;  // [R * d] shifted integers of the block's rows
     const u64 l = blockIdx.y;
-    // at most one column per thread (d mmax <= blockDim): its load goes out with those of S,
-    // and when mmax divides 64 a dimension's columns sit in consecutive lanes of one wave,
-    // so their prefix XORs are a lane scan instead of a walk through LDS
-    const bool one = d * mmax <= (u64)blockDim.x;
-    const bool scan = one && (64 % mmax) == 0;
+    // Only columns c < nbits are read below: every index of the call is below 2^nbits, so
+    // are its Gray codes and the trailing-zero counts of its increments.  When d cb threads
+    // suffice (cb = nbits rounded up to a power of two), thread (j, c) = (tid / cb, tid % cb)
+    // builds one column: its load goes out with those of S, and the prefix XORs of a
+    // dimension's columns are a scan over cb consecutive lanes of one wave.
+    const u64 top = n_start + n - 1;
+    const int nbits = top ? 64 - __clzll((long long)top) : 1;
+    int cb = 1;
+    while (cb < nbits) cb <<= 1;
+    const bool fast = (u64)d * (u64)cb <= (u64)blockDim.x;
+    const u32 fj = threadIdx.x / (u32)cb, fc = threadIdx.x % (u32)cb;
+    const bool act = fast && fj < d && fc < (u32)nbits;
     u64 col0 = 0;
-    if (one && threadIdx.x < d * mmax) col0 = C[(l * d * mmax + threadIdx.x) % nC];
+    if (act) col0 = C[(l * d * mmax + fj * mmax + fc) % nC];
     for (u64 e = threadIdx.x; e < d * tmax_new; e += blockDim.x) srows[e] = S[l * d * tmax_new + e];
     __syncthreads();
-    u64 mine = 0;
-    for (u64 e = threadIdx.x; e < d * mmax; e += blockDim.x) {
-        const u64 j = e / mmax;
-        const u64 col = one ? col0 : C[(l * d * mmax + e) % nC];
-        const u64 *srow = srows + j * tmax_new;
+    if (fast) {
         u64 out = 0;
-        for (u64 t = 0; t < tmax_new; t++)
-            if (__popcll(srow[t] & col) & 1) out += 1ull << ((tmax_new - t - 1) & 63);
-        cl[e] = out;
-        mine = out;
-    }
-    if (scan) {
-        const u32 c = threadIdx.x % (u32)mmax;
-        u64 acc = mine;
-        for (u32 off = 1; off < (u32)mmax; off <<= 1) {
-            const u64 o = __shfl_up(acc, off, 64);
-            if (c >= off) acc ^= o;
+        if (act) {
+            const u64 *srow = srows + fj * tmax_new;
+            if (tmax_new <= 32 && (col0 >> 32) == 0) {
+                // 32-bit columns: the parity of row t is bit tmax_new - 1 - t, shifted in
+                const u32 c32 = (u32)col0;
+                u32 o = 0;
+                for (u64 t = 0; t < tmax_new; t++) o = (o << 1) | (__popc((u32)srow[t] & c32) & 1u);
+                out = o;
+            } else {
+                for (u64 t = 0; t < tmax_new; t++)
+                    if (__popcll(srow[t] & col0) & 1) out += 1ull << ((tmax_new - t - 1) & 63);
+            }
         }
-        if (threadIdx.x < d * mmax) pre[threadIdx.x] = acc;
+        u64 acc = out;
+        for (u32 off = 1; off < (u32)cb; off <<= 1) {
+            const u64 o = __shfl_up(acc, off, 64);
+            if (fc >= off) acc ^= o;
+        }
+        if (act) {
+            cl[fj * mmax + fc] = out;
+            pre[fj * mmax + fc] = acc;
+        }
     } else {
+        for (u64 e = threadIdx.x; e < d * mmax; e += blockDim.x) {
+            const u64 j = e / mmax;
+            const u64 col = C[(l * d * mmax + e) % nC];
+            const u64 *srow = srows + j * tmax_new;
+            u64 out = 0;
+            for (u64 t = 0; t < tmax_new; t++)
+                if (__popcll(srow[t] & col) & 1) out += 1ull << ((tmax_new - t - 1) & 63);
+            cl[e] = out;
+        }
         __syncthreads();
         for (u64 e = threadIdx.x; e < d * mmax; e += blockDim.x) {
             const u64 j = e / mmax;

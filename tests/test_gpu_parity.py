@@ -198,6 +198,9 @@ def test_fused_dnb2_equals_chain():
     (1, 1 << 12, 0, 3, 16, 48, 16, 16),
     (0, 1 << 11, 5, 2, 64, 0, 4, 64),
     (1, 1 << 10, 0, 1, 64, 0, 4, 64),
+    # more dimensions x index bits than threads: the columns in strided loops
+    (1, 1 << 10, 0, 2, 32, 32, 32, None),
+    (0, 3000, 100, 3, 64, 0, 24, None),
 ])
 def test_fused_dnb2_lms_equals_oracle_chain(natural, n, n_start, r_C, tmax_new, lsh, d, mm):
     """qmc_dnb2_lms_gen_shift_float_dev against the oracle's four-call chain
