@@ -163,7 +163,8 @@ __global__ __launch_bounds__(kBlock) void gdn_gen_chain_kernel(u64 n, u64 d, u64
                 } else {
                     // k + 1 ends in c >= 1 zero digits: c = 1 + trailing zeros of (k + 1) / b
                     dlow = 0;
-                    u64 x = (k + 1) / b;
+                    const u64 k1 = k + 1;
+                    u64 x = (k1 >> 32) ? k1 / b : (u64)div_q((u32)k1, bdiv);
                     u32 c = 1;
                     while (x != 0 && c + 1 < (u32)mmax) {  // c < mmax whenever the column qualifies
                         u64 qq;
@@ -551,7 +552,10 @@ __device__ __forceinline__ u32 walk_carry(u32 &dlow, u64 k, u64 b, u32 bb, const
         dlow++;
     } else {
         dlow = 0;
-        u64 xq = (k + 1) / b;
+        // (k + 1) / b: 32-bit reciprocal division while the index fits 32 bits (a wave takes
+        // this branch whenever any of its lanes carries)
+        const u64 k1 = k + 1;
+        u64 xq = (k1 >> 32) ? k1 / b : (u64)div_q((u32)k1, bdiv);
         c = 1;
         while (xq != 0 && c + 1 < mmax) {
             const u64 qq = (xq >> 32) ? xq / b : (u64)div_q((u32)xq, bdiv);
