@@ -803,9 +803,10 @@ __global__ __launch_bounds__(TAB ? kFuseTabBlock : kBlock) void gdn_fused_kernel
             constexpr int TL = decltype(tl)::value;
             const double thl = sThr[(u32)TL < tnew ? (u32)TL : tnew];
             for (u32 h = 0;; h++) {
-                double v = 0.;
+                // the reference adds the first term to 0.0: the same value, as no term is -0.0
+                double v = *(__attribute__((address_space(3))) const double *)(uintptr_t)z[0];
 #pragma unroll
-                for (int t = 0; t < TL; t++)  // + 0 past tmax_new
+                for (int t = 1; t < TL; t++)  // + 0 past tmax_new
                     v += *(__attribute__((address_space(3))) const double *)(uintptr_t)(z[t] + 8u * t);
                 if ((u32)TL < tnew && __builtin_amdgcn_ballot_w64(v < thl) != 0) {
                     // the levels from TL on, for the lanes still below them:
