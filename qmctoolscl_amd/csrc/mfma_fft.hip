@@ -596,16 +596,19 @@ int run_mfma_fft(const char *who, u64 d1, u64 d2, u64 n_half, double *xr, double
     });
     if (arc) return arc;
     const u64 cus = (u64)device_cus();
-    // FP64 twiddle tables (one product per twiddle), 2^klo + 2^(m-klo) entries
-    Twiddles tw;
-    tw.klo = (m + 1) / 2;
-    const u64 ntw = (1ull << tw.klo) + (1ull << (m - tw.klo));
+    // FP64 twiddle tables (one product per twiddle), 2^klo + 2^(m-klo) entries, for the per-tile
+    // factors of the multi-pass lengths (a whole-vector tile has none)
+    Twiddles tw{};
     f64x2 *tt = nullptr;
-    QMC_HIP_CHECK(hipMallocAsync((void **)&tt, sizeof(f64x2) * ntw, s));
-    tw.lo = tt;
-    tw.hi = tt + (1ull << tw.klo);
-    hipLaunchKernelGGL(twiddle_tables_kernel, dim3((unsigned)((ntw + kThreads / 4 - 1) / (kThreads / 4))),
-                       dim3(kThreads / 4), 0, s, m, tw.klo, tt, tt + (1ull << tw.klo));
+    if (m > kT) {
+        tw.klo = (m + 1) / 2;
+        const u64 ntw = (1ull << tw.klo) + (1ull << (m - tw.klo));
+        QMC_HIP_CHECK(hipMallocAsync((void **)&tt, sizeof(f64x2) * ntw, s));
+        tw.lo = tt;
+        tw.hi = tt + (1ull << tw.klo);
+        hipLaunchKernelGGL(twiddle_tables_kernel, dim3((unsigned)((ntw + kThreads / 4 - 1) / (kThreads / 4))),
+                           dim3(kThreads / 4), 0, s, m, tw.klo, tt, tt + (1ull << tw.klo));
+    }
     // persistent grid: as many workgroups as stay resident, at most one per tile
     auto grid_of = [&](const void *fn, int threads, size_t shm, u64 ntiles) -> unsigned {
         return (unsigned)std::min<u64>(ntiles, (u64)resident_per_cu(fn, threads, shm) * cus);
@@ -659,7 +662,7 @@ int run_mfma_fft(const char *who, u64 d1, u64 d2, u64 n_half, double *xr, double
         }
     }
     if (buf && !work) (void)hipFreeAsync(buf, s);
-    (void)hipFreeAsync(tt, s);
+    if (tt) (void)hipFreeAsync(tt, s);
     return rc;
 }
 
