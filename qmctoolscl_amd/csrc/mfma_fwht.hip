@@ -45,13 +45,13 @@ static inline size_t padded_bytes(int T) { return sizeof(float) * (((size_t)1 <<
 // (lane % 16, lane / 16) holds A[n][g] and B[g][n] of its k-block and gets
 // D[4g + i][n], i < 4.  The sign dither (one +-1 per column, undone on the
 // way out) keeps the accumulation's rounding bias from adding up over axes.
-__device__ void h16_axis(float *tile, int T, int a, int wave, int lane, u32 salt) {
+__device__ __forceinline__ void h16_axis_body(float *tile, int T, int a, int wave, int nwave, int lane, u32 salt) {
     const int nblk = 1 << (T - 8);
     const int n = lane & 15, g = lane >> 4;
     float A[4];  // H16[n][4kb + g]
 #pragma unroll
     for (int kb = 0; kb < 4; kb++) A[kb] = (__popc(n & (4 * kb + g)) & 1) ? -1.0f : 1.0f;
-    for (int blk = wave; blk < nblk; blk += (int)(blockDim.x >> 6)) {
+    for (int blk = wave; blk < nblk; blk += nwave) {
         const int col = blk * 16 + n;
         const int lo = col & ((1 << a) - 1);
         const int base = ((col >> a) << (a + 4)) | lo;
@@ -65,6 +65,11 @@ __device__ void h16_axis(float *tile, int T, int a, int wave, int lane, u32 salt
 #pragma unroll
         for (int i = 0; i < 4; i++) tile[pad(base + ((g * 4 + i) << a))] = flip(acc[i], sg);
     }
+}
+
+// the same with the wave count of the block, for the passes whose geometry is known at run time only
+__device__ void h16_axis(float *tile, int T, int a, int wave, int lane, u32 salt) {
+    h16_axis_body(tile, T, a, wave, (int)(blockDim.x >> 6), lane, salt);
 }
 
 // unnormalized radix-2 butterflies along tile bits [a, a+k), k < 4
@@ -88,13 +93,23 @@ __device__ void h2_axes(float *tile, int T, int a, int k) {
 // +gridDim.x, ..., with all NP value pairs of a thread (2^T = 2 NP
 // blockDim.x) loaded at once, the next tile's loads issued as soon as the
 // current tile is staged in LDS.
-template <int IN64, int OUT64, int NP>
+// TT > 0: a whole vector of 2^TT values per tile (lr = 0, lc = tb0 + nb = TT), the
+// length at compile time so the axes unroll with their strides fixed.
+template <int IN64, int OUT64, int NP, int TT = 0>
 __global__ __launch_bounds__(kThreads) void fwht_mfma_pass(int m, int lr, int lc, u64 row_stride, int tb0, int nb,
                                                          const void *__restrict__ src, void *__restrict__ dst,
                                                          double scale, u64 ntiles) {
     typedef typename std::conditional<IN64 != 0, f64x2, f32x2>::type In2;
     extern __shared__ float tile[];
+    if (TT) {
+        m = TT;
+        lr = 0;
+        lc = TT;
+        tb0 = 0;
+        nb = TT;
+    }
     const int T = lr + lc;
+    const int nwave = TT ? (1 << TT) / (128 * NP) : 0;  // waves of a whole-vector block
     const int tlog = m - T;  // log2 of tiles per vector
     const u64 cmask_tile = (1ull << tlog) - 1;
     const int cmask = (1 << lc) - 1;
@@ -137,10 +152,20 @@ __global__ __launch_bounds__(kThreads) void fwht_mfma_pass(int m, int lr, int lc
         // the next tile's loads are in flight while this one is transformed
         if (tix + gridDim.x < ntiles) load(tix + gridDim.x);
         int a = tb0;
-        for (; a + 4 <= tb0 + nb; a += 4) {
-            const int ltid = opaque(threadIdx.x);
-            h16_axis(tile, T, a, ltid >> 6, ltid & 63, (u32)tix * 0x632BE5ABu + (u32)a * 0x27D4EB2Fu);
-            __syncthreads();
+        if constexpr (TT != 0) {
+#pragma unroll
+            for (int aa = 0; aa + 4 <= TT; aa += 4) {
+                const int ltid = opaque(threadIdx.x);
+                h16_axis_body(tile, T, aa, ltid >> 6, nwave, ltid & 63, (u32)tix * 0x632BE5ABu + (u32)aa * 0x27D4EB2Fu);
+                __syncthreads();
+            }
+            a = TT & ~3;
+        } else {
+            for (; a + 4 <= tb0 + nb; a += 4) {
+                const int ltid = opaque(threadIdx.x);
+                h16_axis(tile, T, a, ltid >> 6, ltid & 63, (u32)tix * 0x632BE5ABu + (u32)a * 0x27D4EB2Fu);
+                __syncthreads();
+            }
         }
         if (a < tb0 + nb) h2_axes(tile, T, a, tb0 + nb - a);
         const int stid = opaque(threadIdx.x);
@@ -180,6 +205,22 @@ __global__ void mfma_probe_kernel(const float *a, const float *b, float *d) {
 
 }  // namespace
 
+namespace {
+// the whole-vector kernel for length 2^m (m <= kTileLog2): 8 value pairs per thread and the
+// length at compile time from 2^10 up, fewer pairs below
+const void *whole_fn(int m) {
+    switch (m) {
+    case 8: return (const void *)fwht_mfma_pass<1, 1, 2>;
+    case 9: return (const void *)fwht_mfma_pass<1, 1, 4>;
+    case 10: return (const void *)fwht_mfma_pass<1, 1, 8, 10>;
+    case 11: return (const void *)fwht_mfma_pass<1, 1, 8, 11>;
+    case 12: return (const void *)fwht_mfma_pass<1, 1, 8, 12>;
+    case 13: return (const void *)fwht_mfma_pass<1, 1, 8, 13>;
+    default: return (const void *)fwht_mfma_pass<1, 1, 8, 14>;
+    }
+}
+}  // namespace
+
 extern "C" {
 
 // Orthonormal FWHT of d1*d2 vectors of length 2*n_half (in place, float64 in
@@ -202,7 +243,7 @@ QMC_EXPORT int qmc_fwht_mfma_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, double 
     static PerDeviceOnce attrs;  // tiles of 2^14 padded values need more than the default 64 KiB of LDS
     const int arc = attrs.run([]() -> int {
         const int bytes = (int)padded_bytes(kTileLog2);
-        QMC_HIP_CHECK(hipFuncSetAttribute((const void *)fwht_mfma_pass<1, 1, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+        QMC_HIP_CHECK(hipFuncSetAttribute(whole_fn(kTileLog2), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
         QMC_HIP_CHECK(hipFuncSetAttribute((const void *)fwht_mfma_pass<1, 0, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
         QMC_HIP_CHECK(hipFuncSetAttribute((const void *)fwht_mfma_pass<0, 1, 8>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
         return QMC_OK;
@@ -216,25 +257,16 @@ QMC_EXPORT int qmc_fwht_mfma_dev(qmc_u64 d1, qmc_u64 d2, qmc_u64 n_half, double 
         // resident workgroups looping over vectors, each loading its next vector while it transforms
         // the current one (2^12: 1-2% faster than one workgroup per vector; half as many resident
         // workgroups: 10% slower)
-        const void *fn = np == 8   ? (const void *)fwht_mfma_pass<1, 1, 8>
-                         : np == 4 ? (const void *)fwht_mfma_pass<1, 1, 4>
-                                   : (const void *)fwht_mfma_pass<1, 1, 2>;
+        const void *fn = whole_fn(m);
         const unsigned grid =
             (unsigned)std::min<u64>(V, (u64)resident_per_cu(fn, (int)threads, shm) * (u64)device_cus());
-        switch (np) {
-            case 8:
-                hipLaunchKernelGGL((fwht_mfma_pass<1, 1, 8>), dim3(grid), dim3(threads), shm, s, m, 0, m,
-                                   (u64)0, 0, m, (const void *)x, (void *)x, scale, V);
-                break;
-            case 4:
-                hipLaunchKernelGGL((fwht_mfma_pass<1, 1, 4>), dim3(grid), dim3(threads), shm, s, m, 0, m,
-                                   (u64)0, 0, m, (const void *)x, (void *)x, scale, V);
-                break;
-            default:
-                hipLaunchKernelGGL((fwht_mfma_pass<1, 1, 2>), dim3(grid), dim3(threads), shm, s, m, 0, m,
-                                   (u64)0, 0, m, (const void *)x, (void *)x, scale, V);
-                break;
-        }
+        int a_m = m, a_lr = 0, a_lc = m, a_tb0 = 0, a_nb = m;
+        u64 a_rs = 0, a_nt = V;
+        const void *a_src = x;
+        void *a_dst = x;
+        double a_scale = scale;
+        void *args[] = {&a_m, &a_lr, &a_lc, &a_rs, &a_tb0, &a_nb, &a_src, &a_dst, &a_scale, &a_nt};
+        QMC_HIP_CHECK(hipLaunchKernel(fn, dim3(grid), dim3(threads), args, shm, s));
         QMC_HIP_CHECK(hipGetLastError());
         return QMC_OK;
     }
