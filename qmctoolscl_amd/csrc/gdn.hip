@@ -689,18 +689,25 @@ __global__ __launch_bounds__(TAB ? kFuseTabBlock : kBlock) void gdn_fused_kernel
         // thread per level (terms are >= 0, so their bit patterns order as the
         // values; LDS atomics on one word per level serialized the fill);
         // levels past the table have one tail term each
-        for (u32 t = threadIdx.x; t < tnew; t += blockDim.x) {
+        // four consecutive lanes per level (tnew <= 64, so 4 tnew <= 256 lanes, one pass), each
+        // scanning every fourth row, their maxima combined within the lane group
+        for (u32 e = threadIdx.x; e < 4 * tnew; e += blockDim.x) {
+            const u32 t = e >> 2, part = e & 3;
             unsigned long long mx = 0;
             if (t < (u32)TP) {
-                for (u32 zz = 0; zz < vrows; zz++) {
+                for (u32 zz = part; zz < vrows; zz += 4) {
                     const unsigned long long w = (unsigned long long)__double_as_longlong(V[zz * (TP + 1) + t]);
                     mx = w > mx ? w : mx;
                 }
-            } else {
+            } else if (part == 0) {
                 mx = (unsigned long long)__double_as_longlong(srt[t] *
                                                                (double)fuse_tail<MODE>(a, t, pbase, sP, sS, false, bad0));
             }
-            sM[t] = mx;
+            unsigned long long o = __shfl_xor(mx, 1, 4);
+            mx = o > mx ? o : mx;
+            o = __shfl_xor(mx, 2, 4);
+            mx = o > mx ? o : mx;
+            if (part == 0) sM[t] = mx;
         }
         __syncthreads();
         // Exact early exit of the sums (as in the batch's gdn class): B_t =
