@@ -628,6 +628,8 @@ __global__ __launch_bounds__(TAB ? kFuseTabBlock : kBlock) void gdn_fused_kernel
     unsigned long long *sM = (unsigned long long *)(smem + a.off_thr);
     double *sThr = (double *)(smem + a.off_thr + 8 * 64);
     u32 *sTW = (u32 *)(smem + a.off_thr + 8 * 64 + 8 * 65);
+    // base 2: Q rows as bit words (64), the flip word, the flag
+    unsigned long long *sB2 = (unsigned long long *)(smem + a.off_thr + 8 * 64 + 8 * 65 + 16);
     u32 bad0 = 0;
     const u64 pbase = lj * tnew * a.bmax;
     if (ok)
@@ -729,6 +731,31 @@ __global__ __launch_bounds__(TAB ? kFuseTabBlock : kBlock) void gdn_fused_kernel
             }
             sThr[tnew] = 0.0;
             sTW[0] = tw;
+            // Base 2 with every term 0 or recip_t = 2^-(t+1) (a digit map of {0, 1} onto
+            // itself; the raw tail values 0 or 1): every partial sum of the reference's
+            // loop is a multiple of 2^-tnew below 1, exact for tnew <= 53, so the sum
+            // is Y 2^-tnew with y_t the bit tnew - 1 - t of Y, and Y = Z xor F, Z the
+            // digits as bits and F the images of digit 0
+            unsigned long long okb = 0, F = 0;
+            if (b == 2 && tnew <= (u32)TP && tnew <= 53) {
+                okb = 1;
+                for (u32 t = 0; t < tnew; t++) {
+                    const double v0 = V[t], v1 = V[(TP + 1) + t], rt = srt[t];
+                    const bool good = t < tmax ? (v0 == 0.0 && v1 == rt) || (v0 == rt && v1 == 0.0)
+                                               : (v0 == 0.0 || v0 == rt);
+                    okb &= good ? 1ull : 0ull;
+                    F |= (unsigned long long)(v0 != 0.0) << (tnew - 1 - t);
+                }
+            }
+            sB2[64] = F;
+            sB2[65] = okb;
+        }
+        if (b == 2 && threadIdx.x >= 64 && threadIdx.x < 64 + (u32)a.mmax) {
+            const u32 m = threadIdx.x - 64;
+            unsigned long long w = 0;
+            for (u32 t = 0; t < tmax && t < tnew && t < (u32)TP; t++)
+                w |= (unsigned long long)(sQ[m * TP + t] & 1u) << (tnew - 1 - t);
+            sB2[m] = w;
         }
         __syncthreads();
     }
@@ -798,6 +825,23 @@ __global__ __launch_bounds__(TAB ? kFuseTabBlock : kBlock) void gdn_fused_kernel
         }
     }
     u32 dlow = (k >> 32) ? (u32)(k % b) : (u32)k - div_q((u32)k, bdiv) * bb;
+    if (TAB && fast && b == 2 && sB2[65] != 0) {
+        // base 2 (see the prologue): the digits as one bit word, a carry row xor per step
+        unsigned long long Z = 0;
+#pragma unroll
+        for (int t = 0; t < TP; t++)
+            if ((u32)t < tnew) Z |= (unsigned long long)(z[t] & 1u) << (tnew - 1 - t);
+        const unsigned long long F = sB2[64];
+        const double sc = ldexp(1.0, -(int)tnew);
+        for (u32 h = 0;; h++) {
+            xo[(u64)h * a.d] = (double)(Z ^ F) * sc;
+            if (h + 1 == H) break;
+            Z ^= sB2[walk_carry(dlow, k, b, bb, bdiv, (u32)a.mmax)];
+            k++;
+        }
+        if (bad | bad0) flag(a.status, bad | bad0);
+        return;
+    }
     if (TAB && fast) {
         constexpr u32 S = (TP + 1) * 8;
         const u32 tw = __builtin_amdgcn_readfirstlane(sTW[0]);  // levels a point usually needs (uniform)
@@ -1092,7 +1136,7 @@ static int fused_gdn(const char *who, bool noalloc, int mode, u64 r, u64 n, u64 
     a.off_tab = a.off_rt + (u32)(8 * (tmax_new + TP));
     a.off_thr = (u32)((a.off_tab + (mode == 1 ? (a.perm_lds ? 4 * rows * bmax : 0) : mode == 2 ? 8 * rows : 0) + 15) &
                       ~(u64)15);
-    const size_t shm = a.off_thr + (tabm ? 8 * 64 + 8 * 65 + 16 : 0);
+    const size_t shm = a.off_thr + (tabm ? 8 * 64 + 8 * 65 + 16 + 8 * 66 : 0);
     const u64 nblocks = (a.ngroups + 7) / 8 * 8 * d;
     if (nblocks > 0x7fffffffull) return qmc_fail(QMC_ERR_ARG, "%s: r*n*d too large", who);
     u32 *ws = nullptr;

@@ -474,6 +474,12 @@ def _oracle_gdn_chain(mode, r, n, d, r_b, mmax, tmax, tnew, n_start, bmax, bases
     (1, 1, 3000, 3, 1, 12, 40, 52, 0, "zerolead"),    # walked levels (digits from the index), tail past tmax
     (1, 2, 3000, 12, 1, 8, 52, 52, 0, "halton"),      # bases 29..37 in the 16-register instance beside the 52
     (1, 1, 2000, 12, 1, 10, 30, 52, 0, "halton"),     # the same with digits 16..29 past its registers, tail rows
+    # base 2 with maps of {0, 1} onto itself: the digits as one bit word, the exact sum in one step
+    (1, 2, 3000, 5, 1, 8, 52, 52, 0, "haltonperm"),
+    (1, 1, 2000, 4, 1, 12, 30, 52, 3, "haltonperm"),  # tail rows (the image of digit 0)
+    (1, 2, 1500, 3, 1, 8, 53, 53, 0, "haltonperm"),   # 53 digits, the longest exact sum
+    (1, 1, 1500, 3, 1, 8, 54, 54, 0, "haltonperm"),   # 54 digits: the level walk
+    (2, 2, 2500, 3, 1, 12, 40, 48, 7, "b2shift"),     # shifts, raw tail shifts of 0 or 1
 ])
 def test_fused_gdn_equals_chain(mode, r, n, d, r_b, mmax, tmax, tnew, n_start, kind):
     """qmc_gdn_gen_scramble_float_dev against the oracle's three-call chain,
@@ -489,13 +495,14 @@ def test_fused_gdn_equals_chain(mode, r, n, d, r_b, mmax, tmax, tnew, n_start, k
     elif kind in ("b593", "b131"):
         bases = np.full((r_b, d), int(kind[1:]), dtype=np.uint64)
     else:
-        bases = np.stack([primes[:d]] * r_b) if kind == "halton" else rng.choice(primes, size=(r_b, d))
+        halton = kind in ("halton", "haltonperm", "b2shift")
+        bases = np.stack([primes[:d]] * r_b) if halton else rng.choice(primes, size=(r_b, d))
         bases = bases.astype(np.uint64)
     C = np.empty((r, d, mmax, tmax), dtype=np.uint64)
     for l in range(r):
         for j in range(d):
             b = int(bases[l % r_b, j])
-            if kind == "halton":  # identity scrambled by a random lower-triangular S (values < b)
+            if kind in ("halton", "haltonperm", "b2shift"):  # identity scrambled by a random lower-triangular S
                 C[l, j] = 0
                 for m in range(min(mmax, tmax)):
                     C[l, j, m, m] = rng.integers(1, b)
@@ -515,12 +522,24 @@ def test_fused_gdn_equals_chain(mode, r, n, d, r_b, mmax, tmax, tnew, n_start, k
                     tab[l, j, t] = rng.permutation(bmax).astype(np.uint64)
         if kind == "wide":
             tab[:, :, ::2, 1] += np.uint64(2**40)
+        if kind == "haltonperm":  # each level's map is a permutation of the column's digits 0..b-1
+            for l in range(r):
+                for j in range(d):
+                    b = int(bases[l % r_b, j])
+                    for t in range(tnew):
+                        tab[l, j, t, :b] = rng.permutation(b).astype(np.uint64)
+                        tab[l, j, t, b:] = np.arange(b, bmax, dtype=np.uint64)
         if kind == "zerolead":
             # every digit of the first 20 levels maps to 0 (a table the reference
             # accepts): the sums stay 0 there, so no point can stop before level 20
             tab[:, :, :20, :] = 0
     elif mode == 2 and kind == "smallshift":
         tab = rng.integers(0, 2**64 - 1, size=(r, d, tnew), dtype=np.uint64)  # raw past tmax
+        for l in range(r):
+            for j in range(d):
+                tab[l, j, :tmax] = rng.integers(0, int(bases[l % r_b, j]), size=tmax, dtype=np.uint64)
+    elif mode == 2 and kind == "b2shift":
+        tab = rng.integers(0, 2, size=(r, d, tnew), dtype=np.uint64)  # raw tail shifts of 0 or 1
         for l in range(r):
             for j in range(d):
                 tab[l, j, :tmax] = rng.integers(0, int(bases[l % r_b, j]), size=tmax, dtype=np.uint64)
